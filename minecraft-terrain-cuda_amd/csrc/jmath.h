@@ -1,0 +1,156 @@
+// Java-language arithmetic semantics used by the TERRAIN step, for host and gfx950 device
+// code. Every function reproduces the JLS / java.lang.Math / Mth operation it names bit for
+// bit, given IEEE-754 round-to-nearest, no FMA contraction (-ffp-contract=off), no
+// flush-to-zero and correctly rounded division/sqrt.
+// Reference semantics: cpu/include/mccuda/javamath.h:61-181, cpu/include/mccuda/df_ops.h:14-80.
+#pragma once
+
+#include <stdint.h>
+#include <math.h>
+
+#if defined(__HIPCC__)
+#include <hip/hip_runtime.h>
+#define MCT_HD __host__ __device__
+#define MCT_FI __forceinline__
+#else
+#define MCT_HD
+#define MCT_FI inline __attribute__((always_inline))
+#endif
+
+namespace mct {
+
+MCT_HD MCT_FI uint32_t f2u(float f) { uint32_t u; __builtin_memcpy(&u, &f, 4); return u; }
+MCT_HD MCT_FI float u2f(uint32_t u) { float f; __builtin_memcpy(&f, &u, 4); return f; }
+MCT_HD MCT_FI uint64_t d2u(double d) { uint64_t u; __builtin_memcpy(&u, &d, 8); return u; }
+MCT_HD MCT_FI double u2d(uint64_t u) { double d; __builtin_memcpy(&d, &u, 8); return d; }
+
+// (int) v for a double (JLS 5.1.3): NaN -> 0, saturating, otherwise toward zero.
+MCT_HD MCT_FI int32_t d2i(double v) {
+    if (!(v == v)) return 0;
+    if (v >= 2147483647.0) return 2147483647;
+    if (v <= -2147483648.0) return -2147483647 - 1;
+    return static_cast<int32_t>(v);
+}
+
+// Mth.floor(double) = (int) Math.floor(v). floor() is exact; the cast saturates.
+MCT_HD MCT_FI int32_t mfloor(double v) { return d2i(floor(v)); }
+
+// Java int + and * wrap modulo 2^32.
+MCT_HD MCT_FI int32_t iadd(int32_t a, int32_t b) { return static_cast<int32_t>(static_cast<uint32_t>(a) + static_cast<uint32_t>(b)); }
+MCT_HD MCT_FI int32_t imul(int32_t a, int32_t b) { return static_cast<int32_t>(static_cast<uint32_t>(a) * static_cast<uint32_t>(b)); }
+
+// Math.floorDiv / floorMod (int).
+MCT_HD MCT_FI int32_t fdiv(int32_t a, int32_t b) {
+    int32_t q = a / b;
+    return ((a % b) != 0 && ((a ^ b) < 0)) ? q - 1 : q;
+}
+MCT_HD MCT_FI int32_t fmod_(int32_t a, int32_t b) { return a - fdiv(a, b) * b; }
+
+// Math.min / Math.max (float): NaN wins, -0.0 < +0.0.
+MCT_HD MCT_FI float jminf(float a, float b) {
+    if (a != a) return a;
+    if (a == 0.0f && b == 0.0f && (f2u(b) >> 31)) return b;
+    return a <= b ? a : b;
+}
+MCT_HD MCT_FI float jmaxf(float a, float b) {
+    if (a != a) return a;
+    if (a == 0.0f && b == 0.0f && (f2u(a) >> 31)) return b;
+    return a >= b ? a : b;
+}
+MCT_HD MCT_FI double jmind(double a, double b) {
+    if (a != a) return a;
+    if (a == 0.0 && b == 0.0 && (d2u(b) >> 63)) return b;
+    return a <= b ? a : b;
+}
+MCT_HD MCT_FI double jmaxd(double a, double b) {
+    if (a != a) return a;
+    if (a == 0.0 && b == 0.0 && (d2u(a) >> 63)) return b;
+    return a >= b ? a : b;
+}
+MCT_HD MCT_FI int32_t jmini(int32_t a, int32_t b) { return a <= b ? a : b; }
+MCT_HD MCT_FI int32_t jmaxi(int32_t a, int32_t b) { return a >= b ? a : b; }
+
+// Math.abs(float) clears the sign bit (also of NaN and -0.0).
+MCT_HD MCT_FI float jabsf(float a) { return u2f(f2u(a) & 0x7FFFFFFFu); }
+
+// Mth.clamp(float|double|int): v < lo ? lo : Math.min(v, hi)
+MCT_HD MCT_FI float clampf(float v, float lo, float hi) { return v < lo ? lo : jminf(v, hi); }
+MCT_HD MCT_FI double clampd(double v, double lo, double hi) { return v < lo ? lo : jmind(v, hi); }
+MCT_HD MCT_FI int32_t clampi(int32_t v, int32_t lo, int32_t hi) { return jmini(jmaxi(v, lo), hi); }
+
+// Math.round(float) = (int) floor(a + 0.5) evaluated exactly in double.
+MCT_HD MCT_FI int32_t jroundf(float a) { return d2i(floor(static_cast<double>(a) + 0.5)); }
+
+// Mth.lerp(float t, a, b) = a + t * (b - a)
+MCT_HD MCT_FI float lerpf(float t, float a, float b) {
+    const float d = b - a;
+    const float m = t * d;
+    return a + m;
+}
+
+// Mth.lerp3 over the cube corners v[xyz]: x first, then y, then z.
+MCT_HD MCT_FI float lerp3f(float tx, float ty, float tz, float v000, float v100, float v010, float v110, float v001, float v101,
+                           float v011, float v111) {
+    const float a = lerpf(ty, lerpf(tx, v000, v100), lerpf(tx, v010, v110));
+    const float b = lerpf(ty, lerpf(tx, v001, v101), lerpf(tx, v011, v111));
+    return lerpf(tz, a, b);
+}
+
+// Mth.smoothstep(t) = t^3 * (t * (t * 6 - 15) + 10), in float, in that association.
+MCT_HD MCT_FI float smooth(float t) {
+    const float t3 = (t * t) * t;
+    float p = t * 6.0f;
+    p = p - 15.0f;
+    p = t * p;
+    p = p + 10.0f;
+    return t3 * p;
+}
+
+MCT_HD MCT_FI float cubef(float x) { return (x * x) * x; }
+
+// ---- density-function unary ops (UnaryFunction) ----------------------------------
+MCT_HD MCT_FI float op_squeeze(float x) {
+    const float c = clampf(x, -1.0f, 1.0f);
+    return c / 2.0f - cubef(c) / 24.0f;
+}
+MCT_HD MCT_FI float op_leaky(float x, float k) { return x > 0.0f ? x : x * k; }
+// Mth.sqrt(float) = (float) Math.sqrt(x): the double sqrt of a float input, rounded to float.
+MCT_HD MCT_FI float op_sqrt(float x) { return static_cast<float>(sqrt(static_cast<double>(x))); }
+// Math.signum(float)
+MCT_HD MCT_FI float op_sign(float x) { return (x != x || x == 0.0f) ? x : (x > 0.0f ? 1.0f : -1.0f); }
+
+// Volume-path MIN/MAX: the buffered (first) value stays unless the second is strictly
+// smaller / larger.
+MCT_HD MCT_FI float vmin(float first, float second) { return second < first ? second : first; }
+MCT_HD MCT_FI float vmax(float first, float second) { return second > first ? second : first; }
+
+MCT_HD MCT_FI bool in_range(float v, float lo_incl, float hi_excl) { return v >= lo_incl && v < hi_excl; }
+
+// GradientFunction (axis gradients with tiling). mode 0 clamp_to_edge, 1 repeat,
+// 2 mirrored_repeat; factor = (toValue - fromValue) / (to - from) in float.
+MCT_HD MCT_FI float gradient_at(int mode, int32_t from, int32_t lo, int32_t hi, int32_t range, float from_value, float factor, int32_t c) {
+    if (mode == 0) return from_value + static_cast<float>(clampi(c, lo, hi) - from) * factor;
+    const int32_t rel = iadd(c, -from);
+    if (mode == 1) return from_value + static_cast<float>(fmod_(rel, range)) * factor;
+    const int32_t tile = fdiv(rel, range);
+    const int32_t local = rel - tile * range;
+    return (tile & 1) == 0 ? from_value + static_cast<float>(local) * factor : from_value + static_cast<float>(range - local) * factor;
+}
+
+// InterpolatedFunction cell fill for one block from its 8 cell-corner values (v[xyz],
+// x/z cell size 4 or 16, y cell size 8 or 1): lerp along z, then x, then y as the running
+// sum of `step`, exactly as the volume path accumulates it.
+MCT_HD MCT_FI float cell_fill(float v000, float v100, float v010, float v110, float v001, float v101, float v011, float v111, int xi, int yi,
+                              int zi, float inv_xz, float inv_y) {
+    const float az = static_cast<float>(zi) * inv_xz;
+    const float v00 = lerpf(az, v000, v001), v01 = lerpf(az, v010, v011);
+    const float v10 = lerpf(az, v100, v101), v11 = lerpf(az, v110, v111);
+    const float ax = static_cast<float>(xi) * inv_xz;
+    const float lo = lerpf(ax, v00, v10), hi = lerpf(ax, v01, v11);
+    const float step = (hi - lo) * inv_y;
+    float value = lo + step * 0.0f;
+    for (int y = 0; y < yi; ++y) value += step;
+    return value;
+}
+
+}  // namespace mct

@@ -1,0 +1,179 @@
+// Improved-Perlin gradient noise as the TERRAIN step samples it, host and device.
+//
+// Two sampling paths exist and must not be mixed (they round differently):
+//   point  (PerlinNoise.get): coordinates (block * scale) * frequency, gradient dot
+//          (gx*x + gy*y) + gz*z, trilinear lerp at the point;
+//   volume (PerlinNoise.addToVolume): coordinates block * (scale * frequency), dot
+//          (gx*x + gz*z) + gy*y with the xz part shared along y.
+// Smeared octaves (BlendedNoise) shift the y fraction by SmearedPerlinNoise.computeFudgeY.
+// Reference semantics: cpu/include/mccuda/noise_eval.h:21-304.
+#pragma once
+
+#include "jmath.h"
+
+namespace mct {
+
+// One gradient-noise octave of one seed: random origin and permutation.
+struct Octave {
+    double ox, oy, oz;
+    uint8_t p[256];
+};
+
+// GRADIENT[h & 15] components packed as 2-bit fields (value + 1), entry h at bits 2h..2h+1.
+//   gx: 1,-1,1,-1,1,-1,1,-1,0,0,0,0,1,0,-1,0
+//   gy: 1,1,-1,-1,0,0,0,0,1,-1,1,-1,1,-1,1,-1
+//   gz: 0,0,0,0,1,1,-1,-1,1,1,-1,-1,0,1,0,-1
+constexpr uint32_t kGX = 0x46552222u, kGY = 0x2222550Au, kGZ = 0x190A0A55u;
+MCT_HD MCT_FI float gcomp(uint32_t packed, int h) { return static_cast<float>(static_cast<int>((packed >> ((h & 15) << 1)) & 3u) - 1); }
+
+// Math.nextDown(2^24): GradientNoise.wrap leaves |x| below it untouched.
+constexpr double kWrapLimit = 0x1.fffffffffffffp+23;   // bits 0x416FFFFFFFFFFFFF
+
+// GradientNoise.wrap: x - floor(x / 2^25 + 1/2) * 2^25 outside [-limit, limit).
+MCT_HD MCT_FI double wrap(double x) {
+    if (x >= -kWrapLimit && x < kWrapLimit) return x;
+    double q = x / 33554432.0;
+    q = q + 0.5;
+    q = floor(q);
+    q = q * 33554432.0;
+    return x - q;
+}
+
+MCT_HD MCT_FI int P(const Octave& o, int32_t i) { return o.p[i & 0xFF]; }
+
+// Lattice hashes of the 8 cell corners, h[xyz] with bit 4 = x, 2 = y, 1 = z.
+struct Hash8 {
+    int h000, h100, h010, h110, h001, h101, h011, h111;
+};
+
+MCT_HD MCT_FI Hash8 hash8(const Octave& o, int32_t fx, int32_t fy, int32_t fz) {
+    const int a = P(o, fx), b = P(o, iadd(fx, 1));
+    const int a0 = P(o, iadd(a, fy)), a1 = P(o, iadd(iadd(a, fy), 1));
+    const int b0 = P(o, iadd(b, fy)), b1 = P(o, iadd(iadd(b, fy), 1));
+    Hash8 h;
+    h.h000 = P(o, iadd(a0, fz));
+    h.h100 = P(o, iadd(b0, fz));
+    h.h010 = P(o, iadd(a1, fz));
+    h.h110 = P(o, iadd(b1, fz));
+    h.h001 = P(o, iadd(iadd(a0, fz), 1));
+    h.h101 = P(o, iadd(iadd(b0, fz), 1));
+    h.h011 = P(o, iadd(iadd(a1, fz), 1));
+    h.h111 = P(o, iadd(iadd(b1, fz), 1));
+    return h;
+}
+
+// Gradient dot for the point path: (gx*x + gy*y) + gz*z.
+MCT_HD MCT_FI float dot3(int h, float x, float y, float z) {
+    const float a = gcomp(kGX, h) * x;
+    const float b = gcomp(kGY, h) * y;
+    const float c = gcomp(kGZ, h) * z;
+    return (a + b) + c;
+}
+
+// PerlinNoise.sampleAndLerp (point path); ty is the y fraction the smoothstep uses.
+MCT_HD MCT_FI float sample_lerp(const Octave& o, int32_t fx, int32_t fy, int32_t fz, float rx, float ry, float rz, float ty) {
+    const Hash8 h = hash8(o, fx, fy, fz);
+    const float rx1 = rx - 1.0f, ry1 = ry - 1.0f, rz1 = rz - 1.0f;
+    const float d000 = dot3(h.h000, rx, ry, rz);
+    const float d100 = dot3(h.h100, rx1, ry, rz);
+    const float d010 = dot3(h.h010, rx, ry1, rz);
+    const float d110 = dot3(h.h110, rx1, ry1, rz);
+    const float d001 = dot3(h.h001, rx, ry, rz1);
+    const float d101 = dot3(h.h101, rx1, ry, rz1);
+    const float d011 = dot3(h.h011, rx, ry1, rz1);
+    const float d111 = dot3(h.h111, rx1, ry1, rz1);
+    return lerp3f(smooth(rx), smooth(ty), smooth(rz), d000, d100, d010, d110, d001, d101, d011, d111);
+}
+
+// PerlinNoise.get(x, y, z)
+MCT_HD MCT_FI float perlin_point(const Octave& o, double px, double py, double pz) {
+    const double x = wrap(px) + o.ox, y = wrap(py) + o.oy, z = wrap(pz) + o.oz;
+    const int32_t fx = mfloor(x), fy = mfloor(y), fz = mfloor(z);
+    const float rx = static_cast<float>(x - static_cast<double>(fx));
+    const float ry = static_cast<float>(y - static_cast<double>(fy));
+    const float rz = static_cast<float>(z - static_cast<double>(fz));
+    return sample_lerp(o, fx, fy, fz, rx, ry, rz, ry);
+}
+
+// SmearedPerlinNoise.computeFudgeY
+MCT_HD MCT_FI double fudge_y(double original_y, double rel_y, double smear) {
+    const double limit = (original_y >= 0.0 && original_y < rel_y) ? original_y : rel_y;
+    double q = limit / smear;
+    q = q + static_cast<double>(1.0E-7f);
+    return static_cast<double>(mfloor(q)) * smear;
+}
+
+// SmearedPerlinNoise.get(x, y, z)
+MCT_HD MCT_FI float smeared_point(const Octave& o, double smear, double px, double py, double pz) {
+    const double x = wrap(px) + o.ox, y = wrap(py) + o.oy, z = wrap(pz) + o.oz;
+    const int32_t fx = mfloor(x), fy = mfloor(y), fz = mfloor(z);
+    const float rx = static_cast<float>(x - static_cast<double>(fx));
+    const double rel_y = y - static_cast<double>(fy);
+    const float rz = static_cast<float>(z - static_cast<double>(fz));
+    const float fudged = static_cast<float>(rel_y - fudge_y(py, rel_y, smear));
+    return sample_lerp(o, fx, fy, fz, rx, fudged, rz, static_cast<float>(rel_y));
+}
+
+// Coordinate setup of one axis of the volume path, the only FP64 work of a sample:
+// lattice cell, fraction handed to the gradients, smoothstep weight. It depends on one
+// block coordinate and one octave, so it can be tabulated per distinct coordinate.
+struct Axis {
+    int32_t f;
+    float r;
+    float a;
+};
+
+// x and z (and y of plain octaves): v = wrap(b * s) + offset
+MCT_HD MCT_FI Axis axis_plain(int32_t b, double s, double offset) {
+    const double v = wrap(static_cast<double>(b) * s) + offset;
+    const int32_t f = mfloor(v);
+    const float r = static_cast<float>(v - static_cast<double>(f));
+    return {f, r, smooth(r)};
+}
+
+// y of smeared octaves: the smoothstep uses the true fraction, the gradients the fudged one.
+MCT_HD MCT_FI Axis axis_smeared(int32_t b, double s, double offset, double smear) {
+    const double orig = static_cast<double>(b) * s;
+    const double v = wrap(orig) + offset;
+    const int32_t f = mfloor(v);
+    const double rel = v - static_cast<double>(f);
+    const float a = smooth(static_cast<float>(rel));
+    const float r = static_cast<float>(rel - fudge_y(orig, rel, smear));
+    return {f, r, a};
+}
+
+// Volume-path sample from the three axis records: xz dots per corner, plus gy * ry.
+MCT_HD MCT_FI float volume_sample(const Octave& o, const Axis& X, const Axis& Y, const Axis& Z) {
+    const Hash8 h = hash8(o, X.f, Y.f, Z.f);
+    const float rx = X.r, rz = Z.r, rx1 = rx - 1.0f, rz1 = rz - 1.0f;
+    const float ry = Y.r, ry1 = ry - 1.0f;
+#define MCT_XZ(hh, xx, zz) ((gcomp(kGX, hh) * (xx)) + (gcomp(kGZ, hh) * (zz)))
+    const float c000 = MCT_XZ(h.h000, rx, rz) + gcomp(kGY, h.h000) * ry;
+    const float c100 = MCT_XZ(h.h100, rx1, rz) + gcomp(kGY, h.h100) * ry;
+    const float c010 = MCT_XZ(h.h010, rx, rz) + gcomp(kGY, h.h010) * ry1;
+    const float c110 = MCT_XZ(h.h110, rx1, rz) + gcomp(kGY, h.h110) * ry1;
+    const float c001 = MCT_XZ(h.h001, rx, rz1) + gcomp(kGY, h.h001) * ry;
+    const float c101 = MCT_XZ(h.h101, rx1, rz1) + gcomp(kGY, h.h101) * ry;
+    const float c011 = MCT_XZ(h.h011, rx, rz1) + gcomp(kGY, h.h011) * ry1;
+    const float c111 = MCT_XZ(h.h111, rx1, rz1) + gcomp(kGY, h.h111) * ry1;
+#undef MCT_XZ
+    return lerp3f(X.a, Y.a, Z.a, c000, c100, c010, c110, c001, c101, c011, c111);
+}
+
+// PerlinNoise.addToVolume, one sample (before the amplitude multiply).
+MCT_HD MCT_FI float perlin_volume(const Octave& o, int32_t bx, int32_t by, int32_t bz, double xs, double ys) {
+    const Axis Z = axis_plain(bz, xs, o.oz);
+    const Axis X = axis_plain(bx, xs, o.ox);
+    const Axis Y = axis_plain(by, ys, o.oy);
+    return volume_sample(o, X, Y, Z);
+}
+
+// SmearedPerlinNoise.addToVolume, one sample.
+MCT_HD MCT_FI float smeared_volume(const Octave& o, double smear, int32_t bx, int32_t by, int32_t bz, double xs, double ys) {
+    const Axis Z = axis_plain(bz, xs, o.oz);
+    const Axis X = axis_plain(bx, xs, o.ox);
+    const Axis Y = axis_smeared(by, ys, o.oy, smear);
+    return volume_sample(o, X, Y, Z);
+}
+
+}  // namespace mct
