@@ -306,6 +306,8 @@ class Gen:
 
     def struct(self, ctx, roots, extra=None):
         reach = self.reach(ctx, roots)
+        self.reaches = getattr(self, 'reaches', {})
+        self.reaches[ctx.name] = reach
         o = ['struct %s {' % ctx.name, '    const mct::Octave* O;', '    int32_t x, y, z;']
         if ctx.slices == 'col':
             o.append('    const float* col;')
@@ -540,18 +542,19 @@ class Gen:
         for st in range(n - 1):
             i3 = i2 + '        '
             L.append('%s    case %d: {' % (i2, st))
-            L.append('%sconstexpr float x1 = %s, x2 = %s, d1 = %s, d2 = %s;' % (i3, fl(loc[st]), fl(loc[st + 1]), fl(der[st]), fl(der[st + 1])))
-            L.append('%sconst float t = (in%s - x1) / (x2 - x1);' % (i3, q))
-            L.append('%sfloat y1, y2;' % i3)
-            self.emit_spline(s['val'][st], 'y1', L, idx, counter, i3)
-            self.emit_spline(s['val'][st + 1], 'y2', L, idx, counter, i3)
-            L.append('%sconst float dx = x2 - x1, dy = y2 - y1;' % i3)
-            L.append('%sconst float a = d1 * dx - dy;' % i3)
-            L.append('%sconst float b = -d2 * dx + dy;' % i3)
-            L.append('%sconst float l1 = mct::lerpf(t, y1, y2);' % i3)
-            L.append('%sconst float w = t * (1.0f - t);' % i3)
-            L.append('%sconst float l2 = mct::lerpf(t, a, b);' % i3)
-            L.append('%s%s = l1 + w * l2;' % (i3, target))
+            L.append('%sconstexpr float x1%s = %s, x2%s = %s, d1%s = %s, d2%s = %s;' % (i3, q, fl(loc[st]), q, fl(loc[st + 1]), q, fl(der[st]), q,
+                                                                                 fl(der[st + 1])))
+            L.append('%sconst float t%s = (in%s - x1%s) / (x2%s - x1%s);' % (i3, q, q, q, q, q))
+            L.append('%sfloat y1%s, y2%s;' % (i3, q, q))
+            self.emit_spline(s['val'][st], 'y1' + q, L, idx, counter, i3)
+            self.emit_spline(s['val'][st + 1], 'y2' + q, L, idx, counter, i3)
+            L.append('%sconst float dx%s = x2%s - x1%s, dy%s = y2%s - y1%s;' % (i3, q, q, q, q, q, q))
+            L.append('%sconst float a%s = d1%s * dx%s - dy%s;' % (i3, q, q, q, q))
+            L.append('%sconst float b%s = -d2%s * dx%s + dy%s;' % (i3, q, q, q, q))
+            L.append('%sconst float l1%s = mct::lerpf(t%s, y1%s, y2%s);' % (i3, q, q, q, q))
+            L.append('%sconst float w%s = t%s * (1.0f - t%s);' % (i3, q, q, q))
+            L.append('%sconst float l2%s = mct::lerpf(t%s, a%s, b%s);' % (i3, q, q, q, q))
+            L.append('%s%s = l1%s + w%s * l2%s;' % (i3, target, q, q, q))
             L.append('%sbreak;' % i3)
             L.append('%s    }' % i2)
         L.append('%s    default: %s = 0.0f; break;' % (i2, target))
@@ -657,6 +660,8 @@ class Gen:
     def model_header(self):
         P, tm, N = self.plan, self.tm, self.ir.nodes
         o = ['// GENERATED by codegen/gen_hip.py from the IR and terrain model. Do not edit.', '#pragma once', '#include <stdint.h>',
+             '// Tables the device reads (seed setup) live in constant memory in the device pass.',
+             '#if defined(__HIP_DEVICE_COMPILE__)', '#define MCT_TABLE static __constant__ const', '#else', '#define MCT_TABLE static const', '#endif',
              'namespace mctgen {']
         o.append('constexpr uint64_t kIrHash = %s, kTerrainHash = %s;' % (u64(self.ir_hash), u64(self.tm_hash)))
         o.append('constexpr int kMinY = %d, kHeight = %d, kSeaLevel = %d;' % (self.ir.doc['min_y'], self.ir.doc['height'], self.ir.doc['sea_level']))
@@ -670,7 +675,7 @@ class Gen:
         o.append('static const char* const kNoiseKeys[kNumNoises] = {%s};' % ', '.join('"%s"' % k for k in P.keys))
         o.append('// per layer: kind 0 = NormalNoise (noise key, octave name, fork), 1 = BlendedNoise (node, position in its 40 draws)')
         o.append('struct LayerRecipe { int kind, a, b, c; };')
-        o.append('static const LayerRecipe kLayerRecipe[kNumLayers] = {%s};' % ', '.join('{%d, %d, %d, %d}' % r for r in P.recipe))
+        o.append('MCT_TABLE LayerRecipe kLayerRecipe[kNumLayers] = {%s};' % ', '.join('{%d, %d, %d, %d}' % r for r in P.recipe))
         o.append('static const double kLayerSmear[kNumLayers] = {%s};' % ', '.join(dl(s) for s in P.smear))
         o.append('static const double kLayerFreq[kNumLayers] = {%s};' % ', '.join(dl(s) for s in P.freq))
         o.append('static const float kLayerAmp[kNumLayers] = {%s};' % ', '.join(fl(s) for s in P.amp))
@@ -678,14 +683,14 @@ class Gen:
         o.append('static const int kNoiseCount[kNumNoises] = {%s};' % ', '.join(str(b) for _, b in P.noise_range))
         md = lambda s: '{%s, %s}' % tuple(u64(v) for v in M.md5_longs(s))
         o.append('struct Md5 { uint64_t lo, hi; };')
-        o.append('static const Md5 kNoiseDigest[kNumNoises] = {%s};' % ', '.join(md(k) for k in P.keys))
+        o.append('MCT_TABLE Md5 kNoiseDigest[kNumNoises] = {%s};' % ', '.join(md(k) for k in P.keys))
         o.append('constexpr int kNumOctaveNames = %d;' % len(P.octave_names))
-        o.append('static const Md5 kOctaveDigest[kNumOctaveNames] = {%s};' % ', '.join(md(k) for k in P.octave_names))
-        o.append('static const Md5 kTerrainDigest = %s, kAquiferDigest = %s, kOreDigest = %s, kClayBandsDigest = %s;' % (
+        o.append('MCT_TABLE Md5 kOctaveDigest[kNumOctaveNames] = {%s};' % ', '.join(md(k) for k in P.octave_names))
+        o.append('MCT_TABLE Md5 kTerrainDigest = %s, kAquiferDigest = %s, kOreDigest = %s, kClayBandsDigest = %s;' % (
             md('minecraft:terrain'), md('minecraft:aquifer'), md('minecraft:ore'), md('minecraft:clay_bands')))
         o.append('constexpr int kNumRandomNames = %d;' % len(self.rand_names))
         o.append('static const char* const kRandomNames[%d] = {%s};' % (max(1, len(self.rand_names)), ', '.join('"%s"' % k for k in self.rand_names) or '""'))
-        o.append('static const Md5 kRandomDigest[%d] = {%s};' % (max(1, len(self.rand_names)), ', '.join(md(k) for k in self.rand_names) or '{0, 0}'))
+        o.append('MCT_TABLE Md5 kRandomDigest[%d] = {%s};' % (max(1, len(self.rand_names)), ', '.join(md(k) for k in self.rand_names) or '{0, 0}'))
         # terrain model
         pal = tm['palette']
         if len(pal) > 64:
@@ -820,6 +825,25 @@ class Gen:
         o += main
         o += e
         o.append('}  // namespace mctgen')
+        dbg = ['// GENERATED debug dispatchers (bisection against the interpreter); not part of the library.', '#pragma once',
+               '#include "evaluators.h"', 'namespace mctgen {']
+        dbg.append('inline float debug_corner_node(const mct::Octave* O, int32_t x, int32_t y, int32_t z, const float* col, int id) {')
+        dbg.append('    Corner e{O, x, y, z, col};')
+        dbg.append('    switch (id) {')
+        dbg += ['    case %d: return e.n%d();' % (i, i) for i in self.reaches['Corner']]
+        dbg.append('    }')
+        dbg.append('    return __builtin_nanf("");')
+        dbg.append('}')
+        dbg.append('inline float debug_point_node(const mct::Octave* O, int32_t x, int32_t y, int32_t z, int id) {')
+        dbg.append('    PointY0 sub{O, x, 0, z};')
+        dbg.append('    Point e{O, x, y, z, &sub};')
+        dbg.append('    switch (id) {')
+        dbg += ['    case %d: return e.n%d();' % (i, i) for i in self.reaches['Point']]
+        dbg.append('    }')
+        dbg.append('    return __builtin_nanf("");')
+        dbg.append('}')
+        dbg.append('}  // namespace mctgen')
+        self.debug_lines = dbg
         r = ['// GENERATED by codegen/gen_hip.py from the terrain model. Do not edit.', '#pragma once', '#include <stdint.h>',
              '#include "evaluators.h"', 'namespace mctgen {']
         r += rules
@@ -844,7 +868,7 @@ def main():
     ev, rules = g.generate()
     mh = g.model_header()
     os.makedirs(a.outdir, exist_ok=True)
-    for name, lines in (('evaluators.h', ev), ('rules.h', rules), ('model.h', mh)):
+    for name, lines in (('evaluators.h', ev), ('rules.h', rules), ('model.h', mh), ('debug.h', g.debug_lines)):
         path = os.path.join(a.outdir, name)
         text = '\n'.join(lines) + '\n'
         old = open(path).read() if os.path.exists(path) else None

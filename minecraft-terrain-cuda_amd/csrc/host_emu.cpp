@@ -1,0 +1,220 @@
+// Host emulation of the GPU TERRAIN pipeline: the kernel bodies of pipeline.h and the
+// generated evaluators, compiled for the CPU and run stage by stage in the same order and
+// with the same data layouts as the device launches (the dedupe uses std::sort instead of
+// the device radix sort). It shares the product's code paths, so it is a debugging and
+// bisection tool for them (per stage, per value), not an independent check: the
+// independent checks are the oracle and the reference port.
+#include <algorithm>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#include "pipeline.h"
+
+using namespace mct;
+
+namespace {
+
+thread_local std::string g_err;
+
+struct Emu {
+    ModelDev M{};
+    std::vector<TreeNodeDev> tree;
+    std::vector<uint8_t> leaf_biome;
+    std::vector<int32_t> leaf_node;
+    std::unique_ptr<SeedDev> seed;
+    int64_t seed_value = 0;
+    bool has_seed = false;
+};
+
+void fill_model(Emu& e) {
+    e.tree.resize(mctgen::kTreeNodes);
+    for (int i = 0; i < mctgen::kTreeNodes; ++i) {
+        const mctgen::TreeNode& t = mctgen::kTree[i];
+        TreeNodeDev& d = e.tree[static_cast<size_t>(i)];
+        for (int k = 0; k < 7; ++k) { d.lo[k] = t.lo[k]; d.hi[k] = t.hi[k]; }
+        d.first = t.first;
+        d.count = t.count;
+        d.leaf = t.leaf;
+        d.pad = 0;
+    }
+    e.leaf_biome.assign(mctgen::kLeafBiome, mctgen::kLeafBiome + mctgen::kTreeLeaves);
+    e.leaf_node.assign(mctgen::kLeafNode, mctgen::kLeafNode + mctgen::kTreeLeaves);
+    ModelDev& M = e.M;
+    M.tree = e.tree.data();
+    M.leaf_biome = e.leaf_biome.data();
+    M.leaf_node = e.leaf_node.data();
+    std::memcpy(M.block_flags, mctgen::kBlockFlags, 64);
+    std::memcpy(M.biome_temp, mctgen::kBiomeTemp, sizeof M.biome_temp);
+    std::memcpy(M.biome_frozen, mctgen::kBiomeFrozen, sizeof M.biome_frozen);
+    std::memcpy(M.simplex, mctgen::kSimplexPerm, sizeof M.simplex);
+    for (int k = 0; k < 7; ++k) M.terracotta[k] = mctgen::kTerracotta[k];
+    for (int k = 0; k < mctgen::kNumOreRules; ++k) {
+        for (int j = 0; j < 3; ++j) M.ore_state[k][j] = mctgen::kOreState[k][j];
+        M.ore_raw[k] = mctgen::kOreRawChance[k];
+    }
+}
+
+void build_seed(Emu& e, int64_t seed) {
+    if (e.has_seed && e.seed_value == seed) return;
+    if (!e.seed) e.seed = std::make_unique<SeedDev>();
+    SeedDev& S = *e.seed;
+    uint8_t scratch[256];
+    int blended_first[8];
+    for (int b = 0; b < 8; ++b) blended_first[b] = -1;
+    for (int l = 0; l < mctgen::kNumLayers; ++l) {
+        const mctgen::LayerRecipe& r = mctgen::kLayerRecipe[l];
+        if (r.kind == 0) seed_layer(S, seed, l, scratch);
+        else if (blended_first[r.a] < 0) blended_first[r.a] = l;
+    }
+    for (int b = 0; b < mctgen::kNumBlended; ++b) seed_blended(S, seed, b, blended_first[b], scratch);
+    seed_misc(S, seed, e.M);
+    e.seed_value = seed;
+    e.has_seed = true;
+}
+
+// unique sorted keys and the map key -> unique index
+void dedupe(const std::vector<uint64_t>& keys, std::vector<uint64_t>& uniq, std::vector<int>& map) {
+    uniq = keys;
+    std::sort(uniq.begin(), uniq.end());
+    uniq.erase(std::unique(uniq.begin(), uniq.end()), uniq.end());
+    map.resize(keys.size());
+    for (size_t i = 0; i < keys.size(); ++i) map[i] = static_cast<int>(std::lower_bound(uniq.begin(), uniq.end(), keys[i]) - uniq.begin());
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* mcte_last_error() { return g_err.c_str(); }
+
+void* mcte_create() {
+    auto e = new Emu;
+    fill_model(*e);
+    return e;
+}
+
+void mcte_destroy(void* p) { delete static_cast<Emu*>(p); }
+
+// Seed tables as the device builds them: per layer {ox, oy, oz} and 256 perm bytes.
+int mcte_seed_tables(void* p, int64_t seed, double* offsets, uint8_t* perms, int64_t* zoom_seed, uint64_t* positional6, int32_t* clay192) {
+    Emu& e = *static_cast<Emu*>(p);
+    build_seed(e, seed);
+    const SeedDev& S = *e.seed;
+    for (int l = 0; l < mctgen::kNumLayers; ++l) {
+        if (offsets) { offsets[3 * l] = S.oct[l].ox; offsets[3 * l + 1] = S.oct[l].oy; offsets[3 * l + 2] = S.oct[l].oz; }
+        if (perms) std::memcpy(perms + 256 * l, S.oct[l].p, 256);
+    }
+    if (zoom_seed) *zoom_seed = S.zoom_seed;
+    if (positional6) {
+        positional6[0] = S.random.lo; positional6[1] = S.random.hi;
+        positional6[2] = S.aquifer.lo; positional6[3] = S.aquifer.hi;
+        positional6[4] = S.ore.lo; positional6[5] = S.ore.hi;
+    }
+    if (clay192) for (int i = 0; i < 192; ++i) clay192[i] = S.clay[i];
+    return mctgen::kNumLayers;
+}
+
+// Full pipeline for one seed and n chunks. Outputs (nullable): terrain and aquifer fill u8 per
+// block, biomes u8 per quart, density f32 per block (canonical layouts, chunks back to back).
+// stats[0] = chunks redone in exact order after the speculative surface pass.
+int mcte_generate(void* p, int64_t seed, const int32_t* xz, int n, uint8_t* terrain, uint8_t* biomes, uint8_t* aquifer, float* density, int64_t* stats) {
+    try {
+        Emu& e = *static_cast<Emu*>(p);
+        build_seed(e, seed);
+        const SeedDev* seeds = e.seed.get();
+        const ModelDev& M = e.M;
+        std::vector<ChunkRef> ch(static_cast<size_t>(n));
+        for (int i = 0; i < n; ++i) ch[i] = {xz[2 * i], xz[2 * i + 1], 0, 0};
+        // biome chunks: union of the 3x3 neighbourhoods
+        std::unordered_map<uint64_t, int> bindex;
+        std::vector<ChunkRef> bch;
+        std::vector<int> nb(static_cast<size_t>(n) * 9);
+        for (int i = 0; i < n; ++i)
+            for (int dz = -1; dz <= 1; ++dz)
+                for (int dx = -1; dx <= 1; ++dx) {
+                    const int bx = ch[i].x + dx, bz = ch[i].z + dz;
+                    const uint64_t key = (static_cast<uint64_t>(static_cast<uint32_t>(bx)) << 32) | static_cast<uint32_t>(bz);
+                    auto it = bindex.find(key);
+                    if (it == bindex.end()) {
+                        it = bindex.emplace(key, static_cast<int>(bch.size())).first;
+                        bch.push_back({bx, bz, 0, 0});
+                    }
+                    nb[static_cast<size_t>(i) * 9 + (dz + 1) * 3 + dx + 1] = it->second;
+                }
+        const int nbk = static_cast<int>(bch.size());
+        std::vector<float> bcol(static_cast<size_t>(nbk) * 16 * (mctgen::kNumClimateColumns > 0 ? mctgen::kNumClimateColumns : 1));
+        std::vector<int64_t> targets(static_cast<size_t>(nbk) * kQPer * 7), mind(static_cast<size_t>(nbk) * kQPer);
+        std::vector<int32_t> lstar(static_cast<size_t>(nbk) * kQPer);
+        std::vector<uint8_t> tie(static_cast<size_t>(nbk) * kQPer), bb(static_cast<size_t>(nbk) * kQPer);
+        for (int i = 0; i < nbk * 16; ++i) biome_column(seeds, bch.data(), bcol.data(), i);
+        for (int i = 0; i < nbk * 16 * kBiomeRuns; ++i) biome_quarts(seeds, M, bch.data(), bcol.data(), targets.data(), lstar.data(), mind.data(), i);
+        for (int i = 0; i < nbk * kQPer; ++i) biome_tie(M, targets.data(), lstar.data(), mind.data(), tie.data(), i);
+        for (int b = 0; b < nbk; ++b) biome_chain_seq(M, targets.data(), lstar.data(), mind.data(), bb.data(), b);
+        // corner columns and corners
+        std::vector<uint64_t> ckeys(static_cast<size_t>(n) * kCols), ucols;
+        std::vector<int> colmap;
+        for (int i = 0; i < n * kCols; ++i) column_key(ch.data(), ckeys.data(), i);
+        dedupe(ckeys, ucols, colmap);
+        const int U = static_cast<int>(ucols.size());
+        std::vector<float> colv(static_cast<size_t>(U) * (mctgen::kNumColumns > 0 ? mctgen::kNumColumns : 1));
+        std::vector<float> corners(static_cast<size_t>(U) * kCh * kRows);
+        for (int u = 0; u < U; ++u) column_values(seeds, ucols.data(), colv.data(), u);
+        for (int i = 0; i < U * kRows; ++i) corner_values(seeds, ucols.data(), colv.data(), corners.data(), i);
+        // surface-level grid and aquifer
+        std::vector<uint64_t> gkeys(static_cast<size_t>(n) * kGridKeys), ug;
+        std::vector<int> gmap;
+        for (int i = 0; i < n * kGridKeys; ++i) grid_key(ch.data(), gkeys.data(), i);
+        dedupe(gkeys, ug, gmap);
+        std::vector<float> gv(ug.size());
+        for (size_t u = 0; u < ug.size(); ++u) fts_value(seeds, ug.data(), gv.data(), static_cast<int>(u));
+        std::vector<int32_t> grid(static_cast<size_t>(n) * kGridCells);
+        std::vector<float> prelim(static_cast<size_t>(n) * 4);
+        for (int i = 0; i < n * kGridKeys; ++i) grid_scatter(gmap.data(), gv.data(), grid.data(), prelim.data(), i);
+        std::vector<int> skip(static_cast<size_t>(n));
+        for (int c = 0; c < n; ++c) aquifer_skip(grid.data(), skip.data(), c);
+        std::vector<int32_t> cloc(static_cast<size_t>(n) * kAqCells * 4), cfl(static_cast<size_t>(n) * kAqCells * 2);
+        for (int i = 0; i < n * kAqCells; ++i) aquifer_cell(seeds, ch.data(), grid.data(), cloc.data(), cfl.data(), i);
+        // fill
+        std::vector<uint8_t> blocks(static_cast<size_t>(n) * kPer);
+        FillArgs F{seeds, ch.data(), corners.data(), colmap.data(), cloc.data(), cfl.data(), skip.data(), blocks.data()};
+        for (int c = 0; c < n; ++c)
+            for (int i = 0; i < kPer; ++i)
+                blocks[static_cast<size_t>(c) * kPer + i] = fill_block(F, c, i, density ? density + static_cast<size_t>(c) * kPer + i : nullptr);
+        if (aquifer) std::memcpy(aquifer, blocks.data(), blocks.size());
+        const std::vector<uint8_t> filled = blocks;
+        // surface
+        std::vector<int> h0(static_cast<size_t>(n) * 256), h1(h0.size()), h2(h0.size()), hf(h0.size());
+        for (int c = 0; c < n; ++c)
+            for (int col = 0; col < 256; ++col) h0[c * 256 + col] = column_height(M, blocks.data() + static_cast<size_t>(c) * kPer, col >> 4, col & 15);
+        SurfArgs A{seeds, &M, ch.data(), blocks.data(), nb.data(), bb.data(), corners.data(), colmap.data(), prelim.data()};
+        for (int c = 0; c < n; ++c)
+            for (int col = 0; col < 256; ++col) surface_pass1(A, h0.data(), h1.data(), h2.data(), c, col);
+        for (int c = 0; c < n; ++c)
+            for (int col = 0; col < 256; ++col) surface_pass2(A, h0.data(), h1.data(), h2.data(), hf.data(), c, col);
+        int64_t redone = 0;
+        for (int c = 0; c < n; ++c) {
+            bool ok = true;
+            for (int col = 0; col < 256; ++col) ok &= hf[c * 256 + col] == h2[c * 256 + col];
+            if (ok) continue;
+            ++redone;
+            std::memcpy(blocks.data() + static_cast<size_t>(c) * kPer, filled.data() + static_cast<size_t>(c) * kPer, kPer);
+            int hm[256];
+            for (int col = 0; col < 256; ++col) hm[col] = h0[c * 256 + col];
+            for (int x = 0; x < 16; ++x)
+                for (int z = 0; z < 16; ++z) surface_exact_column(A, hm, c, x * 16 + z);
+        }
+        if (stats) stats[0] = redone;
+        if (terrain) std::memcpy(terrain, blocks.data(), blocks.size());
+        if (biomes)
+            for (int c = 0; c < n; ++c) std::memcpy(biomes + static_cast<size_t>(c) * kQPer, bb.data() + static_cast<size_t>(nb[c * 9 + 4]) * kQPer, kQPer);
+        return 0;
+    } catch (const std::exception& ex) {
+        g_err = ex.what();
+        return 1;
+    }
+}
+
+}  // extern "C"

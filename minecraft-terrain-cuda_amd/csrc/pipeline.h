@@ -1,0 +1,944 @@
+// Per-item bodies of the TERRAIN pipeline, written once for the gfx950 kernels and for the
+// host emulation build (the same source compiled by g++), so every stage can be checked on
+// a machine without a GPU. A kernel maps its thread index to one call of a body here.
+//
+// Stages (one batch of chunks, each chunk tagged with a resident seed slot):
+//   biomes   climate columns -> per-quart climate target, first minimal R-tree leaf L* and
+//            its distance -> tie flags -> last-result chain (per biome chunk)
+//   noise    distinct corner columns (dedupe) -> column values -> channel values at the
+//            4x8 cell corners
+//   aquifer  distinct surface-level cells (dedupe) -> find_top_surface -> per-chunk grid,
+//            skip height, aquifer cell locations and fluid status
+//   fill     per block: channel interpolation, final density, aquifer substance
+//   surface  heights, eroded-badlands / frozen-ocean extensions, material rules
+// Reference semantics: cpu/src/terrain.cpp:152-885 (chunk biomes, NoiseBasedAquifer,
+// Surface), cpu/src/biome.cpp:201-289 (R-tree search, zoom).
+#pragma once
+
+#include "jmath.h"
+#include "noise.h"
+#include "rng.h"
+#include "gen/evaluators.h"
+#include "gen/rules.h"
+
+namespace mct {
+
+constexpr int kMinY = mctgen::kMinY, kHeight = mctgen::kHeight, kSea = mctgen::kSeaLevel;
+constexpr int kQH = kHeight / 4;                   // quart rows
+constexpr int kQPer = 16 * kQH;                    // quarts per chunk
+constexpr int kPer = 256 * kHeight;                // blocks per chunk
+constexpr int kRows = kHeight / 8 + 1;             // corner rows (4x8 cells)
+constexpr int kCh = mctgen::kNumChannels + mctgen::kNumOreChannels;
+constexpr int kCols = 25;                          // corner columns per chunk (5 x 5)
+constexpr int kGridX = 27, kGridZ = 19, kGridCells = kGridX * kGridZ;   // surface-level quarts per chunk
+constexpr int kGridKeys = kGridCells + 4;          // plus the 4 chunk corners of the preliminary surface
+constexpr int kWayBelow = -2032 * 16;              // DimensionType.WAY_BELOW_MIN_Y (-2032 << 4)
+constexpr int kAqMinGY = -1 + ((kMinY + 1) >= 0 ? (kMinY + 1) / 12 : -((-(kMinY + 1) + 11) / 12));
+constexpr int kAqMaxGY = 1 + ((kMinY + kHeight) >= 0 ? (kMinY + kHeight) / 12 : -((-(kMinY + kHeight) + 11) / 12));
+constexpr int kAqGY = kAqMaxGY - kAqMinGY + 1;
+constexpr int kAqCells = 9 * kAqGY;                // 3 x gridY x 3 per chunk
+constexpr int kMaxRandomNames = 8;
+constexpr int kMaxOre = 4;
+static_assert(mctgen::kNumRandomNames <= kMaxRandomNames, "too many vertical-gradient random names");
+static_assert(mctgen::kNumOreRules <= kMaxOre, "too many ore-vein rules");
+static_assert(kCh <= 16, "too many interpolation channels");
+
+enum BlockFlag : uint8_t { kFAir = 1, kFWater = 2, kFLava = 4, kFFluid = 8 };
+
+struct ChunkRef {
+    int32_t x, z;
+    int32_t seed;      // resident seed slot
+    int32_t pad;
+};
+
+// Per-seed state, built on the device from the seed (seed_layer / seed_blended / seed_misc).
+struct SeedDev {
+    Octave oct[mctgen::kNumLayers];
+    Positional random, aquifer, ore, grad[kMaxRandomNames];
+    int64_t zoom_seed;
+    int64_t seed;
+    uint8_t clay[192];
+};
+
+struct TreeNodeDev {
+    int64_t lo[7], hi[7];
+    int32_t first, count, leaf, pad;
+};
+
+// Seed-independent model tables (uploaded once).
+struct ModelDev {
+    const TreeNodeDev* tree;
+    const uint8_t* leaf_biome;
+    const int32_t* leaf_node;
+    uint8_t block_flags[64];
+    float biome_temp[128];
+    uint8_t biome_frozen[128];
+    uint8_t simplex[5][256];       // temperature, frozen x3, biome info
+    int32_t terracotta[7];
+    int32_t ore_state[kMaxOre][3];
+    float ore_raw[kMaxOre];
+};
+
+// ---- keys for the position dedupe -------------------------------------------------------
+// seed slot (12 bits) | flag (1) | qx + 2^23 (24) | qz + 2^23 (24): sorted order groups seeds.
+constexpr int kKeyBits = 61;
+MCT_HD MCT_FI uint64_t make_key(int32_t seed, int flag, int32_t qx, int32_t qz) {
+    return (static_cast<uint64_t>(seed) << 49) | (static_cast<uint64_t>(flag) << 48) | (static_cast<uint64_t>(static_cast<uint32_t>(qx + (1 << 23))) << 24) |
+           static_cast<uint64_t>(static_cast<uint32_t>(qz + (1 << 23)));
+}
+MCT_HD MCT_FI void split_key(uint64_t k, int32_t& seed, int& flag, int32_t& qx, int32_t& qz) {
+    seed = static_cast<int32_t>(k >> 49);
+    flag = static_cast<int>((k >> 48) & 1);
+    qx = static_cast<int32_t>((k >> 24) & 0xFFFFFF) - (1 << 23);
+    qz = static_cast<int32_t>(k & 0xFFFFFF) - (1 << 23);
+}
+
+// ---- per-seed setup ----------------------------------------------------------------------
+// GradientNoise(random, 256): three offsets, then the 256-entry permutation shuffle.
+MCT_HD MCT_FI void make_octave(Xoro& r, uint8_t* p, Octave& out) {
+    out.ox = next_double(r) * 256.0;
+    out.oy = next_double(r) * 256.0;
+    out.oz = next_double(r) * 256.0;
+    for (int i = 0; i < 256; ++i) p[i] = static_cast<uint8_t>(i);
+    for (int i = 0; i < 256; ++i) {
+        const int o = next_int(r, 256 - i);
+        const uint8_t t = p[i];
+        p[i] = p[o + i];
+        p[o + i] = t;
+    }
+    for (int i = 0; i < 256; ++i) out.p[i] = p[i];
+}
+
+MCT_HD MCT_FI Positional world_random(int64_t seed) {
+    Xoro r = xoro_from_seed(seed);
+    return fork_positional(r);
+}
+
+MCT_HD MCT_FI Digest dg(const mctgen::Md5& m) { return {m.lo, m.hi}; }
+
+// One NormalNoise layer: random.fromHashOf(key) -> first/second positional fork ->
+// fromHashOf("octave_N") -> GradientNoise.
+MCT_HD inline void seed_layer(SeedDev& S, int64_t seed, int layer, uint8_t* scratch) {
+    const mctgen::LayerRecipe& rc = mctgen::kLayerRecipe[layer];
+    const Positional P = world_random(seed);
+    Xoro r = from_hash(P, dg(mctgen::kNoiseDigest[rc.a]));
+    Positional f = fork_positional(r);
+    if (rc.c == 1) f = fork_positional(r);
+    Xoro o = from_hash(f, dg(mctgen::kOctaveDigest[rc.b]));
+    make_octave(o, scratch, S.oct[layer]);
+}
+
+// One BlendedNoise node: its min_limit, max_limit and main fbm sets are drawn in sequence
+// from random.fromHashOf("minecraft:terrain").
+MCT_HD inline void seed_blended(SeedDev& S, int64_t seed, int node, int first_layer, uint8_t* scratch) {
+    const Positional P = world_random(seed);
+    Xoro r = from_hash(P, dg(mctgen::kTerrainDigest));
+    for (int k = 0; k < 40; ++k) make_octave(r, scratch, S.oct[first_layer + k]);
+    (void)node;
+}
+
+// Positional factories, clay bands (MaterialSystem.generateBands) and the zoom seed.
+MCT_HD inline void seed_misc(SeedDev& S, int64_t seed, const ModelDev& M) {
+    const Positional P = world_random(seed);
+    S.seed = seed;
+    S.random = P;
+    Xoro a = from_hash(P, dg(mctgen::kAquiferDigest));
+    S.aquifer = fork_positional(a);
+    Xoro b = from_hash(P, dg(mctgen::kOreDigest));
+    S.ore = fork_positional(b);
+    for (int k = 0; k < mctgen::kNumRandomNames; ++k) {
+        Xoro g = from_hash(P, dg(mctgen::kRandomDigest[k]));
+        S.grad[k] = fork_positional(g);
+    }
+    S.zoom_seed = static_cast<int64_t>(sha256_long_le(seed));
+    Xoro r = from_hash(P, dg(mctgen::kClayBandsDigest));
+    const int T = M.terracotta[0], WHITE = M.terracotta[1], ORANGE = M.terracotta[2], YELLOW = M.terracotta[3], BROWN = M.terracotta[4],
+              RED = M.terracotta[5], LGRAY = M.terracotta[6];
+    uint8_t* c = S.clay;
+    for (int i = 0; i < 192; ++i) c[i] = static_cast<uint8_t>(T);
+    for (int i = 0; i < 192; ++i) {
+        i += next_int(r, 5) + 1;
+        if (i < 192) c[i] = static_cast<uint8_t>(ORANGE);
+    }
+    const int widths[3] = {1, 2, 1}, states[3] = {YELLOW, BROWN, RED};
+    for (int k = 0; k < 3; ++k) {
+        const int count = next_int(r, 15 - 6 + 1) + 6;
+        for (int i = 0; i < count; ++i) {
+            const int width = widths[k] + next_int(r, 3);
+            const int start = next_int(r, 192);
+            for (int p = 0; start + p < 192 && p < width; ++p) c[start + p] = static_cast<uint8_t>(states[k]);
+        }
+    }
+    const int whites = next_int(r, 15 - 9 + 1) + 9;
+    int i = 0;
+    for (int start = 0; i < whites && start < 192; start += next_int(r, 16) + 4) {
+        c[start] = static_cast<uint8_t>(WHITE);
+        if (start - 1 > 0 && next_bool(r)) c[start - 1] = static_cast<uint8_t>(LGRAY);
+        if (start + 1 < 192 && next_bool(r)) c[start + 1] = static_cast<uint8_t>(LGRAY);
+        ++i;
+    }
+}
+
+// ---- biomes ------------------------------------------------------------------------------
+MCT_HD MCT_FI int64_t quantize(float v) {
+    const float q = v * 10000.0f;
+    if (q != q) return 0;
+    if (q >= 9.2233720368547758e18f) return INT64_MAX;
+    if (q <= -9.2233720368547758e18f) return INT64_MIN;
+    return static_cast<int64_t>(q);
+}
+
+// Squared distance of a target to a parameter box, in Java long arithmetic (wrapping).
+MCT_HD MCT_FI int64_t box_distance(const TreeNodeDev& n, const int64_t* t) {
+    uint64_t d = 0;
+    for (int i = 0; i < 7; ++i) {
+        const int64_t above = static_cast<int64_t>(static_cast<uint64_t>(t[i]) - static_cast<uint64_t>(n.hi[i]));
+        const int64_t below = static_cast<int64_t>(static_cast<uint64_t>(n.lo[i]) - static_cast<uint64_t>(t[i]));
+        const int64_t p = above > 0 ? above : (below > 0 ? below : 0);
+        d += static_cast<uint64_t>(p) * static_cast<uint64_t>(p);
+    }
+    return static_cast<int64_t>(d);
+}
+
+// First leaf in DFS order at minimal distance (what Climate.RTree.search returns without a
+// candidate). Any bound above the minimal distance gives the same leaf, so a neighbouring
+// quart's leaf distance + 1 prunes most of the tree. Returns -1 if nothing beats `bound`.
+MCT_HD inline int tree_first_min(const TreeNodeDev* T, const int64_t* t, int64_t bound, int64_t& out_d) {
+    int stack_node[12], stack_child[12];
+    int sp = 0;
+    int64_t best = bound;
+    int closest = -1;
+    if (T[0].leaf >= 0) {
+        const int64_t d = box_distance(T[0], t);
+        if (best > d) { best = d; closest = 0; }
+    } else {
+        stack_node[0] = 0;
+        stack_child[0] = 0;
+        sp = 1;
+    }
+    while (sp > 0) {
+        const TreeNodeDev& n = T[stack_node[sp - 1]];
+        const int k = stack_child[sp - 1];
+        if (k >= n.count) { --sp; continue; }
+        stack_child[sp - 1] = k + 1;
+        const int c = n.first + k;
+        const int64_t cd = box_distance(T[c], t);
+        if (best > cd) {
+            if (T[c].leaf >= 0) {
+                best = cd;
+                closest = c;
+            } else if (sp < 12) {
+                stack_node[sp] = c;
+                stack_child[sp] = 0;
+                ++sp;
+            }
+        }
+    }
+    out_d = best;
+    return closest >= 0 ? T[closest].leaf : -1;
+}
+
+// Climate columns of one biome chunk column (y-independent parts at y = 0).
+MCT_HD MCT_FI void biome_column(const SeedDev* seeds, const ChunkRef* bch, float* bcol, int idx) {
+    const int b = idx >> 4, k = idx & 15;
+    const ChunkRef c = bch[b];
+    mctgen::eval_climate_columns(seeds[c.seed].oct, c.x * 16 + (k & 3) * 4, c.z * 16 + (k >> 2) * 4, bcol + static_cast<size_t>(idx) * mctgen::kNumClimateColumns);
+}
+
+constexpr int kBiomeRun = 8;
+constexpr int kBiomeRuns = (kQH + kBiomeRun - 1) / kBiomeRun;
+
+// A run of kBiomeRun quarts down one quart column: climate target, L*, minimal distance.
+MCT_HD inline void biome_quarts(const SeedDev* seeds, const ModelDev& M, const ChunkRef* bch, const float* bcol, int64_t* targets, int32_t* lstar,
+                                int64_t* mind, int idx) {
+    const int b = idx / (16 * kBiomeRuns), rem = idx % (16 * kBiomeRuns);
+    const int colk = rem & 15, run = rem >> 4;
+    const int qx = colk & 3, qz = colk >> 2;
+    const ChunkRef c = bch[b];
+    const Octave* O = seeds[c.seed].oct;
+    const float* cv = bcol + (static_cast<size_t>(b) * 16 + qz * 4 + qx) * mctgen::kNumClimateColumns;
+    int prev = -1;
+    const int q1 = (run + 1) * kBiomeRun < kQH ? (run + 1) * kBiomeRun : kQH;
+    for (int qy = run * kBiomeRun; qy < q1; ++qy) {
+        const size_t i = static_cast<size_t>(b) * kQPer + (qy * 4 + qz) * 4 + qx;
+        float clim[6];
+        mctgen::eval_climate(O, c.x * 16 + qx * 4, kMinY + qy * 4, c.z * 16 + qz * 4, cv, clim);
+        int64_t* t = targets + i * 7;
+        for (int k = 0; k < 6; ++k) t[k] = quantize(clim[k]);
+        t[6] = 0;
+        int64_t bound = INT64_MAX, d = 0;
+        if (prev >= 0) {
+            const int64_t dp = box_distance(M.tree[M.leaf_node[prev]], t);
+            if (dp < INT64_MAX) bound = dp + 1;
+        }
+        int leaf = tree_first_min(M.tree, t, bound, d);
+        if (leaf < 0) leaf = tree_first_min(M.tree, t, INT64_MAX, d);
+        lstar[i] = leaf;
+        mind[i] = d;
+        prev = leaf;
+    }
+}
+
+// Chain order of doCreateBiomes inside a chunk: 4-quart sections, then x, y, z.
+MCT_HD MCT_FI int chain_quart(int p) {
+    const int z = p & 3, y = (p >> 2) & 3, x = (p >> 4) & 3, sy = p >> 6;
+    return ((sy * 4 + y) * 4 + z) * 4 + x;
+}
+
+// tie[p]: the previous chain position's L* is also at minimal distance at p.
+MCT_HD MCT_FI void biome_tie(const ModelDev& M, const int64_t* targets, const int32_t* lstar, const int64_t* mind, uint8_t* tie, int idx) {
+    const int b = idx / kQPer, p = idx % kQPer;
+    if (p == 0) { tie[idx] = 0; return; }
+    const size_t i = static_cast<size_t>(b) * kQPer + chain_quart(p), ip = static_cast<size_t>(b) * kQPer + chain_quart(p - 1);
+    tie[idx] = box_distance(M.tree[M.leaf_node[lstar[ip]]], targets + i * 7) == mind[i] ? 1 : 0;
+}
+
+// The last-result rule along the chain of one biome chunk (sequential form): the result is
+// the previous result if it reaches the minimal distance, else L*.
+MCT_HD inline void biome_chain_seq(const ModelDev& M, const int64_t* targets, const int32_t* lstar, const int64_t* mind, uint8_t* biomes, int b) {
+    const size_t base = static_cast<size_t>(b) * kQPer;
+    int prev = -1;
+    for (int p = 0; p < kQPer; ++p) {
+        const size_t i = base + chain_quart(p);
+        int cur = lstar[i];
+        if (prev >= 0 && box_distance(M.tree[M.leaf_node[prev]], targets + i * 7) == mind[i]) cur = prev;
+        biomes[i] = M.leaf_biome[cur];
+        prev = cur;
+    }
+}
+
+// ---- density ---------------------------------------------------------------------------
+MCT_HD MCT_FI void column_key(const ChunkRef* ch, uint64_t* keys, int idx) {
+    const int c = idx / kCols, k = idx % kCols;
+    keys[idx] = make_key(ch[c].seed, 0, ch[c].x * 4 + k % 5, ch[c].z * 4 + k / 5);
+}
+
+MCT_HD MCT_FI void column_values(const SeedDev* seeds, const uint64_t* ucols, float* colv, int u) {
+    int32_t s, qx, qz;
+    int f;
+    split_key(ucols[u], s, f, qx, qz);
+    mctgen::eval_columns(seeds[s].oct, qx * 4, qz * 4, colv + static_cast<size_t>(u) * (mctgen::kNumColumns > 0 ? mctgen::kNumColumns : 1));
+}
+
+// Channel inputs at one cell corner: corners[(u * kCh + ch) * kRows + row].
+MCT_HD MCT_FI void corner_values(const SeedDev* seeds, const uint64_t* ucols, const float* colv, float* corners, int idx) {
+    const int u = idx / kRows, row = idx % kRows;
+    int32_t s, qx, qz;
+    int f;
+    split_key(ucols[u], s, f, qx, qz);
+    float out[kCh];
+    mctgen::eval_corner(seeds[s].oct, qx * 4, kMinY + row * 8, qz * 4, colv + static_cast<size_t>(u) * (mctgen::kNumColumns > 0 ? mctgen::kNumColumns : 1), out);
+    for (int ch = 0; ch < kCh; ++ch) corners[(static_cast<size_t>(u) * kCh + ch) * kRows + row] = out[ch];
+}
+
+// Channel values of a block by the cell fill (z, then x, then the y steps).
+MCT_HD MCT_FI void interp_channels(const float* corners, const int* cmap, int x, int yy, int z, int first, int count, float* out) {
+    const int cx = x >> 2, xi = x & 3, cz = z >> 2, zi = z & 3, cy = yy >> 3, yi = yy & 7;
+    const size_t u00 = cmap[cz * 5 + cx], u10 = cmap[cz * 5 + cx + 1], u01 = cmap[(cz + 1) * 5 + cx], u11 = cmap[(cz + 1) * 5 + cx + 1];
+    for (int k = 0; k < count; ++k) {
+        const int ch = first + k;
+        const float* a = corners + (u00 * kCh + ch) * kRows;
+        const float* b = corners + (u10 * kCh + ch) * kRows;
+        const float* c = corners + (u01 * kCh + ch) * kRows;
+        const float* d = corners + (u11 * kCh + ch) * kRows;
+        out[k] = cell_fill(a[cy], b[cy], a[cy + 1], b[cy + 1], c[cy], d[cy], c[cy + 1], d[cy + 1], xi, yi, zi, 0.25f, 0.125f);
+    }
+}
+
+// ---- surface-level grid (aquifer) and preliminary surface corners ------------------------
+// Per chunk: 27 x 19 surface-level quarts around it; inside NoiseBasedAquifer's constructor
+// prefill window (quarts 4c-4 .. 4c+6) the volume path is used, elsewhere the point path;
+// plus the 4 chunk corners of chunk_surface_level (volume path).
+MCT_HD MCT_FI void grid_key(const ChunkRef* ch, uint64_t* keys, int idx) {
+    const int c = idx / kGridKeys, e = idx % kGridKeys;
+    const int cx = ch[c].x, cz = ch[c].z;
+    if (e < kGridCells) {
+        const int qx = 4 * cx - 16 + e % kGridX, qz = 4 * cz - 8 + e / kGridX;
+        const bool vol = qx >= 4 * cx - 4 && qx <= 4 * cx + 6 && qz >= 4 * cz - 4 && qz <= 4 * cz + 6;
+        keys[idx] = make_key(ch[c].seed, vol ? 1 : 0, qx, qz);
+    } else {
+        const int k = e - kGridCells;
+        keys[idx] = make_key(ch[c].seed, 1, 4 * cx + 4 * (k & 1), 4 * cz + 4 * (k >> 1));
+    }
+}
+
+MCT_HD MCT_FI void fts_value(const SeedDev* seeds, const uint64_t* ukeys, float* vals, int u) {
+    int32_t s, qx, qz;
+    int vol;
+    split_key(ukeys[u], s, vol, qx, qz);
+    const Octave* O = seeds[s].oct;
+    vals[u] = vol ? mctgen::fts_volume(O, qx * 4, qz * 4) : mctgen::fts_point(O, qx * 4, qz * 4);
+}
+
+MCT_HD MCT_FI void grid_scatter(const int* map, const float* vals, int32_t* grid, float* prelim, int idx) {
+    const int c = idx / kGridKeys, e = idx % kGridKeys;
+    const float v = vals[map[idx]];
+    if (e < kGridCells) grid[static_cast<size_t>(c) * kGridCells + e] = mfloor(static_cast<double>(v));
+    else prelim[static_cast<size_t>(c) * 4 + (e - kGridCells)] = v;
+}
+
+// ---- aquifer (NoiseBasedAquifer) ----------------------------------------------------------
+struct Fluid {
+    int level, type;
+};
+MCT_HD MCT_FI int fluid_at(Fluid f, int y) { return y < f.level ? f.type : mctgen::kAir; }
+MCT_HD MCT_FI Fluid global_fluid(int y) {
+    const int lava_level = kSea < -54 ? kSea : -54;
+    return y < lava_level ? Fluid{-54, mctgen::kLava} : Fluid{kSea, mctgen::kDefaultFluid};
+}
+MCT_HD MCT_FI int grid_yf(int b) { return fdiv(b, 12); }
+
+// Skip height: blocks above it take the global fluid directly (max surface level over the
+// prefill window, +8, up one grid cell).
+MCT_HD MCT_FI void aquifer_skip(const int32_t* grid, int* skip, int c) {
+    int m = INT32_MIN;
+    for (int gz = 4; gz <= 14; ++gz)
+        for (int gx = 12; gx <= 22; ++gx) m = jmaxi(m, grid[static_cast<size_t>(c) * kGridCells + gz * kGridX + gx]);
+    const int adjusted = m + 8;
+    const int skip_gy = grid_yf(adjusted + 12) + 1;
+    skip[c] = skip_gy * 12 + 11 - 1;
+}
+
+MCT_HD MCT_FI int surface_at(const int32_t* g, int cx, int cz, int x, int z) {
+    return g[((x >> 2) - (4 * cx - 16)) + ((z >> 2) - (4 * cz - 8)) * kGridX];
+}
+
+// Location (aquifer random) and computeFluid status of one aquifer cell of a chunk.
+MCT_HD inline void aquifer_cell(const SeedDev* seeds, const ChunkRef* ch, const int32_t* grid, int32_t* cell_loc, int32_t* cell_fluid, int idx) {
+    const int c = idx / kAqCells, i = idx % kAqCells;
+    const int cx = ch[c].x, cz = ch[c].z;
+    const SeedDev& S = seeds[ch[c].seed];
+    const int lx = i % 3, lz = (i / 3) % 3, ly = i / 9;
+    const int gx = cx - 1 + lx, gz = cz - 1 + lz, gy = kAqMinGY + ly;
+    Xoro r = at(S.aquifer, gx, gy, gz);
+    const int x = (gx << 4) + next_int(r, 10);
+    const int y = gy * 12 + next_int(r, 9);
+    const int z = (gz << 4) + next_int(r, 10);
+    const int32_t* g = grid + static_cast<size_t>(c) * kGridCells;
+    const Fluid global = global_fluid(y);
+    Fluid res = global;
+    bool done = false;
+    int lowest = INT32_MAX;
+    const int top = y + 12, bottom = y - 12;
+    bool center_under = false;
+    const int off[13][2] = {{0, 0}, {-2, -1}, {-1, -1}, {0, -1}, {1, -1}, {-3, 0}, {-2, 0}, {-1, 0}, {1, 0}, {-2, 1}, {-1, 1}, {0, 1}, {1, 1}};
+    for (int k = 0; k < 13; ++k) {
+        const int sx = x + (off[k][0] << 4), sz = z + (off[k][1] << 4);
+        const int level = surface_at(g, cx, cz, sx, sz);
+        const int adjusted = level + 8;
+        const bool start = k == 0;
+        if (start && bottom > adjusted) { res = global; done = true; break; }
+        const bool pokes = top > adjusted;
+        if (pokes || start) {
+            const Fluid at_surface = global_fluid(adjusted);
+            if (fluid_at(at_surface, adjusted) != mctgen::kAir) {
+                if (start) center_under = true;
+                if (pokes) { res = at_surface; done = true; break; }
+            }
+        }
+        lowest = jmini(lowest, level);
+    }
+    if (!done) {
+        const Octave* O = S.oct;
+        double partially, fully;
+        if (mctgen::eval_point(O, x, y, z, 4) > 0.0f) {
+            partially = -1.0;
+            fully = -1.0;
+        } else {
+            const int below = lowest + 8 - y;
+            double factor = 0.0;
+            if (center_under) {
+                const double f = (static_cast<double>(below) - 0.0) / (64.0 - 0.0);
+                factor = f < 0.0 ? 1.0 : (f > 1.0 ? 0.0 : 1.0 + f * (0.0 - 1.0));
+            }
+            const double noise = clampd(static_cast<double>(mctgen::eval_point(O, x, y, z, 1)), -1.0, 1.0);
+            const double fully_t = -0.3 + ((factor - 1.0) / (0.0 - 1.0)) * (0.8 - -0.3);
+            const double partial_t = -0.8 + ((factor - 1.0) / (0.0 - 1.0)) * (0.4 - -0.8);
+            partially = noise - partial_t;
+            fully = noise - fully_t;
+        }
+        int level;
+        if (fully > 0.0) {
+            level = global.level;
+        } else if (partially > 0.0) {
+            const int fcx = fdiv(x, 16), fcy = fdiv(y, 40), fcz = fdiv(z, 16);
+            const int middle = fcy * 40 + 20;
+            const double spread = static_cast<double>(mctgen::eval_point(O, fcx, fcy, fcz, 2) * 10.0f);
+            const int q = mfloor(spread / 3.0) * 3;
+            level = jmini(lowest, middle + q);
+        } else {
+            level = kWayBelow;
+        }
+        int type = global.type;
+        if (level <= -10 && level != kWayBelow && global.type != mctgen::kLava) {
+            const int tcx = fdiv(x, 64), tcy = fdiv(y, 40), tcz = fdiv(z, 64);
+            const double lava = static_cast<double>(mctgen::eval_point(O, tcx, tcy, tcz, 3));
+            if (fabs(lava) > 0.3) type = mctgen::kLava;
+        }
+        res = Fluid{level, type};
+    }
+    cell_loc[static_cast<size_t>(idx) * 4 + 0] = x;
+    cell_loc[static_cast<size_t>(idx) * 4 + 1] = y;
+    cell_loc[static_cast<size_t>(idx) * 4 + 2] = z;
+    cell_loc[static_cast<size_t>(idx) * 4 + 3] = 0;
+    cell_fluid[static_cast<size_t>(idx) * 2 + 0] = res.level;
+    cell_fluid[static_cast<size_t>(idx) * 2 + 1] = res.type;
+}
+
+MCT_HD MCT_FI double aquifer_pressure(const Octave* O, int x, int y, int z, double& barrier, bool& barrier_set, Fluid a, Fluid b) {
+    const int t1 = fluid_at(a, y), t2 = fluid_at(b, y);
+    if ((t1 == mctgen::kLava && t2 == mctgen::kWater) || (t1 == mctgen::kWater && t2 == mctgen::kLava)) return 2.0;
+    const int diff = a.level - b.level < 0 ? b.level - a.level : a.level - b.level;
+    if (diff == 0) return 0.0;
+    const double average = 0.5 * static_cast<double>(a.level + b.level);
+    const double above = static_cast<double>(y) + 0.5 - average;
+    const double base = static_cast<double>(diff) / 2.0;
+    const double towards = base - fabs(above);
+    double gradient;
+    if (above > 0.0) {
+        const double center = 0.0 + towards;
+        gradient = center > 0.0 ? center / 1.5 : center / 2.5;
+    } else {
+        const double center = 3.0 + towards;
+        gradient = center > 0.0 ? center / 3.0 : center / 10.0;
+    }
+    double noise;
+    if (!(gradient < -2.0) && !(gradient > 2.0)) {
+        if (!barrier_set) {
+            barrier = static_cast<double>(mctgen::eval_point(O, x, y, z, 0));
+            barrier_set = true;
+        }
+        noise = barrier;
+    } else {
+        noise = 0.0;
+    }
+    return 2.0 * (noise + gradient);
+}
+
+// NoiseBasedAquifer.computeSubstance; -1 means solid (the default block).
+MCT_HD inline int aquifer_substance(const Octave* O, int cx, int cz, const int32_t* loc, const int32_t* fl, int skip_above, int x, int y, int z,
+                                    double density) {
+    if (density > 0.0) return -1;
+    const Fluid global = global_fluid(y);
+    if (y > skip_above) return fluid_at(global, y);
+    if (fluid_at(global, y) == mctgen::kLava) return mctgen::kLava;
+    const int xa = (x - 5) >> 4, ya = grid_yf(y + 1), za = (z - 5) >> 4;
+    int d1 = INT32_MAX, d2 = INT32_MAX, d3 = INT32_MAX, d4 = INT32_MAX;
+    int i1 = 0, i2 = 0, i3 = 0;
+    for (int ox = 0; ox <= 1; ++ox)
+        for (int oy = -1; oy <= 1; ++oy)
+            for (int oz = 0; oz <= 1; ++oz) {
+                const int gx = xa + ox, gy = ya + oy, gz = za + oz;
+                const int index = ((gy - kAqMinGY) * 3 + (gz - (cz - 1))) * 3 + (gx - (cx - 1));
+                const int dx = loc[index * 4] - x, dy = loc[index * 4 + 1] - y, dz = loc[index * 4 + 2] - z;
+                const int nd = dx * dx + dy * dy + dz * dz;
+                if (d1 >= nd) { i3 = i2; i2 = i1; i1 = index; d4 = d3; d3 = d2; d2 = d1; d1 = nd; }
+                else if (d2 >= nd) { i3 = i2; i2 = index; d4 = d3; d3 = d2; d2 = nd; }
+                else if (d3 >= nd) { i3 = index; d4 = d3; d3 = nd; }
+                else if (d4 >= nd) { d4 = nd; }
+            }
+    const Fluid s1{fl[i1 * 2], fl[i1 * 2 + 1]};
+    const double sim12 = 1.0 - static_cast<double>(d2 - d1) / 25.0;
+    const int fluid = fluid_at(s1, y);
+    if (sim12 <= 0.0) return fluid;
+    if (fluid == mctgen::kWater && fluid_at(global_fluid(y - 1), y - 1) == mctgen::kLava) return fluid;
+    double barrier = 0.0;
+    bool barrier_set = false;
+    const Fluid s2{fl[i2 * 2], fl[i2 * 2 + 1]};
+    const double b12 = sim12 * aquifer_pressure(O, x, y, z, barrier, barrier_set, s1, s2);
+    if (density + b12 > 0.0) return -1;
+    const Fluid s3{fl[i3 * 2], fl[i3 * 2 + 1]};
+    const double sim13 = 1.0 - static_cast<double>(d3 - d1) / 25.0;
+    if (sim13 > 0.0) {
+        const double b13 = sim12 * sim13 * aquifer_pressure(O, x, y, z, barrier, barrier_set, s1, s3);
+        if (density + b13 > 0.0) return -1;
+    }
+    const double sim23 = 1.0 - static_cast<double>(d3 - d2) / 25.0;
+    if (sim23 > 0.0) {
+        const double b23 = sim12 * sim23 * aquifer_pressure(O, x, y, z, barrier, barrier_set, s2, s3);
+        if (density + b23 > 0.0) return -1;
+    }
+    return fluid;
+}
+
+// ---- fill: one block -------------------------------------------------------------------
+struct FillArgs {
+    const SeedDev* seeds;
+    const ChunkRef* chunks;
+    const float* corners;
+    const int* colmap;          // n x 25 corner-column indices
+    const int32_t* cell_loc;    // n x kAqCells x 4
+    const int32_t* cell_fluid;  // n x kAqCells x 2
+    const int* skip;
+    uint8_t* blocks;
+};
+
+MCT_HD inline uint8_t fill_block(const FillArgs& A, int c, int i, float* density_out) {
+    const int x = i & 15, z = (i >> 4) & 15, yy = i >> 8;
+    float ch[mctgen::kNumChannels > 0 ? mctgen::kNumChannels : 1];
+    interp_channels(A.corners, A.colmap + static_cast<size_t>(c) * kCols, x, yy, z, 0, mctgen::kNumChannels, ch);
+    const ChunkRef cr = A.chunks[c];
+    const Octave* O = A.seeds[cr.seed].oct;
+    const int bx = cr.x * 16 + x, by = kMinY + yy, bz = cr.z * 16 + z;
+    const float d = mctgen::eval_block(O, bx, by, bz, ch);
+    if (density_out) *density_out = d;
+    const int s = aquifer_substance(O, cr.x, cr.z, A.cell_loc + static_cast<size_t>(c) * kAqCells * 4, A.cell_fluid + static_cast<size_t>(c) * kAqCells * 2,
+                                    A.skip[c], bx, by, bz, static_cast<double>(d));
+    return static_cast<uint8_t>(s < 0 ? mctgen::kDefaultBlock : s);
+}
+
+// Initial WORLD_SURFACE_WG height of a column (highest non-air block + 1, or minY).
+MCT_HD MCT_FI int column_height(const ModelDev& M, const uint8_t* blocks, int x, int z) {
+    for (int yy = kHeight - 1; yy >= 0; --yy)
+        if (!(M.block_flags[blocks[(yy * 16 + z) * 16 + x]] & kFAir)) return kMinY + yy + 1;
+    return kMinY;
+}
+
+// ---- legacy simplex noise (biome temperature) -------------------------------------------
+MCT_HD MCT_FI double simplex_corner(int g, double x, double y) {
+    double t0 = 0.5 - x * x - y * y - 0.0 * 0.0;
+    if (t0 < 0.0) return 0.0;
+    t0 *= t0;
+    double dot = static_cast<double>(gcomp(kGX, g)) * x + static_cast<double>(gcomp(kGY, g)) * y;
+    dot = dot + static_cast<double>(gcomp(kGZ, g)) * 0.0;
+    return t0 * t0 * dot;
+}
+
+MCT_HD inline float simplex2(const uint8_t* perm, double px, double py) {
+    const double s3 = 1.7320508075688772;   // Math.sqrt(3.0)
+    const double F2 = 0.5 * (s3 - 1.0), G2 = (3.0 - s3) / 6.0;
+    const double xin = px + 0.0, yin = py + 0.0;
+    const double s = (xin + yin) * F2;
+    const int32_t i = mfloor(xin + s), j = mfloor(yin + s);
+    const double t = static_cast<double>(iadd(i, j)) * G2;
+    const double x0 = xin - (static_cast<double>(i) - t), y0 = yin - (static_cast<double>(j) - t);
+    int i1, j1;
+    if (x0 > y0) { i1 = 1; j1 = 0; } else { i1 = 0; j1 = 1; }
+    const double x1 = x0 - static_cast<double>(i1) + G2, y1 = y0 - static_cast<double>(j1) + G2;
+    const double x2 = x0 - 1.0 + 2.0 * G2, y2 = y0 - 1.0 + 2.0 * G2;
+    const int ii = i & 0xFF, jj = j & 0xFF;
+    const int g0 = perm[(ii + perm[jj]) & 0xFF] % 12;
+    const int g1 = perm[(ii + i1 + perm[(jj + j1) & 0xFF]) & 0xFF] % 12;
+    const int g2 = perm[(ii + 1 + perm[(jj + 1) & 0xFF]) & 0xFF] % 12;
+    return static_cast<float>(70.0 * (simplex_corner(g0, x0, y0) + simplex_corner(g1, x1, y1) + simplex_corner(g2, x2, y2)));
+}
+
+// Biome.getHeightAdjustedTemperature
+MCT_HD inline float biome_temperature(const ModelDev& M, int b, int x, int y, int z) {
+    float adjusted = M.biome_temp[b];
+    if (M.biome_frozen[b]) {
+        const double fr[3] = {1.0, 0.5, 0.25};
+        const float amp[3] = {0.14285715f, 0.2857143f, 0.5714286f};
+        float fv = 0.0f;
+        for (int k = 0; k < 3; ++k) fv += amp[k] * simplex2(M.simplex[1 + k], static_cast<double>(x) * 0.05 * fr[k], static_cast<double>(z) * 0.05 * fr[k]);
+        const double large = static_cast<double>(fv * 7.0f);
+        const double edge = static_cast<double>(simplex2(M.simplex[4], static_cast<double>(x) * 0.2, static_cast<double>(z) * 0.2));
+        if (large + edge < 0.3) {
+            const double small = static_cast<double>(simplex2(M.simplex[4], static_cast<double>(x) * 0.09, static_cast<double>(z) * 0.09));
+            if (small < 0.8) adjusted = 0.2f;
+        }
+    }
+    const int snow = kSea + 17;
+    if (y > snow) {
+        const float v = simplex2(M.simplex[0], static_cast<double>(static_cast<float>(x) / 8.0f), static_cast<double>(static_cast<float>(z) / 8.0f)) * 8.0f;
+        float t = v + static_cast<float>(y);
+        t = t - static_cast<float>(snow);
+        t = t * 0.05f;
+        t = t / 40.0f;
+        return adjusted - t;
+    }
+    return adjusted;
+}
+
+// ---- material system: one column ----------------------------------------------------------
+struct SurfArgs {
+    const SeedDev* seeds;
+    const ModelDev* M;
+    const ChunkRef* chunks;
+    uint8_t* blocks;
+    const int* nb_index;        // n x 9 biome-chunk indices
+    const uint8_t* bbiomes;     // biome chunks x kQPer
+    const float* corners;
+    const int* colmap;
+    const float* prelim;        // n x 4 chunk-corner preliminary surface values
+};
+
+struct Surf {
+    const SurfArgs* A;
+    const Octave* O;
+    const SeedDev* S;
+    int cx, cz, lx, lz;
+    uint8_t* blocks;
+    int* hm;
+    const uint8_t* nb[9];
+    const int* cmap;
+    float prelim;
+    int x, y, z;
+    int grad_x, grad_z, surface_depth;
+    int stone_above, stone_below, water_height;
+    bool sec_set, biome_set, min_set, ore_set;
+    double secondary;
+    int biome_v, min_surface;
+    float ore_ch[kCh];
+
+    MCT_HD MCT_FI int quart_biome(int qx, int qy, int qz) const {
+        const int dx = (qx >> 2) - cx, dz = (qz >> 2) - cz;
+        const int qmin = kMinY >> 2;
+        const int cqy = clampi(qy, qmin, qmin + kQH - 1) - qmin;
+        return nb[(dz + 1) * 3 + dx + 1][(cqy * 4 + (qz & 3)) * 4 + (qx & 3)];
+    }
+    MCT_HD MCT_FI int biome_at(int bx, int by, int bz) const {
+        int qx, qy, qz;
+        zoom_pick(S->zoom_seed, bx, by, bz, qx, qy, qz);
+        return quart_biome(qx, qy, qz);
+    }
+    MCT_HD MCT_FI int biome() {
+        if (!biome_set) { biome_v = biome_at(x, y, z); biome_set = true; }
+        return biome_v;
+    }
+    MCT_HD MCT_FI bool vertical_gradient(int k, int true_at, int false_at) const {
+        if (y <= true_at) return true;
+        if (y >= false_at) return false;
+        const double t = (static_cast<double>(y) - true_at) / static_cast<double>(false_at - true_at);
+        const double probability = 1.0 + t * (0.0 - 1.0);
+        Xoro r = at(S->grad[k], x, y, z);
+        return static_cast<double>(next_float(r)) < probability;
+    }
+    MCT_HD MCT_FI double surface_secondary() {
+        if (!sec_set) { secondary = mctgen::surf_surface_secondary(O, static_cast<double>(x), 0.0, static_cast<double>(z)); sec_set = true; }
+        return secondary;
+    }
+    MCT_HD MCT_FI bool stone_depth(int offset, bool add, int range, bool ceiling) {
+        const int stone = ceiling ? stone_below : stone_above;
+        const int sd = add ? surface_depth : 0;
+        int sec = 0;
+        if (range != 0) {
+            const double t = (surface_secondary() - -1.0) / (1.0 - -1.0);
+            sec = d2i(0.0 + t * (static_cast<double>(range) - 0.0));
+        }
+        return stone <= 1 + offset + sd + sec;
+    }
+    MCT_HD MCT_FI bool cold_enough_to_snow() { return !(biome_temperature(*A->M, biome(), x, y, z) >= 0.15f); }
+    MCT_HD MCT_FI int min_surface_level() {
+        if (!min_set) { min_surface = mfloor(static_cast<double>(prelim)) + surface_depth - 8; min_set = true; }
+        return min_surface;
+    }
+    MCT_HD MCT_FI int band() const {
+        const int offset = jroundf(static_cast<float>(mctgen::surf_clay_bands_offset(O, static_cast<double>(x), 0.0, static_cast<double>(z))) * 4.0f);
+        return S->clay[(y + offset + 192) % 192];
+    }
+    MCT_HD inline int ore_vein(int k) {
+        if (!ore_set) {
+            interp_channels(A->corners, cmap, lx, y - kMinY, lz, 0, kCh, ore_ch);
+            ore_set = true;
+        }
+        const float density = mctgen::eval_ore(O, x, y, z, ore_ch, 2 * k);
+        if (density <= 0.0f) return -1;
+        Xoro r = at(S->ore, x, y, z);
+        if (next_float(r) > density) return -1;
+        const float richness = mctgen::eval_ore(O, x, y, z, ore_ch, 2 * k + 1);
+        if (next_float(r) < richness && mctgen::eval_point(O, x, y, z, 5 + k) < 0.0f)
+            return next_float(r) < A->M->ore_raw[k] ? A->M->ore_state[k][1] : A->M->ore_state[k][0];
+        return A->M->ore_state[k][2];
+    }
+
+    MCT_HD MCT_FI size_t bidx(int by) const { return static_cast<size_t>(((by - kMinY) * 16 + lz) * 16 + lx); }
+    MCT_HD MCT_FI int get(int by) const { return (by < kMinY || by >= kMinY + kHeight) ? mctgen::kAir : blocks[bidx(by)]; }
+    MCT_HD MCT_FI bool air(int s) const { return (A->M->block_flags[s] & kFAir) != 0; }
+    MCT_HD MCT_FI bool fluid(int s) const { return (A->M->block_flags[s] & kFFluid) != 0; }
+    // setBlockState + Heightmap.update(WORLD_SURFACE_WG); `write` false moves only the height
+    MCT_HD inline void set(int by, int state, bool write) {
+        if (by < kMinY || by >= kMinY + kHeight) return;
+        if (write) blocks[bidx(by)] = static_cast<uint8_t>(state);
+        const int first = *hm;
+        if (by <= first - 2) return;
+        if (!air(state)) {
+            if (by >= first) *hm = by + 1;
+        } else if (first - 1 == by) {
+            for (int yy = by - 1; yy >= kMinY; --yy)
+                if (!air(get(yy))) { *hm = yy + 1; return; }
+            *hm = kMinY;
+        }
+    }
+};
+
+MCT_HD inline void surf_init(Surf& c, const SurfArgs& A, int ci, int lx, int lz, int* hm) {
+    c.A = &A;
+    const ChunkRef cr = A.chunks[ci];
+    c.S = &A.seeds[cr.seed];
+    c.O = c.S->oct;
+    c.cx = cr.x;
+    c.cz = cr.z;
+    c.lx = lx;
+    c.lz = lz;
+    c.blocks = A.blocks + static_cast<size_t>(ci) * kPer;
+    c.hm = hm;
+    for (int k = 0; k < 9; ++k) c.nb[k] = A.bbiomes + static_cast<size_t>(A.nb_index[ci * 9 + k]) * kQPer;
+    c.cmap = A.colmap + static_cast<size_t>(ci) * kCols;
+    const float* pc = A.prelim + static_cast<size_t>(ci) * 4;
+    // chunk_surface_level: InterpolatedFunction 16 x 1 over the chunk's 4 corners
+    c.prelim = cell_fill(pc[0], pc[1], pc[0], pc[1], pc[2], pc[3], pc[2], pc[3], lx, 0, lz, 1.0f / 16.0f, 1.0f / 1.0f);
+    c.sec_set = c.biome_set = c.min_set = c.ore_set = false;
+    c.x = cr.x * 16 + lx;
+    c.z = cr.z * 16 + lz;
+    c.y = 0;
+}
+
+MCT_HD inline void surf_update_xz(Surf& c, int gx, int gz) {
+    c.grad_x = gx;
+    c.grad_z = gz;
+    const double n = mctgen::surf_surface(c.O, static_cast<double>(c.x), 0.0, static_cast<double>(c.z));
+    Xoro r = at(c.S->random, c.x, 0, c.z);
+    c.surface_depth = d2i(n * 2.75 + 3.0 + next_double(r) * 0.25);
+    c.sec_set = false;
+    c.min_set = false;
+}
+
+// MaterialSystem eroded-badlands extension (writes blocks and moves the height).
+MCT_HD inline void eroded_badlands(Surf& c, int height) {
+    const double bx = c.x, bz = c.z;
+    const double a = fabs(mctgen::surf_badlands_surface(c.O, bx, 0.0, bz) * 8.25);
+    const double b = static_cast<double>(static_cast<float>(mctgen::surf_badlands_pillar(c.O, bx * 0.2, 0.0, bz * 0.2)) * 15.0f);
+    const double pillar = jmind(a, b);
+    if (pillar <= 0.0) return;
+    const double floor_v = fabs(mctgen::surf_badlands_pillar_roof(c.O, bx * 0.75, 0.0, bz * 0.75) * 1.5);
+    const double top = 64.0 + jmind(pillar * pillar * 2.5, ceil(floor_v * 50.0) + 24.0);
+    const int start_y = mfloor(top);
+    if (height > start_y) return;
+    for (int yy = start_y; yy >= kMinY; --yy) {
+        const int old = c.get(yy);
+        if (old == mctgen::kDefaultBlock) break;
+        if (c.A->M->block_flags[old] & kFWater) return;
+    }
+    for (int yy = start_y; yy >= kMinY && c.air(c.get(yy)); --yy) c.set(yy, mctgen::kDefaultBlock, true);
+}
+
+// MaterialSystem frozen-ocean extension; with write == false only the height moves.
+MCT_HD inline void frozen_ocean(Surf& c, int min_surface, int biome, int height, bool write) {
+    const double bx = c.x, bz = c.z;
+    const double a = fabs(mctgen::surf_iceberg_surface(c.O, bx, 0.0, bz) * 8.25);
+    const double b = static_cast<double>(static_cast<float>(mctgen::surf_iceberg_pillar(c.O, bx * 1.28, 0.0, bz * 1.28)) * 15.0f);
+    const double iceberg = jmind(a, b);
+    if (iceberg <= 1.8) return;
+    const double roof = fabs(mctgen::surf_iceberg_pillar_roof(c.O, bx * 1.17, 0.0, bz * 1.17) * 1.5);
+    double top = jmind(iceberg * iceberg * 1.2, ceil(roof * 40.0) + 14.0);
+    if (biome_temperature(*c.A->M, biome, c.x, kSea, c.z) > 0.1f) top -= 2.0;
+    if (top <= 2.0) return;
+    const double bottom = static_cast<double>(kSea) - top - 7.0;
+    top += static_cast<double>(kSea);
+    Xoro r = at(c.S->random, c.x, 0, c.z);
+    const int max_snow = 2 + next_int(r, 4);
+    const int min_snow_height = kSea + 18 + next_int(r, 10);
+    int snow = 0;
+    const int itop = d2i(top), ibottom = d2i(bottom);
+    for (int yy = jmaxi(height, itop + 1); yy >= min_surface; --yy) {
+        const int cur = c.get(yy);
+        bool place = false;
+        if (c.air(cur) && yy < itop && next_double(r) > 0.01) place = true;
+        else if (cur == mctgen::kWater && yy > ibottom && yy < kSea && next_double(r) > 0.15) place = true;
+        if (place) {
+            if (snow <= max_snow && yy > min_snow_height) {
+                c.set(yy, mctgen::kSnowBlock, write);
+                ++snow;
+            } else {
+                c.set(yy, mctgen::kPackedIce, write);
+            }
+        }
+    }
+}
+
+// The material-rule walk of one column from `height` down, then the frozen-ocean extension.
+MCT_HD inline void surface_walk(Surf& c, int height, int surface_biome, int starting) {
+    int stone_above = 0, water_height = INT32_MIN, next_ceiling = INT32_MAX;
+    for (int yy = height; yy >= kMinY; --yy) {
+        const int old = c.get(yy);
+        if (c.air(old)) {
+            stone_above = 0;
+            water_height = INT32_MIN;
+        } else if (c.fluid(old)) {
+            if (water_height == INT32_MIN) water_height = yy + 1;
+        } else {
+            if (next_ceiling >= yy) {
+                next_ceiling = kWayBelow;
+                for (int la = yy - 1; la >= kMinY - 1; --la) {
+                    const int s = c.get(la);
+                    if (c.air(s) || c.fluid(s)) { next_ceiling = la + 1; break; }
+                }
+            }
+            ++stone_above;
+            c.stone_above = stone_above;
+            c.stone_below = yy - next_ceiling + 1;
+            c.water_height = water_height;
+            c.y = yy;
+            c.biome_set = false;
+            c.ore_set = false;
+            const int state = mctgen::apply_rules(c);
+            if (state >= 0) c.set(yy, state, true);
+        }
+    }
+    if (surface_biome == mctgen::kBiomeFrozenOcean || surface_biome == mctgen::kBiomeDeepFrozenOcean)
+        frozen_ocean(c, c.min_surface_level(), surface_biome, starting, true);
+}
+
+// Pass 1 of a column: eroded-badlands extension (real) -> h1, frozen-ocean dry run -> h2.
+MCT_HD inline void surface_pass1(const SurfArgs& A, const int* h0, int* h1, int* h2, int ci, int col) {
+    const int lx = col >> 4, lz = col & 15;
+    const int hidx = ci * 256 + col;
+    int hm = h0[hidx];
+    Surf c;
+    surf_init(c, A, ci, lx, lz, &hm);
+    const int starting = hm;
+    const int sb = c.biome_at(c.x, starting, c.z);
+    if (sb == mctgen::kBiomeErodedBadlands) eroded_badlands(c, starting);
+    h1[hidx] = hm;
+    if (sb == mctgen::kBiomeFrozenOcean || sb == mctgen::kBiomeDeepFrozenOcean) {
+        surf_update_xz(c, 0, 0);
+        frozen_ocean(c, c.min_surface_level(), sb, starting, false);
+    }
+    h2[hidx] = hm;
+}
+
+// Pass 2 of a column, assuming earlier columns (x-major, z-minor order) end at h2 and later
+// ones are still at h0; the final height goes to hf (checked against h2 afterwards).
+MCT_HD inline void surface_pass2(const SurfArgs& A, const int* h0, const int* h1, const int* h2, int* hf, int ci, int col) {
+    const int lx = col >> 4, lz = col & 15;
+    const int hidx = ci * 256 + col;
+    auto H = [&](int ox, int oz) {
+        const int j = ci * 256 + ox * 16 + oz;
+        int v;
+        if (ox == lx && oz == lz) v = h1[j];
+        else if (ox < lx || (ox == lx && oz < lz)) v = h2[j];
+        else v = h0[j];
+        return v - 1;
+    };
+    int hm = h1[hidx];
+    Surf c;
+    surf_init(c, A, ci, lx, lz, &hm);
+    const int starting = h0[hidx];
+    const int sb = c.biome_at(c.x, starting, c.z);
+    const int height = H(lx, lz) + 1;
+    const int gx = H(lx + 1 < 15 ? lx + 1 : 15, lz) - H(lx - 1 > 0 ? lx - 1 : 0, lz);
+    const int gz = H(lx, lz + 1 < 15 ? lz + 1 : 15) - H(lx, lz - 1 > 0 ? lz - 1 : 0);
+    surf_update_xz(c, gx, gz);
+    surface_walk(c, height, sb, starting);
+    hf[hidx] = hm;
+}
+
+// One column in the exact sequential order, with a live height array hm[256] of the chunk
+// (used for chunks whose speculative pass 2 saw a height change it did not assume).
+MCT_HD inline void surface_exact_column(const SurfArgs& A, int* hm, int ci, int col) {
+    const int lx = col >> 4, lz = col & 15;
+    Surf c;
+    surf_init(c, A, ci, lx, lz, hm + col);
+    auto H = [&](int ox, int oz) { return hm[ox * 16 + oz] - 1; };
+    const int starting = H(lx, lz) + 1;
+    const int sb = c.biome_at(c.x, starting, c.z);
+    if (sb == mctgen::kBiomeErodedBadlands) eroded_badlands(c, starting);
+    const int height = H(lx, lz) + 1;
+    const int gx = H(lx + 1 < 15 ? lx + 1 : 15, lz) - H(lx - 1 > 0 ? lx - 1 : 0, lz);
+    const int gz = H(lx, lz + 1 < 15 ? lz + 1 : 15) - H(lx, lz - 1 > 0 ? lz - 1 : 0);
+    surf_update_xz(c, gx, gz);
+    surface_walk(c, height, sb, starting);
+}
+
+}  // namespace mct

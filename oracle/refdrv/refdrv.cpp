@@ -162,6 +162,35 @@ int ref_sample(void* p, int64_t seed, const char* root, const int32_t* vol, floa
     }
 }
 
+// Volume-path sample of any IR node id (debug bisection), fresh caches.
+int ref_sample_id(void* p, int64_t seed, int node, const int32_t* vol, float* out) {
+    try {
+        Handle* h = static_cast<Handle*>(p);
+        TerrainSeed& ts = seed_of(h, seed);
+        VolumeEvaluator ev(h->ir, ts.tables);
+        Volume v{vol[0], vol[1], vol[2], vol[3], vol[4], vol[5], vol[6], vol[7], vol[8]};
+        ev.sample(node, v, out);
+        return 0;
+    } catch (const std::exception& e) {
+        g_err = e.what();
+        return 1;
+    }
+}
+
+// Point-path value of any IR node id at n positions.
+int ref_point_id(void* p, int64_t seed, int node, const int32_t* xyz, int n, float* out) {
+    try {
+        Handle* h = static_cast<Handle*>(p);
+        TerrainSeed& ts = seed_of(h, seed);
+        VolumeEvaluator ev(h->ir, ts.tables);
+        for (int i = 0; i < n; ++i) out[i] = ev.point(node, xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2]);
+        return 0;
+    } catch (const std::exception& e) {
+        g_err = e.what();
+        return 1;
+    }
+}
+
 // Point-path values of a named root at n positions (xyz triples).
 int ref_point(void* p, int64_t seed, const char* root, const int32_t* xyz, int n, float* out) {
     try {
