@@ -1,0 +1,98 @@
+/* C ABI of the MI355X TERRAIN-step generator (libmct.so).
+ *
+ * Drop-in boundary for the reference's GPU generator API, FullTerrainGenerator
+ * (cuda/include/mccuda_cuda/terrain_gpu.h:31-63 in bonquifo/minecraft-terrain-cuda):
+ * a seed and a chunk list go in; per chunk a u8 palette index per block and a u8 biome
+ * index per quart come out. Layouts are the reference's (terrain_gpu.h:41-43):
+ *   blocks  ((y - minY) * 16 + z) * 16 + x, 16 x 384 x 16 = 98,304 bytes per chunk
+ *   biomes  (qy * 4 + qz) * 4 + qx, 4 x 96 x 4 = 1,536 bytes per chunk
+ * chunks back to back in input order. Values index the terrain model's palette / biome list.
+ *
+ * Every entry point returns 0 on success and a non-zero status on failure; the message is
+ * in mct_last_error() (thread-local). No C++ exception crosses this boundary. A generator
+ * handle is not thread-safe: one host thread per handle (as the reference class).
+ * The IR and terrain model are compiled into the library by the code generator; mct_create
+ * checks that the given files are the ones it was generated from.
+ */
+#ifndef MCT_H
+#define MCT_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    MCT_OK = 0,
+    MCT_EINVAL = 1,       /* bad argument (null handle, count < 0, unknown flag) */
+    MCT_EMODEL = 2,       /* IR / terrain model files differ from the compiled-in model */
+    MCT_EDEVICE = 3,      /* HIP runtime error (message carries the HIP error string) */
+    MCT_ECAPACITY = 4,    /* request beyond a configured capacity */
+    MCT_EIO = 5           /* cannot read a file */
+};
+
+typedef struct mct_generator mct_generator;
+
+/* Per-stage times accumulated over a call (ms, HIP events on the batch streams), plus host
+ * wall time; mirrors TerrainTiming (terrain_gpu.h:15-22). */
+typedef struct mct_timing {
+    double h2d_ms, biome_ms, noise_ms, aquifer_ms, fill_ms, surface_ms, d2h_ms, wall_ms;
+} mct_timing;
+
+/* FullTerrainGenerator(ir, tm, max_batch_chunks, pipeline_slots) (terrain_gpu.h:39).
+ * ir_json / terrain_json: paths of the mccuda-ir/1 and mccuda-terrain/1 files.
+ * max_batch_chunks: chunks per device batch (>= 1); pipeline_slots: batches in flight (1..4).
+ * seed_capacity: per-seed noise tables kept resident on the device (>= 1; 0 = default 64). */
+int mct_create(const char* ir_json, const char* terrain_json, int max_batch_chunks, int pipeline_slots, int seed_capacity, mct_generator** out);
+int mct_destroy(mct_generator* g);
+
+/* FullTerrainGenerator::set_seed (terrain_gpu.h:42): the seed of later mct_generate calls.
+ * The seed's tables are built on the device (or found resident) at the next generate. */
+int mct_set_seed(mct_generator* g, int64_t seed);
+
+/* FullTerrainGenerator::generate (terrain_gpu.h:45): chunk_xz = count (x, z) pairs of chunk
+ * coordinates. terrain (count * 98,304 B) is required; biomes (count * 1,536 B) and aquifer
+ * (count * 98,304 B, the fill before material rules) may be NULL. Host memory; pinned
+ * memory from mct_alloc_pinned is faster. Blocks until all results are in place. */
+int mct_generate(mct_generator* g, const int32_t* chunk_xz, int count, uint8_t* terrain, uint8_t* biomes, uint8_t* aquifer, mct_timing* timing);
+
+/* Batched form (no reference counterpart; the reference runs one seed per call): chunk i
+ * uses seeds[i]. Any mix of seeds and uneven per-seed chunk lists; results do not depend on
+ * the mix. Seeds beyond seed_capacity distinct values per call are processed in groups. */
+int mct_generate_multi(mct_generator* g, const int64_t* seeds, const int32_t* chunk_xz, int count, uint8_t* terrain, uint8_t* biomes,
+                       uint8_t* aquifer, mct_timing* timing);
+
+/* Device-output form: terrain / biomes are device pointers (HBM, e.g. from mct_alloc_device);
+ * nothing is copied to the host. Used to measure the generator with its outputs resident. */
+int mct_generate_device(mct_generator* g, const int64_t* seeds, const int32_t* chunk_xz, int count, uint8_t* d_terrain, uint8_t* d_biomes,
+                        mct_timing* timing);
+
+/* FullTerrainGenerator::biome_tie_stats (terrain_gpu.h:47): out[0] quarts searched, out[1]
+ * lookups whose leaf the R-tree's last-result rule changed, out[2] of those, a changed biome.
+ * out[3]: chunks whose surface pass was redone in exact sequential column order. */
+int mct_biome_tie_stats(mct_generator* g, int64_t out[4]);
+
+/* FullTerrainGenerator::last_batch_done_ms (terrain_gpu.h:51): per batch of the last call,
+ * ms after the call started when its results were in place. Writes min(cap, batches) values
+ * and stores the batch count in *n. */
+int mct_last_batch_done_ms(mct_generator* g, double* out, int cap, int* n);
+
+/* alloc_pinned / free_pinned (terrain_gpu.h:61-62), plus device memory for the device form. */
+void* mct_alloc_pinned(size_t bytes);
+void mct_free_pinned(void* p);
+void* mct_alloc_device(size_t bytes);
+void mct_free_device(void* p);
+int mct_device_to_host(void* dst, const void* src, size_t bytes);
+
+/* Build facts: blocks per chunk, quarts per chunk, min y, height, palette size, biome count. */
+int mct_info(int32_t out[6]);
+
+const char* mct_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MCT_H */

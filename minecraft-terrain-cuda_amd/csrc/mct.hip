@@ -1,0 +1,981 @@
+// gfx950 TERRAIN-step generator: kernels, device radix-sort dedupe, per-seed setup on the
+// device, batch pipeline on HIP streams, and the C ABI declared in include/mct.h.
+//
+// Build: hipcc --offload-arch=gfx950 -O3 -ffp-contract=off -fno-fast-math
+//        -fno-gpu-flush-denormals-to-zero -fhip-fp32-correctly-rounded-divide-sqrt
+// (Java semantics need unfused, correctly rounded IEEE arithmetic with denormals).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cstdio>
+#include <cstring>
+#include <fstream>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#include "../../include/mct.h"
+#include "pipeline.h"
+
+using namespace mct;
+
+namespace {
+
+__constant__ ModelDev c_model;
+
+struct Error : std::runtime_error {
+    int code;
+    Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+void hip_check(hipError_t e, const char* what) {
+    if (e != hipSuccess) throw Error(MCT_EDEVICE, std::string("HIP error in ") + what + ": " + hipGetErrorString(e));
+}
+
+// ---- per-seed setup kernels -------------------------------------------------------------
+struct SeedJob {
+    int64_t seed;
+    int32_t slot;
+    int32_t pad;
+};
+
+constexpr int kSetupThreads = 64;
+
+// One thread per (job, NormalNoise layer); the permutation is shuffled in the thread's LDS row.
+__global__ __launch_bounds__(kSetupThreads) void k_seed_layers(SeedDev* seeds, const SeedJob* jobs, int njobs, int nnormal) {
+    __shared__ uint8_t scratch[kSetupThreads][256];
+    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= njobs * nnormal) return;
+    const SeedJob j = jobs[idx / nnormal];
+    seed_layer(seeds[j.slot], j.seed, idx % nnormal, scratch[threadIdx.x]);
+}
+
+// One thread per (job, BlendedNoise node): 40 octaves in sequence; one thread per job for
+// the positional factories, clay bands and zoom seed.
+__global__ __launch_bounds__(kSetupThreads) void k_seed_serial(SeedDev* seeds, const SeedJob* jobs, int njobs, const int* blended_first) {
+    __shared__ uint8_t scratch[kSetupThreads][256];
+    const int per = mctgen::kNumBlended + 1;
+    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= njobs * per) return;
+    const SeedJob j = jobs[idx / per];
+    const int k = idx % per;
+    if (k < mctgen::kNumBlended) seed_blended(seeds[j.slot], j.seed, k, blended_first[k], scratch[threadIdx.x]);
+    else seed_misc(seeds[j.slot], j.seed, c_model);
+}
+
+// ---- biomes ------------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_biome_columns(const SeedDev* seeds, const ChunkRef* bch, int nbk, float* bcol) {
+    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx < nbk * 16) biome_column(seeds, bch, bcol, idx);
+}
+
+__global__ __launch_bounds__(256) void k_biome_quarts(const SeedDev* seeds, const ChunkRef* bch, int nbk, const float* bcol, int64_t* targets,
+                                                      int32_t* lstar, int64_t* mind) {
+    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx < nbk * 16 * kBiomeRuns) biome_quarts(seeds, c_model, bch, bcol, targets, lstar, mind, idx);
+}
+
+__global__ __launch_bounds__(256) void k_biome_ties(int nbk, const int64_t* targets, const int32_t* lstar, const int64_t* mind, uint8_t* tie) {
+    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx < nbk * kQPer) biome_tie(c_model, targets, lstar, mind, tie, idx);
+}
+
+// The last-result rule, one wave64 per biome chunk. Windows of 64 chain positions with no tie
+// flag, entered in sync (previous result == previous L*), resolve to L* in parallel; any other
+// window is walked by lane 0 with the full rule.
+__global__ __launch_bounds__(256) void k_biome_chain(int nbk, const int64_t* targets, const int32_t* lstar, const int64_t* mind, const uint8_t* tie,
+                                                     uint8_t* biomes, unsigned long long* stats) {
+    const int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const int lane = threadIdx.x & 63;
+    if (wave >= nbk) return;
+    const ModelDev& M = c_model;
+    const size_t base = static_cast<size_t>(wave) * kQPer;
+    int prev = -1, prev_l = -1;
+    for (int p0 = 0; p0 < kQPer; p0 += 64) {
+        const int p = p0 + lane;
+        const size_t i = base + chain_quart(p);
+        const int ls = lstar[i];
+        const bool t = p > 0 && tie[base + p];
+        const unsigned long long any = __ballot(t);
+        if (any == 0 && prev == prev_l) {
+            biomes[i] = M.leaf_biome[ls];
+            prev = prev_l = __shfl(ls, 63);
+        } else {
+            if (lane == 0) {
+                unsigned long long changed = 0, changed_biome = 0;
+                for (int k = 0; k < 64; ++k) {
+                    const size_t ik = base + chain_quart(p0 + k);
+                    const int l = lstar[ik];
+                    int cur = l;
+                    if (prev >= 0 && !(prev == prev_l && !tie[base + p0 + k]))
+                        cur = box_distance(M.tree[M.leaf_node[prev]], targets + ik * 7) == mind[ik] ? prev : l;
+                    if (cur != l) {
+                        ++changed;
+                        if (M.leaf_biome[cur] != M.leaf_biome[l]) ++changed_biome;
+                    }
+                    biomes[ik] = M.leaf_biome[cur];
+                    prev = cur;
+                    prev_l = l;
+                }
+                if (changed) {
+                    atomicAdd(stats + 1, changed);
+                    atomicAdd(stats + 2, changed_biome);
+                }
+            }
+            prev = __shfl(prev, 0);
+            prev_l = __shfl(prev_l, 0);
+        }
+    }
+}
+
+// ---- density ------------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_col_keys(const ChunkRef* ch, int n, uint64_t* keys) {
+    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx < n * kCols) column_key(ch, keys, idx);
+}
+
+__global__ __launch_bounds__(256) void k_columns(const SeedDev* seeds, int upper, const uint64_t* ucols, const int* count, float* colv) {
+    const int u = blockIdx.x * blockDim.x + threadIdx.x;
+    if (u < upper && u < *count) column_values(seeds, ucols, colv, u);
+}
+
+__global__ __launch_bounds__(256) void k_corners(const SeedDev* seeds, int upper, const uint64_t* ucols, const int* count, const float* colv,
+                                                 float* corners) {
+    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx < upper * kRows && idx / kRows < *count) corner_values(seeds, ucols, colv, corners, idx);
+}
+
+// ---- surface-level grid / aquifer -----------------------------------------------------------
+__global__ __launch_bounds__(256) void k_grid_keys(const ChunkRef* ch, int n, uint64_t* keys) {
+    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx < n * kGridKeys) grid_key(ch, keys, idx);
+}
+
+__global__ __launch_bounds__(256) void k_fts(const SeedDev* seeds, int upper, const uint64_t* ukeys, const int* count, float* vals) {
+    const int u = blockIdx.x * blockDim.x + threadIdx.x;
+    if (u < upper && u < *count) fts_value(seeds, ukeys, vals, u);
+}
+
+__global__ __launch_bounds__(256) void k_grid_scatter(int n, const int* map, const float* vals, int32_t* grid, float* prelim) {
+    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx < n * kGridKeys) grid_scatter(map, vals, grid, prelim, idx);
+}
+
+__global__ __launch_bounds__(256) void k_aq_skip(int n, const int32_t* grid, int* skip) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c < n) aquifer_skip(grid, skip, c);
+}
+
+__global__ __launch_bounds__(256) void k_aq_cells(const SeedDev* seeds, const ChunkRef* ch, int n, const int32_t* grid, int32_t* loc, int32_t* fl) {
+    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx < n * kAqCells) aquifer_cell(seeds, ch, grid, loc, fl, idx);
+}
+
+// ---- fill --------------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_fill(FillArgs A, int n) {
+    const size_t idx = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (idx >= static_cast<size_t>(n) * kPer) return;
+    const int c = static_cast<int>(idx / kPer), i = static_cast<int>(idx % kPer);
+    A.blocks[idx] = fill_block(A, c, i, nullptr);
+}
+
+// ---- surface -------------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_heights(int n, const uint8_t* blocks, int* h0) {
+    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= n * 256) return;
+    const int c = idx >> 8, t = idx & 255;
+    const int x = t & 15, z = t >> 4;
+    h0[c * 256 + x * 16 + z] = column_height(c_model, blocks + static_cast<size_t>(c) * kPer, x, z);
+}
+
+// Threads of a wave take adjacent x so the column walks read adjacent bytes of a y-layer.
+__global__ __launch_bounds__(64) void k_surface1(SurfArgs A, int n, const int* h0, int* h1, int* h2) {
+    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= n * 256) return;
+    A.M = &c_model;
+    const int c = idx >> 8, t = idx & 255;
+    surface_pass1(A, h0, h1, h2, c, (t & 15) * 16 + (t >> 4));
+}
+
+__global__ __launch_bounds__(64) void k_surface2(SurfArgs A, int n, const int* h0, const int* h1, const int* h2, int* hf) {
+    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= n * 256) return;
+    A.M = &c_model;
+    const int c = idx >> 8, t = idx & 255;
+    surface_pass2(A, h0, h1, h2, hf, c, (t & 15) * 16 + (t >> 4));
+}
+
+// A chunk needs the exact pass if any column's final height differs from the height the
+// speculative pass assumed for it.
+__global__ __launch_bounds__(256) void k_surface_check(int n, const int* h2, const int* hf, int* flags, unsigned long long* stats) {
+    const int c = blockIdx.x;
+    if (c >= n) return;
+    const int t = threadIdx.x;
+    const bool bad = h2[c * 256 + t] != hf[c * 256 + t];
+    const int any = __syncthreads_or(bad ? 1 : 0);
+    if (t == 0) {
+        flags[c] = any;
+        if (any) atomicAdd(stats + 3, 1ull);
+    }
+}
+
+// Exact pass for flagged chunks: refill the chunk, recompute its heights, then process its
+// columns by anti-diagonals (x + z = k): a column reads its x-1 and z-1 neighbours after their
+// full processing and its x+1 and z+1 neighbours before theirs, as the x-major loop does.
+__global__ __launch_bounds__(256) void k_surface_exact(FillArgs F, SurfArgs A, int n, const int* flags) {
+    const int c = blockIdx.x;
+    if (c >= n || flags[c] == 0) return;
+    A.M = &c_model;
+    __shared__ int hm[256];
+    for (int i = threadIdx.x; i < kPer; i += blockDim.x) F.blocks[static_cast<size_t>(c) * kPer + i] = fill_block(F, c, i, nullptr);
+    __syncthreads();
+    {
+        const int t = threadIdx.x;
+        hm[t] = column_height(c_model, F.blocks + static_cast<size_t>(c) * kPer, t >> 4, t & 15);
+    }
+    __syncthreads();
+    for (int k = 0; k <= 30; ++k) {
+        const int t = threadIdx.x;
+        const int lx = t >> 4, lz = t & 15;
+        if (lx + lz == k) surface_exact_column(A, hm, c, t);
+        __syncthreads();
+    }
+}
+
+__global__ __launch_bounds__(256) void k_gather_biomes(int n, const int* nb, const uint8_t* bb, uint8_t* out) {
+    const size_t idx = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (idx >= static_cast<size_t>(n) * kQPer) return;
+    const size_t c = idx / kQPer, q = idx % kQPer;
+    out[idx] = bb[static_cast<size_t>(nb[c * 9 + 4]) * kQPer + q];
+}
+
+__global__ __launch_bounds__(256) void k_copy16(size_t n16, const uint4* src, uint4* dst) {
+    const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n16) dst[i] = src[i];
+}
+
+// ---- dedupe: LSD radix sort of (key, index), run heads, scan, scatter ------------------------
+constexpr int kRsThreads = 256, kRsItems = 8, kRsTile = kRsThreads * kRsItems;
+constexpr int kRsWaves = kRsThreads / 64, kRsRounds = kRsItems;   // per wave: kRsRounds rounds of 64 keys
+
+__global__ __launch_bounds__(kRsThreads) void k_rs_hist(const uint64_t* keys, int n, int shift, int nblocks, int* hist) {
+    __shared__ int h[256];
+    h[threadIdx.x] = 0;
+    __syncthreads();
+    const int base = blockIdx.x * kRsTile;
+    for (int k = 0; k < kRsItems; ++k) {
+        const int i = base + k * kRsThreads + threadIdx.x;
+        if (i < n) atomicAdd(&h[(keys[i] >> shift) & 255], 1);
+    }
+    __syncthreads();
+    hist[threadIdx.x * nblocks + blockIdx.x] = h[threadIdx.x];
+}
+
+// Exclusive scan of `total` ints in one workgroup (each thread a contiguous run).
+__global__ __launch_bounds__(1024) void k_scan_small(int* v, int total, int* total_out) {
+    __shared__ int part[1024];
+    const int t = threadIdx.x;
+    const int per = (total + 1023) / 1024;
+    const int b = t * per, e = b + per < total ? b + per : total;
+    int s = 0;
+    for (int i = b; i < e; ++i) s += v[i];
+    part[t] = s;
+    __syncthreads();
+    for (int off = 1; off < 1024; off <<= 1) {
+        const int a = t >= off ? part[t - off] : 0;
+        __syncthreads();
+        part[t] += a;
+        __syncthreads();
+    }
+    int run = part[t] - s;
+    for (int i = b; i < e; ++i) {
+        const int x = v[i];
+        v[i] = run;
+        run += x;
+    }
+    if (t == 1023 && total_out) *total_out = part[1023];
+}
+
+__device__ __forceinline__ unsigned long long match_digit(int d, bool valid) {
+    unsigned long long m = __ballot(valid);
+    for (int b = 0; b < 8; ++b) {
+        const unsigned long long bb = __ballot(valid && ((d >> b) & 1));
+        m &= ((d >> b) & 1) ? bb : ~bb;
+    }
+    return m;
+}
+
+// Stable scatter: wave w owns keys [base + w*512, +512) in rounds of 64; ranks come from
+// 64-lane ballots, per-wave digit counts from a first pass.
+__global__ __launch_bounds__(kRsThreads) void k_rs_scatter(const uint64_t* kin, const int* vin, int n, int shift, int nblocks, const int* hist,
+                                                          uint64_t* kout, int* vout) {
+    __shared__ int cnt[kRsWaves][256];
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    for (int i = threadIdx.x; i < kRsWaves * 256; i += kRsThreads) (&cnt[0][0])[i] = 0;
+    __syncthreads();
+    const int base = blockIdx.x * kRsTile + w * (kRsRounds * 64);
+    const unsigned long long below = (1ull << lane) - 1ull;
+    for (int r = 0; r < kRsRounds; ++r) {
+        const int i = base + r * 64 + lane;
+        const bool valid = i < n;
+        const int d = valid ? static_cast<int>((kin[i] >> shift) & 255) : 0;
+        const unsigned long long peers = match_digit(d, valid);
+        if (valid && (peers & below) == 0) cnt[w][d] += __popcll(peers);
+        __builtin_amdgcn_wave_barrier();
+    }
+    __syncthreads();
+    {
+        const int d = threadIdx.x;   // 256 threads, one digit each
+        int run = hist[d * nblocks + blockIdx.x];
+        for (int ww = 0; ww < kRsWaves; ++ww) {
+            const int c = cnt[ww][d];
+            cnt[ww][d] = run;
+            run += c;
+        }
+    }
+    __syncthreads();
+    for (int r = 0; r < kRsRounds; ++r) {
+        const int i = base + r * 64 + lane;
+        const bool valid = i < n;
+        const uint64_t key = valid ? kin[i] : 0;
+        const int d = static_cast<int>((key >> shift) & 255);
+        const unsigned long long peers = match_digit(d, valid);
+        if (valid) {
+            const int pos = cnt[w][d] + __popcll(peers & below);
+            kout[pos] = key;
+            vout[pos] = vin ? vin[i] : i;
+        }
+        __builtin_amdgcn_wave_barrier();
+        if (valid && (peers & below) == 0) cnt[w][d] += __popcll(peers);
+        __builtin_amdgcn_wave_barrier();
+    }
+}
+
+__global__ __launch_bounds__(256) void k_heads(const uint64_t* sorted, int n, int* flag) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) flag[i] = (i == 0 || sorted[i] != sorted[i - 1]) ? 1 : 0;
+}
+
+// Per-tile sums of the head flags (tile = 2048), then in-tile prefix + tile offset.
+__global__ __launch_bounds__(256) void k_tile_sums(const int* flag, int n, int* sums) {
+    __shared__ int s[256];
+    const int base = blockIdx.x * kRsTile;
+    int a = 0;
+    for (int k = 0; k < kRsItems; ++k) {
+        const int i = base + k * 256 + threadIdx.x;
+        if (i < n) a += flag[i];
+    }
+    s[threadIdx.x] = a;
+    __syncthreads();
+    for (int off = 128; off > 0; off >>= 1) {
+        if (threadIdx.x < off) s[threadIdx.x] += s[threadIdx.x + off];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) sums[blockIdx.x] = s[0];
+}
+
+// unique[u] = key at each run head, map[index] = u, with u from the exclusive tile offsets.
+__global__ __launch_bounds__(256) void k_unique_scatter(const uint64_t* sorted, const int* idx, const int* flag, int n, const int* tile_off,
+                                                        uint64_t* unique, int* map, int* count, const int* total) {
+    __shared__ int s[kRsTile];
+    const int base = blockIdx.x * kRsTile;
+    for (int k = 0; k < kRsItems; ++k) {
+        const int i = base + k * 256 + threadIdx.x;
+        s[k * 256 + threadIdx.x] = i < n ? flag[i] : 0;
+    }
+    __syncthreads();
+    // inclusive scan of the tile's 2048 flags: each thread scans 8 consecutive, then a
+    // 256-wide scan of the thread sums
+    __shared__ int ts[256];
+    int local[kRsItems];
+    int a = 0;
+    for (int k = 0; k < kRsItems; ++k) {
+        a += s[threadIdx.x * kRsItems + k];
+        local[k] = a;
+    }
+    ts[threadIdx.x] = a;
+    __syncthreads();
+    for (int off = 1; off < 256; off <<= 1) {
+        const int v = threadIdx.x >= off ? ts[threadIdx.x - off] : 0;
+        __syncthreads();
+        ts[threadIdx.x] += v;
+        __syncthreads();
+    }
+    const int before = ts[threadIdx.x] - a + tile_off[blockIdx.x];
+    for (int k = 0; k < kRsItems; ++k) {
+        const int i = base + threadIdx.x * kRsItems + k;
+        if (i < n) {
+            const int u = before + local[k] - 1;
+            if (s[threadIdx.x * kRsItems + k]) unique[u] = sorted[i];
+            map[idx[i]] = u;
+        }
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) *count = *total;
+}
+
+unsigned grid_of(size_t threads, int block) { return static_cast<unsigned>((threads + block - 1) / block); }
+
+// Device dedupe of 64-bit keys: unique[0..*count) ascending, map[i] = index of keys[i].
+struct Dedupe {
+    int cap = 0;
+    uint64_t *ka = nullptr, *kb = nullptr;
+    int *va = nullptr, *vb = nullptr, *hist = nullptr, *flag = nullptr, *tsum = nullptr, *total = nullptr;
+    explicit Dedupe(int capacity) : cap(capacity) {
+        const int nb = (cap + kRsTile - 1) / kRsTile;
+        hip_check(hipMalloc(&ka, sizeof(uint64_t) * cap), "malloc");
+        hip_check(hipMalloc(&kb, sizeof(uint64_t) * cap), "malloc");
+        hip_check(hipMalloc(&va, sizeof(int) * cap), "malloc");
+        hip_check(hipMalloc(&vb, sizeof(int) * cap), "malloc");
+        hip_check(hipMalloc(&flag, sizeof(int) * cap), "malloc");
+        hip_check(hipMalloc(&hist, sizeof(int) * 256 * nb), "malloc");
+        hip_check(hipMalloc(&tsum, sizeof(int) * (nb + 1)), "malloc");
+        hip_check(hipMalloc(&total, sizeof(int)), "malloc");
+    }
+    ~Dedupe() {
+        void* p[] = {ka, kb, va, vb, flag, hist, tsum, total};
+        for (void* q : p) (void)hipFree(q);
+    }
+    void run(const uint64_t* keys, int n, int key_bits, uint64_t* unique, int* map, int* count, hipStream_t s) {
+        if (n > cap) throw Error(MCT_ECAPACITY, "dedupe capacity exceeded");
+        if (n == 0) return;
+        const int nb = (n + kRsTile - 1) / kRsTile;
+        const uint64_t* kin = keys;
+        const int* vin = nullptr;
+        uint64_t* kout = ka;
+        int* vout = va;
+        for (int shift = 0; shift < key_bits; shift += 8) {
+            k_rs_hist<<<nb, kRsThreads, 0, s>>>(kin, n, shift, nb, hist);
+            k_scan_small<<<1, 1024, 0, s>>>(hist, 256 * nb, nullptr);
+            k_rs_scatter<<<nb, kRsThreads, 0, s>>>(kin, vin, n, shift, nb, hist, kout, vout);
+            kin = kout;
+            vin = vout;
+            kout = kout == ka ? kb : ka;
+            vout = vout == va ? vb : va;
+        }
+        k_heads<<<grid_of(n, 256), 256, 0, s>>>(kin, n, flag);
+        k_tile_sums<<<nb, 256, 0, s>>>(flag, n, tsum);
+        k_scan_small<<<1, 1024, 0, s>>>(tsum, nb, total);
+        k_unique_scatter<<<nb, 256, 0, s>>>(kin, vin, flag, n, tsum, unique, map, count, total);
+        hip_check(hipGetLastError(), "dedupe kernels");
+    }
+};
+
+// ---- generator -------------------------------------------------------------------------------
+struct Slot {
+    hipStream_t stream = nullptr;
+    hipEvent_t ev[9] = {};
+    bool pending = false;
+    int batch = -1;
+    int cap;                        // chunks
+    int bcap = 0;                   // biome chunks
+    // pinned staging
+    ChunkRef *h_ch = nullptr, *h_bch = nullptr;
+    int* h_nb = nullptr;
+    // device
+    ChunkRef *d_ch = nullptr, *d_bch = nullptr;
+    int* d_nb = nullptr;
+    float *d_bcol = nullptr, *d_colv = nullptr, *d_corners = nullptr, *d_gv = nullptr, *d_prelim = nullptr;
+    int64_t *d_targets = nullptr, *d_mind = nullptr;
+    int32_t *d_lstar = nullptr, *d_grid = nullptr, *d_cloc = nullptr, *d_cfl = nullptr;
+    uint8_t *d_tie = nullptr, *d_bb = nullptr, *d_blocks = nullptr, *d_bout = nullptr, *d_aq = nullptr;
+    uint64_t *d_ckeys = nullptr, *d_ucols = nullptr, *d_gkeys = nullptr, *d_ug = nullptr;
+    int *d_colmap = nullptr, *d_gmap = nullptr, *d_counts = nullptr, *d_skip = nullptr;
+    int *d_h0 = nullptr, *d_h1 = nullptr, *d_h2 = nullptr, *d_hf = nullptr, *d_flags = nullptr;
+    std::unique_ptr<Dedupe> dedupe;
+    // destination of the batch being generated
+    int n = 0;
+    uint8_t *out_terrain = nullptr, *out_biomes = nullptr, *out_aquifer = nullptr;
+    bool to_device = false;
+
+    explicit Slot(int mb) : cap(mb) {
+        hip_check(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking), "stream");
+        for (auto& e : ev) hip_check(hipEventCreate(&e), "event");
+        hip_check(hipHostMalloc(&h_ch, sizeof(ChunkRef) * mb, hipHostMallocDefault), "pinned");
+        hip_check(hipHostMalloc(&h_bch, sizeof(ChunkRef) * mb * 9, hipHostMallocDefault), "pinned");
+        hip_check(hipHostMalloc(&h_nb, sizeof(int) * mb * 9, hipHostMallocDefault), "pinned");
+        auto M = [](auto** p, size_t bytes) { hip_check(hipMalloc(reinterpret_cast<void**>(p), bytes > 0 ? bytes : 16), "malloc"); };
+        const size_t ncol = std::max(mctgen::kNumColumns, 1);
+        M(&d_ch, sizeof(ChunkRef) * mb);
+        M(&d_nb, sizeof(int) * mb * 9);
+        M(&d_ckeys, sizeof(uint64_t) * mb * kCols);
+        M(&d_ucols, sizeof(uint64_t) * mb * kCols);
+        M(&d_colmap, sizeof(int) * mb * kCols);
+        M(&d_colv, sizeof(float) * mb * kCols * ncol);
+        M(&d_corners, sizeof(float) * mb * kCols * kCh * kRows);
+        M(&d_gkeys, sizeof(uint64_t) * mb * kGridKeys);
+        M(&d_ug, sizeof(uint64_t) * mb * kGridKeys);
+        M(&d_gmap, sizeof(int) * mb * kGridKeys);
+        M(&d_gv, sizeof(float) * mb * kGridKeys);
+        M(&d_grid, sizeof(int32_t) * mb * kGridCells);
+        M(&d_prelim, sizeof(float) * mb * 4);
+        M(&d_skip, sizeof(int) * mb);
+        M(&d_cloc, sizeof(int32_t) * mb * kAqCells * 4);
+        M(&d_cfl, sizeof(int32_t) * mb * kAqCells * 2);
+        M(&d_blocks, static_cast<size_t>(mb) * kPer);
+        M(&d_bout, static_cast<size_t>(mb) * kQPer);
+        M(&d_h0, sizeof(int) * mb * 256);
+        M(&d_h1, sizeof(int) * mb * 256);
+        M(&d_h2, sizeof(int) * mb * 256);
+        M(&d_hf, sizeof(int) * mb * 256);
+        M(&d_flags, sizeof(int) * mb);
+        M(&d_counts, sizeof(int) * 4);
+        dedupe = std::make_unique<Dedupe>(mb * kGridKeys);
+    }
+    ~Slot() {
+        if (stream) (void)hipStreamSynchronize(stream);
+        void* p[] = {d_ch, d_bch, d_nb, d_bcol, d_colv, d_corners, d_gv, d_prelim, d_targets, d_mind, d_lstar, d_grid, d_cloc, d_cfl, d_tie, d_bb,
+                     d_blocks, d_bout, d_aq, d_ckeys, d_ucols, d_gkeys, d_ug, d_colmap, d_gmap, d_counts, d_skip, d_h0, d_h1, d_h2, d_hf, d_flags};
+        for (void* q : p) (void)hipFree(q);
+        dedupe.reset();
+        (void)hipHostFree(h_ch);
+        (void)hipHostFree(h_bch);
+        (void)hipHostFree(h_nb);
+        for (auto& e : ev) (void)hipEventDestroy(e);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+    void ensure_biome_capacity(int nbk) {
+        if (nbk <= bcap) return;
+        void* old[] = {d_bch, d_bcol, d_targets, d_mind, d_lstar, d_tie, d_bb};
+        for (void* q : old) (void)hipFree(q);
+        const size_t q = static_cast<size_t>(nbk) * kQPer;
+        hip_check(hipMalloc(&d_bch, sizeof(ChunkRef) * nbk), "malloc");
+        hip_check(hipMalloc(&d_bcol, sizeof(float) * nbk * 16 * std::max(mctgen::kNumClimateColumns, 1)), "malloc");
+        hip_check(hipMalloc(&d_targets, sizeof(int64_t) * q * 7), "malloc");
+        hip_check(hipMalloc(&d_mind, sizeof(int64_t) * q), "malloc");
+        hip_check(hipMalloc(&d_lstar, sizeof(int32_t) * q), "malloc");
+        hip_check(hipMalloc(&d_tie, q), "malloc");
+        hip_check(hipMalloc(&d_bb, q), "malloc");
+        bcap = nbk;
+    }
+};
+
+struct Generator {
+    int max_batch, nslots, seed_cap;
+    std::vector<std::unique_ptr<Slot>> slots;
+    size_t next_slot = 0;
+    // model
+    TreeNodeDev* d_tree = nullptr;
+    uint8_t* d_leaf_biome = nullptr;
+    int32_t* d_leaf_node = nullptr;
+    int* d_blended_first = nullptr;
+    int nnormal = 0;
+    // resident seeds
+    SeedDev* d_seeds = nullptr;
+    std::vector<int64_t> slot_seed;
+    std::vector<uint64_t> slot_used;
+    std::unordered_map<int64_t, int> resident;
+    uint64_t use_clock = 0;
+    SeedJob* d_jobs = nullptr;
+    SeedJob* h_jobs = nullptr;
+    int64_t current_seed = 0;
+    unsigned long long* d_stats = nullptr;   // [0] quarts, [1] overrides, [2] biome changes, [3] exact redo
+    uint64_t quarts = 0;
+    hipEvent_t t0 = nullptr;
+    std::vector<double> done_ms;
+
+    Generator(int mb, int ns, int sc) : max_batch(mb), nslots(ns), seed_cap(sc) {
+        std::vector<TreeNodeDev> tree(mctgen::kTreeNodes);
+        for (int i = 0; i < mctgen::kTreeNodes; ++i) {
+            const mctgen::TreeNode& t = mctgen::kTree[i];
+            for (int k = 0; k < 7; ++k) { tree[i].lo[k] = t.lo[k]; tree[i].hi[k] = t.hi[k]; }
+            tree[i].first = t.first;
+            tree[i].count = t.count;
+            tree[i].leaf = t.leaf;
+            tree[i].pad = 0;
+        }
+        hip_check(hipMalloc(&d_tree, sizeof(TreeNodeDev) * tree.size()), "malloc");
+        hip_check(hipMemcpy(d_tree, tree.data(), sizeof(TreeNodeDev) * tree.size(), hipMemcpyHostToDevice), "upload tree");
+        hip_check(hipMalloc(&d_leaf_biome, mctgen::kTreeLeaves), "malloc");
+        hip_check(hipMemcpy(d_leaf_biome, mctgen::kLeafBiome, mctgen::kTreeLeaves, hipMemcpyHostToDevice), "upload");
+        hip_check(hipMalloc(&d_leaf_node, sizeof(int32_t) * mctgen::kTreeLeaves), "malloc");
+        hip_check(hipMemcpy(d_leaf_node, mctgen::kLeafNode, sizeof(int32_t) * mctgen::kTreeLeaves, hipMemcpyHostToDevice), "upload");
+        ModelDev M{};
+        M.tree = d_tree;
+        M.leaf_biome = d_leaf_biome;
+        M.leaf_node = d_leaf_node;
+        std::memcpy(M.block_flags, mctgen::kBlockFlags, 64);
+        std::memcpy(M.biome_temp, mctgen::kBiomeTemp, sizeof M.biome_temp);
+        std::memcpy(M.biome_frozen, mctgen::kBiomeFrozen, sizeof M.biome_frozen);
+        std::memcpy(M.simplex, mctgen::kSimplexPerm, sizeof M.simplex);
+        for (int k = 0; k < 7; ++k) M.terracotta[k] = mctgen::kTerracotta[k];
+        for (int k = 0; k < mctgen::kNumOreRules; ++k) {
+            for (int j = 0; j < 3; ++j) M.ore_state[k][j] = mctgen::kOreState[k][j];
+            M.ore_raw[k] = mctgen::kOreRawChance[k];
+        }
+        hip_check(hipMemcpyToSymbol(HIP_SYMBOL(c_model), &M, sizeof M), "upload model");
+        std::vector<int> bf(std::max(mctgen::kNumBlended, 1), -1);
+        for (int l = 0; l < mctgen::kNumLayers; ++l) {
+            const mctgen::LayerRecipe& r = mctgen::kLayerRecipe[l];
+            if (r.kind == 0) nnormal = l + 1;
+            else if (bf[r.a] < 0) bf[r.a] = l;
+        }
+        hip_check(hipMalloc(&d_blended_first, sizeof(int) * bf.size()), "malloc");
+        hip_check(hipMemcpy(d_blended_first, bf.data(), sizeof(int) * bf.size(), hipMemcpyHostToDevice), "upload");
+        hip_check(hipMalloc(&d_seeds, sizeof(SeedDev) * seed_cap), "malloc seed tables");
+        slot_seed.assign(seed_cap, 0);
+        slot_used.assign(seed_cap, 0);
+        hip_check(hipMalloc(&d_jobs, sizeof(SeedJob) * seed_cap), "malloc");
+        hip_check(hipHostMalloc(&h_jobs, sizeof(SeedJob) * seed_cap, hipHostMallocDefault), "pinned");
+        hip_check(hipMalloc(&d_stats, sizeof(unsigned long long) * 4), "malloc");
+        hip_check(hipMemset(d_stats, 0, sizeof(unsigned long long) * 4), "memset");
+        hip_check(hipEventCreate(&t0), "event");
+        for (int i = 0; i < nslots; ++i) slots.push_back(std::make_unique<Slot>(mb));
+        hip_check(hipDeviceSynchronize(), "init");
+    }
+
+    ~Generator() {
+        (void)hipDeviceSynchronize();
+        slots.clear();
+        void* p[] = {d_tree, d_leaf_biome, d_leaf_node, d_blended_first, d_seeds, d_jobs, d_stats};
+        for (void* q : p) (void)hipFree(q);
+        (void)hipHostFree(h_jobs);
+        if (t0) (void)hipEventDestroy(t0);
+    }
+
+    void collect(Slot& S, mct_timing* tm) {
+        if (!S.pending) return;
+        hip_check(hipEventSynchronize(S.ev[8]), "sync");
+        S.pending = false;
+        if (S.batch >= 0 && S.batch < static_cast<int>(done_ms.size())) {
+            float ms = 0;
+            hip_check(hipEventElapsedTime(&ms, t0, S.ev[8]), "event time");
+            done_ms[static_cast<size_t>(S.batch)] = ms;
+        }
+        S.batch = -1;
+        if (!tm) return;
+        float t[8];
+        for (int i = 0; i < 8; ++i) hip_check(hipEventElapsedTime(&t[i], S.ev[i], S.ev[i + 1]), "event time");
+        tm->h2d_ms += t[0];
+        tm->biome_ms += t[1];
+        tm->noise_ms += t[2];
+        tm->aquifer_ms += t[3];
+        tm->fill_ms += t[4];
+        tm->surface_ms += t[5] + t[6];
+        tm->d2h_ms += t[7];
+    }
+    void collect_all(mct_timing* tm) {
+        for (auto& s : slots) collect(*s, tm);
+    }
+
+    // Slots of the given seeds (building missing tables on `stream`); at most seed_cap distinct.
+    void ensure_seeds(const std::vector<int64_t>& need, hipStream_t stream) {
+        int njobs = 0;
+        ++use_clock;
+        for (int64_t s : need) {
+            auto it = resident.find(s);
+            if (it != resident.end()) {
+                slot_used[it->second] = use_clock;
+                continue;
+            }
+            int victim = -1;
+            for (int k = 0; k < seed_cap; ++k)
+                if (slot_used[k] != use_clock && (victim < 0 || slot_used[k] < slot_used[victim])) victim = k;
+            if (victim < 0) throw Error(MCT_ECAPACITY, "more distinct seeds in one batch than seed_capacity");
+            if (slot_used[victim] != 0) {
+                collect_all(nullptr);   // in-flight batches may still read the victim's tables
+                resident.erase(slot_seed[victim]);
+            }
+            slot_seed[victim] = s;
+            slot_used[victim] = use_clock;
+            resident[s] = victim;
+            h_jobs[njobs++] = SeedJob{s, victim, 0};
+        }
+        if (njobs == 0) return;
+        hip_check(hipStreamSynchronize(stream), "sync");   // h_jobs staging is shared
+        hip_check(hipMemcpyAsync(d_jobs, h_jobs, sizeof(SeedJob) * njobs, hipMemcpyHostToDevice, stream), "upload seed jobs");
+        k_seed_layers<<<grid_of(static_cast<size_t>(njobs) * nnormal, kSetupThreads), kSetupThreads, 0, stream>>>(d_seeds, d_jobs, njobs, nnormal);
+        k_seed_serial<<<grid_of(static_cast<size_t>(njobs) * (mctgen::kNumBlended + 1), kSetupThreads), kSetupThreads, 0, stream>>>(d_seeds, d_jobs, njobs,
+                                                                                                                               d_blended_first);
+        hip_check(hipGetLastError(), "seed setup kernels");
+        hip_check(hipStreamSynchronize(stream), "seed setup");
+    }
+
+    // Enqueues one batch on slot S (uploads, kernels, downloads); returns without waiting.
+    void run_batch(Slot& S, const int64_t* seeds, const int32_t* xz, int n, uint8_t* terrain, uint8_t* biomes, uint8_t* aquifer, bool to_device) {
+        S.n = n;
+        S.out_terrain = terrain;
+        S.out_biomes = biomes;
+        S.out_aquifer = aquifer;
+        S.to_device = to_device;
+        if (aquifer && !S.d_aq) hip_check(hipMalloc(&S.d_aq, static_cast<size_t>(S.cap) * kPer), "malloc");
+        // seeds of this batch -> resident slots
+        std::vector<int64_t> distinct;
+        {
+            std::unordered_map<int64_t, int> seen;
+            for (int i = 0; i < n; ++i)
+                if (seen.emplace(seeds[i], 1).second) distinct.push_back(seeds[i]);
+        }
+        ensure_seeds(distinct, S.stream);
+        // chunks and the union of their 3x3 biome neighbourhoods (per seed)
+        std::unordered_map<uint64_t, int> bindex;
+        bindex.reserve(static_cast<size_t>(n) * 4);
+        int nbk = 0;
+        for (int i = 0; i < n; ++i) {
+            const int slot = resident.at(seeds[i]);
+            S.h_ch[i] = ChunkRef{xz[2 * i], xz[2 * i + 1], slot, 0};
+            for (int dz = -1; dz <= 1; ++dz)
+                for (int dx = -1; dx <= 1; ++dx) {
+                    const int bx = xz[2 * i] + dx, bz = xz[2 * i + 1] + dz;
+                    const uint64_t key = (static_cast<uint64_t>(slot) << 48) ^ (static_cast<uint64_t>(static_cast<uint32_t>(bx) & 0xFFFFFF) << 24) ^
+                                         static_cast<uint64_t>(static_cast<uint32_t>(bz) & 0xFFFFFF);
+                    auto it = bindex.find(key);
+                    if (it == bindex.end()) {
+                        it = bindex.emplace(key, nbk).first;
+                        S.h_bch[nbk++] = ChunkRef{bx, bz, slot, 0};
+                    }
+                    S.h_nb[static_cast<size_t>(i) * 9 + (dz + 1) * 3 + dx + 1] = it->second;
+                }
+        }
+        S.ensure_biome_capacity(nbk);
+        hipStream_t st = S.stream;
+        hip_check(hipEventRecord(S.ev[0], st), "event");
+        hip_check(hipMemcpyAsync(S.d_ch, S.h_ch, sizeof(ChunkRef) * n, hipMemcpyHostToDevice, st), "upload chunks");
+        hip_check(hipMemcpyAsync(S.d_bch, S.h_bch, sizeof(ChunkRef) * nbk, hipMemcpyHostToDevice, st), "upload biome chunks");
+        hip_check(hipMemcpyAsync(S.d_nb, S.h_nb, sizeof(int) * n * 9, hipMemcpyHostToDevice, st), "upload neighbours");
+        hip_check(hipEventRecord(S.ev[1], st), "event");
+        const SeedDev* sd = d_seeds;
+        // biomes
+        k_biome_columns<<<grid_of(static_cast<size_t>(nbk) * 16, 256), 256, 0, st>>>(sd, S.d_bch, nbk, S.d_bcol);
+        k_biome_quarts<<<grid_of(static_cast<size_t>(nbk) * 16 * kBiomeRuns, 256), 256, 0, st>>>(sd, S.d_bch, nbk, S.d_bcol, S.d_targets, S.d_lstar,
+                                                                                                 S.d_mind);
+        k_biome_ties<<<grid_of(static_cast<size_t>(nbk) * kQPer, 256), 256, 0, st>>>(nbk, S.d_targets, S.d_lstar, S.d_mind, S.d_tie);
+        k_biome_chain<<<grid_of(static_cast<size_t>(nbk) * 64, 256), 256, 0, st>>>(nbk, S.d_targets, S.d_lstar, S.d_mind, S.d_tie, S.d_bb, d_stats);
+        quarts += static_cast<uint64_t>(nbk) * kQPer;
+        hip_check(hipGetLastError(), "biome kernels");
+        hip_check(hipEventRecord(S.ev[2], st), "event");
+        // density corners
+        const int ck = n * kCols;
+        k_col_keys<<<grid_of(ck, 256), 256, 0, st>>>(S.d_ch, n, S.d_ckeys);
+        S.dedupe->run(S.d_ckeys, ck, kKeyBits, S.d_ucols, S.d_colmap, S.d_counts, st);
+        k_columns<<<grid_of(ck, 256), 256, 0, st>>>(sd, ck, S.d_ucols, S.d_counts, S.d_colv);
+        k_corners<<<grid_of(static_cast<size_t>(ck) * kRows, 256), 256, 0, st>>>(sd, ck, S.d_ucols, S.d_counts, S.d_colv, S.d_corners);
+        hip_check(hipGetLastError(), "noise kernels");
+        hip_check(hipEventRecord(S.ev[3], st), "event");
+        // surface-level grid and aquifer cells
+        const int gk = n * kGridKeys;
+        k_grid_keys<<<grid_of(gk, 256), 256, 0, st>>>(S.d_ch, n, S.d_gkeys);
+        S.dedupe->run(S.d_gkeys, gk, kKeyBits, S.d_ug, S.d_gmap, S.d_counts + 1, st);
+        k_fts<<<grid_of(gk, 256), 256, 0, st>>>(sd, gk, S.d_ug, S.d_counts + 1, S.d_gv);
+        k_grid_scatter<<<grid_of(gk, 256), 256, 0, st>>>(n, S.d_gmap, S.d_gv, S.d_grid, S.d_prelim);
+        k_aq_skip<<<grid_of(n, 256), 256, 0, st>>>(n, S.d_grid, S.d_skip);
+        k_aq_cells<<<grid_of(static_cast<size_t>(n) * kAqCells, 256), 256, 0, st>>>(sd, S.d_ch, n, S.d_grid, S.d_cloc, S.d_cfl);
+        hip_check(hipGetLastError(), "aquifer kernels");
+        hip_check(hipEventRecord(S.ev[4], st), "event");
+        // fill
+        uint8_t* blocks = to_device ? terrain : S.d_blocks;
+        FillArgs F{sd, S.d_ch, S.d_corners, S.d_colmap, S.d_cloc, S.d_cfl, S.d_skip, blocks};
+        k_fill<<<grid_of(static_cast<size_t>(n) * kPer, 256), 256, 0, st>>>(F, n);
+        hip_check(hipGetLastError(), "fill kernel");
+        hip_check(hipEventRecord(S.ev[5], st), "event");
+        if (aquifer) k_copy16<<<grid_of(static_cast<size_t>(n) * kPer / 16, 256), 256, 0, st>>>(static_cast<size_t>(n) * kPer / 16,
+                                                                                               reinterpret_cast<const uint4*>(blocks),
+                                                                                               reinterpret_cast<uint4*>(S.d_aq));
+        hip_check(hipEventRecord(S.ev[6], st), "event");
+        // surface
+        SurfArgs A{sd, nullptr, S.d_ch, blocks, S.d_nb, S.d_bb, S.d_corners, S.d_colmap, S.d_prelim};
+        k_heights<<<grid_of(static_cast<size_t>(n) * 256, 256), 256, 0, st>>>(n, blocks, S.d_h0);
+        k_surface1<<<grid_of(static_cast<size_t>(n) * 256, 64), 64, 0, st>>>(A, n, S.d_h0, S.d_h1, S.d_h2);
+        k_surface2<<<grid_of(static_cast<size_t>(n) * 256, 64), 64, 0, st>>>(A, n, S.d_h0, S.d_h1, S.d_h2, S.d_hf);
+        k_surface_check<<<n, 256, 0, st>>>(n, S.d_h2, S.d_hf, S.d_flags, d_stats);
+        k_surface_exact<<<n, 256, 0, st>>>(F, A, n, S.d_flags);
+        uint8_t* bout = to_device ? biomes : S.d_bout;
+        if (bout) k_gather_biomes<<<grid_of(static_cast<size_t>(n) * kQPer, 256), 256, 0, st>>>(n, S.d_nb, S.d_bb, bout);
+        hip_check(hipGetLastError(), "surface kernels");
+        hip_check(hipEventRecord(S.ev[7], st), "event");
+        if (!to_device) {
+            hip_check(hipMemcpyAsync(terrain, S.d_blocks, static_cast<size_t>(n) * kPer, hipMemcpyDeviceToHost, st), "download terrain");
+            if (biomes) hip_check(hipMemcpyAsync(biomes, S.d_bout, static_cast<size_t>(n) * kQPer, hipMemcpyDeviceToHost, st), "download biomes");
+            if (aquifer) hip_check(hipMemcpyAsync(aquifer, S.d_aq, static_cast<size_t>(n) * kPer, hipMemcpyDeviceToHost, st), "download aquifer");
+        }
+        hip_check(hipEventRecord(S.ev[8], st), "event");
+        S.pending = true;
+    }
+
+    void generate(const int64_t* seeds, const int32_t* xz, int count, uint8_t* terrain, uint8_t* biomes, uint8_t* aquifer, mct_timing* tm,
+                  bool to_device) {
+        const auto w0 = std::chrono::steady_clock::now();
+        collect_all(nullptr);
+        done_ms.assign(static_cast<size_t>((count + max_batch - 1) / max_batch), 0.0);
+        hip_check(hipEventRecord(t0, slots[0]->stream), "event");
+        int batch = 0;
+        for (int off = 0; off < count; off += max_batch, ++batch) {
+            const int n = std::min(max_batch, count - off);
+            Slot& S = *slots[next_slot];
+            next_slot = (next_slot + 1) % slots.size();
+            collect(S, tm);
+            S.batch = batch;
+            run_batch(S, seeds + off, xz + 2 * static_cast<size_t>(off), n, terrain + static_cast<size_t>(off) * kPer,
+                      biomes ? biomes + static_cast<size_t>(off) * kQPer : nullptr, aquifer ? aquifer + static_cast<size_t>(off) * kPer : nullptr,
+                      to_device);
+        }
+        collect_all(tm);
+        if (tm) tm->wall_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - w0).count();
+    }
+};
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& m) {
+    g_err = m;
+    return code;
+}
+
+template <typename F>
+int guard(F&& f) {
+    try {
+        f();
+        return MCT_OK;
+    } catch (const Error& e) {
+        return fail(e.code, e.what());
+    } catch (const std::bad_alloc&) {
+        return fail(MCT_ECAPACITY, "host allocation failed");
+    } catch (const std::exception& e) {
+        return fail(MCT_EINVAL, e.what());
+    }
+}
+
+uint64_t file_fnv1a(const char* path) {
+    std::ifstream in(path, std::ios::binary);
+    if (!in) throw Error(MCT_EIO, std::string("cannot open ") + path);
+    uint64_t h = 0xcbf29ce484222325ull;
+    char buf[1 << 16];
+    while (in) {
+        in.read(buf, sizeof buf);
+        const std::streamsize got = in.gcount();
+        for (std::streamsize i = 0; i < got; ++i) {
+            h ^= static_cast<uint8_t>(buf[i]);
+            h *= 0x100000001b3ull;
+        }
+    }
+    return h;
+}
+
+}  // namespace
+
+struct mct_generator {
+    std::unique_ptr<Generator> g;
+};
+
+extern "C" {
+
+const char* mct_last_error(void) { return g_err.c_str(); }
+
+int mct_create(const char* ir_json, const char* terrain_json, int max_batch_chunks, int pipeline_slots, int seed_capacity, mct_generator** out) {
+    return guard([&] {
+        if (!out) throw Error(MCT_EINVAL, "out is null");
+        *out = nullptr;
+        if (!ir_json || !terrain_json) throw Error(MCT_EINVAL, "null path");
+        if (max_batch_chunks < 1 || max_batch_chunks > (1 << 20)) throw Error(MCT_EINVAL, "max_batch_chunks out of range");
+        if (pipeline_slots < 1 || pipeline_slots > 4) throw Error(MCT_EINVAL, "pipeline_slots must be 1..4");
+        if (seed_capacity == 0) seed_capacity = 64;
+        if (seed_capacity < 1 || seed_capacity > 4096) throw Error(MCT_EINVAL, "seed_capacity must be 1..4096");
+        if (file_fnv1a(ir_json) != mctgen::kIrHash) throw Error(MCT_EMODEL, "the IR file differs from the one this library was generated from (rebuild)");
+        if (file_fnv1a(terrain_json) != mctgen::kTerrainHash)
+            throw Error(MCT_EMODEL, "the terrain model file differs from the one this library was generated from (rebuild)");
+        int dev = 0;
+        hip_check(hipGetDevice(&dev), "get device");
+        auto h = std::make_unique<mct_generator>();
+        h->g = std::make_unique<Generator>(max_batch_chunks, pipeline_slots, seed_capacity);
+        *out = h.release();
+    });
+}
+
+int mct_destroy(mct_generator* g) {
+    return guard([&] { delete g; });
+}
+
+int mct_set_seed(mct_generator* g, int64_t seed) {
+    return guard([&] {
+        if (!g) throw Error(MCT_EINVAL, "null generator");
+        g->g->collect_all(nullptr);
+        g->g->current_seed = seed;
+    });
+}
+
+int mct_generate(mct_generator* g, const int32_t* chunk_xz, int count, uint8_t* terrain, uint8_t* biomes, uint8_t* aquifer, mct_timing* timing) {
+    return guard([&] {
+        if (!g || count < 0 || (count > 0 && (!chunk_xz || !terrain))) throw Error(MCT_EINVAL, "bad arguments");
+        std::vector<int64_t> seeds(static_cast<size_t>(count), g->g->current_seed);
+        g->g->generate(seeds.data(), chunk_xz, count, terrain, biomes, aquifer, timing, false);
+    });
+}
+
+int mct_generate_multi(mct_generator* g, const int64_t* seeds, const int32_t* chunk_xz, int count, uint8_t* terrain, uint8_t* biomes, uint8_t* aquifer,
+                       mct_timing* timing) {
+    return guard([&] {
+        if (!g || count < 0 || (count > 0 && (!seeds || !chunk_xz || !terrain))) throw Error(MCT_EINVAL, "bad arguments");
+        g->g->generate(seeds, chunk_xz, count, terrain, biomes, aquifer, timing, false);
+    });
+}
+
+int mct_generate_device(mct_generator* g, const int64_t* seeds, const int32_t* chunk_xz, int count, uint8_t* d_terrain, uint8_t* d_biomes,
+                        mct_timing* timing) {
+    return guard([&] {
+        if (!g || count < 0 || (count > 0 && (!seeds || !chunk_xz || !d_terrain))) throw Error(MCT_EINVAL, "bad arguments");
+        g->g->generate(seeds, chunk_xz, count, d_terrain, d_biomes, nullptr, timing, true);
+    });
+}
+
+int mct_biome_tie_stats(mct_generator* g, int64_t out[4]) {
+    return guard([&] {
+        if (!g || !out) throw Error(MCT_EINVAL, "bad arguments");
+        g->g->collect_all(nullptr);
+        unsigned long long s[4];
+        hip_check(hipMemcpy(s, g->g->d_stats, sizeof s, hipMemcpyDeviceToHost), "download stats");
+        out[0] = static_cast<int64_t>(g->g->quarts);
+        out[1] = static_cast<int64_t>(s[1]);
+        out[2] = static_cast<int64_t>(s[2]);
+        out[3] = static_cast<int64_t>(s[3]);
+    });
+}
+
+int mct_last_batch_done_ms(mct_generator* g, double* out, int cap, int* n) {
+    return guard([&] {
+        if (!g || !n) throw Error(MCT_EINVAL, "bad arguments");
+        const auto& d = g->g->done_ms;
+        *n = static_cast<int>(d.size());
+        for (int i = 0; i < cap && i < static_cast<int>(d.size()); ++i) out[i] = d[static_cast<size_t>(i)];
+    });
+}
+
+void* mct_alloc_pinned(size_t bytes) {
+    void* p = nullptr;
+    if (hipHostMalloc(&p, bytes, hipHostMallocDefault) != hipSuccess) {
+        g_err = "pinned host allocation failed";
+        return nullptr;
+    }
+    return p;
+}
+
+void mct_free_pinned(void* p) { (void)hipHostFree(p); }
+
+void* mct_alloc_device(size_t bytes) {
+    void* p = nullptr;
+    if (hipMalloc(&p, bytes) != hipSuccess) {
+        g_err = "device allocation failed";
+        return nullptr;
+    }
+    return p;
+}
+
+void mct_free_device(void* p) { (void)hipFree(p); }
+
+int mct_device_to_host(void* dst, const void* src, size_t bytes) {
+    return guard([&] { hip_check(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost), "device to host"); });
+}
+
+int mct_info(int32_t out[6]) {
+    if (!out) return fail(MCT_EINVAL, "null out");
+    out[0] = kPer;
+    out[1] = kQPer;
+    out[2] = kMinY;
+    out[3] = kHeight;
+    out[4] = mctgen::kPaletteSize;
+    out[5] = mctgen::kNumBiomes;
+    return MCT_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,161 @@
+"""Python binding of libmct.so (the C ABI in include/mct.h), shaped like the reference's
+generator class FullTerrainGenerator (cuda/include/mccuda_cuda/terrain_gpu.h:37-55).
+
+Arrays are numpy; the library is loaded from this directory and must have been built
+(`python build.py`). Nothing here computes terrain: every call goes to the HIP library,
+and a missing or unloadable library is an error.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, 'libmct.so')
+IR_PATH = os.path.join(HERE, 'data', 'synthetic_overworld_ir.json')
+TERRAIN_PATH = os.path.join(HERE, 'data', 'synthetic_overworld_terrain.json')
+
+BLOCKS_PER_CHUNK = 16 * 384 * 16
+QUARTS_PER_CHUNK = 4 * 96 * 4
+
+_lib = None
+
+
+class MctError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__('mct error %d: %s' % (code, msg))
+        self.code = code
+
+
+class Timing(ctypes.Structure):
+    _fields_ = [(k, ctypes.c_double) for k in ('h2d_ms', 'biome_ms', 'noise_ms', 'aquifer_ms', 'fill_ms', 'surface_ms', 'd2h_ms', 'wall_ms')]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise MctError(-1, 'libmct.so is not built (run build.py): %s' % LIB_PATH)
+        L = ctypes.CDLL(LIB_PATH)
+        P, I, I64, SZ = ctypes.c_void_p, ctypes.c_int, ctypes.c_int64, ctypes.c_size_t
+        L.mct_last_error.restype = ctypes.c_char_p
+        L.mct_create.argtypes = [ctypes.c_char_p, ctypes.c_char_p, I, I, I, ctypes.POINTER(P)]
+        L.mct_destroy.argtypes = [P]
+        L.mct_set_seed.argtypes = [P, I64]
+        L.mct_generate.argtypes = [P, P, I, P, P, P, P]
+        L.mct_generate_multi.argtypes = [P, P, P, I, P, P, P, P]
+        L.mct_generate_device.argtypes = [P, P, P, I, P, P, P]
+        L.mct_biome_tie_stats.argtypes = [P, P]
+        L.mct_last_batch_done_ms.argtypes = [P, P, I, ctypes.POINTER(I)]
+        L.mct_alloc_pinned.argtypes = [SZ]
+        L.mct_alloc_pinned.restype = P
+        L.mct_free_pinned.argtypes = [P]
+        L.mct_alloc_device.argtypes = [SZ]
+        L.mct_alloc_device.restype = P
+        L.mct_free_device.argtypes = [P]
+        L.mct_device_to_host.argtypes = [P, P, SZ]
+        L.mct_info.argtypes = [P]
+        _lib = L
+    return _lib
+
+
+def _check(rc):
+    if rc != 0:
+        raise MctError(rc, lib().mct_last_error().decode())
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data
+
+
+def _xz(chunks):
+    a = np.ascontiguousarray(np.asarray(chunks, dtype=np.int32).reshape(-1, 2))
+    return a
+
+
+class FullTerrainGenerator:
+    """generate(chunks) -> (terrain u8[n, 384, 16, 16], biomes u8[n, 96, 4, 4])."""
+
+    def __init__(self, max_batch_chunks=256, pipeline_slots=2, seed_capacity=64, ir_path=IR_PATH, terrain_path=TERRAIN_PATH):
+        h = ctypes.c_void_p()
+        _check(lib().mct_create(ir_path.encode(), terrain_path.encode(), max_batch_chunks, pipeline_slots, seed_capacity, ctypes.byref(h)))
+        self.h = h
+        self.max_batch = max_batch_chunks
+
+    def close(self):
+        if self.h:
+            lib().mct_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_seed(self, seed):
+        _check(lib().mct_set_seed(self.h, int(seed)))
+
+    def generate(self, chunks, biomes=True, aquifer=False, timing=None, out=None):
+        xz = _xz(chunks)
+        n = len(xz)
+        t = out if out is not None else np.empty((n, 384, 16, 16), np.uint8)
+        b = np.empty((n, 96, 4, 4), np.uint8) if biomes else None
+        a = np.empty((n, 384, 16, 16), np.uint8) if aquifer else None
+        _check(lib().mct_generate(self.h, _ptr(xz), n, _ptr(t), _ptr(b), _ptr(a), ctypes.byref(timing) if timing is not None else None))
+        return (t, b, a) if aquifer else (t, b)
+
+    def generate_multi(self, seeds, chunks, biomes=True, aquifer=False, timing=None):
+        xz = _xz(chunks)
+        n = len(xz)
+        s = np.ascontiguousarray(np.asarray(seeds, dtype=np.int64))
+        if len(s) != n:
+            raise ValueError('one seed per chunk')
+        t = np.empty((n, 384, 16, 16), np.uint8)
+        b = np.empty((n, 96, 4, 4), np.uint8) if biomes else None
+        a = np.empty((n, 384, 16, 16), np.uint8) if aquifer else None
+        _check(lib().mct_generate_multi(self.h, _ptr(s), _ptr(xz), n, _ptr(t), _ptr(b), _ptr(a), ctypes.byref(timing) if timing is not None else None))
+        return (t, b, a) if aquifer else (t, b)
+
+    def generate_device(self, seeds, chunks, d_terrain, d_biomes=None, timing=None):
+        """Outputs stay in device memory (pointers from alloc_device)."""
+        xz = _xz(chunks)
+        s = np.ascontiguousarray(np.asarray(seeds, dtype=np.int64))
+        _check(lib().mct_generate_device(self.h, _ptr(s), _ptr(xz), len(xz), d_terrain, d_biomes, ctypes.byref(timing) if timing is not None else None))
+
+    def biome_tie_stats(self):
+        out = np.zeros(4, np.int64)
+        _check(lib().mct_biome_tie_stats(self.h, _ptr(out)))
+        return {'quarts': int(out[0]), 'overrides': int(out[1]), 'biome_changes': int(out[2]), 'exact_surface_redo': int(out[3])}
+
+    def last_batch_done_ms(self):
+        n = ctypes.c_int()
+        buf = np.zeros(1 << 16, np.float64)
+        _check(lib().mct_last_batch_done_ms(self.h, _ptr(buf), len(buf), ctypes.byref(n)))
+        return buf[:min(n.value, len(buf))].copy()
+
+
+def alloc_device(nbytes):
+    p = lib().mct_alloc_device(nbytes)
+    if not p:
+        raise MctError(3, lib().mct_last_error().decode())
+    return p
+
+
+def free_device(p):
+    lib().mct_free_device(p)
+
+
+def device_to_host(p, nbytes):
+    out = np.empty(nbytes, np.uint8)
+    _check(lib().mct_device_to_host(_ptr(out), p, nbytes))
+    return out
+
+
+def info():
+    out = np.zeros(6, np.int32)
+    _check(lib().mct_info(_ptr(out)))
+    return dict(zip(('blocks_per_chunk', 'quarts_per_chunk', 'min_y', 'height', 'palette', 'biomes'), (int(v) for v in out)))
