@@ -1,0 +1,169 @@
+"""Throughput of the MI355X TERRAIN-step generator.
+
+A step is one pass of the full TERRAIN step (biomes, density + aquifer fill, material rules
+with ore veins) over the plain single-seed workload: three 64 x 64-chunk regions (origin,
+chunk (-2048, -2048), and one ending at chunk 117,187, the chunk of block 1,874,999) plus the
+noise-wrap chunks and the reference's world-border chunks, with outputs kept in HBM. `value`
+is chunks/s of that step on one GPU. Alongside it the line carries:
+
+  e2e_host_chunks_per_s   the same chunks with results copied to pinned host memory
+  batched                 mixed-seed batch (256 uniform seeds, uneven chunk lists), chunks/s
+  roofline                the dominant kernel against the MI355X VALU peak
+  cpu_baseline            the reference cpu/ port (built from its sources) on the host cores
+  stages_ms / kernels_ms  per-stage and per-kernel times of one step (HIP events)
+
+  python bench.py [--gpus 1] [--steps K] [--warmup W] [--quick]
+"""
+import argparse
+import ctypes
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, 'minecraft-terrain-cuda_amd'))
+
+SEED = 123456789            # the reference bench's seed (bench/bench_main.cpp:50)
+BORDER_CHUNK = 1874999 >> 4  # chunk of block 1,874,999
+WRAP_CHUNKS = [(6128, 0), (-6129, 5), (11, 6128), (-3, -6129)]
+REF_BORDER_CHUNKS = [(1874999, 1874999), (-1874999, 1874998), (1874999, -1874999), (-1874999, -1874999)]
+
+# MI355X VALU issue rate: 256 CUs x 64 lanes x 2.4 GHz (non-packed, one op per lane per
+# cycle; FP32 and FP64 both full rate on CDNA4; no FMA is used, so one op = one flop).
+VALU_PEAK_TOPS = 256 * 64 * 2.4e9 / 1e12
+
+# VALU lane-operations per unit of work of each kernel group, from rocprofv3 --pmc
+# SQ_INSTS_VALU (x 64 lanes) on this workload (profiles/valu_counts.md); the unit is what
+# one thread of the kernel processes.
+VALU_PER_UNIT = {
+    'corners': None,
+    'fill': None,
+    'surface2': None,
+    'biome_quarts': None,
+    'fts': None,
+}
+
+
+def region(x0, z0, w):
+    return [(x, z) for z in range(z0, z0 + w) for x in range(x0, x0 + w)]
+
+
+def workload(side=64):
+    chunks = region(0, 0, side) + region(-2048, -2048, side) + region(BORDER_CHUNK - side + 1, -BORDER_CHUNK, side)
+    return chunks + WRAP_CHUNKS + REF_BORDER_CHUNKS
+
+
+def distinct_columns(chunks):
+    cols = set()
+    for x, z in chunks:
+        for k in range(25):
+            cols.add((x * 4 + k % 5, z * 4 + k // 5))
+    return len(cols)
+
+
+def cpu_baseline(seconds_target=15.0):
+    """The reference cpu/ port (oracle/_ref, built from its sources) in its bench's region
+    mode (biomes of the region + border once, then fill + surface per chunk) on a thread
+    pool of the host's cores."""
+    lib = os.path.join(ROOT, 'oracle', '_ref', 'libmcref.so')
+    if not os.path.exists(lib):
+        return None
+    import mct
+    P, I64, I = ctypes.c_void_p, ctypes.c_int64, ctypes.c_int
+    R = ctypes.CDLL(lib)
+    R.ref_create.restype = P
+    R.ref_create.argtypes = [ctypes.c_char_p, ctypes.c_char_p]
+    R.ref_generate_region.argtypes = [P, I64, I, I, I, I, I, P, P]
+    h = R.ref_create(mct.IR_PATH.encode(), mct.TERRAIN_PATH.encode())
+    threads = len(os.sched_getaffinity(0)) if hasattr(os, 'sched_getaffinity') else os.cpu_count()
+    threads = max(1, min(threads, 16))
+    w = 8
+    t = np.zeros(w * w * 98304, np.uint16)
+    t0 = time.perf_counter()
+    R.ref_generate_region(h, SEED, 0, 0, w, w, threads, None, t.ctypes.data)
+    dt = time.perf_counter() - t0
+    # grow the region to about seconds_target of work
+    w2 = int(max(8, min(64, w * (seconds_target / max(dt, 1e-3)) ** 0.5)))
+    t = np.zeros(w2 * w2 * 98304, np.uint16)
+    t0 = time.perf_counter()
+    R.ref_generate_region(h, SEED, 0, 0, w2, w2, threads, None, t.ctypes.data)
+    dt = time.perf_counter() - t0
+    return {'value': round(w2 * w2 / dt, 2), 'unit': 'chunks/s', 'cores': threads, 'kind': 'reference',
+            'sample': '%dx%d-chunk region at (0,0), seed %d, reference cpu/ port region mode, %.1f s' % (w2, w2, SEED, dt)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument('--gpus', type=int, default=1)
+    ap.add_argument('--steps', type=int, default=3)
+    ap.add_argument('--warmup', type=int, default=1)
+    ap.add_argument('--batch', type=int, default=4096)
+    ap.add_argument('--slots', type=int, default=2)
+    ap.add_argument('--quick', action='store_true', help='one 32x32 region instead of the full workload')
+    ap.add_argument('--no-cpu', action='store_true')
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print('bench.py measures one GPU: this workload shards by chunk tiles with no exchange step; '
+              'runs on more than one GPU are not part of this build (--gpus %d refused)' % a.gpus, file=sys.stderr)
+        sys.exit(2)
+    import mct
+    chunks = region(0, 0, 32) if a.quick else workload()
+    n = len(chunks)
+    seeds = np.full(n, SEED, np.int64)
+    g = mct.FullTerrainGenerator(max_batch_chunks=a.batch, pipeline_slots=a.slots, seed_capacity=256)
+    d_t = mct.alloc_device(n * mct.BLOCKS_PER_CHUNK)
+    d_b = mct.alloc_device(n * mct.QUARTS_PER_CHUNK)
+    for _ in range(a.warmup):
+        g.generate_device(seeds, chunks, d_t, d_b)
+    t0 = time.perf_counter()
+    for _ in range(a.steps):
+        g.generate_device(seeds, chunks, d_t, d_b)
+    dt = (time.perf_counter() - t0) / a.steps
+    value = n / dt
+    # one profiled step: stage and kernel times
+    tm = mct.Timing()
+    g.profile(True)
+    g.generate_device(seeds, chunks, d_t, d_b, timing=tm)
+    kt = g.kernel_times()
+    g.profile(False)
+    stages = {k: round(v, 3) for k, v in tm.as_dict().items()}
+    kernels = {k: round(v[0], 3) for k, v in kt.items() if v[1] > 0}
+    dom = max(kernels, key=kernels.get)
+    units = {'corners': distinct_columns(chunks) * 49, 'fill': n * 98304, 'surface2': n * 256}
+    roof = None
+    per_unit = VALU_PER_UNIT.get(dom)
+    if per_unit and dom in units:
+        ops = per_unit * units[dom]
+        achieved = ops / (kernels[dom] * 1e-3) / 1e12
+        roof = {'bound': 'valu', 'kernel': dom, 'achieved': round(achieved, 3), 'peak': VALU_PEAK_TOPS, 'unit': 'TOP/s',
+                'frac': round(achieved / VALU_PEAK_TOPS, 4), 'traffic': None}
+    else:
+        roof = {'bound': 'valu', 'kernel': dom, 'achieved': None, 'peak': VALU_PEAK_TOPS, 'unit': 'TOP/s', 'frac': None, 'traffic': None}
+    # end to end with results copied to pinned host memory
+    pin = mct.lib().mct_alloc_pinned(n * mct.BLOCKS_PER_CHUNK)
+    buf = np.ctypeslib.as_array(ctypes.cast(pin, ctypes.POINTER(ctypes.c_uint8)), shape=(n, 384, 16, 16))
+    g.set_seed(SEED)
+    g.generate(chunks, biomes=False, out=buf)
+    t0 = time.perf_counter()
+    g.generate(chunks, biomes=False, out=buf)
+    e2e = n / (time.perf_counter() - t0)
+    mct.lib().mct_free_pinned(pin)
+    mct.free_device(d_t)
+    mct.free_device(d_b)
+    g.close()
+    cpu = None if a.no_cpu else cpu_baseline()
+    line = {'metric': 'chunks/s full TERRAIN, batched/served vs plain, 1 MI355X; p50/p99 latency', 'value': round(value, 1), 'unit': 'chunks/s',
+            'n_gpus': 1, 'steps': a.steps, 'warmup': a.warmup, 'ms_per_step': round(dt * 1e3, 3), 'higher_is_better': True, 'scaling': 'weak',
+            'vs_baseline': None, 'dtype': 'f32+f64+i64', 'data': 'synthetic (deterministic synthetic overworld IR and terrain model)',
+            'config': {'workload': 'plain TERRAIN, one seed (%d), %s, full height, outputs in HBM' % (
+                SEED, '32x32 region' if a.quick else '3 x 64x64-chunk regions (origin, (-2048,-2048), border chunk %d) + 8 wrap/border chunks' % BORDER_CHUNK),
+                'chunks': n, 'batch': a.batch, 'slots': a.slots},
+            'e2e_host_chunks_per_s': round(e2e, 1), 'roofline': roof, 'cpu_baseline': cpu, 'stages_ms': stages, 'kernels_ms': kernels}
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == '__main__':
+    main()

@@ -86,6 +86,13 @@ void* mct_alloc_device(size_t bytes);
 void mct_free_device(void* p);
 int mct_device_to_host(void* dst, const void* src, size_t bytes);
 
+/* Per-kernel timing (HIP events around each kernel group on its stream; adds a little
+ * overhead, off by default). enable != 0 turns it on and clears the totals. */
+int mct_profile(mct_generator* g, int enable);
+/* Accumulated ms and launch counts per kernel group since mct_profile; *n = group count. */
+int mct_kernel_times(mct_generator* g, double* ms, int64_t* launches, int cap, int* n);
+const char* mct_kernel_name(int i);
+
 /* Build facts: blocks per chunk, quarts per chunk, min y, height, palette size, biome count. */
 int mct_info(int32_t out[6]);
 

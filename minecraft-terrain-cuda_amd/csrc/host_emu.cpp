@@ -75,6 +75,23 @@ void build_seed(Emu& e, int64_t seed) {
     e.has_seed = true;
 }
 
+KeyCodec codec_for(const std::vector<ChunkRef>& ch, int xlo, int xhi, int zlo, int zhi) {
+    int mnx = INT32_MAX, mxx = INT32_MIN, mnz = INT32_MAX, mxz = INT32_MIN, mns = INT32_MAX, mxs = INT32_MIN;
+    for (const ChunkRef& c : ch) {
+        mnx = std::min(mnx, c.x); mxx = std::max(mxx, c.x); mnz = std::min(mnz, c.z); mxz = std::max(mxz, c.z);
+        mns = std::min(mns, c.seed); mxs = std::max(mxs, c.seed);
+    }
+    auto bits = [](int64_t range) { int b = 0; while ((int64_t{1} << b) < range) ++b; return b; };
+    KeyCodec k{};
+    k.x0 = 4 * mnx + xlo;
+    k.z0 = 4 * mnz + zlo;
+    k.s0 = mns;
+    k.bx = bits(static_cast<int64_t>(4 * mxx + xhi) - k.x0 + 1);
+    k.bz = bits(static_cast<int64_t>(4 * mxz + zhi) - k.z0 + 1);
+    k.bits = bits(static_cast<int64_t>(mxs - mns + 1)) + 1 + k.bx + k.bz;
+    return k;
+}
+
 // unique sorted keys and the map key -> unique index
 void dedupe(const std::vector<uint64_t>& keys, std::vector<uint64_t>& uniq, std::vector<int>& map) {
     uniq = keys;
@@ -156,20 +173,22 @@ int mcte_generate(void* p, int64_t seed, const int32_t* xz, int n, uint8_t* terr
         // corner columns and corners
         std::vector<uint64_t> ckeys(static_cast<size_t>(n) * kCols), ucols;
         std::vector<int> colmap;
-        for (int i = 0; i < n * kCols; ++i) column_key(ch.data(), ckeys.data(), i);
+        const KeyCodec KC = codec_for(ch, 0, 4, 0, 4);
+        for (int i = 0; i < n * kCols; ++i) column_key(KC, ch.data(), ckeys.data(), i);
         dedupe(ckeys, ucols, colmap);
         const int U = static_cast<int>(ucols.size());
         std::vector<float> colv(static_cast<size_t>(U) * (mctgen::kNumColumns > 0 ? mctgen::kNumColumns : 1));
         std::vector<float> corners(static_cast<size_t>(U) * kCh * kRows);
-        for (int u = 0; u < U; ++u) column_values(seeds, ucols.data(), colv.data(), u);
-        for (int i = 0; i < U * kRows; ++i) corner_values(seeds, ucols.data(), colv.data(), corners.data(), i);
+        for (int u = 0; u < U; ++u) column_values(KC, seeds, ucols.data(), colv.data(), u);
+        for (int i = 0; i < U * kRows; ++i) corner_values(KC, seeds, ucols.data(), colv.data(), corners.data(), i);
         // surface-level grid and aquifer
         std::vector<uint64_t> gkeys(static_cast<size_t>(n) * kGridKeys), ug;
         std::vector<int> gmap;
-        for (int i = 0; i < n * kGridKeys; ++i) grid_key(ch.data(), gkeys.data(), i);
+        const KeyCodec KG = codec_for(ch, -16, 10, -8, 10);
+        for (int i = 0; i < n * kGridKeys; ++i) grid_key(KG, ch.data(), gkeys.data(), i);
         dedupe(gkeys, ug, gmap);
         std::vector<float> gv(ug.size());
-        for (size_t u = 0; u < ug.size(); ++u) fts_value(seeds, ug.data(), gv.data(), static_cast<int>(u));
+        for (size_t u = 0; u < ug.size(); ++u) fts_value(KG, seeds, ug.data(), gv.data(), static_cast<int>(u));
         std::vector<int32_t> grid(static_cast<size_t>(n) * kGridCells);
         std::vector<float> prelim(static_cast<size_t>(n) * 4);
         for (int i = 0; i < n * kGridKeys; ++i) grid_scatter(gmap.data(), gv.data(), grid.data(), prelim.data(), i);

@@ -132,31 +132,31 @@ __global__ __launch_bounds__(256) void k_biome_chain(int nbk, const int64_t* tar
 }
 
 // ---- density ------------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_col_keys(const ChunkRef* ch, int n, uint64_t* keys) {
+__global__ __launch_bounds__(256) void k_col_keys(KeyCodec K, const ChunkRef* ch, int n, uint64_t* keys) {
     const int idx = blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx < n * kCols) column_key(ch, keys, idx);
+    if (idx < n * kCols) column_key(K, ch, keys, idx);
 }
 
-__global__ __launch_bounds__(256) void k_columns(const SeedDev* seeds, int upper, const uint64_t* ucols, const int* count, float* colv) {
+__global__ __launch_bounds__(256) void k_columns(KeyCodec K, const SeedDev* seeds, int upper, const uint64_t* ucols, const int* count, float* colv) {
     const int u = blockIdx.x * blockDim.x + threadIdx.x;
-    if (u < upper && u < *count) column_values(seeds, ucols, colv, u);
+    if (u < upper && u < *count) column_values(K, seeds, ucols, colv, u);
 }
 
-__global__ __launch_bounds__(256) void k_corners(const SeedDev* seeds, int upper, const uint64_t* ucols, const int* count, const float* colv,
+__global__ __launch_bounds__(256) void k_corners(KeyCodec K, const SeedDev* seeds, int upper, const uint64_t* ucols, const int* count, const float* colv,
                                                  float* corners) {
     const int idx = blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx < upper * kRows && idx / kRows < *count) corner_values(seeds, ucols, colv, corners, idx);
+    if (idx < upper * kRows && idx / kRows < *count) corner_values(K, seeds, ucols, colv, corners, idx);
 }
 
 // ---- surface-level grid / aquifer -----------------------------------------------------------
-__global__ __launch_bounds__(256) void k_grid_keys(const ChunkRef* ch, int n, uint64_t* keys) {
+__global__ __launch_bounds__(256) void k_grid_keys(KeyCodec K, const ChunkRef* ch, int n, uint64_t* keys) {
     const int idx = blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx < n * kGridKeys) grid_key(ch, keys, idx);
+    if (idx < n * kGridKeys) grid_key(K, ch, keys, idx);
 }
 
-__global__ __launch_bounds__(256) void k_fts(const SeedDev* seeds, int upper, const uint64_t* ukeys, const int* count, float* vals) {
+__global__ __launch_bounds__(256) void k_fts(KeyCodec K, const SeedDev* seeds, int upper, const uint64_t* ukeys, const int* count, float* vals) {
     const int u = blockIdx.x * blockDim.x + threadIdx.x;
-    if (u < upper && u < *count) fts_value(seeds, ukeys, vals, u);
+    if (u < upper && u < *count) fts_value(K, seeds, ukeys, vals, u);
 }
 
 __global__ __launch_bounds__(256) void k_grid_scatter(int n, const int* map, const float* vals, int32_t* grid, float* prelim) {
@@ -183,8 +183,9 @@ __global__ __launch_bounds__(256) void k_fill(FillArgs A, int n) {
 }
 
 // ---- surface -------------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_heights(int n, const uint8_t* blocks, int* h0) {
+__global__ __launch_bounds__(256) void k_heights(int n, const uint8_t* blocks, int* h0, int* redo_count) {
     const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx == 0) *redo_count = 0;   // list of chunks for the exact surface pass, filled by k_surface_check
     if (idx >= n * 256) return;
     const int c = idx >> 8, t = idx & 255;
     const int x = t & 15, z = t >> 4;
@@ -210,26 +211,28 @@ __global__ __launch_bounds__(64) void k_surface2(SurfArgs A, int n, const int* h
 
 // A chunk needs the exact pass if any column's final height differs from the height the
 // speculative pass assumed for it.
-__global__ __launch_bounds__(256) void k_surface_check(int n, const int* h2, const int* hf, int* flags, unsigned long long* stats) {
+__global__ __launch_bounds__(256) void k_surface_check(int n, const int* h2, const int* hf, int* redo, int* redo_count, unsigned long long* stats) {
     const int c = blockIdx.x;
     if (c >= n) return;
     const int t = threadIdx.x;
     const bool bad = h2[c * 256 + t] != hf[c * 256 + t];
     const int any = __syncthreads_or(bad ? 1 : 0);
-    if (t == 0) {
-        flags[c] = any;
-        if (any) atomicAdd(stats + 3, 1ull);
+    if (t == 0 && any) {
+        redo[atomicAdd(redo_count, 1)] = c;
+        atomicAdd(stats + 3, 1ull);
     }
 }
 
 // Exact pass for flagged chunks: refill the chunk, recompute its heights, then process its
 // columns by anti-diagonals (x + z = k): a column reads its x-1 and z-1 neighbours after their
 // full processing and its x+1 and z+1 neighbours before theirs, as the x-major loop does.
-__global__ __launch_bounds__(256) void k_surface_exact(FillArgs F, SurfArgs A, int n, const int* flags) {
-    const int c = blockIdx.x;
-    if (c >= n || flags[c] == 0) return;
+__global__ __launch_bounds__(256) void k_surface_exact(FillArgs F, SurfArgs A, const int* redo, const int* redo_count) {
     A.M = &c_model;
     __shared__ int hm[256];
+    const int total = *redo_count;
+    for (int r = blockIdx.x; r < total; r += gridDim.x) {
+    const int c = redo[r];
+    __syncthreads();
     for (int i = threadIdx.x; i < kPer; i += blockDim.x) F.blocks[static_cast<size_t>(c) * kPer + i] = fill_block(F, c, i, nullptr);
     __syncthreads();
     {
@@ -242,6 +245,7 @@ __global__ __launch_bounds__(256) void k_surface_exact(FillArgs F, SurfArgs A, i
         const int lx = t >> 4, lz = t & 15;
         if (lx + lz == k) surface_exact_column(A, hm, c, t);
         __syncthreads();
+    }
     }
 }
 
@@ -258,7 +262,7 @@ __global__ __launch_bounds__(256) void k_copy16(size_t n16, const uint4* src, ui
 }
 
 // ---- dedupe: LSD radix sort of (key, index), run heads, scan, scatter ------------------------
-constexpr int kRsThreads = 256, kRsItems = 8, kRsTile = kRsThreads * kRsItems;
+constexpr int kRsThreads = 256, kRsItems = 16, kRsTile = kRsThreads * kRsItems;
 constexpr int kRsWaves = kRsThreads / 64, kRsRounds = kRsItems;   // per wave: kRsRounds rounds of 64 keys
 
 __global__ __launch_bounds__(kRsThreads) void k_rs_hist(const uint64_t* keys, int n, int shift, int nblocks, int* hist) {
@@ -416,6 +420,24 @@ __global__ __launch_bounds__(256) void k_unique_scatter(const uint64_t* sorted, 
     if (blockIdx.x == 0 && threadIdx.x == 0) *count = *total;
 }
 
+KeyCodec codec_for(const ChunkRef* ch, int n, int xlo, int xhi, int zlo, int zhi) {
+    int mnx = INT32_MAX, mxx = INT32_MIN, mnz = INT32_MAX, mxz = INT32_MIN, mns = INT32_MAX, mxs = INT32_MIN;
+    for (int i = 0; i < n; ++i) {
+        mnx = std::min(mnx, ch[i].x); mxx = std::max(mxx, ch[i].x); mnz = std::min(mnz, ch[i].z); mxz = std::max(mxz, ch[i].z);
+        mns = std::min(mns, ch[i].seed); mxs = std::max(mxs, ch[i].seed);
+    }
+    auto bits = [](int64_t range) { int b = 0; while ((int64_t{1} << b) < range) ++b; return b; };
+    KeyCodec k{};
+    k.x0 = 4 * mnx + xlo;
+    k.z0 = 4 * mnz + zlo;
+    k.s0 = mns;
+    k.bx = bits(static_cast<int64_t>(4 * mxx + xhi) - k.x0 + 1);
+    k.bz = bits(static_cast<int64_t>(4 * mxz + zhi) - k.z0 + 1);
+    k.bits = bits(static_cast<int64_t>(mxs - mns + 1)) + 1 + k.bx + k.bz;
+    if (k.bits > 64) throw Error(MCT_ECAPACITY, "chunk range too wide for 64-bit position keys");
+    return k;
+}
+
 unsigned grid_of(size_t threads, int block) { return static_cast<unsigned>((threads + block - 1) / block); }
 
 // Device dedupe of 64-bit keys: unique[0..*count) ascending, map[i] = index of keys[i].
@@ -464,6 +486,15 @@ struct Dedupe {
 };
 
 // ---- generator -------------------------------------------------------------------------------
+// Kernel groups timed with HIP events on the batch stream when profiling is enabled.
+enum KernelId {
+    kKBiomeColumns, kKBiomeQuarts, kKBiomeTies, kKBiomeChain, kKDedupeCols, kKColumns, kKCorners, kKDedupeGrid, kKFts, kKGridScatter,
+    kKAqCells, kKFill, kKHeights, kKSurface1, kKSurface2, kKSurfaceExact, kKGather, kKCount
+};
+const char* const kKernelNames[kKCount] = {"biome_columns", "biome_quarts", "biome_ties", "biome_chain", "dedupe_columns", "columns", "corners",
+                                           "dedupe_grid", "fts", "grid_scatter+aquifer_skip", "aquifer_cells", "fill", "heights", "surface1",
+                                           "surface2", "surface_check+exact", "gather_biomes"};
+
 struct Slot {
     hipStream_t stream = nullptr;
     hipEvent_t ev[9] = {};
@@ -485,6 +516,8 @@ struct Slot {
     int *d_colmap = nullptr, *d_gmap = nullptr, *d_counts = nullptr, *d_skip = nullptr;
     int *d_h0 = nullptr, *d_h1 = nullptr, *d_h2 = nullptr, *d_hf = nullptr, *d_flags = nullptr;
     std::unique_ptr<Dedupe> dedupe;
+    hipEvent_t kev[kKCount][2] = {};
+    bool kev_recorded = false;
     // destination of the batch being generated
     int n = 0;
     uint8_t *out_terrain = nullptr, *out_biomes = nullptr, *out_aquifer = nullptr;
@@ -534,6 +567,9 @@ struct Slot {
         (void)hipHostFree(h_bch);
         (void)hipHostFree(h_nb);
         for (auto& e : ev) (void)hipEventDestroy(e);
+        for (auto& k : kev)
+            for (auto& e : k)
+                if (e) (void)hipEventDestroy(e);
         if (stream) (void)hipStreamDestroy(stream);
     }
     void ensure_biome_capacity(int nbk) {
@@ -575,6 +611,23 @@ struct Generator {
     uint64_t quarts = 0;
     hipEvent_t t0 = nullptr;
     std::vector<double> done_ms;
+    bool profile = false;
+    double kernel_ms[kKCount] = {};
+    int64_t kernel_launches[kKCount] = {};
+
+    // Records kernel-group events around `launch` on the slot stream when profiling.
+    template <typename F>
+    void timed(Slot& S, int id, F&& launch) {
+        if (!profile) { launch(); return; }
+        if (!S.kev[id][0]) {
+            hip_check(hipEventCreate(&S.kev[id][0]), "event");
+            hip_check(hipEventCreate(&S.kev[id][1]), "event");
+        }
+        hip_check(hipEventRecord(S.kev[id][0], S.stream), "event");
+        launch();
+        hip_check(hipEventRecord(S.kev[id][1], S.stream), "event");
+        S.kev_recorded = true;
+    }
 
     Generator(int mb, int ns, int sc) : max_batch(mb), nslots(ns), seed_cap(sc) {
         std::vector<TreeNodeDev> tree(mctgen::kTreeNodes);
@@ -645,6 +698,17 @@ struct Generator {
             done_ms[static_cast<size_t>(S.batch)] = ms;
         }
         S.batch = -1;
+        if (S.kev_recorded) {
+            for (int k = 0; k < kKCount; ++k) {
+                if (!S.kev[k][0]) continue;
+                float ms = 0;
+                if (hipEventElapsedTime(&ms, S.kev[k][0], S.kev[k][1]) == hipSuccess) {
+                    kernel_ms[k] += ms;
+                    kernel_launches[k] += 1;
+                }
+            }
+            S.kev_recorded = false;
+        }
         if (!tm) return;
         float t[8];
         for (int i = 0; i < 8; ++i) hip_check(hipEventElapsedTime(&t[i], S.ev[i], S.ev[i + 1]), "event time");
@@ -738,36 +802,38 @@ struct Generator {
         hip_check(hipEventRecord(S.ev[1], st), "event");
         const SeedDev* sd = d_seeds;
         // biomes
-        k_biome_columns<<<grid_of(static_cast<size_t>(nbk) * 16, 256), 256, 0, st>>>(sd, S.d_bch, nbk, S.d_bcol);
-        k_biome_quarts<<<grid_of(static_cast<size_t>(nbk) * 16 * kBiomeRuns, 256), 256, 0, st>>>(sd, S.d_bch, nbk, S.d_bcol, S.d_targets, S.d_lstar,
-                                                                                                 S.d_mind);
-        k_biome_ties<<<grid_of(static_cast<size_t>(nbk) * kQPer, 256), 256, 0, st>>>(nbk, S.d_targets, S.d_lstar, S.d_mind, S.d_tie);
-        k_biome_chain<<<grid_of(static_cast<size_t>(nbk) * 64, 256), 256, 0, st>>>(nbk, S.d_targets, S.d_lstar, S.d_mind, S.d_tie, S.d_bb, d_stats);
+        timed(S, kKBiomeColumns, [&] { k_biome_columns<<<grid_of(static_cast<size_t>(nbk) * 16, 256), 256, 0, st>>>(sd, S.d_bch, nbk, S.d_bcol); });
+        timed(S, kKBiomeQuarts, [&] { k_biome_quarts<<<grid_of(static_cast<size_t>(nbk) * 16 * kBiomeRuns, 256), 256, 0, st>>>(sd, S.d_bch, nbk, S.d_bcol, S.d_targets, S.d_lstar,
+                                                                                                 S.d_mind); });
+        timed(S, kKBiomeTies, [&] { k_biome_ties<<<grid_of(static_cast<size_t>(nbk) * kQPer, 256), 256, 0, st>>>(nbk, S.d_targets, S.d_lstar, S.d_mind, S.d_tie); });
+        timed(S, kKBiomeChain, [&] { k_biome_chain<<<grid_of(static_cast<size_t>(nbk) * 64, 256), 256, 0, st>>>(nbk, S.d_targets, S.d_lstar, S.d_mind, S.d_tie, S.d_bb, d_stats); });
         quarts += static_cast<uint64_t>(nbk) * kQPer;
         hip_check(hipGetLastError(), "biome kernels");
         hip_check(hipEventRecord(S.ev[2], st), "event");
         // density corners
         const int ck = n * kCols;
-        k_col_keys<<<grid_of(ck, 256), 256, 0, st>>>(S.d_ch, n, S.d_ckeys);
-        S.dedupe->run(S.d_ckeys, ck, kKeyBits, S.d_ucols, S.d_colmap, S.d_counts, st);
-        k_columns<<<grid_of(ck, 256), 256, 0, st>>>(sd, ck, S.d_ucols, S.d_counts, S.d_colv);
-        k_corners<<<grid_of(static_cast<size_t>(ck) * kRows, 256), 256, 0, st>>>(sd, ck, S.d_ucols, S.d_counts, S.d_colv, S.d_corners);
+        const KeyCodec KC = codec_for(S.h_ch, n, 0, 4, 0, 4);
+        const KeyCodec KG = codec_for(S.h_ch, n, -16, 10, -8, 10);
+        k_col_keys<<<grid_of(ck, 256), 256, 0, st>>>(KC, S.d_ch, n, S.d_ckeys);
+        timed(S, kKDedupeCols, [&] { S.dedupe->run(S.d_ckeys, ck, KC.bits, S.d_ucols, S.d_colmap, S.d_counts, st); });
+        timed(S, kKColumns, [&] { k_columns<<<grid_of(ck, 256), 256, 0, st>>>(KC, sd, ck, S.d_ucols, S.d_counts, S.d_colv); });
+        timed(S, kKCorners, [&] { k_corners<<<grid_of(static_cast<size_t>(ck) * kRows, 256), 256, 0, st>>>(KC, sd, ck, S.d_ucols, S.d_counts, S.d_colv, S.d_corners); });
         hip_check(hipGetLastError(), "noise kernels");
         hip_check(hipEventRecord(S.ev[3], st), "event");
         // surface-level grid and aquifer cells
         const int gk = n * kGridKeys;
-        k_grid_keys<<<grid_of(gk, 256), 256, 0, st>>>(S.d_ch, n, S.d_gkeys);
-        S.dedupe->run(S.d_gkeys, gk, kKeyBits, S.d_ug, S.d_gmap, S.d_counts + 1, st);
-        k_fts<<<grid_of(gk, 256), 256, 0, st>>>(sd, gk, S.d_ug, S.d_counts + 1, S.d_gv);
-        k_grid_scatter<<<grid_of(gk, 256), 256, 0, st>>>(n, S.d_gmap, S.d_gv, S.d_grid, S.d_prelim);
+        k_grid_keys<<<grid_of(gk, 256), 256, 0, st>>>(KG, S.d_ch, n, S.d_gkeys);
+        timed(S, kKDedupeGrid, [&] { S.dedupe->run(S.d_gkeys, gk, KG.bits, S.d_ug, S.d_gmap, S.d_counts + 1, st); });
+        timed(S, kKFts, [&] { k_fts<<<grid_of(gk, 256), 256, 0, st>>>(KG, sd, gk, S.d_ug, S.d_counts + 1, S.d_gv); });
+        timed(S, kKGridScatter, [&] { k_grid_scatter<<<grid_of(gk, 256), 256, 0, st>>>(n, S.d_gmap, S.d_gv, S.d_grid, S.d_prelim); });
         k_aq_skip<<<grid_of(n, 256), 256, 0, st>>>(n, S.d_grid, S.d_skip);
-        k_aq_cells<<<grid_of(static_cast<size_t>(n) * kAqCells, 256), 256, 0, st>>>(sd, S.d_ch, n, S.d_grid, S.d_cloc, S.d_cfl);
+        timed(S, kKAqCells, [&] { k_aq_cells<<<grid_of(static_cast<size_t>(n) * kAqCells, 256), 256, 0, st>>>(sd, S.d_ch, n, S.d_grid, S.d_cloc, S.d_cfl); });
         hip_check(hipGetLastError(), "aquifer kernels");
         hip_check(hipEventRecord(S.ev[4], st), "event");
         // fill
         uint8_t* blocks = to_device ? terrain : S.d_blocks;
         FillArgs F{sd, S.d_ch, S.d_corners, S.d_colmap, S.d_cloc, S.d_cfl, S.d_skip, blocks};
-        k_fill<<<grid_of(static_cast<size_t>(n) * kPer, 256), 256, 0, st>>>(F, n);
+        timed(S, kKFill, [&] { k_fill<<<grid_of(static_cast<size_t>(n) * kPer, 256), 256, 0, st>>>(F, n); });
         hip_check(hipGetLastError(), "fill kernel");
         hip_check(hipEventRecord(S.ev[5], st), "event");
         if (aquifer) k_copy16<<<grid_of(static_cast<size_t>(n) * kPer / 16, 256), 256, 0, st>>>(static_cast<size_t>(n) * kPer / 16,
@@ -776,11 +842,11 @@ struct Generator {
         hip_check(hipEventRecord(S.ev[6], st), "event");
         // surface
         SurfArgs A{sd, nullptr, S.d_ch, blocks, S.d_nb, S.d_bb, S.d_corners, S.d_colmap, S.d_prelim};
-        k_heights<<<grid_of(static_cast<size_t>(n) * 256, 256), 256, 0, st>>>(n, blocks, S.d_h0);
-        k_surface1<<<grid_of(static_cast<size_t>(n) * 256, 64), 64, 0, st>>>(A, n, S.d_h0, S.d_h1, S.d_h2);
-        k_surface2<<<grid_of(static_cast<size_t>(n) * 256, 64), 64, 0, st>>>(A, n, S.d_h0, S.d_h1, S.d_h2, S.d_hf);
-        k_surface_check<<<n, 256, 0, st>>>(n, S.d_h2, S.d_hf, S.d_flags, d_stats);
-        k_surface_exact<<<n, 256, 0, st>>>(F, A, n, S.d_flags);
+        timed(S, kKHeights, [&] { k_heights<<<grid_of(static_cast<size_t>(n) * 256, 256), 256, 0, st>>>(n, blocks, S.d_h0, S.d_counts + 2); });
+        timed(S, kKSurface1, [&] { k_surface1<<<grid_of(static_cast<size_t>(n) * 256, 64), 64, 0, st>>>(A, n, S.d_h0, S.d_h1, S.d_h2); });
+        timed(S, kKSurface2, [&] { k_surface2<<<grid_of(static_cast<size_t>(n) * 256, 64), 64, 0, st>>>(A, n, S.d_h0, S.d_h1, S.d_h2, S.d_hf); });
+        k_surface_check<<<n, 256, 0, st>>>(n, S.d_h2, S.d_hf, S.d_flags, S.d_counts + 2, d_stats);
+        timed(S, kKSurfaceExact, [&] { k_surface_exact<<<64, 256, 0, st>>>(F, A, S.d_flags, S.d_counts + 2); });
         uint8_t* bout = to_device ? biomes : S.d_bout;
         if (bout) k_gather_biomes<<<grid_of(static_cast<size_t>(n) * kQPer, 256), 256, 0, st>>>(n, S.d_nb, S.d_bb, bout);
         hip_check(hipGetLastError(), "surface kernels");
@@ -801,8 +867,15 @@ struct Generator {
         done_ms.assign(static_cast<size_t>((count + max_batch - 1) / max_batch), 0.0);
         hip_check(hipEventRecord(t0, slots[0]->stream), "event");
         int batch = 0;
-        for (int off = 0; off < count; off += max_batch, ++batch) {
-            const int n = std::min(max_batch, count - off);
+        for (int off = 0; off < count; ++batch) {
+            int n = std::min(max_batch, count - off);
+            {   // at most seed_cap distinct seeds per batch
+                std::unordered_map<int64_t, int> seen;
+                for (int i = 0; i < n; ++i) {
+                    if (seen.emplace(seeds[off + i], 1).second && static_cast<int>(seen.size()) > seed_cap) { n = i; break; }
+                }
+            }
+            if (batch >= static_cast<int>(done_ms.size())) done_ms.push_back(0.0);
             Slot& S = *slots[next_slot];
             next_slot = (next_slot + 1) % slots.size();
             collect(S, tm);
@@ -810,6 +883,7 @@ struct Generator {
             run_batch(S, seeds + off, xz + 2 * static_cast<size_t>(off), n, terrain + static_cast<size_t>(off) * kPer,
                       biomes ? biomes + static_cast<size_t>(off) * kQPer : nullptr, aquifer ? aquifer + static_cast<size_t>(off) * kPer : nullptr,
                       to_device);
+            off += n;
         }
         collect_all(tm);
         if (tm) tm->wall_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - w0).count();
@@ -966,6 +1040,32 @@ void mct_free_device(void* p) { (void)hipFree(p); }
 int mct_device_to_host(void* dst, const void* src, size_t bytes) {
     return guard([&] { hip_check(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost), "device to host"); });
 }
+
+int mct_profile(mct_generator* g, int enable) {
+    return guard([&] {
+        if (!g) throw Error(MCT_EINVAL, "null generator");
+        g->g->collect_all(nullptr);
+        g->g->profile = enable != 0;
+        for (int k = 0; k < kKCount; ++k) {
+            g->g->kernel_ms[k] = 0;
+            g->g->kernel_launches[k] = 0;
+        }
+    });
+}
+
+int mct_kernel_times(mct_generator* g, double* ms, int64_t* launches, int cap, int* n) {
+    return guard([&] {
+        if (!g || !n) throw Error(MCT_EINVAL, "bad arguments");
+        g->g->collect_all(nullptr);
+        *n = kKCount;
+        for (int k = 0; k < cap && k < kKCount; ++k) {
+            if (ms) ms[k] = g->g->kernel_ms[k];
+            if (launches) launches[k] = g->g->kernel_launches[k];
+        }
+    });
+}
+
+const char* mct_kernel_name(int i) { return (i >= 0 && i < kKCount) ? kKernelNames[i] : ""; }
 
 int mct_info(int32_t out[6]) {
     if (!out) return fail(MCT_EINVAL, "null out");

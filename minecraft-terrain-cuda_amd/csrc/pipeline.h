@@ -80,17 +80,24 @@ struct ModelDev {
 };
 
 // ---- keys for the position dedupe -------------------------------------------------------
-// seed slot (12 bits) | flag (1) | qx + 2^23 (24) | qz + 2^23 (24): sorted order groups seeds.
-constexpr int kKeyBits = 61;
-MCT_HD MCT_FI uint64_t make_key(int32_t seed, int flag, int32_t qx, int32_t qz) {
-    return (static_cast<uint64_t>(seed) << 49) | (static_cast<uint64_t>(flag) << 48) | (static_cast<uint64_t>(static_cast<uint32_t>(qx + (1 << 23))) << 24) |
-           static_cast<uint64_t>(static_cast<uint32_t>(qz + (1 << 23)));
+// A key packs (seed slot, flag, quart x, quart z) relative to the batch's minima, with each
+// field as wide as the batch's range needs, so the radix sort runs as few passes as possible:
+// key = (((slot - s0) * 2 + flag) << (bx + bz)) | ((qx - x0) << bz) | (qz - z0).
+struct KeyCodec {
+    int32_t x0, z0, s0;
+    int32_t bx, bz;
+    int32_t bits;     // total key bits
+};
+MCT_HD MCT_FI uint64_t make_key(const KeyCodec& k, int32_t slot, int flag, int32_t qx, int32_t qz) {
+    const uint64_t hi = static_cast<uint64_t>(slot - k.s0) * 2u + static_cast<uint64_t>(flag);
+    return (hi << (k.bx + k.bz)) | (static_cast<uint64_t>(static_cast<uint32_t>(qx - k.x0)) << k.bz) | static_cast<uint64_t>(static_cast<uint32_t>(qz - k.z0));
 }
-MCT_HD MCT_FI void split_key(uint64_t k, int32_t& seed, int& flag, int32_t& qx, int32_t& qz) {
-    seed = static_cast<int32_t>(k >> 49);
-    flag = static_cast<int>((k >> 48) & 1);
-    qx = static_cast<int32_t>((k >> 24) & 0xFFFFFF) - (1 << 23);
-    qz = static_cast<int32_t>(k & 0xFFFFFF) - (1 << 23);
+MCT_HD MCT_FI void split_key(const KeyCodec& k, uint64_t key, int32_t& slot, int& flag, int32_t& qx, int32_t& qz) {
+    qz = static_cast<int32_t>(key & ((1ull << k.bz) - 1)) + k.z0;
+    qx = static_cast<int32_t>((key >> k.bz) & ((1ull << k.bx) - 1)) + k.x0;
+    const uint64_t hi = key >> (k.bx + k.bz);
+    flag = static_cast<int>(hi & 1u);
+    slot = static_cast<int32_t>(hi >> 1) + k.s0;
 }
 
 // ---- per-seed setup ----------------------------------------------------------------------
@@ -285,12 +292,15 @@ MCT_HD MCT_FI int chain_quart(int p) {
     return ((sy * 4 + y) * 4 + z) * 4 + x;
 }
 
-// tie[p]: the previous chain position's L* is also at minimal distance at p.
+// tie[p]: the previous chain position's L* differs from L* at p and is also at minimal
+// distance at p. While the chain is in sync (previous result == previous L*), the result at p
+// differs from L* exactly where tie[p] is set.
 MCT_HD MCT_FI void biome_tie(const ModelDev& M, const int64_t* targets, const int32_t* lstar, const int64_t* mind, uint8_t* tie, int idx) {
     const int b = idx / kQPer, p = idx % kQPer;
     if (p == 0) { tie[idx] = 0; return; }
     const size_t i = static_cast<size_t>(b) * kQPer + chain_quart(p), ip = static_cast<size_t>(b) * kQPer + chain_quart(p - 1);
-    tie[idx] = box_distance(M.tree[M.leaf_node[lstar[ip]]], targets + i * 7) == mind[i] ? 1 : 0;
+    const int prev = lstar[ip];
+    tie[idx] = (prev != lstar[i] && box_distance(M.tree[M.leaf_node[prev]], targets + i * 7) == mind[i]) ? 1 : 0;
 }
 
 // The last-result rule along the chain of one biome chunk (sequential form): the result is
@@ -308,24 +318,24 @@ MCT_HD inline void biome_chain_seq(const ModelDev& M, const int64_t* targets, co
 }
 
 // ---- density ---------------------------------------------------------------------------
-MCT_HD MCT_FI void column_key(const ChunkRef* ch, uint64_t* keys, int idx) {
+MCT_HD MCT_FI void column_key(const KeyCodec& K, const ChunkRef* ch, uint64_t* keys, int idx) {
     const int c = idx / kCols, k = idx % kCols;
-    keys[idx] = make_key(ch[c].seed, 0, ch[c].x * 4 + k % 5, ch[c].z * 4 + k / 5);
+    keys[idx] = make_key(K, ch[c].seed, 0, ch[c].x * 4 + k % 5, ch[c].z * 4 + k / 5);
 }
 
-MCT_HD MCT_FI void column_values(const SeedDev* seeds, const uint64_t* ucols, float* colv, int u) {
+MCT_HD MCT_FI void column_values(const KeyCodec& K, const SeedDev* seeds, const uint64_t* ucols, float* colv, int u) {
     int32_t s, qx, qz;
     int f;
-    split_key(ucols[u], s, f, qx, qz);
+    split_key(K, ucols[u], s, f, qx, qz);
     mctgen::eval_columns(seeds[s].oct, qx * 4, qz * 4, colv + static_cast<size_t>(u) * (mctgen::kNumColumns > 0 ? mctgen::kNumColumns : 1));
 }
 
 // Channel inputs at one cell corner: corners[(u * kCh + ch) * kRows + row].
-MCT_HD MCT_FI void corner_values(const SeedDev* seeds, const uint64_t* ucols, const float* colv, float* corners, int idx) {
+MCT_HD MCT_FI void corner_values(const KeyCodec& K, const SeedDev* seeds, const uint64_t* ucols, const float* colv, float* corners, int idx) {
     const int u = idx / kRows, row = idx % kRows;
     int32_t s, qx, qz;
     int f;
-    split_key(ucols[u], s, f, qx, qz);
+    split_key(K, ucols[u], s, f, qx, qz);
     float out[kCh];
     mctgen::eval_corner(seeds[s].oct, qx * 4, kMinY + row * 8, qz * 4, colv + static_cast<size_t>(u) * (mctgen::kNumColumns > 0 ? mctgen::kNumColumns : 1), out);
     for (int ch = 0; ch < kCh; ++ch) corners[(static_cast<size_t>(u) * kCh + ch) * kRows + row] = out[ch];
@@ -349,23 +359,23 @@ MCT_HD MCT_FI void interp_channels(const float* corners, const int* cmap, int x,
 // Per chunk: 27 x 19 surface-level quarts around it; inside NoiseBasedAquifer's constructor
 // prefill window (quarts 4c-4 .. 4c+6) the volume path is used, elsewhere the point path;
 // plus the 4 chunk corners of chunk_surface_level (volume path).
-MCT_HD MCT_FI void grid_key(const ChunkRef* ch, uint64_t* keys, int idx) {
+MCT_HD MCT_FI void grid_key(const KeyCodec& K, const ChunkRef* ch, uint64_t* keys, int idx) {
     const int c = idx / kGridKeys, e = idx % kGridKeys;
     const int cx = ch[c].x, cz = ch[c].z;
     if (e < kGridCells) {
         const int qx = 4 * cx - 16 + e % kGridX, qz = 4 * cz - 8 + e / kGridX;
         const bool vol = qx >= 4 * cx - 4 && qx <= 4 * cx + 6 && qz >= 4 * cz - 4 && qz <= 4 * cz + 6;
-        keys[idx] = make_key(ch[c].seed, vol ? 1 : 0, qx, qz);
+        keys[idx] = make_key(K, ch[c].seed, vol ? 1 : 0, qx, qz);
     } else {
         const int k = e - kGridCells;
-        keys[idx] = make_key(ch[c].seed, 1, 4 * cx + 4 * (k & 1), 4 * cz + 4 * (k >> 1));
+        keys[idx] = make_key(K, ch[c].seed, 1, 4 * cx + 4 * (k & 1), 4 * cz + 4 * (k >> 1));
     }
 }
 
-MCT_HD MCT_FI void fts_value(const SeedDev* seeds, const uint64_t* ukeys, float* vals, int u) {
+MCT_HD MCT_FI void fts_value(const KeyCodec& K, const SeedDev* seeds, const uint64_t* ukeys, float* vals, int u) {
     int32_t s, qx, qz;
     int vol;
-    split_key(ukeys[u], s, vol, qx, qz);
+    split_key(K, ukeys[u], s, vol, qx, qz);
     const Octave* O = seeds[s].oct;
     vals[u] = vol ? mctgen::fts_volume(O, qx * 4, qz * 4) : mctgen::fts_point(O, qx * 4, qz * 4);
 }

@@ -58,6 +58,10 @@ def lib():
         L.mct_free_device.argtypes = [P]
         L.mct_device_to_host.argtypes = [P, P, SZ]
         L.mct_info.argtypes = [P]
+        L.mct_profile.argtypes = [P, I]
+        L.mct_kernel_times.argtypes = [P, P, P, I, ctypes.POINTER(I)]
+        L.mct_kernel_name.argtypes = [I]
+        L.mct_kernel_name.restype = ctypes.c_char_p
         _lib = L
     return _lib
 
@@ -125,6 +129,16 @@ class FullTerrainGenerator:
         xz = _xz(chunks)
         s = np.ascontiguousarray(np.asarray(seeds, dtype=np.int64))
         _check(lib().mct_generate_device(self.h, _ptr(s), _ptr(xz), len(xz), d_terrain, d_biomes, ctypes.byref(timing) if timing is not None else None))
+
+    def profile(self, enable=True):
+        _check(lib().mct_profile(self.h, 1 if enable else 0))
+
+    def kernel_times(self):
+        ms = np.zeros(64, np.float64)
+        cnt = np.zeros(64, np.int64)
+        n = ctypes.c_int()
+        _check(lib().mct_kernel_times(self.h, _ptr(ms), _ptr(cnt), 64, ctypes.byref(n)))
+        return {lib().mct_kernel_name(i).decode(): (float(ms[i]), int(cnt[i])) for i in range(n.value)}
 
     def biome_tie_stats(self):
         out = np.zeros(4, np.int64)
