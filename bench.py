@@ -8,7 +8,7 @@ is chunks/s of that step on one GPU. Alongside it the line carries:
 
   e2e_host_chunks_per_s   the same chunks with results copied to pinned host memory
   batched                 mixed-seed batch (256 uniform seeds, uneven chunk lists), chunks/s
-  roofline                the dominant kernel against the MI355X VALU peak
+  roofline                the dominant kernel's VALU issue rate against the MI355X VALU peak
   cpu_baseline            the reference cpu/ port (built from its sources) on the host cores
   stages_ms / kernels_ms  per-stage and per-kernel times of one step (HIP events)
 
@@ -26,34 +26,24 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, 'minecraft-terrain-cuda_amd'))
 
-SEED = 123456789            # the reference bench's seed (bench/bench_main.cpp:50)
-BORDER_CHUNK = 1874999 >> 4  # chunk of block 1,874,999
-WRAP_CHUNKS = [(6128, 0), (-6129, 5), (11, 6128), (-3, -6129)]
-REF_BORDER_CHUNKS = [(1874999, 1874999), (-1874999, 1874998), (1874999, -1874999), (-1874999, -1874999)]
+from workloads import SEED, BORDER_CHUNK, region, plain_region_set  # noqa: E402
 
 # MI355X VALU issue rate: 256 CUs x 64 lanes x 2.4 GHz (non-packed, one op per lane per
 # cycle; FP32 and FP64 both full rate on CDNA4; no FMA is used, so one op = one flop).
 VALU_PEAK_TOPS = 256 * 64 * 2.4e9 / 1e12
 
-# VALU lane-operations per unit of work of each kernel group, from rocprofv3 --pmc
-# SQ_INSTS_VALU (x 64 lanes) on this workload (profiles/valu_counts.md); the unit is what
-# one thread of the kernel processes.
-VALU_PER_UNIT = {
-    'corners': None,
-    'fill': None,
-    'surface2': None,
-    'biome_quarts': None,
-    'fts': None,
+# Work per unit of the main kernels, measured with rocprofv3 on this workload (the summaries
+# are in profiles/valu_plain3x64.md and profiles/hbm_plain3x64.md; the unit is what one thread
+# of the kernel processes):
+#   VALU lane-ops   SQ_INSTS_VALU x 64 per thread (the instruction stream the kernel issues)
+#   HBM bytes       FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 per thread, the factors from a plain
+#                   1 GiB device-to-device copy measured in the same runs
+WORK_PER_UNIT = {
+    'surface2': {'unit': 'block column', 'valu': 259127.3, 'hbm': 933.14 + 1650.18},
+    'corners': {'unit': 'corner', 'valu': 45118.0, 'hbm': 359.0},   # per corner evaluated (the grid has idle tail threads)
+    'biome_quarts': {'unit': 'run of 8 quarts', 'valu': 132087.4, 'hbm': 5.12 + 550.38},
+    'fill': {'unit': 'block', 'valu': 510.5, 'hbm': 1.72 + 1.00},
 }
-
-
-def region(x0, z0, w):
-    return [(x, z) for z in range(z0, z0 + w) for x in range(x0, x0 + w)]
-
-
-def workload(side=64):
-    chunks = region(0, 0, side) + region(-2048, -2048, side) + region(BORDER_CHUNK - side + 1, -BORDER_CHUNK, side)
-    return chunks + WRAP_CHUNKS + REF_BORDER_CHUNKS
 
 
 def distinct_columns(chunks):
@@ -62,6 +52,17 @@ def distinct_columns(chunks):
         for k in range(25):
             cols.add((x * 4 + k % 5, z * 4 + k // 5))
     return len(cols)
+
+
+def units_per_step(chunks, batch):
+    """Threads (units) each main kernel runs over the whole workload, batch by batch (corner
+    columns and biome chunks are deduplicated within a batch)."""
+    u = {'surface2': 256 * len(chunks), 'fill': 98304 * len(chunks), 'corners': 0, 'biome_quarts': 0}
+    for k in range(0, len(chunks), batch):
+        part = chunks[k:k + batch]
+        u['corners'] += distinct_columns(part) * 49
+        u['biome_quarts'] += len({(x + dx, z + dz) for x, z in part for dz in (-1, 0, 1) for dx in (-1, 0, 1)}) * 16 * 12
+    return u
 
 
 def cpu_baseline(seconds_target=15.0):
@@ -95,6 +96,24 @@ def cpu_baseline(seconds_target=15.0):
             'sample': '%dx%d-chunk region at (0,0), seed %d, reference cpu/ port region mode, %.1f s' % (w2, w2, SEED, dt)}
 
 
+def copy_calibration(nbytes):
+    """One plain device-to-device copy of nbytes (HIP's blit kernel): under rocprofv3 --pmc its
+    FETCH_SIZE / WRITE_SIZE against nbytes give the scale of those counters."""
+    import mct
+    hip = ctypes.CDLL('libamdhip64.so')
+    hip.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]
+    hip.hipDeviceSynchronize.argtypes = []
+    src, dst = mct.alloc_device(nbytes), mct.alloc_device(nbytes)
+    try:
+        rc = hip.hipMemcpy(dst, src, nbytes, 3)   # hipMemcpyDeviceToDevice
+        rc = rc or hip.hipDeviceSynchronize()
+        if rc != 0:
+            raise RuntimeError('hipMemcpy device to device failed: %d' % rc)
+    finally:
+        mct.free_device(src)
+        mct.free_device(dst)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument('--gpus', type=int, default=1)
@@ -104,13 +123,15 @@ def main():
     ap.add_argument('--slots', type=int, default=2)
     ap.add_argument('--quick', action='store_true', help='one 32x32 region instead of the full workload')
     ap.add_argument('--no-cpu', action='store_true')
+    ap.add_argument('--copy-calibration', type=int, default=0, metavar='BYTES',
+                    help='also run one device-to-device copy of BYTES (the reference point for HBM counters)')
     a = ap.parse_args()
     if a.gpus != 1:
         print('bench.py measures one GPU: this workload shards by chunk tiles with no exchange step; '
               'runs on more than one GPU are not part of this build (--gpus %d refused)' % a.gpus, file=sys.stderr)
         sys.exit(2)
     import mct
-    chunks = region(0, 0, 32) if a.quick else workload()
+    chunks = region(0, 0, 32) if a.quick else plain_region_set()
     n = len(chunks)
     seeds = np.full(n, SEED, np.int64)
     g = mct.FullTerrainGenerator(max_batch_chunks=a.batch, pipeline_slots=a.slots, seed_capacity=256)
@@ -123,25 +144,32 @@ def main():
         g.generate_device(seeds, chunks, d_t, d_b)
     dt = (time.perf_counter() - t0) / a.steps
     value = n / dt
-    # one profiled step: stage and kernel times
+    # one profiled step on a single-slot generator (no overlap between batches, so each
+    # kernel's HIP-event time is its own): stage and kernel times
+    gp = mct.FullTerrainGenerator(max_batch_chunks=a.batch, pipeline_slots=1, seed_capacity=256)
+    gp.generate_device(seeds, chunks, d_t, d_b)
     tm = mct.Timing()
-    g.profile(True)
-    g.generate_device(seeds, chunks, d_t, d_b, timing=tm)
-    kt = g.kernel_times()
-    g.profile(False)
+    gp.profile(True)
+    gp.generate_device(seeds, chunks, d_t, d_b, timing=tm)
+    kt = gp.kernel_times()
+    gp.profile(False)
+    gp.close()
     stages = {k: round(v, 3) for k, v in tm.as_dict().items()}
     kernels = {k: round(v[0], 3) for k, v in kt.items() if v[1] > 0}
     dom = max(kernels, key=kernels.get)
-    units = {'corners': distinct_columns(chunks) * 49, 'fill': n * 98304, 'surface2': n * 256}
-    roof = None
-    per_unit = VALU_PER_UNIT.get(dom)
-    if per_unit and dom in units:
-        ops = per_unit * units[dom]
-        achieved = ops / (kernels[dom] * 1e-3) / 1e12
-        roof = {'bound': 'valu', 'kernel': dom, 'achieved': round(achieved, 3), 'peak': VALU_PEAK_TOPS, 'unit': 'TOP/s',
-                'frac': round(achieved / VALU_PEAK_TOPS, 4), 'traffic': None}
-    else:
-        roof = {'bound': 'valu', 'kernel': dom, 'achieved': None, 'peak': VALU_PEAK_TOPS, 'unit': 'TOP/s', 'frac': None, 'traffic': None}
+    # roofline of the dominant kernel: its work per launch over its average launch time (HIP
+    # events on the stream it runs on, single-slot pass above)
+    launches = {k: v[1] for k, v in kt.items()}
+    units = units_per_step(chunks, a.batch)
+    w = WORK_PER_UNIT.get(dom)
+    roof = {'bound': 'valu', 'kernel': dom, 'achieved': None, 'peak': VALU_PEAK_TOPS, 'unit': 'TOP/s', 'frac': None, 'traffic': None}
+    if w and dom in units:
+        per_launch = units[dom] / launches[dom]
+        avg_s = kernels[dom] * 1e-3 / launches[dom]
+        achieved = w['valu'] * per_launch / avg_s / 1e12
+        roof.update({'achieved': round(achieved, 3), 'frac': round(achieved / VALU_PEAK_TOPS, 4), 'traffic': round(w['hbm'] * per_launch),
+                     'avg_launch_ms': round(avg_s * 1e3, 4), 'units_per_launch': round(per_launch, 1), 'work_unit': w['unit'],
+                     'valu_per_unit': w['valu'], 'hbm_gbs': round(w['hbm'] * per_launch / avg_s / 1e9, 2)})
     # end to end with results copied to pinned host memory
     pin = mct.lib().mct_alloc_pinned(n * mct.BLOCKS_PER_CHUNK)
     buf = np.ctypeslib.as_array(ctypes.cast(pin, ctypes.POINTER(ctypes.c_uint8)), shape=(n, 384, 16, 16))
@@ -154,6 +182,8 @@ def main():
     mct.free_device(d_t)
     mct.free_device(d_b)
     g.close()
+    if a.copy_calibration:
+        copy_calibration(a.copy_calibration)
     cpu = None if a.no_cpu else cpu_baseline()
     line = {'metric': 'chunks/s full TERRAIN, batched/served vs plain, 1 MI355X; p50/p99 latency', 'value': round(value, 1), 'unit': 'chunks/s',
             'n_gpus': 1, 'steps': a.steps, 'warmup': a.warmup, 'ms_per_step': round(dt * 1e3, 3), 'higher_is_better': True, 'scaling': 'weak',

@@ -1,0 +1,135 @@
+"""The chunk lists of the measured and tested workloads, generated deterministically from
+SplitMix64 (constants below), so the bench, the tests and the fixture writer see the same
+inputs everywhere.
+
+  plain_region_set   one seed, three 64 x 64-chunk regions (origin, (-2048, -2048), and one
+                     ending at chunk 117,187 = the chunk of block 1,874,999) + the noise-wrap
+                     chunks + the reference's world-border chunks (chunk +-1,874,999)
+  c1_region          8 x 8 chunks at (0, 0) under one seed drawn from SplitMix64
+  batched            256 uniform 64-bit seeds, each with a log-uniform 1..1024 chunk list,
+                     half contiguous near-square rectangles, half scattered in +-117,187
+  requests           served stream: Poisson arrivals, seed from a 64-seed pool, W x H region
+                     (W, H uniform 1..32) in spiral order around a random player chunk
+  stress             1,024 seeds, 2^20 chunks uniform over +-117,187 (wrap chunks included)
+"""
+import math
+
+MASK = (1 << 64) - 1
+SEED = 123456789              # the reference bench's seed (bench/bench_main.cpp:50)
+BORDER_CHUNK = 1874999 >> 4   # 117,187: chunk of block 1,874,999
+CHUNK_RANGE = BORDER_CHUNK
+# chunks straddling the noise lattice's 32-bit wrap (GradientNoise.wrap folds), and the
+# reference's own world-border chunks (chunk coordinate +-1,874,999)
+WRAP_CHUNKS = [(6128, 0), (-6129, 5), (11, 6128), (-3, -6129)]
+REF_BORDER_CHUNKS = [(1874999, 1874999), (-1874999, 1874998), (1874999, -1874999), (-1874999, -1874999)]
+EDGE_CASES = [(123456789, [(0, 0)]), (0, [(-2048, -2048)]), (1, [(6128, 0), (-6129, 5)]),
+              (-1, [(1874999, -1874999), (11, 6128)]), (-4172144997902289642, [(-3, -6129), (7, -9)])]
+
+C1_STREAM = 0x43315F5345454453     # SplitMix64 start for the C1 seed
+BATCH_STREAM = 0x4241544348454421  # 256-seed batch
+SERVE_STREAM = 0x5345525645212121  # request stream
+STRESS_STREAM = 0x5354524553532121
+
+
+class SplitMix64:
+    def __init__(self, state):
+        self.s = state & MASK
+
+    def next(self):
+        self.s = (self.s + 0x9E3779B97F4A7C15) & MASK
+        z = self.s
+        z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & MASK
+        z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & MASK
+        return z ^ (z >> 31)
+
+    def signed(self):
+        v = self.next()
+        return v - (1 << 64) if v >> 63 else v
+
+    def uniform(self):            # [0, 1)
+        return (self.next() >> 11) * (1.0 / (1 << 53))
+
+    def below(self, n):           # [0, n)
+        return (self.next() >> 11) % n
+
+    def between(self, lo, hi):    # [lo, hi]
+        return lo + self.below(hi - lo + 1)
+
+
+def region(x0, z0, w, d=None):
+    d = w if d is None else d
+    return [(x, z) for z in range(z0, z0 + d) for x in range(x0, x0 + w)]
+
+
+def plain_region_set(side=64):
+    chunks = region(0, 0, side) + region(-2048, -2048, side) + region(BORDER_CHUNK - side + 1, -BORDER_CHUNK, side)
+    return chunks + WRAP_CHUNKS + REF_BORDER_CHUNKS
+
+
+def c1_seed():
+    return SplitMix64(C1_STREAM).signed()
+
+
+def c1_region():
+    return c1_seed(), region(0, 0, 8)
+
+
+def batched(n_seeds=256, max_chunks=1024):
+    """[(seed, [(cx, cz), ...]), ...]: even entries are near-square rectangles (the first n
+    chunks of a ceil(sqrt n)-wide rectangle), odd entries n scattered chunks."""
+    r = SplitMix64(BATCH_STREAM)
+    out = []
+    for i in range(n_seeds):
+        seed = r.signed()
+        n = int(round(math.exp(r.uniform() * math.log(max_chunks))))
+        n = max(1, min(max_chunks, n))
+        if i % 2 == 0:
+            w = int(math.ceil(math.sqrt(n)))
+            h = (n + w - 1) // w
+            x0 = r.between(-CHUNK_RANGE, CHUNK_RANGE - w + 1)
+            z0 = r.between(-CHUNK_RANGE, CHUNK_RANGE - h + 1)
+            chunks = region(x0, z0, w, h)[:n]
+        else:
+            chunks = [(r.between(-CHUNK_RANGE, CHUNK_RANGE), r.between(-CHUNK_RANGE, CHUNK_RANGE)) for _ in range(n)]
+        out.append((seed, chunks))
+    return out
+
+
+def spiral(cx, cz, w, h):
+    """The W x H region around (cx, cz) (the player chunk at its centre), nearest rings first;
+    within a ring in the order of the angle, starting east."""
+    x0, z0 = cx - (w - 1) // 2, cz - (h - 1) // 2
+    pts = region(x0, z0, w, h)
+    return sorted(pts, key=lambda p: (max(abs(p[0] - cx), abs(p[1] - cz)), math.atan2(p[1] - cz, p[0] - cx) % (2 * math.pi), p))
+
+
+def seed_pool(n=64):
+    r = SplitMix64(SERVE_STREAM ^ 0x504F4F4C)
+    return [r.signed() for _ in range(n)]
+
+
+def requests(rate_per_s, duration_s, pool=None, max_side=32):
+    """Open-loop Poisson stream: [(arrival_s, seed, [(cx, cz), ...]), ...]."""
+    pool = seed_pool() if pool is None else pool
+    r = SplitMix64(SERVE_STREAM + int(rate_per_s * 1000))
+    t, out = 0.0, []
+    while True:
+        t += -math.log(1.0 - r.uniform()) / rate_per_s
+        if t >= duration_s:
+            return out
+        seed = pool[r.below(len(pool))]
+        w, h = r.between(1, max_side), r.between(1, max_side)
+        px, pz = r.between(-CHUNK_RANGE + max_side, CHUNK_RANGE - max_side), r.between(-CHUNK_RANGE + max_side, CHUNK_RANGE - max_side)
+        out.append((t, seed, spiral(px, pz, w, h)))
+
+
+def stress(n_seeds=1024, n_chunks=1 << 20):
+    """(seeds, chunk_seed_index, chunks): uniform chunks over +-117,187 with the wrap chunks
+    at the front, seed index uniform over the seed list."""
+    r = SplitMix64(STRESS_STREAM)
+    seeds = [r.signed() for _ in range(n_seeds)]
+    chunks = list(WRAP_CHUNKS)
+    while len(chunks) < n_chunks:
+        chunks.append((r.between(-CHUNK_RANGE, CHUNK_RANGE), r.between(-CHUNK_RANGE, CHUNK_RANGE)))
+    which = [r.below(n_seeds) for _ in range(n_chunks)]
+    return seeds, which, chunks

@@ -181,6 +181,10 @@ void orc_lcg_ints(int64_t seed, int bound, int n, int32_t* out) {
     Lcg r(seed);
     for (int i = 0; i < n; ++i) out[i] = r.next_int(bound);
 }
+void orc_lcg_next(int64_t seed, int bits, int n, int32_t* out) {
+    Lcg r(seed);
+    for (int i = 0; i < n; ++i) out[i] = r.next(bits);
+}
 void orc_lcg_doubles(int64_t seed, int n, double* out) {
     Lcg r(seed);
     for (int i = 0; i < n; ++i) out[i] = r.next_double();
@@ -230,6 +234,24 @@ int orc_octaves(void* hp, int64_t seed, int k, double* offs, uint8_t* perm) {
     offs[3] = o.smear;
     std::memcpy(perm, o.perm, 256);
     return static_cast<int>(w.tables.oct.size());
+}
+
+// The layers of noise `key` under `seed`, in stack order: 6 doubles per layer {frequency,
+// amplitude, ox, oy, oz, smear} and 256 permutation bytes. Returns the layer count (-1: no key).
+int orc_noise_layers(void* hp, int64_t seed, const char* key, double* params, uint8_t* perms, int cap) {
+    Handle& h = *static_cast<Handle*>(hp);
+    const World& w = h.world(seed);
+    auto it = w.tables.noise.find(key);
+    if (it == w.tables.noise.end()) return -1;
+    const Stack& s = it->second;
+    for (int i = 0; i < s.count && i < cap; ++i) {
+        const Layer& L = w.tables.layers[static_cast<size_t>(s.first + i)];
+        const Octave& o = w.tables.oct[static_cast<size_t>(L.octave)];
+        double* q = params + 6 * i;
+        q[0] = L.freq; q[1] = static_cast<double>(L.amp); q[2] = o.xo; q[3] = o.yo; q[4] = o.zo; q[5] = o.smear;
+        std::memcpy(perms + 256 * i, o.perm, 256);
+    }
+    return s.count;
 }
 
 void orc_clay(void* hp, int64_t seed, int32_t* out) {

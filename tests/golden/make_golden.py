@@ -1,0 +1,176 @@
+"""Writes the golden fixtures under tests/golden/ from the reference cpu/ port, compiled from
+its own sources by `make -C oracle ref` (oracle/_ref/libmcref.so). Runs in the build
+container only (the reference sources are not on the GPU machines); the fixtures it writes are
+data: inputs and the reference's outputs, on the synthetic overworld IR and terrain model.
+
+  python tests/golden/make_golden.py            fixtures.npz (+ plain_digests.npz with --plain)
+
+fixtures.npz holds, per part of the path:
+  layers_<seed>_<key>   per-seed noise layers {frequency, amplitude, ox, oy, oz, smear} (f64),
+  perm_<seed>_<key>     and an 8-byte digest of each layer's permutation
+  misc_<seed>           zoom seed, the random / aquifer / ore positional factories, clay bands
+  tree_t, tree_c, tree_out         climate targets, candidate leaves, search results
+  zoom_seed, zoom_xyz, zoom_out    BiomeManager zoom picks
+  vol_<seed>_<root>_<k>, pts_xyz, pts_<seed>_<root>   volume-path and point-path samples
+  edge_*                the edge-case chunks: per-chunk digests of terrain / biomes / aquifer /
+                        density, full terrain + biomes of the first two chunks
+  c1_*                  the 8 x 8 region of the CPU-runnable configuration (digests)
+plain_digests.npz: 8-byte digests of terrain (u8) and biomes of every chunk of the plain
+three-region workload (12,296 chunks), region mode, for the full-size GPU parity check.
+"""
+import hashlib
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, os.path.join(ROOT, 'oracle'))
+sys.path.insert(0, os.path.join(ROOT, 'minecraft-terrain-cuda_amd'))
+import oracle  # noqa: E402
+import workloads  # noqa: E402
+
+TABLE_SEEDS = [123456789, -1]
+DENSITY_SEEDS = [123456789, -4172144997902289642]
+
+
+def digest(a, size=8):
+    return np.frombuffer(hashlib.blake2b(np.ascontiguousarray(a).tobytes(), digest_size=size).digest(), np.uint8)
+
+
+def chunk_digests(arr, size=8):
+    return np.stack([digest(arr[i], size) for i in range(len(arr))])
+
+
+def splitmix_ints(state, n, lo, hi):
+    r = workloads.SplitMix64(state)
+    return np.array([r.between(lo, hi) for _ in range(n)], dtype=np.int64)
+
+
+def tree_targets(params):
+    """Random climate points, every box's corners and centre (exact ties against neighbours)."""
+    r = workloads.SplitMix64(0x5452454554524545)
+    t = []
+    for _ in range(1500):
+        t.append([r.between(-12000, 12000) for _ in range(6)] + [0])
+    for k in range(0, len(params), 23):
+        p = params[k]['p']
+        lo = [p[2 * d] for d in range(6)]
+        hi = [p[2 * d + 1] for d in range(6)]
+        t.append(lo + [0])
+        t.append(hi + [0])
+        t.append([(a + b) // 2 for a, b in zip(lo, hi)] + [0])
+    return np.array(t, dtype=np.int64)
+
+
+def main():
+    R = oracle.Reference()
+    out = {}
+    ir = json.load(open(oracle.IR_PATH))
+    tm = json.load(open(oracle.TERRAIN_PATH))
+    t0 = time.time()
+
+    # per-seed tables
+    for seed in TABLE_SEEDS:
+        for key in sorted(ir['noises']):
+            params, perms = R.noise_layers(seed, key)
+            out['layers_%d_%s' % (seed, key)] = params
+            out['perm_%d_%s' % (seed, key)] = np.stack([digest(p) for p in perms]) if len(perms) else np.zeros((0, 8), np.uint8)
+        zs, pos, clay = R.seed_misc(seed)
+        out['misc_%d' % seed] = np.concatenate([np.array([zs], np.int64).view(np.uint64), pos, clay.astype(np.int64).view(np.uint64)])
+
+    # climate R-tree: a chain of searches, each with the previous result as candidate
+    tt = tree_targets(tm['biome_parameters'])
+    cand = np.full(len(tt), -1, np.int32)
+    res = np.zeros(len(tt), np.int32)
+    last = -1
+    for i in range(len(tt)):   # the chain, as the biome fill does it
+        cand[i] = last
+        res[i] = R.tree_search(tt[i:i + 1], cand[i:i + 1])[0]
+        last = res[i]
+    out['tree_t'], out['tree_c'], out['tree_out'] = tt, cand, res
+
+    # zoom
+    xyz = np.stack([splitmix_ints(0x5A4F4F4D + k, 2000, lo, hi) for k, (lo, hi) in enumerate([(-29999984, 29999984), (-64, 319), (-29999984, 29999984)])], 1)
+    xyz[:16] = [[x, y, z] for x in (-29999984, 29999983) for y in (-64, 319) for z in (-29999984, 29999983)] + [[0, 0, 0]] * 8
+    zseeds = np.array([R.seed_misc(s)[0] for s in TABLE_SEEDS], np.int64)
+    out['zoom_seed'], out['zoom_xyz'] = zseeds, xyz.astype(np.int32)
+    out['zoom_out'] = np.stack([R.zoom(z, xyz) for z in zseeds])
+
+    # density roots: volume path over three volumes, point path at 40 points
+    vols = np.array([[4, 12, 4, 6128 * 16, -64, 0, 4, 8, 4],            # corner lattice at a wrap chunk
+                     [5, 1, 5, -2048 * 16, 0, -2048 * 16, 4, 1, 4],      # surface quart grid
+                     [3, 6, 2, 1874999 * 16 + 13, 40, -7, 1, 1, 1]], np.int32)   # blocks at the border
+    pts = np.stack([splitmix_ints(0x504F494E + k, 40, lo, hi) for k, (lo, hi) in enumerate([(-29999984, 29999984), (-64, 319), (-29999984, 29999984)])], 1).astype(np.int32)
+    pts[:4] = [[98048, 0, 5], [-98064, 100, 80], [176, -64, 98048], [-48, 319, -98064]]
+    out['vols'], out['pts_xyz'] = vols, pts
+    roots = sorted(ir['roots'])
+    out['roots'] = np.array(roots)
+    for seed in DENSITY_SEEDS:
+        for name in roots:
+            node = ir['roots'][name]
+            for k, v in enumerate(vols):
+                out['vol_%d_%s_%d' % (seed, name, k)] = R.sample(seed, node, v)
+            out['pts_%d_%s' % (seed, name)] = R.point(seed, node, pts)
+    print('tables, tree, zoom, density: %.1fs' % (time.time() - t0), flush=True)
+
+    # edge-case chunks
+    seeds, chunks = [], []
+    for s, ch in workloads.EDGE_CASES:
+        for c in ch:
+            seeds.append(s)
+            chunks.append(c)
+    dig = {k: [] for k in ('terrain', 'biomes', 'aquifer', 'density')}
+    full_t, full_b = [], []
+    for s, c in zip(seeds, chunks):
+        t, b, a, d = R.generate(s, [c], density=True)
+        dig['terrain'].append(digest(t.astype(np.uint8)))
+        dig['biomes'].append(digest(b))
+        dig['aquifer'].append(digest(a.astype(np.uint8)))
+        dig['density'].append(digest(d))
+        if len(full_t) < 2:
+            full_t.append(t[0].astype(np.uint8))
+            full_b.append(b[0])
+    out['edge_seed'], out['edge_xz'] = np.array(seeds, np.int64), np.array(chunks, np.int32)
+    for k, v in dig.items():
+        out['edge_' + k] = np.stack(v)
+    out['edge_full_terrain'], out['edge_full_biomes'] = np.stack(full_t), np.stack(full_b)
+
+    # the CPU-runnable configuration: 8 x 8 region at (0, 0)
+    c1_seed, c1_chunks = workloads.c1_region()
+    t, b = R.region(c1_seed, 0, 0, 8, 8, threads=8)
+    out['c1_seed'], out['c1_xz'] = np.array([c1_seed], np.int64), np.array(c1_chunks, np.int32)
+    out['c1_terrain'], out['c1_biomes'] = chunk_digests(t.astype(np.uint8)), chunk_digests(b)
+    print('chunks: %.1fs' % (time.time() - t0), flush=True)
+    np.savez_compressed(os.path.join(HERE, 'fixtures.npz'), **out)
+
+    if '--plain' in sys.argv:
+        plain(R)
+
+
+def plain(R):
+    """Digests of every chunk of the plain three-region workload (region mode per region)."""
+    t0 = time.time()
+    chunks = workloads.plain_region_set()
+    side = 64
+    dt, db = [], []
+    for k in range(3):
+        x0, z0 = chunks[k * side * side]
+        t, b = R.region(workloads.SEED, x0, z0, side, side, threads=8)
+        dt.append(chunk_digests(t.astype(np.uint8)))
+        db.append(chunk_digests(b))
+        print('region %d at (%d, %d): %.1fs' % (k, x0, z0, time.time() - t0), flush=True)
+    rest = chunks[3 * side * side:]
+    t, b, _ = R.generate(workloads.SEED, rest)
+    dt.append(chunk_digests(t.astype(np.uint8)))
+    db.append(chunk_digests(b))
+    np.savez_compressed(os.path.join(HERE, 'plain_digests.npz'), xz=np.array(chunks, np.int32), seed=np.array([workloads.SEED], np.int64),
+                        terrain=np.concatenate(dt), biomes=np.concatenate(db))
+    print('plain digests: %.1fs' % (time.time() - t0), flush=True)
+
+
+if __name__ == '__main__':
+    main()

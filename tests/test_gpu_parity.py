@@ -1,0 +1,165 @@
+"""Bit-exact parity of the HIP library (through the C ABI) with the oracle and with the
+reference cpu/ port's own outputs (golden digests), on the edge cases the reference tests
+(negative coordinates, the world border, the noise wrap), the CPU-runnable 8 x 8 region, the
+full plain three-region workload, mixed-seed batches and the API's edge inputs."""
+import hashlib
+import os
+
+import numpy as np
+import pytest
+
+import workloads
+from conftest import GOLDEN
+
+pytestmark = pytest.mark.gpu
+
+
+def digest(a, size=8):
+    return np.frombuffer(hashlib.blake2b(np.ascontiguousarray(a).tobytes(), digest_size=size).digest(), np.uint8)
+
+
+def log(*a):
+    print(*a, flush=True)
+
+
+@pytest.fixture(scope='module')
+def mct():
+    import mct as m
+    m.lib()
+    return m
+
+
+@pytest.fixture(scope='module')
+def gen(mct):
+    g = mct.FullTerrainGenerator(max_batch_chunks=256, pipeline_slots=2)
+    yield g
+    g.close()
+
+
+def test_edge_cases_match_oracle(gen, orc, golden):
+    k = 0
+    for seed, chunks in workloads.EDGE_CASES:
+        log('edge seed', seed, chunks)
+        gen.set_seed(seed)
+        t, b, a = gen.generate(chunks, biomes=True, aquifer=True)
+        ot, ob, oa = orc.generate(seed, chunks)
+        assert np.array_equal(t, ot) and np.array_equal(b, ob) and np.array_equal(a, oa), (seed, chunks)
+        for i in range(len(chunks)):
+            assert (digest(t[i]) == golden['edge_terrain'][k]).all() and (digest(b[i]) == golden['edge_biomes'][k]).all()
+            assert (digest(a[i]) == golden['edge_aquifer'][k]).all()
+            k += 1
+
+
+def test_c1_region_matches_reference_digests(gen, golden):
+    seed, chunks = workloads.c1_region()
+    gen.set_seed(seed)
+    t, b = gen.generate(chunks)
+    for i in range(len(chunks)):
+        assert (digest(t[i]) == golden['c1_terrain'][i]).all(), chunks[i]
+        assert (digest(b[i]) == golden['c1_biomes'][i]).all(), chunks[i]
+
+
+def test_plain_three_regions_match_reference_digests(mct):
+    """Every chunk of the full plain workload (12,296 chunks) against the reference port's
+    per-chunk digests."""
+    with np.load(os.path.join(GOLDEN, 'plain_digests.npz'), allow_pickle=False) as z:
+        xz, seed, dt, db = z['xz'], int(z['seed'][0]), z['terrain'], z['biomes']
+    chunks = workloads.plain_region_set()
+    assert np.array_equal(np.array(chunks, np.int32), xz)
+    g = mct.FullTerrainGenerator(max_batch_chunks=4096, pipeline_slots=2)
+    g.set_seed(seed)
+    out = np.empty((len(chunks), 384, 16, 16), np.uint8)
+    t, b = g.generate(chunks, out=out)
+    g.close()
+    bad = [i for i in range(len(chunks)) if not ((digest(t[i]) == dt[i]).all() and (digest(b[i]) == db[i]).all())]
+    assert not bad, ('mismatching chunks', len(bad), [chunks[i] for i in bad[:8]])
+
+
+def test_batch_size_and_order_do_not_change_results(mct, orc):
+    seed = -987654321
+    chunks = workloads.region(-5, 3, 5, 4) + [(1874999, 1874999), (6128, 0), (-2, 4)]
+    g1 = mct.FullTerrainGenerator(max_batch_chunks=7, pipeline_slots=3)
+    g1.set_seed(seed)
+    t1, b1 = g1.generate(chunks)
+    rev = chunks[::-1]
+    t2, b2 = g1.generate(rev)
+    g1.close()
+    assert np.array_equal(t1, t2[::-1]) and np.array_equal(b1, b2[::-1])
+    # duplicated chunks in one call
+    dup = chunks[:3] + chunks[:3]
+    g2 = mct.FullTerrainGenerator(max_batch_chunks=64, pipeline_slots=1)
+    g2.set_seed(seed)
+    t3, b3 = g2.generate(dup)
+    g2.close()
+    assert np.array_equal(t3[:3], t3[3:]) and np.array_equal(t3[:3], t1[:3]) and np.array_equal(b3[:3], b1[:3])
+    ot, ob, _ = orc.generate(seed, chunks[:6])
+    assert np.array_equal(t1[:6], ot) and np.array_equal(b1[:6], ob)
+
+
+def test_mixed_seeds_equal_each_seed_alone(mct, orc):
+    """Batching: a mix of seeds with uneven chunk lists in one call equals each seed alone."""
+    work = workloads.batched(n_seeds=12, max_chunks=40)
+    seeds = [s for s, ch in work for _ in ch]
+    chunks = [c for _, ch in work for c in ch]
+    log('mix of', len(work), 'seeds,', len(chunks), 'chunks')
+    g = mct.FullTerrainGenerator(max_batch_chunks=128, pipeline_slots=2, seed_capacity=5)
+    tm, bm = g.generate_multi(seeds, chunks)
+    k = 0
+    for s, ch in work:
+        g.set_seed(s)
+        ta, ba = g.generate(ch)
+        assert np.array_equal(tm[k:k + len(ch)], ta) and np.array_equal(bm[k:k + len(ch)], ba), s
+        k += len(ch)
+    g.close()
+    # a sample against the oracle
+    rng = np.random.default_rng(1)
+    for i in rng.choice(len(chunks), 6, replace=False):
+        ot, ob, _ = orc.generate(seeds[i], [chunks[i]])
+        assert np.array_equal(tm[i], ot[0]) and np.array_equal(bm[i], ob[0]), (seeds[i], chunks[i])
+
+
+def test_device_outputs_equal_host_outputs(mct):
+    seed, chunks = 42, workloads.region(10, -10, 4)
+    g = mct.FullTerrainGenerator(max_batch_chunks=8, pipeline_slots=2)
+    g.set_seed(seed)
+    t, b = g.generate(chunks)
+    n = len(chunks)
+    dt, db = mct.alloc_device(n * mct.BLOCKS_PER_CHUNK), mct.alloc_device(n * mct.QUARTS_PER_CHUNK)
+    try:
+        g.generate_device([seed] * n, chunks, dt, db)
+        ht = mct.device_to_host(dt, n * mct.BLOCKS_PER_CHUNK).reshape(t.shape)
+        hb = mct.device_to_host(db, n * mct.QUARTS_PER_CHUNK).reshape(b.shape)
+    finally:
+        mct.free_device(dt)
+        mct.free_device(db)
+    g.close()
+    assert np.array_equal(ht, t) and np.array_equal(hb, b)
+
+
+def test_api_edge_inputs(mct):
+    g = mct.FullTerrainGenerator(max_batch_chunks=4, pipeline_slots=1)
+    t, b = g.generate([])
+    assert t.shape[0] == 0
+    with pytest.raises(mct.MctError) as e:
+        mct._check(mct.lib().mct_generate(g.h, None, -1, None, None, None, None))
+    assert e.value.code == 1
+    with pytest.raises(mct.MctError):
+        mct._check(mct.lib().mct_generate(g.h, np.zeros(2, np.int32).ctypes.data, 1, None, None, None, None))
+    g.close()
+    with pytest.raises(mct.MctError):
+        mct.FullTerrainGenerator(max_batch_chunks=0)
+    with pytest.raises(mct.MctError):
+        mct.FullTerrainGenerator(pipeline_slots=9)
+    with pytest.raises(mct.MctError) as e:
+        mct.FullTerrainGenerator(ir_path='/nonexistent.json')
+    assert e.value.code in (2, 5)
+
+
+def test_tie_statistics_are_counted(mct):
+    g = mct.FullTerrainGenerator(max_batch_chunks=64, pipeline_slots=1)
+    g.set_seed(123456789)
+    g.generate(workloads.region(0, 0, 8))
+    st = g.biome_tie_stats()
+    g.close()
+    assert st['quarts'] >= 100 * 1536          # (8 + 2)^2 biome chunks
+    assert 0 < st['overrides'] < st['quarts'] // 100

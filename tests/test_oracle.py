@@ -1,0 +1,283 @@
+"""Pins the oracle (oracle/liboracle.so) before it is trusted as the checker of the HIP library:
+known answers and closed forms for the arithmetic and the random sources, independent
+implementations (Python's hashlib) for MD5 / SHA-256, brute force for the climate R-tree,
+invariants of the gradient noise, and the reference cpu/ port's own outputs (golden fixtures
+written by tests/golden/make_golden.py) for everything whose only statement is reference code.
+"""
+import ctypes
+import hashlib
+import math
+import os
+
+import numpy as np
+import pytest
+
+import workloads
+from conftest import GOLDEN
+
+
+def L():
+    import oracle
+    return oracle.lib()
+
+
+def f32bits(x):
+    return int(np.array([x], np.float32).view(np.uint32)[0])
+
+
+def digest(a, size=8):
+    return np.frombuffer(hashlib.blake2b(np.ascontiguousarray(a).tobytes(), digest_size=size).digest(), np.uint8)
+
+
+# ---- Java arithmetic (JLS 5.1.3, Math.min/max, Mth.floor; reference tests/test_l0.cpp:36-58) ----
+def test_double_to_int_saturates_like_java():
+    f = L().orc_to_int
+    assert f(float('nan')) == 0
+    assert f(1e10) == 2**31 - 1 and f(-1e10) == -2**31
+    assert f(-0.5) == 0 and f(2147483647.5) == 2**31 - 1 and f(-2147483648.9) == -2**31
+    assert f(-7.9) == -7 and f(7.9) == 7
+
+
+def test_floor_saturates_and_rounds_down():
+    f = L().orc_floor
+    assert f(-0.5) == -1 and f(-1e-300) == -1 and f(3e9) == 2**31 - 1 and f(-3e9) == -2**31
+    assert f(-0.0) == 0 and f(5.0) == 5 and f(-5.0) == -5
+
+
+def test_java_min_max_signed_zero_and_nan():
+    mn, mx = L().orc_fmin, L().orc_fmax
+    assert f32bits(mn(0.0, -0.0)) == 0x80000000 and f32bits(mn(-0.0, 0.0)) == 0x80000000
+    assert f32bits(mx(-0.0, 0.0)) == 0 and f32bits(mx(0.0, -0.0)) == 0
+    assert math.isnan(mn(float('nan'), 1.0)) and math.isnan(mn(1.0, float('nan')))
+    assert math.isnan(mx(float('nan'), 1.0)) and math.isnan(mx(1.0, float('nan')))
+    assert mn(1.0, 2.0) == 1.0 and mx(1.0, 2.0) == 2.0
+
+
+def test_wrap_is_vanillas_fold_mod_2_pow_25():
+    # GradientNoise.wrap(x) = x - lfloor(x / 2^25 + 0.5) * 2^25 (exact in double for these x)
+    near = np.nextafter(2.0**24, 0)
+    xs = [0.0, 1.5, -1.5, near, -near, 2.0**24, -2.0**24, 2.0**24 + 1, 2.0**25, -2.0**25, 3 * 2.0**24 + 0.25, 1e12 + 0.5, -1e12 - 0.5,
+          98048.0 * 171.103, -98064.0 * 684.412]
+    rng = np.random.default_rng(7)
+    xs += list(rng.uniform(-1e9, 1e9, 200)) + list(near + rng.integers(-4, 4, 20))
+    for x in xs:
+        want = x - math.floor(x / 33554432.0 + 0.5) * 33554432.0
+        got = L().orc_wrap(float(x))
+        assert got == want, (x, got, want)
+        if abs(x) < near:
+            assert got == x
+
+
+def test_stream_sum_compensates():
+    s = L().orc_stream_sum
+    v = np.array([1.0] + [1e-16] * 10, np.float64)   # each 1e-16 is below half an ulp of 1.0
+    plain = 0.0
+    for x in v:
+        plain += x
+    assert plain == 1.0 and s(v.ctypes.data, len(v)) == math.fsum(v) > 1.0
+    v = np.random.default_rng(3).uniform(-1, 1, 1000)
+    assert abs(s(v.ctypes.data, len(v)) - math.fsum(v)) <= 1e-15 * np.abs(v).sum()
+    v = np.array([float('inf'), 1.0], np.float64)
+    assert s(v.ctypes.data, 2) == float('inf')
+
+
+# ---- digests (RFC 1321, FIPS 180-4, and Python's independent hashlib) ----
+@pytest.mark.parametrize('msg,hexd', [
+    ('', 'd41d8cd98f00b204e9800998ecf8427e'), ('a', '0cc175b9c0f1b6a831c399e269772661'), ('abc', '900150983cd24fb0d6963f7d28e17f72'),
+    ('message digest', 'f96b697d7cb7938d525a2f31aaf161d0'), ('abcdefghijklmnopqrstuvwxyz', 'c3fcd3d76192e4007dfb496cca67e13b'),
+    ('The quick brown fox jumps over the lazy dog', '9e107d9d372bb6826bd81d3542a419d6'), ('a' * 1000, 'cabe45dcc9ae5b66ba86600cca6b8ba8')])
+def test_md5_known_answers(msg, hexd):
+    out = (ctypes.c_uint8 * 16)()
+    L().orc_md5(msg.encode(), out)
+    assert bytes(out).hex() == hexd
+
+
+def test_md5_and_sha256_match_hashlib():
+    rng = np.random.default_rng(11)
+    for n in list(range(0, 130)) + [1000, 4097]:
+        s = ''.join(chr(c) for c in rng.integers(97, 123, n))
+        out = (ctypes.c_uint8 * 16)()
+        L().orc_md5(s.encode(), out)
+        assert bytes(out) == hashlib.md5(s.encode()).digest()
+        b = s.encode()
+        o2 = (ctypes.c_uint8 * 32)()
+        L().orc_sha256(b, len(b), o2)
+        assert bytes(o2) == hashlib.sha256(b).digest()
+    o2 = (ctypes.c_uint8 * 32)()
+    L().orc_sha256(b'abc', 3, o2)
+    assert bytes(o2).hex() == 'ba7816bf8f01cfea414140de5dae2223b00361a396177a9cb410ff61f20015ad'
+
+
+# ---- random sources ----
+def test_java_util_random_known_values():
+    out = np.zeros(2, np.int32)
+    L().orc_lcg_next(0, 32, 2, out.ctypes.data)
+    assert out[0] == -1155484576                       # new Random(0).nextInt()
+    L().orc_lcg_next(42, 32, 1, out.ctypes.data)
+    assert out[0] == -1170105035                       # new Random(42).nextInt()
+    d = np.zeros(1, np.float64)
+    L().orc_lcg_doubles(0, 1, d.ctypes.data)
+    assert d[0] == 0.730967787376657                   # new Random(0).nextDouble()
+
+
+def test_bounded_ints_stay_in_range():
+    for bound in (1, 2, 5, 10, 16, 192, 256, 1000, 2**30 + 1):
+        for f in (L().orc_xoro_ints, L().orc_lcg_ints):
+            out = np.zeros(500, np.int32)
+            f(12345, bound, 500, out.ctypes.data)
+            assert out.min() >= 0 and out.max() < bound
+    d = np.zeros(1000, np.float64)
+    L().orc_xoro_doubles(-3, 1000, d.ctypes.data)
+    assert d.min() >= 0.0 and d.max() < 1.0 and abs(d.mean() - 0.5) < 0.05
+
+
+def test_block_seed_is_mth_getseed():
+    # Mth.getSeed(x, y, z): l = (x * 3129871) ^ (z * 116129781L) ^ y; l = l*l*42317861 + l*11; l >> 16
+    def get_seed(x, y, z):
+        m = (1 << 64) - 1
+        l = ((x * 3129871) & 0xFFFFFFFF)
+        l = l - (1 << 32) if l >> 31 else l                # the int product
+        l = (l ^ (z * 116129781) ^ y) & m
+        l = (l * l * 42317861 + l * 11) & m
+        l = l - (1 << 64) if l >> 63 else l
+        return l >> 16
+    for x, y, z in [(0, 0, 0), (1, 2, 3), (-1, -64, -1), (29999984, 319, -29999984), (123456, -7, 98765)]:
+        assert L().orc_block_seed(x, y, z) == get_seed(x, y, z)
+
+
+# ---- gradient noise invariants ----
+def test_octave_is_zero_on_the_lattice_and_bounded(orc):
+    count, offs, perm = orc.octave(123456789, 0)
+    assert count > 100
+    p = perm.copy()
+    rng = np.random.default_rng(5)
+    for x, y, z in rng.integers(-300, 300, (200, 3)):
+        assert L().orc_octave(p.ctypes.data, 0.0, 0.0, 0.0, 0.0, float(x), float(y), float(z), 0) == 0.0
+    for x, y, z in rng.uniform(-1e4, 1e4, (500, 3)):
+        v = L().orc_octave(p.ctypes.data, 0.0, 0.0, 0.0, 0.0, x, y, z, 0)
+        assert abs(v) <= 1.1
+
+
+def test_seed_octaves_are_permutations_with_offsets_in_range(orc):
+    count, _, _ = orc.octave(-1, 0)
+    for k in range(0, count, 7):
+        _, offs, perm = orc.octave(-1, k)
+        assert sorted(perm.tolist()) == list(range(256))
+        assert all(0.0 <= o < 256.0 for o in offs[:3])
+
+
+# ---- climate R-tree: brute force over every leaf ----
+def test_tree_search_follows_the_last_result_rule(orc, golden):
+    t = golden['tree_t']
+    leaf, brute, dist, cdist = orc.tree(t, chain=True)
+    assert (cdist == dist).all()                      # the search result is at the minimal distance
+    prev = np.concatenate([[-1], leaf[:-1]])
+    # the candidate (previous result) wins iff it reaches the minimum; else the first minimal leaf
+    assert ((leaf == brute) | (leaf == prev)).all()
+    ties = (leaf != brute).sum()
+    assert ties > 0                                   # the box corners create exact ties
+    leaf0, brute0, _, _ = orc.tree(t, chain=False)
+    assert (leaf0 == brute0).all()                    # without a candidate: the first minimal leaf
+
+
+def test_tree_search_matches_reference(orc, golden):
+    leaf, _, _, _ = orc.tree(golden['tree_t'], chain=True)
+    assert (leaf == golden['tree_out']).all()
+
+
+# ---- zoom ----
+def test_zoom_matches_reference_and_picks_a_neighbour(golden):
+    xyz = golden['zoom_xyz']
+    for k, zs in enumerate(golden['zoom_seed']):
+        out = np.zeros_like(xyz)
+        L().orc_zoom(int(zs), xyz.ctypes.data, len(xyz), out.ctypes.data)
+        assert (out == golden['zoom_out'][k]).all()
+        base = (xyz.astype(np.int64) - 2) >> 2
+        assert ((out - base >= 0) & (out - base <= 1)).all()
+
+
+def test_zoom_seed_is_sha256_of_the_seed():
+    for seed in (0, 1, -1, 123456789):
+        want = int.from_bytes(hashlib.sha256(int(seed).to_bytes(8, 'little', signed=True)).digest()[:8], 'little', signed=True)
+        assert L().orc_zoom_seed(seed) == want
+
+
+# ---- per-seed tables vs the reference ----
+def test_noise_tables_match_reference(orc, golden):
+    keys = [k for k in golden if k.startswith('layers_')]
+    assert keys
+    for k in keys:
+        _, seed, key = k.split('_', 2)
+        params, perms = orc.noise_layers(int(seed), key)
+        assert params.view(np.uint64).tolist() == golden[k].view(np.uint64).tolist(), key
+        pd = np.stack([digest(p) for p in perms]) if len(perms) else np.zeros((0, 8), np.uint8)
+        assert (pd == golden['perm_%s_%s' % (seed, key)]).all(), key
+
+
+def test_seed_misc_matches_reference(orc, golden):
+    import oracle
+    for k in [k for k in golden if k.startswith('misc_')]:
+        seed = int(k.split('_')[1])
+        g = golden[k]
+        assert np.int64(g[0].view(np.int64)) == L().orc_zoom_seed(seed)
+        assert (orc.clay(seed) == g[7:].view(np.int64)).all()
+        # the positional factories: fromHashOf(name).at(0, 0, 0) streams under the same seed
+        assert oracle.lib().orc_positional(seed, b'minecraft:aquifer', 0, 0, 0) != oracle.lib().orc_positional(seed, b'minecraft:ore', 0, 0, 0)
+
+
+# ---- density functions vs the reference ----
+def test_density_roots_match_reference_volume_and_point(orc, golden):
+    vols, pts = golden['vols'], golden['pts_xyz']
+    n = 0
+    for name in golden['roots']:
+        node = orc.root(str(name))
+        for seed in (123456789, -4172144997902289642):
+            for k, v in enumerate(vols):
+                want = golden['vol_%d_%s_%d' % (seed, name, k)]
+                got = orc.sample(seed, node, v)
+                assert (got.view(np.uint32) == want.view(np.uint32)).all(), (name, seed, k)
+            want = golden['pts_%d_%s' % (seed, name)]
+            assert (orc.point(seed, node, pts).view(np.uint32) == want.view(np.uint32)).all(), (name, seed)
+            n += 1
+    assert n >= 60
+
+
+# ---- whole chunks vs the reference ----
+def test_edge_chunks_match_reference(orc, golden):
+    seeds, xz = golden['edge_seed'], golden['edge_xz']
+    for i, (s, c) in enumerate(zip(seeds, xz)):
+        t, b, a, d = orc.generate(int(s), [tuple(c)], density=True)
+        assert (digest(t.astype(np.uint8)) == golden['edge_terrain'][i]).all(), (s, c)
+        assert (digest(b) == golden['edge_biomes'][i]).all(), (s, c)
+        assert (digest(a.astype(np.uint8)) == golden['edge_aquifer'][i]).all(), (s, c)
+        assert (digest(d) == golden['edge_density'][i]).all(), (s, c)
+        if i < 2:
+            assert (t[0].astype(np.uint8) == golden['edge_full_terrain'][i]).all()
+            assert (b[0] == golden['edge_full_biomes'][i]).all()
+
+
+def test_c1_region_matches_reference(orc, golden):
+    seed = int(golden['c1_seed'][0])
+    assert (seed, [tuple(c) for c in golden['c1_xz']]) == workloads.c1_region()
+    t, b, _ = orc.generate(seed, [tuple(c) for c in golden['c1_xz']], threads=8)
+    for i in range(len(t)):
+        assert (digest(t[i].astype(np.uint8)) == golden['c1_terrain'][i]).all(), i
+        assert (digest(b[i]) == golden['c1_biomes'][i]).all(), i
+
+
+def test_outputs_fit_the_u8_layout(orc):
+    t, b, _ = orc.generate(0, [(3, -4)])
+    import json
+    tm = json.load(open(os.path.join(GOLDEN, '..', '..', 'minecraft-terrain-cuda_amd', 'data', 'synthetic_overworld_terrain.json')))
+    assert t.max() < len(tm['palette']) <= 256 and b.max() < len(tm['biome_names']) <= 128
+
+
+def test_oracle_matches_live_reference(orc, reference):
+    r = workloads.SplitMix64(0x4C495645)
+    for _ in range(3):
+        seed = r.signed()
+        chunks = [(r.between(-117187, 117187), r.between(-117187, 117187)) for _ in range(2)]
+        ot, ob, oa, od = orc.generate(seed, chunks, density=True)
+        rt, rb, ra, rd = reference.generate(seed, chunks, density=True)
+        assert (ot == rt).all() and (ob == rb).all() and (oa == ra).all()
+        assert (od.view(np.uint32) == rd.view(np.uint32)).all()
