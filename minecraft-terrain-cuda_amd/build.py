@@ -2,6 +2,7 @@
 
   python build.py            codegen from data/*.json, then hipcc (device) and g++ (emulation)
   python build.py --no-emu   skip the host emulation library
+  python build.py --remarks F  also write the compiler's per-kernel resource usage to F
 
 The exactness flags are part of the semantics (Java arithmetic): no FMA contraction, no
 fast-math, denormals kept, correctly rounded f32 division and square root; packed FP32
@@ -45,10 +46,14 @@ def codegen(ir=IR, terrain=TERRAIN):
     run([sys.executable, os.path.join(HERE, 'codegen', 'gen_hip.py'), ir, terrain, os.path.join(CSRC, 'gen')], 'codegen')
 
 
-def build_device(out=LIB, extra=()):
+def build_device(out=LIB, extra=(), remarks=None):
     hipcc = shutil.which('hipcc') or '/opt/rocm/bin/hipcc'
     tmp = out + '.tmp'
-    run([hipcc] + HIP_FLAGS + list(extra) + [os.path.join(CSRC, 'mct.hip'), '-o', tmp], 'hipcc gfx950 -> %s' % os.path.basename(out))
+    extra = list(extra) + (['-Rpass-analysis=kernel-resource-usage'] if remarks else [])
+    text = run([hipcc] + HIP_FLAGS + extra + [os.path.join(CSRC, 'mct.hip'), '-o', tmp], 'hipcc gfx950 -> %s' % os.path.basename(out))
+    if remarks:
+        with open(remarks, 'w') as fh:
+            fh.write(text)
     os.replace(tmp, out)
 
 
@@ -63,10 +68,11 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument('--no-emu', action='store_true')
     ap.add_argument('--no-device', action='store_true')
+    ap.add_argument('--remarks', metavar='FILE', help='write the per-kernel register / scratch / occupancy report to FILE')
     a = ap.parse_args()
     codegen()
     if not a.no_device:
-        build_device()
+        build_device(remarks=a.remarks)
     if not a.no_emu:
         build_emu()
 

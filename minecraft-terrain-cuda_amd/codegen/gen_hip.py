@@ -569,9 +569,24 @@ class Gen:
         cnt = {'c': 0, 'r': 0}
         key_index = self.plan.key_index
 
-        def noise_call(key, is3d):
+        # 2D noises (at (x, 0, z)) depend on the column only: each gets a slot of the column
+        # record the surface kernels compute once per column
+        self.col_noise = []
+
+        def col_slot(key):
             g = self.noise_fn('g', key)
-            return '%s(c.O, static_cast<double>(c.x), %s, static_cast<double>(c.z))' % (g, 'static_cast<double>(c.y)' if is3d else '0.0')
+            if g not in self.col_noise:
+                self.col_noise.append(g)
+            return self.col_noise.index(g)
+
+        for k in ('surface', 'surface_secondary', 'clay_bands_offset'):
+            col_slot('minecraft:' + k)
+
+        def noise_call(key, is3d):
+            if not is3d:
+                return 'c.col_noise(%d)' % col_slot(key)
+            g = self.noise_fn('g', key)
+            return '%s(c.O, static_cast<double>(c.x), static_cast<double>(c.y), static_cast<double>(c.z))' % g
 
         def cond(cn):
             t = cn['t']
@@ -654,6 +669,12 @@ class Gen:
         for name, body in rules:
             o.append('template <class C> MCT_HD MCT_FI int %s(C& c) {\n    %s\n}' % (name, body))
         o.append('template <class C> MCT_HD MCT_FI int apply_rules(C& c) { return %s(c); }' % root)
+        o.append('constexpr int kColNoises = %d;   // per-column noise slots' % len(self.col_noise))
+        o.append('constexpr int kColSurface = 0, kColSecondary = 1, kColClay = 2;')
+        o.append('MCT_HD MCT_FI void surf_column_noises(const mct::Octave* O, double x, double z, float* out) {')
+        for k, g in enumerate(self.col_noise):
+            o.append('    out[%d] = %s(O, x, 0.0, z);' % (k, g))
+        o.append('}')
         return o
 
     # ---- model tables --------------------------------------------------------------------

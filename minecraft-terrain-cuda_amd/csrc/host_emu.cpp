@@ -22,6 +22,7 @@ thread_local std::string g_err;
 struct Emu {
     ModelDev M{};
     std::vector<TreeNodeDev> tree;
+    std::vector<TreeNodeF> treef;
     std::vector<uint8_t> leaf_biome;
     std::vector<int32_t> leaf_node;
     std::unique_ptr<SeedDev> seed;
@@ -40,10 +41,12 @@ void fill_model(Emu& e) {
         d.leaf = t.leaf;
         d.pad = 0;
     }
+    e.treef = float_tree(e.tree);
     e.leaf_biome.assign(mctgen::kLeafBiome, mctgen::kLeafBiome + mctgen::kTreeLeaves);
     e.leaf_node.assign(mctgen::kLeafNode, mctgen::kLeafNode + mctgen::kTreeLeaves);
     ModelDev& M = e.M;
     M.tree = e.tree.data();
+    M.treef = e.treef.empty() ? nullptr : e.treef.data();
     M.leaf_biome = e.leaf_biome.data();
     M.leaf_node = e.leaf_node.data();
     std::memcpy(M.block_flags, mctgen::kBlockFlags, 64);
@@ -208,7 +211,9 @@ int mcte_generate(void* p, int64_t seed, const int32_t* xz, int n, uint8_t* terr
         std::vector<int> h0(static_cast<size_t>(n) * 256), h1(h0.size()), h2(h0.size()), hf(h0.size());
         for (int c = 0; c < n; ++c)
             for (int col = 0; col < 256; ++col) h0[c * 256 + col] = column_height(M, blocks.data() + static_cast<size_t>(c) * kPer, col >> 4, col & 15);
-        SurfArgs A{seeds, &M, ch.data(), blocks.data(), nb.data(), bb.data(), corners.data(), colmap.data(), prelim.data()};
+        std::vector<float> colrec(static_cast<size_t>(n) * 256 * kColF);
+        for (int i = 0; i < n * 256; ++i) surface_column(seeds, M, ch.data(), colrec.data(), i);
+        SurfArgs A{seeds, &M, ch.data(), blocks.data(), nb.data(), bb.data(), corners.data(), colmap.data(), prelim.data(), colrec.data()};
         for (int c = 0; c < n; ++c)
             for (int col = 0; col < 256; ++col) surface_pass1(A, h0.data(), h1.data(), h2.data(), c, col);
         for (int c = 0; c < n; ++c)

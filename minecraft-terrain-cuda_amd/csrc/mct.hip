@@ -192,6 +192,13 @@ __global__ __launch_bounds__(256) void k_heights(int n, const uint8_t* blocks, i
     h0[c * 256 + x * 16 + z] = column_height(c_model, blocks + static_cast<size_t>(c) * kPer, x, z);
 }
 
+// Column records of the surface pass (2D rule noises, snow-line noise, frozen-patch flag).
+__global__ __launch_bounds__(256) void k_surface_cols(const SeedDev* seeds, const ChunkRef* chunks, int n, float* rec) {
+    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= n * 256) return;
+    surface_column(seeds, c_model, chunks, rec, idx);
+}
+
 // Threads of a wave take adjacent x so the column walks read adjacent bytes of a y-layer.
 __global__ __launch_bounds__(64) void k_surface1(SurfArgs A, int n, const int* h0, int* h1, int* h2) {
     const int idx = blockIdx.x * blockDim.x + threadIdx.x;
@@ -489,11 +496,11 @@ struct Dedupe {
 // Kernel groups timed with HIP events on the batch stream when profiling is enabled.
 enum KernelId {
     kKBiomeColumns, kKBiomeQuarts, kKBiomeTies, kKBiomeChain, kKDedupeCols, kKColumns, kKCorners, kKDedupeGrid, kKFts, kKGridScatter,
-    kKAqCells, kKFill, kKHeights, kKSurface1, kKSurface2, kKSurfaceExact, kKGather, kKCount
+    kKAqCells, kKFill, kKHeights, kKSurface1, kKSurface2, kKSurfaceExact, kKGather, kKSurfaceCols, kKCount
 };
 const char* const kKernelNames[kKCount] = {"biome_columns", "biome_quarts", "biome_ties", "biome_chain", "dedupe_columns", "columns", "corners",
                                            "dedupe_grid", "fts", "grid_scatter+aquifer_skip", "aquifer_cells", "fill", "heights", "surface1",
-                                           "surface2", "surface_check+exact", "gather_biomes"};
+                                           "surface2", "surface_check+exact", "gather_biomes", "surface_columns"};
 
 struct Slot {
     hipStream_t stream = nullptr;
@@ -515,6 +522,7 @@ struct Slot {
     uint64_t *d_ckeys = nullptr, *d_ucols = nullptr, *d_gkeys = nullptr, *d_ug = nullptr;
     int *d_colmap = nullptr, *d_gmap = nullptr, *d_counts = nullptr, *d_skip = nullptr;
     int *d_h0 = nullptr, *d_h1 = nullptr, *d_h2 = nullptr, *d_hf = nullptr, *d_flags = nullptr;
+    float* d_colrec = nullptr;
     std::unique_ptr<Dedupe> dedupe;
     hipEvent_t kev[kKCount][2] = {};
     bool kev_recorded = false;
@@ -553,6 +561,7 @@ struct Slot {
         M(&d_h1, sizeof(int) * mb * 256);
         M(&d_h2, sizeof(int) * mb * 256);
         M(&d_hf, sizeof(int) * mb * 256);
+        M(&d_colrec, sizeof(float) * mb * 256 * kColF);
         M(&d_flags, sizeof(int) * mb);
         M(&d_counts, sizeof(int) * 4);
         dedupe = std::make_unique<Dedupe>(mb * kGridKeys);
@@ -560,7 +569,7 @@ struct Slot {
     ~Slot() {
         if (stream) (void)hipStreamSynchronize(stream);
         void* p[] = {d_ch, d_bch, d_nb, d_bcol, d_colv, d_corners, d_gv, d_prelim, d_targets, d_mind, d_lstar, d_grid, d_cloc, d_cfl, d_tie, d_bb,
-                     d_blocks, d_bout, d_aq, d_ckeys, d_ucols, d_gkeys, d_ug, d_colmap, d_gmap, d_counts, d_skip, d_h0, d_h1, d_h2, d_hf, d_flags};
+                     d_blocks, d_bout, d_aq, d_ckeys, d_ucols, d_gkeys, d_ug, d_colmap, d_gmap, d_counts, d_skip, d_h0, d_h1, d_h2, d_hf, d_flags, d_colrec};
         for (void* q : p) (void)hipFree(q);
         dedupe.reset();
         (void)hipHostFree(h_ch);
@@ -594,6 +603,7 @@ struct Generator {
     size_t next_slot = 0;
     // model
     TreeNodeDev* d_tree = nullptr;
+    TreeNodeF* d_treef = nullptr;
     uint8_t* d_leaf_biome = nullptr;
     int32_t* d_leaf_node = nullptr;
     int* d_blended_first = nullptr;
@@ -641,12 +651,18 @@ struct Generator {
         }
         hip_check(hipMalloc(&d_tree, sizeof(TreeNodeDev) * tree.size()), "malloc");
         hip_check(hipMemcpy(d_tree, tree.data(), sizeof(TreeNodeDev) * tree.size(), hipMemcpyHostToDevice), "upload tree");
+        const std::vector<TreeNodeF> treef = float_tree(tree);
+        if (!treef.empty()) {
+            hip_check(hipMalloc(&d_treef, sizeof(TreeNodeF) * treef.size()), "malloc");
+            hip_check(hipMemcpy(d_treef, treef.data(), sizeof(TreeNodeF) * treef.size(), hipMemcpyHostToDevice), "upload tree");
+        }
         hip_check(hipMalloc(&d_leaf_biome, mctgen::kTreeLeaves), "malloc");
         hip_check(hipMemcpy(d_leaf_biome, mctgen::kLeafBiome, mctgen::kTreeLeaves, hipMemcpyHostToDevice), "upload");
         hip_check(hipMalloc(&d_leaf_node, sizeof(int32_t) * mctgen::kTreeLeaves), "malloc");
         hip_check(hipMemcpy(d_leaf_node, mctgen::kLeafNode, sizeof(int32_t) * mctgen::kTreeLeaves, hipMemcpyHostToDevice), "upload");
         ModelDev M{};
         M.tree = d_tree;
+        M.treef = d_treef;
         M.leaf_biome = d_leaf_biome;
         M.leaf_node = d_leaf_node;
         std::memcpy(M.block_flags, mctgen::kBlockFlags, 64);
@@ -682,7 +698,7 @@ struct Generator {
     ~Generator() {
         (void)hipDeviceSynchronize();
         slots.clear();
-        void* p[] = {d_tree, d_leaf_biome, d_leaf_node, d_blended_first, d_seeds, d_jobs, d_stats};
+        void* p[] = {d_tree, d_treef, d_leaf_biome, d_leaf_node, d_blended_first, d_seeds, d_jobs, d_stats};
         for (void* q : p) (void)hipFree(q);
         (void)hipHostFree(h_jobs);
         if (t0) (void)hipEventDestroy(t0);
@@ -841,7 +857,8 @@ struct Generator {
                                                                                                reinterpret_cast<uint4*>(S.d_aq));
         hip_check(hipEventRecord(S.ev[6], st), "event");
         // surface
-        SurfArgs A{sd, nullptr, S.d_ch, blocks, S.d_nb, S.d_bb, S.d_corners, S.d_colmap, S.d_prelim};
+        SurfArgs A{sd, nullptr, S.d_ch, blocks, S.d_nb, S.d_bb, S.d_corners, S.d_colmap, S.d_prelim, S.d_colrec};
+        timed(S, kKSurfaceCols, [&] { k_surface_cols<<<grid_of(static_cast<size_t>(n) * 256, 256), 256, 0, st>>>(sd, S.d_ch, n, S.d_colrec); });
         timed(S, kKHeights, [&] { k_heights<<<grid_of(static_cast<size_t>(n) * 256, 256), 256, 0, st>>>(n, blocks, S.d_h0, S.d_counts + 2); });
         timed(S, kKSurface1, [&] { k_surface1<<<grid_of(static_cast<size_t>(n) * 256, 64), 64, 0, st>>>(A, n, S.d_h0, S.d_h1, S.d_h2); });
         timed(S, kKSurface2, [&] { k_surface2<<<grid_of(static_cast<size_t>(n) * 256, 64), 64, 0, st>>>(A, n, S.d_h0, S.d_h1, S.d_h2, S.d_hf); });

@@ -15,6 +15,8 @@
 // Surface), cpu/src/biome.cpp:201-289 (R-tree search, zoom).
 #pragma once
 
+#include <vector>
+
 #include "jmath.h"
 #include "noise.h"
 #include "rng.h"
@@ -65,9 +67,18 @@ struct TreeNodeDev {
     int32_t first, count, leaf, pad;
 };
 
+// The same tree with the bounds as floats (exact while every bound is an integer of magnitude
+// at most 2^24), one 64-byte record per node: first child | child count << 24, leaf index.
+struct TreeNodeF {
+    float lo[7], hi[7];
+    int32_t first_count, leaf;
+};
+constexpr double kTreeExact = 16777216.0;   // 2^24
+
 // Seed-independent model tables (uploaded once).
 struct ModelDev {
     const TreeNodeDev* tree;
+    const TreeNodeF* treef;         // null when some bound is beyond 2^24 (int64 search only)
     const uint8_t* leaf_biome;
     const int32_t* leaf_node;
     uint8_t block_flags[64];
@@ -78,6 +89,24 @@ struct ModelDev {
     int32_t ore_state[kMaxOre][3];
     float ore_raw[kMaxOre];
 };
+
+// Float copy of the tree, or an empty vector when a bound is not exactly representable.
+inline std::vector<TreeNodeF> float_tree(const std::vector<TreeNodeDev>& tree) {
+    std::vector<TreeNodeF> out(tree.size());
+    for (size_t i = 0; i < tree.size(); ++i) {
+        for (int k = 0; k < 7; ++k) {
+            const int64_t lo = tree[i].lo[k], hi = tree[i].hi[k];
+            const int64_t lim = static_cast<int64_t>(kTreeExact);
+            if (lo < -lim || lo > lim || hi < -lim || hi > lim) return {};
+            out[i].lo[k] = static_cast<float>(lo);
+            out[i].hi[k] = static_cast<float>(hi);
+        }
+        if (tree[i].first >= (1 << 24) || tree[i].count > 127) return {};
+        out[i].first_count = tree[i].first | (tree[i].count << 24);
+        out[i].leaf = tree[i].leaf;
+    }
+    return out;
+}
 
 // ---- keys for the position dedupe -------------------------------------------------------
 // A key packs (seed slot, flag, quart x, quart z) relative to the batch's minima, with each
@@ -245,6 +274,56 @@ MCT_HD inline int tree_first_min(const TreeNodeDev* T, const int64_t* t, int64_t
     return closest >= 0 ? T[closest].leaf : -1;
 }
 
+// The search over the float tree with FP64 distances. With every target coordinate and bound
+// an integer of magnitude <= 2^24, each per-axis distance is <= 2^25, its square <= 2^50 and a
+// sum of seven < 2^53: all exact in FP64, so the distances, their order and the leaf found are
+// those of the int64 search above.
+MCT_HD MCT_FI double box_distance_f(const TreeNodeF& n, const double* t) {
+    double d = 0.0;
+    for (int i = 0; i < 7; ++i) {
+        const double above = t[i] - static_cast<double>(n.hi[i]);
+        const double below = static_cast<double>(n.lo[i]) - t[i];
+        const double p = above > 0.0 ? above : (below > 0.0 ? below : 0.0);
+        d = d + p * p;
+    }
+    return d;
+}
+
+MCT_HD inline int tree_first_min_f(const TreeNodeF* T, const double* t, double bound, double& out_d) {
+    int stack_node[12], stack_child[12];
+    int sp = 0;
+    double best = bound;
+    int closest = -1;
+    if (T[0].leaf >= 0) {
+        const double d = box_distance_f(T[0], t);
+        if (best > d) { best = d; closest = 0; }
+    } else {
+        stack_node[0] = 0;
+        stack_child[0] = 0;
+        sp = 1;
+    }
+    while (sp > 0) {
+        const int fc = T[stack_node[sp - 1]].first_count;
+        const int k = stack_child[sp - 1];
+        if (k >= (fc >> 24)) { --sp; continue; }
+        stack_child[sp - 1] = k + 1;
+        const int c = (fc & 0xFFFFFF) + k;
+        const double cd = box_distance_f(T[c], t);
+        if (best > cd) {
+            if (T[c].leaf >= 0) {
+                best = cd;
+                closest = c;
+            } else if (sp < 12) {
+                stack_node[sp] = c;
+                stack_child[sp] = 0;
+                ++sp;
+            }
+        }
+    }
+    out_d = best;
+    return closest >= 0 ? T[closest].leaf : -1;
+}
+
 // Climate columns of one biome chunk column (y-independent parts at y = 0).
 MCT_HD MCT_FI void biome_column(const SeedDev* seeds, const ChunkRef* bch, float* bcol, int idx) {
     const int b = idx >> 4, k = idx & 15;
@@ -273,13 +352,29 @@ MCT_HD inline void biome_quarts(const SeedDev* seeds, const ModelDev& M, const C
         int64_t* t = targets + i * 7;
         for (int k = 0; k < 6; ++k) t[k] = quantize(clim[k]);
         t[6] = 0;
-        int64_t bound = INT64_MAX, d = 0;
-        if (prev >= 0) {
-            const int64_t dp = box_distance(M.tree[M.leaf_node[prev]], t);
-            if (dp < INT64_MAX) bound = dp + 1;
+        bool small = M.treef != nullptr;
+        double td[7];
+        for (int k = 0; k < 7; ++k) {
+            small = small && t[k] >= -static_cast<int64_t>(kTreeExact) && t[k] <= static_cast<int64_t>(kTreeExact);
+            td[k] = static_cast<double>(t[k]);
         }
-        int leaf = tree_first_min(M.tree, t, bound, d);
-        if (leaf < 0) leaf = tree_first_min(M.tree, t, INT64_MAX, d);
+        int leaf;
+        int64_t d = 0;
+        if (small) {
+            double bound = u2d(0x7FF0000000000000ull), dd = 0.0;
+            if (prev >= 0) bound = box_distance_f(M.treef[M.leaf_node[prev]], td) + 1.0;
+            leaf = tree_first_min_f(M.treef, td, bound, dd);
+            if (leaf < 0) leaf = tree_first_min_f(M.treef, td, u2d(0x7FF0000000000000ull), dd);
+            d = static_cast<int64_t>(dd);
+        } else {
+            int64_t bound = INT64_MAX;
+            if (prev >= 0) {
+                const int64_t dp = box_distance(M.tree[M.leaf_node[prev]], t);
+                if (dp < INT64_MAX) bound = dp + 1;
+            }
+            leaf = tree_first_min(M.tree, t, bound, d);
+            if (leaf < 0) leaf = tree_first_min(M.tree, t, INT64_MAX, d);
+        }
         lstar[i] = leaf;
         mind[i] = d;
         prev = leaf;
@@ -633,31 +728,54 @@ MCT_HD inline float simplex2(const uint8_t* perm, double px, double py) {
     return static_cast<float>(70.0 * (simplex_corner(g0, x0, y0) + simplex_corner(g1, x1, y1) + simplex_corner(g2, x2, y2)));
 }
 
-// Biome.getHeightAdjustedTemperature
-MCT_HD inline float biome_temperature(const ModelDev& M, int b, int x, int y, int z) {
-    float adjusted = M.biome_temp[b];
-    if (M.biome_frozen[b]) {
-        const double fr[3] = {1.0, 0.5, 0.25};
-        const float amp[3] = {0.14285715f, 0.2857143f, 0.5714286f};
-        float fv = 0.0f;
-        for (int k = 0; k < 3; ++k) fv += amp[k] * simplex2(M.simplex[1 + k], static_cast<double>(x) * 0.05 * fr[k], static_cast<double>(z) * 0.05 * fr[k]);
-        const double large = static_cast<double>(fv * 7.0f);
-        const double edge = static_cast<double>(simplex2(M.simplex[4], static_cast<double>(x) * 0.2, static_cast<double>(z) * 0.2));
-        if (large + edge < 0.3) {
-            const double small = static_cast<double>(simplex2(M.simplex[4], static_cast<double>(x) * 0.09, static_cast<double>(z) * 0.09));
-            if (small < 0.8) adjusted = 0.2f;
-        }
-    }
+// Biome.getHeightAdjustedTemperature, split into what depends on the column only (the
+// frozen-modifier patch test and the snow-line noise, kept in the column record) and the rest.
+// frozen patch: large (three frozen simplex layers x 7) + edge < 0.3 and small < 0.8.
+MCT_HD inline bool frozen_patch(const ModelDev& M, int x, int z) {
+    const double fr[3] = {1.0, 0.5, 0.25};
+    const float amp[3] = {0.14285715f, 0.2857143f, 0.5714286f};
+    float fv = 0.0f;
+    for (int k = 0; k < 3; ++k) fv += amp[k] * simplex2(M.simplex[1 + k], static_cast<double>(x) * 0.05 * fr[k], static_cast<double>(z) * 0.05 * fr[k]);
+    const double large = static_cast<double>(fv * 7.0f);
+    const double edge = static_cast<double>(simplex2(M.simplex[4], static_cast<double>(x) * 0.2, static_cast<double>(z) * 0.2));
+    if (!(large + edge < 0.3)) return false;
+    const double small = static_cast<double>(simplex2(M.simplex[4], static_cast<double>(x) * 0.09, static_cast<double>(z) * 0.09));
+    return small < 0.8;
+}
+
+// the temperature noise above the snow line: simplex(x / 8f, z / 8f) * 8f
+MCT_HD inline float snow_noise(const ModelDev& M, int x, int z) {
+    return simplex2(M.simplex[0], static_cast<double>(static_cast<float>(x) / 8.0f), static_cast<double>(static_cast<float>(z) / 8.0f)) * 8.0f;
+}
+
+MCT_HD MCT_FI float biome_temperature_col(const ModelDev& M, int b, int y, bool patch, float snow_v) {
+    const float adjusted = (M.biome_frozen[b] && patch) ? 0.2f : M.biome_temp[b];
     const int snow = kSea + 17;
     if (y > snow) {
-        const float v = simplex2(M.simplex[0], static_cast<double>(static_cast<float>(x) / 8.0f), static_cast<double>(static_cast<float>(z) / 8.0f)) * 8.0f;
-        float t = v + static_cast<float>(y);
+        float t = snow_v + static_cast<float>(y);
         t = t - static_cast<float>(snow);
         t = t * 0.05f;
         t = t / 40.0f;
         return adjusted - t;
     }
     return adjusted;
+}
+
+// Column record: the rule tree's 2D noises (slots of surf_column_noises, including surface,
+// surface_secondary and clay_bands_offset), the snow-line noise and the frozen-patch flag.
+constexpr int kColF = mctgen::kColNoises + 2;
+constexpr int kColSnow = mctgen::kColNoises, kColPatch = mctgen::kColNoises + 1;
+
+MCT_HD inline void surface_column(const SeedDev* seeds, const ModelDev& M, const ChunkRef* chunks, float* rec, int idx) {
+    const int ci = idx >> 8, col = idx & 255;
+    const ChunkRef cr = chunks[ci];
+    const int x = cr.x * 16 + (col >> 4), z = cr.z * 16 + (col & 15);
+    float* r = rec + static_cast<size_t>(idx) * kColF;
+    float v[mctgen::kColNoises];
+    mctgen::surf_column_noises(seeds[cr.seed].oct, static_cast<double>(x), static_cast<double>(z), v);
+    for (int k = 0; k < mctgen::kColNoises; ++k) r[k] = v[k];
+    r[kColSnow] = snow_noise(M, x, z);
+    r[kColPatch] = frozen_patch(M, x, z) ? 1.0f : 0.0f;
 }
 
 // ---- material system: one column ----------------------------------------------------------
@@ -671,17 +789,23 @@ struct SurfArgs {
     const float* corners;
     const int* colmap;
     const float* prelim;        // n x 4 chunk-corner preliminary surface values
+    const float* colrec;        // n x 256 column records (kColF floats, column = lx * 16 + lz)
 };
 
 struct Surf {
     const SurfArgs* A;
     const Octave* O;
     const SeedDev* S;
-    int cx, cz, lx, lz;
+    int ci, cx, cz, lx, lz;
     uint8_t* blocks;
     int* hm;
-    const uint8_t* nb[9];
     const int* cmap;
+    const float* rec;
+    // zoom: the column's fixed x / z parts and the fiddle indices of the corner quarts at
+    // quart rows zc_py (zc[0..3]) and zc_py + 1 (zc[4..7]), corner k = xf * 2 + zf, packed
+    // mx | my << 10 | mz << 20
+    int zpx, zpz, zfx, zfz, zc_py;
+    uint32_t zc[8];
     float prelim;
     int x, y, z;
     int grad_x, grad_z, surface_depth;
@@ -695,17 +819,50 @@ struct Surf {
         const int dx = (qx >> 2) - cx, dz = (qz >> 2) - cz;
         const int qmin = kMinY >> 2;
         const int cqy = clampi(qy, qmin, qmin + kQH - 1) - qmin;
-        return nb[(dz + 1) * 3 + dx + 1][(cqy * 4 + (qz & 3)) * 4 + (qx & 3)];
+        const size_t bc = static_cast<size_t>(A->nb_index[ci * 9 + (dz + 1) * 3 + dx + 1]);
+        return A->bbiomes[bc * kQPer + (cqy * 4 + (qz & 3)) * 4 + (qx & 3)];
     }
-    MCT_HD MCT_FI int biome_at(int bx, int by, int bz) const {
-        int qx, qy, qz;
-        zoom_pick(S->zoom_seed, bx, by, bz, qx, qy, qz);
-        return quart_biome(qx, qy, qz);
+    MCT_HD MCT_FI void zoom_layer(int cy, uint32_t* L) const {
+        for (int k = 0; k < 4; ++k) {
+            int mx, my, mz;
+            corner_fiddles(S->zoom_seed, zpx + (k >> 1), cy, zpz + (k & 1), mx, my, mz);
+            L[k] = static_cast<uint32_t>(mx) | (static_cast<uint32_t>(my) << 10) | (static_cast<uint32_t>(mz) << 20);
+        }
+    }
+    // BiomeManager.getBiome at (x, by, z) of this column (zoom_pick with the corner fiddles of
+    // the two quart rows kept while the walk moves down)
+    MCT_HD MCT_FI int biome_y(int by) {
+        const int ay = by - 2, py = ay >> 2, fy = ay & 3;
+        if (py != zc_py) {
+            if (py == zc_py - 1) {
+                for (int k = 0; k < 4; ++k) zc[4 + k] = zc[k];
+                zoom_layer(py, zc);
+            } else {
+                zoom_layer(py, zc);
+                zoom_layer(py + 1, zc + 4);
+            }
+            zc_py = py;
+        }
+        int best = 0;
+        double best_d = u2d(0x7FF0000000000000ull);
+        for (int i = 0; i < 8; ++i) {
+            const bool xf = (i & 4) != 0, yf = (i & 2) != 0, zf = (i & 1) != 0;
+            const uint32_t pk = zc[(yf ? 4 : 0) + (xf ? 2 : 0) + (zf ? 1 : 0)];
+            const int mx = static_cast<int>(pk & 1023u), my = static_cast<int>((pk >> 10) & 1023u), mz = static_cast<int>(pk >> 20);
+            const double d = (zoom_sq(zfz, zf, mz) + zoom_sq(fy, yf, my)) + zoom_sq(zfx, xf, mx);
+            if (best_d > d) {
+                best = i;
+                best_d = d;
+            }
+        }
+        return quart_biome((best & 4) ? zpx + 1 : zpx, (best & 2) ? py + 1 : py, (best & 1) ? zpz + 1 : zpz);
     }
     MCT_HD MCT_FI int biome() {
-        if (!biome_set) { biome_v = biome_at(x, y, z); biome_set = true; }
+        if (!biome_set) { biome_v = biome_y(y); biome_set = true; }
         return biome_v;
     }
+    MCT_HD MCT_FI float col_noise(int k) const { return rec[k]; }
+    MCT_HD MCT_FI float temperature(int b, int by) const { return biome_temperature_col(*A->M, b, by, rec[kColPatch] != 0.0f, rec[kColSnow]); }
     MCT_HD MCT_FI bool vertical_gradient(int k, int true_at, int false_at) const {
         if (y <= true_at) return true;
         if (y >= false_at) return false;
@@ -715,7 +872,7 @@ struct Surf {
         return static_cast<double>(next_float(r)) < probability;
     }
     MCT_HD MCT_FI double surface_secondary() {
-        if (!sec_set) { secondary = mctgen::surf_surface_secondary(O, static_cast<double>(x), 0.0, static_cast<double>(z)); sec_set = true; }
+        if (!sec_set) { secondary = static_cast<double>(rec[mctgen::kColSecondary]); sec_set = true; }
         return secondary;
     }
     MCT_HD MCT_FI bool stone_depth(int offset, bool add, int range, bool ceiling) {
@@ -728,16 +885,16 @@ struct Surf {
         }
         return stone <= 1 + offset + sd + sec;
     }
-    MCT_HD MCT_FI bool cold_enough_to_snow() { return !(biome_temperature(*A->M, biome(), x, y, z) >= 0.15f); }
+    MCT_HD MCT_FI bool cold_enough_to_snow() { return !(temperature(biome(), y) >= 0.15f); }
     MCT_HD MCT_FI int min_surface_level() {
         if (!min_set) { min_surface = mfloor(static_cast<double>(prelim)) + surface_depth - 8; min_set = true; }
         return min_surface;
     }
     MCT_HD MCT_FI int band() const {
-        const int offset = jroundf(static_cast<float>(mctgen::surf_clay_bands_offset(O, static_cast<double>(x), 0.0, static_cast<double>(z))) * 4.0f);
+        const int offset = jroundf(rec[mctgen::kColClay] * 4.0f);
         return S->clay[(y + offset + 192) % 192];
     }
-    MCT_HD inline int ore_vein(int k) {
+    MCT_HD MCT_FI int ore_vein(int k) {
         if (!ore_set) {
             interp_channels(A->corners, cmap, lx, y - kMinY, lz, 0, kCh, ore_ch);
             ore_set = true;
@@ -757,7 +914,7 @@ struct Surf {
     MCT_HD MCT_FI bool air(int s) const { return (A->M->block_flags[s] & kFAir) != 0; }
     MCT_HD MCT_FI bool fluid(int s) const { return (A->M->block_flags[s] & kFFluid) != 0; }
     // setBlockState + Heightmap.update(WORLD_SURFACE_WG); `write` false moves only the height
-    MCT_HD inline void set(int by, int state, bool write) {
+    MCT_HD MCT_FI void set(int by, int state, bool write) {
         if (by < kMinY || by >= kMinY + kHeight) return;
         if (write) blocks[bidx(by)] = static_cast<uint8_t>(state);
         const int first = *hm;
@@ -772,7 +929,7 @@ struct Surf {
     }
 };
 
-MCT_HD inline void surf_init(Surf& c, const SurfArgs& A, int ci, int lx, int lz, int* hm) {
+MCT_HD MCT_FI void surf_init(Surf& c, const SurfArgs& A, int ci, int lx, int lz, int* hm) {
     c.A = &A;
     const ChunkRef cr = A.chunks[ci];
     c.S = &A.seeds[cr.seed];
@@ -783,7 +940,8 @@ MCT_HD inline void surf_init(Surf& c, const SurfArgs& A, int ci, int lx, int lz,
     c.lz = lz;
     c.blocks = A.blocks + static_cast<size_t>(ci) * kPer;
     c.hm = hm;
-    for (int k = 0; k < 9; ++k) c.nb[k] = A.bbiomes + static_cast<size_t>(A.nb_index[ci * 9 + k]) * kQPer;
+    c.ci = ci;
+    c.rec = A.colrec + (static_cast<size_t>(ci) * 256 + static_cast<size_t>(lx * 16 + lz)) * kColF;
     c.cmap = A.colmap + static_cast<size_t>(ci) * kCols;
     const float* pc = A.prelim + static_cast<size_t>(ci) * 4;
     // chunk_surface_level: InterpolatedFunction 16 x 1 over the chunk's 4 corners
@@ -792,12 +950,17 @@ MCT_HD inline void surf_init(Surf& c, const SurfArgs& A, int ci, int lx, int lz,
     c.x = cr.x * 16 + lx;
     c.z = cr.z * 16 + lz;
     c.y = 0;
+    c.zpx = (c.x - 2) >> 2;
+    c.zpz = (c.z - 2) >> 2;
+    c.zfx = (c.x - 2) & 3;
+    c.zfz = (c.z - 2) & 3;
+    c.zc_py = INT32_MIN;
 }
 
-MCT_HD inline void surf_update_xz(Surf& c, int gx, int gz) {
+MCT_HD MCT_FI void surf_update_xz(Surf& c, int gx, int gz) {
     c.grad_x = gx;
     c.grad_z = gz;
-    const double n = mctgen::surf_surface(c.O, static_cast<double>(c.x), 0.0, static_cast<double>(c.z));
+    const double n = static_cast<double>(c.rec[mctgen::kColSurface]);
     Xoro r = at(c.S->random, c.x, 0, c.z);
     c.surface_depth = d2i(n * 2.75 + 3.0 + next_double(r) * 0.25);
     c.sec_set = false;
@@ -805,7 +968,7 @@ MCT_HD inline void surf_update_xz(Surf& c, int gx, int gz) {
 }
 
 // MaterialSystem eroded-badlands extension (writes blocks and moves the height).
-MCT_HD inline void eroded_badlands(Surf& c, int height) {
+MCT_HD MCT_FI void eroded_badlands(Surf& c, int height) {
     const double bx = c.x, bz = c.z;
     const double a = fabs(mctgen::surf_badlands_surface(c.O, bx, 0.0, bz) * 8.25);
     const double b = static_cast<double>(static_cast<float>(mctgen::surf_badlands_pillar(c.O, bx * 0.2, 0.0, bz * 0.2)) * 15.0f);
@@ -824,7 +987,7 @@ MCT_HD inline void eroded_badlands(Surf& c, int height) {
 }
 
 // MaterialSystem frozen-ocean extension; with write == false only the height moves.
-MCT_HD inline void frozen_ocean(Surf& c, int min_surface, int biome, int height, bool write) {
+MCT_HD MCT_FI void frozen_ocean(Surf& c, int min_surface, int biome, int height, bool write) {
     const double bx = c.x, bz = c.z;
     const double a = fabs(mctgen::surf_iceberg_surface(c.O, bx, 0.0, bz) * 8.25);
     const double b = static_cast<double>(static_cast<float>(mctgen::surf_iceberg_pillar(c.O, bx * 1.28, 0.0, bz * 1.28)) * 15.0f);
@@ -832,7 +995,7 @@ MCT_HD inline void frozen_ocean(Surf& c, int min_surface, int biome, int height,
     if (iceberg <= 1.8) return;
     const double roof = fabs(mctgen::surf_iceberg_pillar_roof(c.O, bx * 1.17, 0.0, bz * 1.17) * 1.5);
     double top = jmind(iceberg * iceberg * 1.2, ceil(roof * 40.0) + 14.0);
-    if (biome_temperature(*c.A->M, biome, c.x, kSea, c.z) > 0.1f) top -= 2.0;
+    if (c.temperature(biome, kSea) > 0.1f) top -= 2.0;
     if (top <= 2.0) return;
     const double bottom = static_cast<double>(kSea) - top - 7.0;
     top += static_cast<double>(kSea);
@@ -858,7 +1021,7 @@ MCT_HD inline void frozen_ocean(Surf& c, int min_surface, int biome, int height,
 }
 
 // The material-rule walk of one column from `height` down, then the frozen-ocean extension.
-MCT_HD inline void surface_walk(Surf& c, int height, int surface_biome, int starting) {
+MCT_HD MCT_FI void surface_walk(Surf& c, int height, int surface_biome, int starting) {
     int stone_above = 0, water_height = INT32_MIN, next_ceiling = INT32_MAX;
     for (int yy = height; yy >= kMinY; --yy) {
         const int old = c.get(yy);
@@ -891,14 +1054,14 @@ MCT_HD inline void surface_walk(Surf& c, int height, int surface_biome, int star
 }
 
 // Pass 1 of a column: eroded-badlands extension (real) -> h1, frozen-ocean dry run -> h2.
-MCT_HD inline void surface_pass1(const SurfArgs& A, const int* h0, int* h1, int* h2, int ci, int col) {
+MCT_HD MCT_FI void surface_pass1(const SurfArgs& A, const int* h0, int* h1, int* h2, int ci, int col) {
     const int lx = col >> 4, lz = col & 15;
     const int hidx = ci * 256 + col;
     int hm = h0[hidx];
     Surf c;
     surf_init(c, A, ci, lx, lz, &hm);
     const int starting = hm;
-    const int sb = c.biome_at(c.x, starting, c.z);
+    const int sb = c.biome_y(starting);
     if (sb == mctgen::kBiomeErodedBadlands) eroded_badlands(c, starting);
     h1[hidx] = hm;
     if (sb == mctgen::kBiomeFrozenOcean || sb == mctgen::kBiomeDeepFrozenOcean) {
@@ -910,7 +1073,7 @@ MCT_HD inline void surface_pass1(const SurfArgs& A, const int* h0, int* h1, int*
 
 // Pass 2 of a column, assuming earlier columns (x-major, z-minor order) end at h2 and later
 // ones are still at h0; the final height goes to hf (checked against h2 afterwards).
-MCT_HD inline void surface_pass2(const SurfArgs& A, const int* h0, const int* h1, const int* h2, int* hf, int ci, int col) {
+MCT_HD MCT_FI void surface_pass2(const SurfArgs& A, const int* h0, const int* h1, const int* h2, int* hf, int ci, int col) {
     const int lx = col >> 4, lz = col & 15;
     const int hidx = ci * 256 + col;
     auto H = [&](int ox, int oz) {
@@ -925,7 +1088,7 @@ MCT_HD inline void surface_pass2(const SurfArgs& A, const int* h0, const int* h1
     Surf c;
     surf_init(c, A, ci, lx, lz, &hm);
     const int starting = h0[hidx];
-    const int sb = c.biome_at(c.x, starting, c.z);
+    const int sb = c.biome_y(starting);
     const int height = H(lx, lz) + 1;
     const int gx = H(lx + 1 < 15 ? lx + 1 : 15, lz) - H(lx - 1 > 0 ? lx - 1 : 0, lz);
     const int gz = H(lx, lz + 1 < 15 ? lz + 1 : 15) - H(lx, lz - 1 > 0 ? lz - 1 : 0);
@@ -936,13 +1099,13 @@ MCT_HD inline void surface_pass2(const SurfArgs& A, const int* h0, const int* h1
 
 // One column in the exact sequential order, with a live height array hm[256] of the chunk
 // (used for chunks whose speculative pass 2 saw a height change it did not assume).
-MCT_HD inline void surface_exact_column(const SurfArgs& A, int* hm, int ci, int col) {
+MCT_HD MCT_FI void surface_exact_column(const SurfArgs& A, int* hm, int ci, int col) {
     const int lx = col >> 4, lz = col & 15;
     Surf c;
     surf_init(c, A, ci, lx, lz, hm + col);
     auto H = [&](int ox, int oz) { return hm[ox * 16 + oz] - 1; };
     const int starting = H(lx, lz) + 1;
-    const int sb = c.biome_at(c.x, starting, c.z);
+    const int sb = c.biome_y(starting);
     if (sb == mctgen::kBiomeErodedBadlands) eroded_badlands(c, starting);
     const int height = H(lx, lz) + 1;
     const int gx = H(lx + 1 < 15 ? lx + 1 : 15, lz) - H(lx - 1 > 0 ? lx - 1 : 0, lz);
