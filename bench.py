@@ -96,6 +96,108 @@ def cpu_baseline(seconds_target=15.0):
             'sample': '%dx%d-chunk region at (0,0), seed %d, reference cpu/ port region mode, %.1f s' % (w2, w2, SEED, dt)}
 
 
+def batched_vs_plain(mct, batch):
+    """The 256-seed uneven workload in one mixed-seed call against the same chunks generated
+    plainly, one seed's list per call (outputs in HBM both ways)."""
+    import workloads
+    work = workloads.batched()
+    seeds = np.array([s for s, ch in work for _ in ch], np.int64)
+    chunks = [c for _, ch in work for c in ch]
+    n = len(chunks)
+    d_t, d_b = mct.alloc_device(n * mct.BLOCKS_PER_CHUNK), mct.alloc_device(n * mct.QUARTS_PER_CHUNK)
+    g = mct.FullTerrainGenerator(max_batch_chunks=batch, pipeline_slots=2, seed_capacity=256)
+    g.generate_device(seeds, chunks, d_t, d_b)          # tables resident, buffers sized
+    t0 = time.perf_counter()
+    g.generate_device(seeds, chunks, d_t, d_b)
+    tb = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    off = 0
+    for s, ch in work:                                  # plain: one seed per call
+        g.generate_device(np.full(len(ch), s, np.int64), ch, d_t + off * mct.BLOCKS_PER_CHUNK, d_b + off * mct.QUARTS_PER_CHUNK)
+        off += len(ch)
+    tp = time.perf_counter() - t0
+    g.close()
+    mct.free_device(d_t)
+    mct.free_device(d_b)
+    return {'workload': '256 uniform seeds, log-uniform 1..1024 chunks each (half near-square rectangles, half scattered in +-117,187)',
+            'chunks': n, 'seeds': len(work), 'chunks_per_s': round(n / tb, 1), 'plain_chunks_per_s': round(n / tp, 1), 'speedup': round(tp / tb, 2)}
+
+
+def served(mct, plain_rate, seconds=4.0, fractions=(0.25, 0.5, 0.8, 3.0), max_batch=1024):
+    """Open-loop Poisson request streams at fractions of the plain generator's capacity on the
+    same request mix: p50 / p99 latency (submit to results in host memory) and throughput. The
+    last rate (3x) overloads the server on purpose: its chunks/s is the served capacity."""
+    import threading
+    import workloads
+    pool = workloads.seed_pool()
+    mean_chunks = 16.5 * 16.5
+    ring_chunks = 40000
+    ring = mct.PinnedBuffer((ring_chunks, 384, 16, 16))
+    bring = mct.PinnedBuffer((ring_chunks, 96, 4, 4))
+    out = []
+    srv = mct.Server(max_batch_chunks=max_batch, seed_capacity=64, hbm_budget_bytes=48 << 30)
+    for f in fractions:
+        rps = f * plain_rate / mean_chunks
+        reqs = workloads.requests(rps, seconds, pool=pool)
+        srv.poll()
+        pos = [0]
+        sub = {}
+
+        def client():
+            t0 = time.perf_counter()
+            for t, seed, chunks in reqs:
+                dt = t - (time.perf_counter() - t0)
+                if dt > 0:
+                    time.sleep(dt)
+                k = len(chunks)
+                if pos[0] + k > ring_chunks:
+                    pos[0] = 0                           # the ring is far larger than what is in flight
+                sub[srv.submit(seed, chunks, ring.array[pos[0]:pos[0] + k], bring.array[pos[0]:pos[0] + k])] = k
+                pos[0] += k
+        t0 = time.perf_counter()
+        th = threading.Thread(target=client)
+        th.start()
+        th.join()
+        srv.drain()
+        wall = time.perf_counter() - t0
+        lat = np.array([l for _, l in srv.poll(1 << 20)])
+        nch = sum(len(c) for _, _, c in reqs)
+        out.append({'offered_fraction': f, 'offered_requests_per_s': round(rps, 1), 'requests': len(reqs), 'chunks': nch,
+                    'chunks_per_s': round(nch / wall, 1), 'p50_ms': round(float(np.percentile(lat, 50)), 3),
+                    'p99_ms': round(float(np.percentile(lat, 99)), 3), 'max_ms': round(float(lat.max()), 3)})
+    st = srv.stats()
+    srv.close()
+    ring.free()
+    bring.free()
+    return {'workload': 'Poisson arrivals, seed from a 64-seed pool, W x H region (W, H uniform 1..32) in spiral order around a random player chunk',
+            'plain_capacity_chunks_per_s': round(plain_rate, 1), 'seconds_per_rate': seconds, 'rates': out,
+            'served_capacity_chunks_per_s': out[-1]['chunks_per_s'], 'served_vs_plain': round(out[-1]['chunks_per_s'] / plain_rate, 2),
+            'batches': st['batches'], 'mixed_batches': st['mixed_batches'], 'max_batch': max_batch}
+
+
+def plain_request_capacity(mct, n_requests=200):
+    """Chunks/s of the plain generator on the served request mix: one request per call,
+    results copied to pinned host memory (what a server without batching does)."""
+    import workloads
+    pool = workloads.seed_pool()
+    reqs = workloads.requests(1000.0, 10.0, pool=pool)[:n_requests]
+    g = mct.FullTerrainGenerator(max_batch_chunks=1024, pipeline_slots=2, seed_capacity=64)
+    buf = mct.PinnedBuffer((1024, 384, 16, 16))
+    for s in pool:
+        g.set_seed(s)
+        g.generate([(0, 0)], out=buf.array[:1])
+    t0 = time.perf_counter()
+    nch = 0
+    for _, seed, chunks in reqs:
+        g.set_seed(seed)
+        g.generate(chunks, out=buf.array[:len(chunks)])
+        nch += len(chunks)
+    dt = time.perf_counter() - t0
+    g.close()
+    buf.free()
+    return nch / dt
+
+
 def copy_calibration(nbytes):
     """One plain device-to-device copy of nbytes (HIP's blit kernel): under rocprofv3 --pmc its
     FETCH_SIZE / WRITE_SIZE against nbytes give the scale of those counters."""
@@ -123,6 +225,7 @@ def main():
     ap.add_argument('--slots', type=int, default=2)
     ap.add_argument('--quick', action='store_true', help='one 32x32 region instead of the full workload')
     ap.add_argument('--no-cpu', action='store_true')
+    ap.add_argument('--no-serve', action='store_true', help='skip the served-request measurement')
     ap.add_argument('--copy-calibration', type=int, default=0, metavar='BYTES',
                     help='also run one device-to-device copy of BYTES (the reference point for HBM counters)')
     a = ap.parse_args()
@@ -182,6 +285,10 @@ def main():
     mct.free_device(d_t)
     mct.free_device(d_b)
     g.close()
+    batched = None if a.quick else batched_vs_plain(mct, a.batch)
+    serve = None
+    if not a.quick and not a.no_serve:
+        serve = served(mct, plain_request_capacity(mct))
     if a.copy_calibration:
         copy_calibration(a.copy_calibration)
     cpu = None if a.no_cpu else cpu_baseline()
@@ -191,7 +298,8 @@ def main():
             'config': {'workload': 'plain TERRAIN, one seed (%d), %s, full height, outputs in HBM' % (
                 SEED, '32x32 region' if a.quick else '3 x 64x64-chunk regions (origin, (-2048,-2048), border chunk %d) + 8 wrap/border chunks' % BORDER_CHUNK),
                 'chunks': n, 'batch': a.batch, 'slots': a.slots},
-            'e2e_host_chunks_per_s': round(e2e, 1), 'roofline': roof, 'cpu_baseline': cpu, 'stages_ms': stages, 'kernels_ms': kernels}
+            'e2e_host_chunks_per_s': round(e2e, 1), 'batched': batched, 'served': serve, 'roofline': roof, 'cpu_baseline': cpu,
+            'stages_ms': stages, 'kernels_ms': kernels}
     print(json.dumps(line), flush=True)
 
 

@@ -93,6 +93,35 @@ int mct_profile(mct_generator* g, int enable);
 int mct_kernel_times(mct_generator* g, double* ms, int64_t* launches, int cap, int* n);
 const char* mct_kernel_name(int i);
 
+/* ---- Serving (no counterpart in the reference, which serves one synchronous call at a time) ----
+ * A server owns a generator with two pipeline slots, the per-seed tables of up to
+ * seed_capacity seeds, and a result arena in HBM sized by hbm_budget_bytes (the budget minus
+ * the generator's own buffers). Region requests are admitted, in arrival order, while the
+ * arena has room for their outputs. A scheduler thread batches the admitted requests' chunks
+ * in arrival order (one batch mixes requests and seeds; a request arriving while others run
+ * joins the next batch), writes results into the arena, and copies each finished request to
+ * its host buffers on a separate stream while later batches compute. Results are the same
+ * bytes as mct_generate gives for the request alone. Submit / poll / drain may be called from
+ * any thread. */
+typedef struct mct_server mct_server;
+
+/* max_batch_chunks: chunks per device batch (1..65536); seed_capacity as in mct_create. */
+int mct_server_create(const char* ir_json, const char* terrain_json, int max_batch_chunks, int seed_capacity, size_t hbm_budget_bytes,
+                      mct_server** out);
+/* Queues a request: count chunks of seed; terrain (count * 98,304 B) and biomes (count * 1,536
+ * B, may be NULL) are host buffers (pinned is faster) written before the request completes.
+ * MCT_ECAPACITY if the request alone exceeds the arena. */
+int mct_server_submit(mct_server* s, int64_t seed, const int32_t* chunk_xz, int count, uint8_t* terrain, uint8_t* biomes, int64_t* request_id);
+/* Up to cap requests completed since the last poll: their ids and their latency (ms from
+ * submit to the end of the copy to the host); *n = how many. */
+int mct_server_poll(mct_server* s, int64_t* ids, double* latency_ms, int cap, int* n);
+/* Blocks until every submitted request has completed. */
+int mct_server_drain(mct_server* s);
+/* submitted, completed, batches, chunks, arena chunk slots, most slots held at once, batches
+ * that mixed requests, seed capacity. */
+int mct_server_stats(mct_server* s, int64_t out[8]);
+int mct_server_destroy(mct_server* s);
+
 /* Build facts: blocks per chunk, quarts per chunk, min y, height, palette size, biome count. */
 int mct_info(int32_t out[6]);
 

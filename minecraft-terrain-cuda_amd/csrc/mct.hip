@@ -509,6 +509,7 @@ struct Slot {
     int batch = -1;
     int cap;                        // chunks
     int bcap = 0;                   // biome chunks
+    size_t bytes = 0;               // device memory held (without the biome-chunk buffers)
     // pinned staging
     ChunkRef *h_ch = nullptr, *h_bch = nullptr;
     int* h_nb = nullptr;
@@ -537,7 +538,10 @@ struct Slot {
         hip_check(hipHostMalloc(&h_ch, sizeof(ChunkRef) * mb, hipHostMallocDefault), "pinned");
         hip_check(hipHostMalloc(&h_bch, sizeof(ChunkRef) * mb * 9, hipHostMallocDefault), "pinned");
         hip_check(hipHostMalloc(&h_nb, sizeof(int) * mb * 9, hipHostMallocDefault), "pinned");
-        auto M = [](auto** p, size_t bytes) { hip_check(hipMalloc(reinterpret_cast<void**>(p), bytes > 0 ? bytes : 16), "malloc"); };
+        auto M = [this](auto** p, size_t bytes) {
+            hip_check(hipMalloc(reinterpret_cast<void**>(p), bytes > 0 ? bytes : 16), "malloc");
+            this->bytes += bytes;
+        };
         const size_t ncol = std::max(mctgen::kNumColumns, 1);
         M(&d_ch, sizeof(ChunkRef) * mb);
         M(&d_nb, sizeof(int) * mb * 9);
@@ -594,6 +598,10 @@ struct Slot {
         hip_check(hipMalloc(&d_tie, q), "malloc");
         hip_check(hipMalloc(&d_bb, q), "malloc");
         bcap = nbk;
+    }
+    size_t biome_bytes() const {
+        const size_t q = static_cast<size_t>(bcap) * kQPer;
+        return sizeof(ChunkRef) * bcap + sizeof(float) * bcap * 16 * std::max(mctgen::kNumClimateColumns, 1) + q * (7 * 8 + 8 + 4 + 1 + 1);
     }
 };
 
@@ -693,6 +701,14 @@ struct Generator {
         hip_check(hipEventCreate(&t0), "event");
         for (int i = 0; i < nslots; ++i) slots.push_back(std::make_unique<Slot>(mb));
         hip_check(hipDeviceSynchronize(), "init");
+    }
+
+    // Device memory held: seed tables, model tables and every slot (biome buffers at their
+    // current capacity).
+    size_t device_bytes() const {
+        size_t b = sizeof(SeedDev) * seed_cap + sizeof(TreeNodeDev) * mctgen::kTreeNodes + sizeof(TreeNodeF) * mctgen::kTreeNodes;
+        for (const auto& s : slots) b += s->bytes + s->biome_bytes();
+        return b;
     }
 
     ~Generator() {
@@ -907,6 +923,12 @@ struct Generator {
     }
 };
 
+}  // namespace
+
+#include "server.h"
+
+namespace {
+
 thread_local std::string g_err;
 
 int fail(int code, const std::string& m) {
@@ -950,6 +972,19 @@ struct mct_generator {
     std::unique_ptr<Generator> g;
 };
 
+struct mct_server {
+    std::unique_ptr<Server> s;
+};
+
+namespace {
+void check_model_files(const char* ir_json, const char* terrain_json) {
+    if (!ir_json || !terrain_json) throw Error(MCT_EINVAL, "null path");
+    if (file_fnv1a(ir_json) != mctgen::kIrHash) throw Error(MCT_EMODEL, "the IR file differs from the one this library was generated from (rebuild)");
+    if (file_fnv1a(terrain_json) != mctgen::kTerrainHash)
+        throw Error(MCT_EMODEL, "the terrain model file differs from the one this library was generated from (rebuild)");
+}
+}  // namespace
+
 extern "C" {
 
 const char* mct_last_error(void) { return g_err.c_str(); }
@@ -963,9 +998,7 @@ int mct_create(const char* ir_json, const char* terrain_json, int max_batch_chun
         if (pipeline_slots < 1 || pipeline_slots > 4) throw Error(MCT_EINVAL, "pipeline_slots must be 1..4");
         if (seed_capacity == 0) seed_capacity = 64;
         if (seed_capacity < 1 || seed_capacity > 4096) throw Error(MCT_EINVAL, "seed_capacity must be 1..4096");
-        if (file_fnv1a(ir_json) != mctgen::kIrHash) throw Error(MCT_EMODEL, "the IR file differs from the one this library was generated from (rebuild)");
-        if (file_fnv1a(terrain_json) != mctgen::kTerrainHash)
-            throw Error(MCT_EMODEL, "the terrain model file differs from the one this library was generated from (rebuild)");
+        check_model_files(ir_json, terrain_json);
         int dev = 0;
         hip_check(hipGetDevice(&dev), "get device");
         auto h = std::make_unique<mct_generator>();
@@ -976,6 +1009,52 @@ int mct_create(const char* ir_json, const char* terrain_json, int max_batch_chun
 
 int mct_destroy(mct_generator* g) {
     return guard([&] { delete g; });
+}
+
+int mct_server_create(const char* ir_json, const char* terrain_json, int max_batch_chunks, int seed_capacity, size_t hbm_budget_bytes, mct_server** out) {
+    return guard([&] {
+        if (!out) throw Error(MCT_EINVAL, "out is null");
+        *out = nullptr;
+        if (max_batch_chunks < 1 || max_batch_chunks > (1 << 16)) throw Error(MCT_EINVAL, "max_batch_chunks out of range");
+        if (seed_capacity == 0) seed_capacity = 64;
+        if (seed_capacity < 1 || seed_capacity > 4096) throw Error(MCT_EINVAL, "seed_capacity must be 1..4096");
+        check_model_files(ir_json, terrain_json);
+        auto h = std::make_unique<mct_server>();
+        h->s = std::make_unique<Server>(max_batch_chunks, seed_capacity, hbm_budget_bytes);
+        *out = h.release();
+    });
+}
+
+int mct_server_submit(mct_server* s, int64_t seed, const int32_t* chunk_xz, int count, uint8_t* terrain, uint8_t* biomes, int64_t* request_id) {
+    return guard([&] {
+        if (!s || count < 1 || !chunk_xz || !terrain || !request_id) throw Error(MCT_EINVAL, "bad arguments");
+        *request_id = s->s->submit(seed, chunk_xz, count, terrain, biomes);
+    });
+}
+
+int mct_server_poll(mct_server* s, int64_t* ids, double* latency_ms, int cap, int* n) {
+    return guard([&] {
+        if (!s || !n || cap < 0 || (cap > 0 && !ids)) throw Error(MCT_EINVAL, "bad arguments");
+        *n = s->s->poll(ids, latency_ms, cap);
+    });
+}
+
+int mct_server_drain(mct_server* s) {
+    return guard([&] {
+        if (!s) throw Error(MCT_EINVAL, "null server");
+        s->s->drain();
+    });
+}
+
+int mct_server_stats(mct_server* s, int64_t out[8]) {
+    return guard([&] {
+        if (!s || !out) throw Error(MCT_EINVAL, "bad arguments");
+        s->s->stats(out);
+    });
+}
+
+int mct_server_destroy(mct_server* s) {
+    return guard([&] { delete s; });
 }
 
 int mct_set_seed(mct_generator* g, int64_t seed) {

@@ -38,6 +38,7 @@ struct Segment {
 class Server {
 public:
     Server(int max_batch, int seed_cap, size_t budget) : gen(max_batch, 2, seed_cap) {
+        for (auto& sl : gen.slots) sl->ensure_biome_capacity(9 * max_batch);   // worst case: scattered chunks
         const size_t per = static_cast<size_t>(kPer) + kQPer;
         const size_t fixed = gen.device_bytes();
         if (budget <= fixed + per) throw Error(MCT_ECAPACITY, "HBM budget below the generator's own buffers (" + std::to_string(fixed) + " bytes)");

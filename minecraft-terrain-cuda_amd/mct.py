@@ -62,6 +62,12 @@ def lib():
         L.mct_kernel_times.argtypes = [P, P, P, I, ctypes.POINTER(I)]
         L.mct_kernel_name.argtypes = [I]
         L.mct_kernel_name.restype = ctypes.c_char_p
+        L.mct_server_create.argtypes = [ctypes.c_char_p, ctypes.c_char_p, I, I, SZ, ctypes.POINTER(P)]
+        L.mct_server_submit.argtypes = [P, I64, P, I, P, P, ctypes.POINTER(I64)]
+        L.mct_server_poll.argtypes = [P, P, P, I, ctypes.POINTER(I)]
+        L.mct_server_drain.argtypes = [P]
+        L.mct_server_stats.argtypes = [P, P]
+        L.mct_server_destroy.argtypes = [P]
         _lib = L
     return _lib
 
@@ -173,3 +179,68 @@ def info():
     out = np.zeros(6, np.int32)
     _check(lib().mct_info(_ptr(out)))
     return dict(zip(('blocks_per_chunk', 'quarts_per_chunk', 'min_y', 'height', 'palette', 'biomes'), (int(v) for v in out)))
+
+
+class Server:
+    """Serving loop (include/mct.h, mct_server_*): submit region requests from any thread,
+    poll completions with their latency. Output buffers must stay alive until the request
+    completes; pinned buffers (PinnedBuffer) make the copy back asynchronous."""
+
+    def __init__(self, max_batch_chunks=1024, seed_capacity=64, hbm_budget_bytes=32 << 30, ir_path=IR_PATH, terrain_path=TERRAIN_PATH):
+        h = ctypes.c_void_p()
+        _check(lib().mct_server_create(ir_path.encode(), terrain_path.encode(), max_batch_chunks, seed_capacity, hbm_budget_bytes, ctypes.byref(h)))
+        self.h = h
+        self._keep = {}
+
+    def submit(self, seed, chunks, terrain, biomes=None):
+        xz = _xz(chunks)
+        rid = ctypes.c_int64()
+        _check(lib().mct_server_submit(self.h, int(seed), _ptr(xz), len(xz), _ptr(terrain), _ptr(biomes), ctypes.byref(rid)))
+        self._keep[rid.value] = (xz, terrain, biomes)
+        return rid.value
+
+    def poll(self, cap=4096):
+        ids = np.zeros(cap, np.int64)
+        lat = np.zeros(cap, np.float64)
+        n = ctypes.c_int()
+        _check(lib().mct_server_poll(self.h, _ptr(ids), _ptr(lat), cap, ctypes.byref(n)))
+        out = list(zip(ids[:n.value].tolist(), lat[:n.value].tolist()))
+        for rid, _ in out:
+            self._keep.pop(rid, None)
+        return out
+
+    def drain(self):
+        _check(lib().mct_server_drain(self.h))
+
+    def stats(self):
+        out = np.zeros(8, np.int64)
+        _check(lib().mct_server_stats(self.h, _ptr(out)))
+        keys = ('submitted', 'completed', 'batches', 'chunks', 'arena_chunks', 'max_arena_used', 'mixed_batches', 'seed_capacity')
+        return dict(zip(keys, (int(v) for v in out)))
+
+    def close(self):
+        if self.h:
+            lib().mct_server_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class PinnedBuffer:
+    """Page-locked host memory (mct_alloc_pinned) viewed as a numpy array."""
+
+    def __init__(self, shape, dtype=np.uint8):
+        n = int(np.prod(shape)) * np.dtype(dtype).itemsize
+        self.p = lib().mct_alloc_pinned(max(n, 1))
+        if not self.p:
+            raise MctError(3, 'pinned allocation failed')
+        self.array = np.ctypeslib.as_array(ctypes.cast(self.p, ctypes.POINTER(ctypes.c_uint8)), shape=(max(n, 1),))[:n].view(dtype).reshape(shape)
+
+    def free(self):
+        if self.p:
+            lib().mct_free_pinned(self.p)
+            self.p = None
