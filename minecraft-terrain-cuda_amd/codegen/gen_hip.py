@@ -240,7 +240,9 @@ class Gen:
         P = self.plan
         L = []
         if kind in ('v', 'sv'):
-            L.append('MCT_HD MCT_FI float %s(const mct::Octave* O, int32_t bx, int32_t by, int32_t bz, double xz, double ys, float amplitude) {' % name)
+            # noise stacks are out-of-line calls: inlined, their octave temporaries pile up on
+            # the evaluators' memoised values and force register spills and occupancy 1
+            L.append('MCT_HD MCT_NI float %s(const mct::Octave* O, int32_t bx, int32_t by, int32_t bz, double xz, double ys, float amplitude) {' % name)
             L.append('    float acc = 0.0f;')
             for i in range(first, first + count):
                 fr = dl(P.freq[i])
@@ -251,7 +253,7 @@ class Gen:
                     L.append('    acc = acc + (%s) * mct::smeared_volume(O[%d], %s, bx, by, bz, xz * %s, ys * %s);' % (amp, i, dl(P.smear[i]), fr, fr))
             L.append('    return acc;')
         else:
-            L.append('MCT_HD MCT_FI float %s(const mct::Octave* O, double x, double y, double z) {' % name)
+            L.append('MCT_HD MCT_NI float %s(const mct::Octave* O, double x, double y, double z) {' % name)
             L.append('    float value = 0.0f;')
             for i in range(first, first + count):
                 fr = dl(P.freq[i])
@@ -671,9 +673,12 @@ class Gen:
         o.append('template <class C> MCT_HD MCT_FI int apply_rules(C& c) { return %s(c); }' % root)
         o.append('constexpr int kColNoises = %d;   // per-column noise slots' % len(self.col_noise))
         o.append('constexpr int kColSurface = 0, kColSecondary = 1, kColClay = 2;')
-        o.append('MCT_HD MCT_FI void surf_column_noises(const mct::Octave* O, double x, double z, float* out) {')
+        o.append('MCT_HD MCT_FI float surf_column_noise(const mct::Octave* O, double x, double z, int k) {')
+        o.append('    switch (k) {')
         for k, g in enumerate(self.col_noise):
-            o.append('    out[%d] = %s(O, x, 0.0, z);' % (k, g))
+            o.append('        case %d: return %s(O, x, 0.0, z);' % (k, g))
+        o.append('    }')
+        o.append('    return 0.0f;')
         o.append('}')
         return o
 

@@ -203,8 +203,7 @@ int mcte_generate(void* p, int64_t seed, const int32_t* xz, int n, uint8_t* terr
         std::vector<uint8_t> blocks(static_cast<size_t>(n) * kPer);
         FillArgs F{seeds, ch.data(), corners.data(), colmap.data(), cloc.data(), cfl.data(), skip.data(), blocks.data()};
         for (int c = 0; c < n; ++c)
-            for (int i = 0; i < kPer; ++i)
-                blocks[static_cast<size_t>(c) * kPer + i] = fill_block(F, c, i, density ? density + static_cast<size_t>(c) * kPer + i : nullptr);
+            for (int t = 0; t < 256 * kCellsY; ++t) fill_cell(F, c, t, density ? density + static_cast<size_t>(c) * kPer : nullptr);
         if (aquifer) std::memcpy(aquifer, blocks.data(), blocks.size());
         const std::vector<uint8_t> filled = blocks;
         // surface

@@ -12,9 +12,11 @@
 #include <hip/hip_runtime.h>
 #define MCT_HD __host__ __device__
 #define MCT_FI __forceinline__
+#define MCT_NI __noinline__
 #else
 #define MCT_HD
 #define MCT_FI inline __attribute__((always_inline))
+#define MCT_NI __attribute__((noinline))
 #endif
 
 namespace mct {

@@ -175,14 +175,13 @@ __global__ __launch_bounds__(256) void k_aq_cells(const SeedDev* seeds, const Ch
 }
 
 // ---- fill --------------------------------------------------------------------------------
+// One thread per block column of a 4x8 cell (8 blocks); adjacent threads take adjacent x.
 __global__ __launch_bounds__(256) void k_fill(FillArgs A, int n) {
-    const size_t idx = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (idx >= static_cast<size_t>(n) * kPer) return;
-    const int c = static_cast<int>(idx / kPer), i = static_cast<int>(idx % kPer);
-    A.blocks[idx] = fill_block(A, c, i, nullptr);
+    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= n * 256 * kCellsY) return;
+    fill_cell(A, idx / (256 * kCellsY), idx % (256 * kCellsY), nullptr);
 }
 
-// ---- surface -------------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_heights(int n, const uint8_t* blocks, int* h0, int* redo_count) {
     const int idx = blockIdx.x * blockDim.x + threadIdx.x;
     if (idx == 0) *redo_count = 0;   // list of chunks for the exact surface pass, filled by k_surface_check
@@ -240,7 +239,7 @@ __global__ __launch_bounds__(256) void k_surface_exact(FillArgs F, SurfArgs A, c
     for (int r = blockIdx.x; r < total; r += gridDim.x) {
     const int c = redo[r];
     __syncthreads();
-    for (int i = threadIdx.x; i < kPer; i += blockDim.x) F.blocks[static_cast<size_t>(c) * kPer + i] = fill_block(F, c, i, nullptr);
+    for (int t = threadIdx.x; t < 256 * kCellsY; t += blockDim.x) fill_cell(F, c, t, nullptr);
     __syncthreads();
     {
         const int t = threadIdx.x;
@@ -865,7 +864,7 @@ struct Generator {
         // fill
         uint8_t* blocks = to_device ? terrain : S.d_blocks;
         FillArgs F{sd, S.d_ch, S.d_corners, S.d_colmap, S.d_cloc, S.d_cfl, S.d_skip, blocks};
-        timed(S, kKFill, [&] { k_fill<<<grid_of(static_cast<size_t>(n) * kPer, 256), 256, 0, st>>>(F, n); });
+        timed(S, kKFill, [&] { k_fill<<<grid_of(static_cast<size_t>(n) * 256 * kCellsY, 256), 256, 0, st>>>(F, n); });
         hip_check(hipGetLastError(), "fill kernel");
         hip_check(hipEventRecord(S.ev[5], st), "event");
         if (aquifer) k_copy16<<<grid_of(static_cast<size_t>(n) * kPer / 16, 256), 256, 0, st>>>(static_cast<size_t>(n) * kPer / 16,

@@ -678,18 +678,44 @@ struct FillArgs {
     uint8_t* blocks;
 };
 
-MCT_HD inline uint8_t fill_block(const FillArgs& A, int c, int i, float* density_out) {
-    const int x = i & 15, z = (i >> 4) & 15, yy = i >> 8;
-    float ch[mctgen::kNumChannels > 0 ? mctgen::kNumChannels : 1];
-    interp_channels(A.corners, A.colmap + static_cast<size_t>(c) * kCols, x, yy, z, 0, mctgen::kNumChannels, ch);
+// The 8 blocks of one block column inside one 4x8 cell (t = (cy * 16 + z) * 16 + x): the z
+// and x lerps of every channel once, then y as the running sum of the step, which is the
+// same sequence of float additions the per-block form repeats from the cell's bottom.
+constexpr int kCellsY = kHeight / 8;
+MCT_HD MCT_FI void fill_cell(const FillArgs& A, int c, int t, float* density_out) {
+    const int x = t & 15, z = (t >> 4) & 15, cy = t >> 8;
+    const int* cmap = A.colmap + static_cast<size_t>(c) * kCols;
+    const int cx = x >> 2, xi = x & 3, cz = z >> 2, zi = z & 3;
+    const size_t u00 = cmap[cz * 5 + cx], u10 = cmap[cz * 5 + cx + 1], u01 = cmap[(cz + 1) * 5 + cx], u11 = cmap[(cz + 1) * 5 + cx + 1];
+    constexpr int NC = mctgen::kNumChannels > 0 ? mctgen::kNumChannels : 1;
+    float val[NC], step[NC];
+    const float az = static_cast<float>(zi) * 0.25f, ax = static_cast<float>(xi) * 0.25f;
+    for (int k = 0; k < mctgen::kNumChannels; ++k) {
+        const float* a = A.corners + (u00 * kCh + k) * kRows + cy;
+        const float* b = A.corners + (u10 * kCh + k) * kRows + cy;
+        const float* cc = A.corners + (u01 * kCh + k) * kRows + cy;
+        const float* d = A.corners + (u11 * kCh + k) * kRows + cy;
+        const float v00 = lerpf(az, a[0], cc[0]), v01 = lerpf(az, a[1], cc[1]);
+        const float v10 = lerpf(az, b[0], d[0]), v11 = lerpf(az, b[1], d[1]);
+        const float lo = lerpf(ax, v00, v10), hi = lerpf(ax, v01, v11);
+        step[k] = (hi - lo) * 0.125f;
+        val[k] = lo + step[k] * 0.0f;
+    }
     const ChunkRef cr = A.chunks[c];
     const Octave* O = A.seeds[cr.seed].oct;
-    const int bx = cr.x * 16 + x, by = kMinY + yy, bz = cr.z * 16 + z;
-    const float d = mctgen::eval_block(O, bx, by, bz, ch);
-    if (density_out) *density_out = d;
-    const int s = aquifer_substance(O, cr.x, cr.z, A.cell_loc + static_cast<size_t>(c) * kAqCells * 4, A.cell_fluid + static_cast<size_t>(c) * kAqCells * 2,
-                                    A.skip[c], bx, by, bz, static_cast<double>(d));
-    return static_cast<uint8_t>(s < 0 ? mctgen::kDefaultBlock : s);
+    const int bx = cr.x * 16 + x, bz = cr.z * 16 + z;
+    const int32_t* loc = A.cell_loc + static_cast<size_t>(c) * kAqCells * 4;
+    const int32_t* fl = A.cell_fluid + static_cast<size_t>(c) * kAqCells * 2;
+    const int skip = A.skip[c];
+    uint8_t* out = A.blocks + static_cast<size_t>(c) * kPer + static_cast<size_t>(cy * 8 * 256 + z * 16 + x);
+    for (int yi = 0; yi < 8; ++yi) {
+        const int by = kMinY + cy * 8 + yi;
+        const float d = mctgen::eval_block(O, bx, by, bz, val);
+        if (density_out) density_out[(cy * 8 + yi) * 256 + z * 16 + x] = d;
+        const int s = aquifer_substance(O, cr.x, cr.z, loc, fl, skip, bx, by, bz, static_cast<double>(d));
+        out[yi * 256] = static_cast<uint8_t>(s < 0 ? mctgen::kDefaultBlock : s);
+        for (int k = 0; k < mctgen::kNumChannels; ++k) val[k] += step[k];
+    }
 }
 
 // Initial WORLD_SURFACE_WG height of a column (highest non-air block + 1, or minY).
@@ -766,14 +792,13 @@ MCT_HD MCT_FI float biome_temperature_col(const ModelDev& M, int b, int y, bool 
 constexpr int kColF = mctgen::kColNoises + 2;
 constexpr int kColSnow = mctgen::kColNoises, kColPatch = mctgen::kColNoises + 1;
 
+// The record of one column (idx = chunk * 256 + lx * 16 + lz).
 MCT_HD inline void surface_column(const SeedDev* seeds, const ModelDev& M, const ChunkRef* chunks, float* rec, int idx) {
     const int ci = idx >> 8, col = idx & 255;
     const ChunkRef cr = chunks[ci];
     const int x = cr.x * 16 + (col >> 4), z = cr.z * 16 + (col & 15);
     float* r = rec + static_cast<size_t>(idx) * kColF;
-    float v[mctgen::kColNoises];
-    mctgen::surf_column_noises(seeds[cr.seed].oct, static_cast<double>(x), static_cast<double>(z), v);
-    for (int k = 0; k < mctgen::kColNoises; ++k) r[k] = v[k];
+    for (int k = 0; k < mctgen::kColNoises; ++k) r[k] = mctgen::surf_column_noise(seeds[cr.seed].oct, static_cast<double>(x), static_cast<double>(z), k);
     r[kColSnow] = snow_noise(M, x, z);
     r[kColPatch] = frozen_patch(M, x, z) ? 1.0f : 0.0f;
 }
