@@ -72,10 +72,35 @@ __global__ __launch_bounds__(256) void k_biome_columns(const SeedDev* seeds, con
     if (idx < nbk * 16) biome_column(seeds, bch, bcol, idx);
 }
 
-__global__ __launch_bounds__(256) void k_biome_quarts(const SeedDev* seeds, const ChunkRef* bch, int nbk, const float* bcol, int64_t* targets,
-                                                      int32_t* lstar, int64_t* mind) {
+__global__ __launch_bounds__(256) void k_biome_targets(const SeedDev* seeds, const ChunkRef* bch, int nbk, const float* bcol, int64_t* targets) {
     const int idx = blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx < nbk * 16 * kBiomeRuns) biome_quarts(seeds, c_model, bch, bcol, targets, lstar, mind, idx);
+    if (idx < nbk * kQPer) biome_target(seeds, bch, bcol, targets, idx);
+}
+
+// R-tree searches, one run of kBiomeRun quarts per thread; the upper levels of the tree (the
+// first kTreeTop nodes of the breadth-first order, 24 KB) are staged in LDS once per
+// workgroup. Quarts with targets beyond the float tree's range are left to k_biome_search_wide.
+constexpr int kTreeTop = 384;
+constexpr int kSearchDepth = 12;
+__global__ __launch_bounds__(256) void k_biome_search(int nbk, const int64_t* targets, int32_t* lstar, int64_t* mind) {
+    __shared__ TreeNodeF top[kTreeTop];
+    const ModelDev& M = c_model;
+    const int ntop = mctgen::kTreeNodes < kTreeTop ? mctgen::kTreeNodes : kTreeTop;
+    for (int i = threadIdx.x; i < ntop * 16; i += blockDim.x) reinterpret_cast<uint32_t*>(top)[i] = reinterpret_cast<const uint32_t*>(M.treef)[i];
+    __syncthreads();
+    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx < nbk * 16 * kBiomeRuns) biome_search(M, top, ntop, targets, lstar, mind, idx, false);
+}
+
+__global__ __launch_bounds__(256) void k_biome_search_wide(int nbk, const int64_t* targets, int32_t* lstar, int64_t* mind) {
+    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx < nbk * kQPer) biome_search_wide(c_model, targets, lstar, mind, idx);
+}
+
+// The search without the float tree (a model whose bounds it cannot hold exactly).
+__global__ __launch_bounds__(256) void k_biome_search_scalar(int nbk, const int64_t* targets, int32_t* lstar, int64_t* mind) {
+    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx < nbk * 16 * kBiomeRuns) biome_search(c_model, nullptr, 0, targets, lstar, mind, idx, true);
 }
 
 __global__ __launch_bounds__(256) void k_biome_ties(int nbk, const int64_t* targets, const int32_t* lstar, const int64_t* mind, uint8_t* tie) {
@@ -207,12 +232,21 @@ __global__ __launch_bounds__(64) void k_surface1(SurfArgs A, int n, const int* h
     surface_pass1(A, h0, h1, h2, c, (t & 15) * 16 + (t >> 4));
 }
 
+// A workgroup is 64 columns of one chunk (x 0..15 by 4 z rows, lane = (z & 3) * 16 + x), whose
+// blocks are one contiguous 64-byte run per y: the 24 KB slab is staged in LDS, walked there
+// and written back.
 __global__ __launch_bounds__(64) void k_surface2(SurfArgs A, int n, const int* h0, const int* h1, const int* h2, int* hf) {
+    __shared__ uint4 slab[kHeight * 4];
     const int idx = blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= n * 256) return;
+    if (idx >= n * 256) return;   // whole workgroups (256 columns per chunk)
     A.M = &c_model;
     const int c = idx >> 8, t = idx & 255;
-    surface_pass2(A, h0, h1, h2, hf, c, (t & 15) * 16 + (t >> 4));
+    uint8_t* base = A.blocks + static_cast<size_t>(c) * kPer + ((t >> 4) & ~3) * 16;
+    for (int k = threadIdx.x; k < kHeight * 4; k += 64) slab[k] = *reinterpret_cast<const uint4*>(base + (k >> 2) * 256 + (k & 3) * 16);
+    __syncthreads();
+    surface_pass2(A, h0, h1, h2, hf, c, (t & 15) * 16 + (t >> 4), reinterpret_cast<uint8_t*>(slab) + threadIdx.x, 64);
+    __syncthreads();
+    for (int k = threadIdx.x; k < kHeight * 4; k += 64) *reinterpret_cast<uint4*>(base + (k >> 2) * 256 + (k & 3) * 16) = slab[k];
 }
 
 // A chunk needs the exact pass if any column's final height differs from the height the
@@ -659,7 +693,7 @@ struct Generator {
         hip_check(hipMalloc(&d_tree, sizeof(TreeNodeDev) * tree.size()), "malloc");
         hip_check(hipMemcpy(d_tree, tree.data(), sizeof(TreeNodeDev) * tree.size(), hipMemcpyHostToDevice), "upload tree");
         const std::vector<TreeNodeF> treef = float_tree(tree);
-        if (!treef.empty()) {
+        if (!treef.empty() && tree_levels(tree) <= kSearchDepth) {
             hip_check(hipMalloc(&d_treef, sizeof(TreeNodeF) * treef.size()), "malloc");
             hip_check(hipMemcpy(d_treef, treef.data(), sizeof(TreeNodeF) * treef.size(), hipMemcpyHostToDevice), "upload tree");
         }
@@ -834,8 +868,15 @@ struct Generator {
         const SeedDev* sd = d_seeds;
         // biomes
         timed(S, kKBiomeColumns, [&] { k_biome_columns<<<grid_of(static_cast<size_t>(nbk) * 16, 256), 256, 0, st>>>(sd, S.d_bch, nbk, S.d_bcol); });
-        timed(S, kKBiomeQuarts, [&] { k_biome_quarts<<<grid_of(static_cast<size_t>(nbk) * 16 * kBiomeRuns, 256), 256, 0, st>>>(sd, S.d_bch, nbk, S.d_bcol, S.d_targets, S.d_lstar,
-                                                                                                 S.d_mind); });
+        timed(S, kKBiomeQuarts, [&] {
+            k_biome_targets<<<grid_of(static_cast<size_t>(nbk) * kQPer, 256), 256, 0, st>>>(sd, S.d_bch, nbk, S.d_bcol, S.d_targets);
+            if (d_treef) {
+                k_biome_search<<<grid_of(static_cast<size_t>(nbk) * 16 * kBiomeRuns, 256), 256, 0, st>>>(nbk, S.d_targets, S.d_lstar, S.d_mind);
+                k_biome_search_wide<<<grid_of(static_cast<size_t>(nbk) * kQPer, 256), 256, 0, st>>>(nbk, S.d_targets, S.d_lstar, S.d_mind);
+            } else {
+                k_biome_search_scalar<<<grid_of(static_cast<size_t>(nbk) * 16 * kBiomeRuns, 256), 256, 0, st>>>(nbk, S.d_targets, S.d_lstar, S.d_mind);
+            }
+        });
         timed(S, kKBiomeTies, [&] { k_biome_ties<<<grid_of(static_cast<size_t>(nbk) * kQPer, 256), 256, 0, st>>>(nbk, S.d_targets, S.d_lstar, S.d_mind, S.d_tie); });
         timed(S, kKBiomeChain, [&] { k_biome_chain<<<grid_of(static_cast<size_t>(nbk) * 64, 256), 256, 0, st>>>(nbk, S.d_targets, S.d_lstar, S.d_mind, S.d_tie, S.d_bb, d_stats); });
         quarts += static_cast<uint64_t>(nbk) * kQPer;

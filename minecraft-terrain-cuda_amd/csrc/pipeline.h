@@ -15,6 +15,7 @@
 // Surface), cpu/src/biome.cpp:201-289 (R-tree search, zoom).
 #pragma once
 
+#include <algorithm>
 #include <vector>
 
 #include "jmath.h"
@@ -89,6 +90,14 @@ struct ModelDev {
     int32_t ore_state[kMaxOre][3];
     float ore_raw[kMaxOre];
 };
+
+// Levels of internal nodes on the longest root-to-leaf path.
+inline int tree_levels(const std::vector<TreeNodeDev>& tree, int n = 0) {
+    if (tree.empty() || tree[static_cast<size_t>(n)].leaf >= 0) return 0;
+    int deepest = 0;
+    for (int k = 0; k < tree[static_cast<size_t>(n)].count; ++k) deepest = std::max(deepest, tree_levels(tree, tree[static_cast<size_t>(n)].first + k));
+    return deepest + 1;
+}
 
 // Float copy of the tree, or an empty vector when a bound is not exactly representable.
 inline std::vector<TreeNodeF> float_tree(const std::vector<TreeNodeDev>& tree) {
@@ -289,39 +298,41 @@ MCT_HD MCT_FI double box_distance_f(const TreeNodeF& n, const double* t) {
     return d;
 }
 
-MCT_HD inline int tree_first_min_f(const TreeNodeF* T, const double* t, double bound, double& out_d) {
-    int stack_node[12], stack_child[12];
+// Nodes below `ntop` are read from `top` (the first nodes of the breadth-first flattening,
+// i.e. the upper levels, staged in LDS by the caller), the rest from T.
+MCT_HD inline int tree_first_min_f(const TreeNodeF* T, const TreeNodeF* top, int ntop, const double* t, double bound, double& out_d) {
+    int stack[12];   // node << 5 | next child (at most 19 children per node)
     int sp = 0;
     double best = bound;
     int closest = -1;
-    if (T[0].leaf >= 0) {
-        const double d = box_distance_f(T[0], t);
+    auto node = [&](int i) -> const TreeNodeF& { return i < ntop ? top[i] : T[i]; };
+    if (node(0).leaf >= 0) {
+        const double d = box_distance_f(node(0), t);
         if (best > d) { best = d; closest = 0; }
     } else {
-        stack_node[0] = 0;
-        stack_child[0] = 0;
+        stack[0] = 0;
         sp = 1;
     }
     while (sp > 0) {
-        const int fc = T[stack_node[sp - 1]].first_count;
-        const int k = stack_child[sp - 1];
+        const int e = stack[sp - 1];
+        const int fc = node(e >> 5).first_count;
+        const int k = e & 31;
         if (k >= (fc >> 24)) { --sp; continue; }
-        stack_child[sp - 1] = k + 1;
+        stack[sp - 1] = e + 1;
         const int c = (fc & 0xFFFFFF) + k;
-        const double cd = box_distance_f(T[c], t);
+        const TreeNodeF& cn = node(c);
+        const double cd = box_distance_f(cn, t);
         if (best > cd) {
-            if (T[c].leaf >= 0) {
+            if (cn.leaf >= 0) {
                 best = cd;
                 closest = c;
             } else if (sp < 12) {
-                stack_node[sp] = c;
-                stack_child[sp] = 0;
-                ++sp;
+                stack[sp++] = c << 5;
             }
         }
     }
     out_d = best;
-    return closest >= 0 ? T[closest].leaf : -1;
+    return closest >= 0 ? node(closest).leaf : -1;
 }
 
 // Climate columns of one biome chunk column (y-independent parts at y = 0).
@@ -334,51 +345,68 @@ MCT_HD MCT_FI void biome_column(const SeedDev* seeds, const ChunkRef* bch, float
 constexpr int kBiomeRun = 8;
 constexpr int kBiomeRuns = (kQH + kBiomeRun - 1) / kBiomeRun;
 
-// A run of kBiomeRun quarts down one quart column: climate target, L*, minimal distance.
-MCT_HD inline void biome_quarts(const SeedDev* seeds, const ModelDev& M, const ChunkRef* bch, const float* bcol, int64_t* targets, int32_t* lstar,
-                                int64_t* mind, int idx) {
+// Climate target of one quart (idx = biome chunk * kQPer + (qy * 4 + qz) * 4 + qx).
+MCT_HD MCT_FI void biome_target(const SeedDev* seeds, const ChunkRef* bch, const float* bcol, int64_t* targets, int idx) {
+    const int b = idx / kQPer, q = idx % kQPer;
+    const int qx = q & 3, qz = (q >> 2) & 3, qy = q >> 4;
+    const ChunkRef c = bch[b];
+    const float* cv = bcol + (static_cast<size_t>(b) * 16 + qz * 4 + qx) * mctgen::kNumClimateColumns;
+    float clim[6];
+    mctgen::eval_climate(seeds[c.seed].oct, c.x * 16 + qx * 4, kMinY + qy * 4, c.z * 16 + qz * 4, cv, clim);
+    int64_t* t = targets + static_cast<size_t>(idx) * 7;
+    for (int k = 0; k < 6; ++k) t[k] = quantize(clim[k]);
+    t[6] = 0;
+}
+
+// L* and its distance for a run of kBiomeRun quarts down one quart column; each search is
+// bounded by the previous quart's leaf distance + 1. Targets beyond the float tree's exact
+// range take the int64 search here when `wide` is set, else they are marked (L* = -2) for a
+// separate pass (biome_search_wide).
+MCT_HD inline void biome_search(const ModelDev& M, const TreeNodeF* top, int ntop, const int64_t* targets, int32_t* lstar, int64_t* mind, int idx,
+                                bool wide) {
     const int b = idx / (16 * kBiomeRuns), rem = idx % (16 * kBiomeRuns);
     const int colk = rem & 15, run = rem >> 4;
     const int qx = colk & 3, qz = colk >> 2;
-    const ChunkRef c = bch[b];
-    const Octave* O = seeds[c.seed].oct;
-    const float* cv = bcol + (static_cast<size_t>(b) * 16 + qz * 4 + qx) * mctgen::kNumClimateColumns;
+    const double inf = u2d(0x7FF0000000000000ull);
     int prev = -1;
     const int q1 = (run + 1) * kBiomeRun < kQH ? (run + 1) * kBiomeRun : kQH;
     for (int qy = run * kBiomeRun; qy < q1; ++qy) {
         const size_t i = static_cast<size_t>(b) * kQPer + (qy * 4 + qz) * 4 + qx;
-        float clim[6];
-        mctgen::eval_climate(O, c.x * 16 + qx * 4, kMinY + qy * 4, c.z * 16 + qz * 4, cv, clim);
-        int64_t* t = targets + i * 7;
-        for (int k = 0; k < 6; ++k) t[k] = quantize(clim[k]);
-        t[6] = 0;
+        const int64_t* t = targets + i * 7;
         bool small = M.treef != nullptr;
         double td[7];
         for (int k = 0; k < 7; ++k) {
             small = small && t[k] >= -static_cast<int64_t>(kTreeExact) && t[k] <= static_cast<int64_t>(kTreeExact);
             td[k] = static_cast<double>(t[k]);
         }
-        int leaf;
-        int64_t d = 0;
-        if (small) {
-            double bound = u2d(0x7FF0000000000000ull), dd = 0.0;
-            if (prev >= 0) bound = box_distance_f(M.treef[M.leaf_node[prev]], td) + 1.0;
-            leaf = tree_first_min_f(M.treef, td, bound, dd);
-            if (leaf < 0) leaf = tree_first_min_f(M.treef, td, u2d(0x7FF0000000000000ull), dd);
-            d = static_cast<int64_t>(dd);
-        } else {
-            int64_t bound = INT64_MAX;
-            if (prev >= 0) {
-                const int64_t dp = box_distance(M.tree[M.leaf_node[prev]], t);
-                if (dp < INT64_MAX) bound = dp + 1;
+        if (!small) {
+            if (wide) {
+                int64_t d = 0;
+                lstar[i] = tree_first_min(M.tree, t, INT64_MAX, d);
+                mind[i] = d;
+            } else {
+                lstar[i] = -2;
+                mind[i] = -1;
             }
-            leaf = tree_first_min(M.tree, t, bound, d);
-            if (leaf < 0) leaf = tree_first_min(M.tree, t, INT64_MAX, d);
+            prev = -1;
+            continue;
         }
+        double dd = 0.0;
+        const double bound = prev >= 0 ? box_distance_f(M.treef[M.leaf_node[prev]], td) + 1.0 : inf;
+        int leaf = tree_first_min_f(M.treef, top, ntop, td, bound, dd);
+        if (leaf < 0) leaf = tree_first_min_f(M.treef, top, ntop, td, inf, dd);
         lstar[i] = leaf;
-        mind[i] = d;
+        mind[i] = static_cast<int64_t>(dd);
         prev = leaf;
     }
+}
+
+// The quarts biome_search marked (targets beyond 2^24): int64 search, unbounded.
+MCT_HD inline void biome_search_wide(const ModelDev& M, const int64_t* targets, int32_t* lstar, int64_t* mind, int idx) {
+    if (lstar[idx] != -2) return;
+    int64_t d = 0;
+    lstar[idx] = tree_first_min(M.tree, targets + static_cast<size_t>(idx) * 7, INT64_MAX, d);
+    mind[idx] = d;
 }
 
 // Chain order of doCreateBiomes inside a chunk: 4-quart sections, then x, y, z.
@@ -823,6 +851,8 @@ struct Surf {
     const SeedDev* S;
     int ci, cx, cz, lx, lz;
     uint8_t* blocks;
+    uint8_t* col;
+    int stride;
     int* hm;
     const int* cmap;
     const float* rec;
@@ -934,14 +964,16 @@ struct Surf {
         return A->M->ore_state[k][2];
     }
 
-    MCT_HD MCT_FI size_t bidx(int by) const { return static_cast<size_t>(((by - kMinY) * 16 + lz) * 16 + lx); }
-    MCT_HD MCT_FI int get(int by) const { return (by < kMinY || by >= kMinY + kHeight) ? mctgen::kAir : blocks[bidx(by)]; }
+    // the column's blocks: col[(y - minY) * stride] (the chunk in global memory, stride 256,
+    // or a workgroup's staged slab in LDS)
+    MCT_HD MCT_FI size_t bidx(int by) const { return static_cast<size_t>(by - kMinY) * static_cast<size_t>(stride); }
+    MCT_HD MCT_FI int get(int by) const { return (by < kMinY || by >= kMinY + kHeight) ? mctgen::kAir : col[bidx(by)]; }
     MCT_HD MCT_FI bool air(int s) const { return (A->M->block_flags[s] & kFAir) != 0; }
     MCT_HD MCT_FI bool fluid(int s) const { return (A->M->block_flags[s] & kFFluid) != 0; }
     // setBlockState + Heightmap.update(WORLD_SURFACE_WG); `write` false moves only the height
     MCT_HD MCT_FI void set(int by, int state, bool write) {
         if (by < kMinY || by >= kMinY + kHeight) return;
-        if (write) blocks[bidx(by)] = static_cast<uint8_t>(state);
+        if (write) col[bidx(by)] = static_cast<uint8_t>(state);
         const int first = *hm;
         if (by <= first - 2) return;
         if (!air(state)) {
@@ -964,6 +996,8 @@ MCT_HD MCT_FI void surf_init(Surf& c, const SurfArgs& A, int ci, int lx, int lz,
     c.lx = lx;
     c.lz = lz;
     c.blocks = A.blocks + static_cast<size_t>(ci) * kPer;
+    c.col = c.blocks + lz * 16 + lx;
+    c.stride = 256;
     c.hm = hm;
     c.ci = ci;
     c.rec = A.colrec + (static_cast<size_t>(ci) * 256 + static_cast<size_t>(lx * 16 + lz)) * kColF;
@@ -1098,7 +1132,9 @@ MCT_HD MCT_FI void surface_pass1(const SurfArgs& A, const int* h0, int* h1, int*
 
 // Pass 2 of a column, assuming earlier columns (x-major, z-minor order) end at h2 and later
 // ones are still at h0; the final height goes to hf (checked against h2 afterwards).
-MCT_HD MCT_FI void surface_pass2(const SurfArgs& A, const int* h0, const int* h1, const int* h2, int* hf, int ci, int col) {
+// slab: the column's blocks staged elsewhere (LDS) as slab[(y - minY) * stride], or null.
+MCT_HD MCT_FI void surface_pass2(const SurfArgs& A, const int* h0, const int* h1, const int* h2, int* hf, int ci, int col, uint8_t* slab = nullptr,
+                                 int stride = 0) {
     const int lx = col >> 4, lz = col & 15;
     const int hidx = ci * 256 + col;
     auto H = [&](int ox, int oz) {
@@ -1112,6 +1148,10 @@ MCT_HD MCT_FI void surface_pass2(const SurfArgs& A, const int* h0, const int* h1
     int hm = h1[hidx];
     Surf c;
     surf_init(c, A, ci, lx, lz, &hm);
+    if (slab) {
+        c.col = slab;
+        c.stride = stride;
+    }
     const int starting = h0[hidx];
     const int sb = c.biome_y(starting);
     const int height = H(lx, lz) + 1;
