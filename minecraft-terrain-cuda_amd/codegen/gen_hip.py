@@ -239,28 +239,31 @@ class Gen:
             return name
         P = self.plan
         L = []
+        # Noise stacks are out-of-line calls with the octaves in a loop (not unrolled) over the
+        # layer tables: inlined and unrolled, their octave temporaries pile up on the
+        # evaluators' memoised values and force register spills and occupancy 1.
+        loop = '#pragma clang loop unroll(disable)'
         if kind in ('v', 'sv'):
-            # noise stacks are out-of-line calls: inlined, their octave temporaries pile up on
-            # the evaluators' memoised values and force register spills and occupancy 1
             L.append('MCT_HD MCT_NI float %s(const mct::Octave* O, int32_t bx, int32_t by, int32_t bz, double xz, double ys, float amplitude) {' % name)
             L.append('    float acc = 0.0f;')
-            for i in range(first, first + count):
-                fr = dl(P.freq[i])
-                amp = 'amplitude * %s' % fl(P.amp[i])
-                if kind == 'v':
-                    L.append('    acc = acc + (%s) * mct::perlin_volume(O[%d], bx, by, bz, xz * %s, ys * %s);' % (amp, i, fr, fr))
-                else:
-                    L.append('    acc = acc + (%s) * mct::smeared_volume(O[%d], %s, bx, by, bz, xz * %s, ys * %s);' % (amp, i, dl(P.smear[i]), fr, fr))
+            L.append(loop)
+            L.append('    for (int i = %d; i < %d; ++i) {' % (first, first + count))
+            if kind == 'v':
+                L.append('        acc = acc + (amplitude * kLayerAmp[i]) * mct::perlin_volume(O[i], bx, by, bz, xz * kLayerFreq[i], ys * kLayerFreq[i]);')
+            else:
+                L.append('        acc = acc + (amplitude * kLayerAmp[i]) * mct::smeared_volume(O[i], kLayerSmear[i], bx, by, bz, xz * kLayerFreq[i], ys * kLayerFreq[i]);')
+            L.append('    }')
             L.append('    return acc;')
         else:
             L.append('MCT_HD MCT_NI float %s(const mct::Octave* O, double x, double y, double z) {' % name)
             L.append('    float value = 0.0f;')
-            for i in range(first, first + count):
-                fr = dl(P.freq[i])
-                if kind == 'g':
-                    L.append('    value += %s * mct::perlin_point(O[%d], x * %s, y * %s, z * %s);' % (fl(P.amp[i]), i, fr, fr, fr))
-                else:
-                    L.append('    value += %s * mct::smeared_point(O[%d], %s, x * %s, y * %s, z * %s);' % (fl(P.amp[i]), i, dl(P.smear[i]), fr, fr, fr))
+            L.append(loop)
+            L.append('    for (int i = %d; i < %d; ++i) {' % (first, first + count))
+            if kind == 'g':
+                L.append('        value += kLayerAmp[i] * mct::perlin_point(O[i], x * kLayerFreq[i], y * kLayerFreq[i], z * kLayerFreq[i]);')
+            else:
+                L.append('        value += kLayerAmp[i] * mct::smeared_point(O[i], kLayerSmear[i], x * kLayerFreq[i], y * kLayerFreq[i], z * kLayerFreq[i]);')
+            L.append('    }')
             L.append('    return value;')
         L.append('}')
         self.noise_fns[name] = L
@@ -702,9 +705,9 @@ class Gen:
         o.append('// per layer: kind 0 = NormalNoise (noise key, octave name, fork), 1 = BlendedNoise (node, position in its 40 draws)')
         o.append('struct LayerRecipe { int kind, a, b, c; };')
         o.append('MCT_TABLE LayerRecipe kLayerRecipe[kNumLayers] = {%s};' % ', '.join('{%d, %d, %d, %d}' % r for r in P.recipe))
-        o.append('static const double kLayerSmear[kNumLayers] = {%s};' % ', '.join(dl(s) for s in P.smear))
-        o.append('static const double kLayerFreq[kNumLayers] = {%s};' % ', '.join(dl(s) for s in P.freq))
-        o.append('static const float kLayerAmp[kNumLayers] = {%s};' % ', '.join(fl(s) for s in P.amp))
+        o.append('MCT_TABLE double kLayerSmear[kNumLayers] = {%s};' % ', '.join(dl(s) for s in P.smear))
+        o.append('MCT_TABLE double kLayerFreq[kNumLayers] = {%s};' % ', '.join(dl(s) for s in P.freq))
+        o.append('MCT_TABLE float kLayerAmp[kNumLayers] = {%s};' % ', '.join(fl(s) for s in P.amp))
         o.append('static const int kNoiseFirst[kNumNoises] = {%s};' % ', '.join(str(a) for a, _ in P.noise_range))
         o.append('static const int kNoiseCount[kNumNoises] = {%s};' % ', '.join(str(b) for _, b in P.noise_range))
         md = lambda s: '{%s, %s}' % tuple(u64(v) for v in M.md5_longs(s))
