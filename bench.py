@@ -39,10 +39,10 @@ VALU_PEAK_TOPS = 256 * 64 * 2.4e9 / 1e12
 #   HBM bytes       FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 per thread, the factors from a plain
 #                   1 GiB device-to-device copy measured in the same runs
 WORK_PER_UNIT = {
-    'surface2': {'unit': 'block column', 'valu': 259127.3, 'hbm': 933.14 + 1650.18},
-    'corners': {'unit': 'corner', 'valu': 45118.0, 'hbm': 359.0},   # per corner evaluated (the grid has idle tail threads)
-    'biome_quarts': {'unit': 'run of 8 quarts', 'valu': 132087.4, 'hbm': 5.12 + 550.38},
-    'fill': {'unit': 'block', 'valu': 510.5, 'hbm': 1.72 + 1.00},
+    'surface2': {'unit': 'block column', 'valu': 106741.6, 'hbm': 1036.64 + 392.00},
+    'corners': {'unit': 'corner', 'valu': 50108.0, 'hbm': 31.3},   # per corner evaluated (the grid has idle tail threads)
+    'biome_quarts': {'unit': 'run of 8 quarts', 'valu': 117936.2 + 8 * 149.0, 'hbm': 449.64 + 101.35 + 8 * (1.51 + 56.06)},   # targets + search
+    'fill': {'unit': 'block column of a cell (8 blocks)', 'valu': 1550.2, 'hbm': 12.65 + 8.00},
 }
 
 
@@ -57,7 +57,7 @@ def distinct_columns(chunks):
 def units_per_step(chunks, batch):
     """Threads (units) each main kernel runs over the whole workload, batch by batch (corner
     columns and biome chunks are deduplicated within a batch)."""
-    u = {'surface2': 256 * len(chunks), 'fill': 98304 * len(chunks), 'corners': 0, 'biome_quarts': 0}
+    u = {'surface2': 256 * len(chunks), 'fill': 256 * 48 * len(chunks), 'corners': 0, 'biome_quarts': 0}
     for k in range(0, len(chunks), batch):
         part = chunks[k:k + batch]
         u['corners'] += distinct_columns(part) * 49
@@ -226,6 +226,7 @@ def main():
     ap.add_argument('--quick', action='store_true', help='one 32x32 region instead of the full workload')
     ap.add_argument('--no-cpu', action='store_true')
     ap.add_argument('--no-serve', action='store_true', help='skip the served-request measurement')
+    ap.add_argument('--plain-only', action='store_true', help='only the plain workload (no batched or served legs), e.g. under a profiler')
     ap.add_argument('--copy-calibration', type=int, default=0, metavar='BYTES',
                     help='also run one device-to-device copy of BYTES (the reference point for HBM counters)')
     a = ap.parse_args()
@@ -285,9 +286,9 @@ def main():
     mct.free_device(d_t)
     mct.free_device(d_b)
     g.close()
-    batched = None if a.quick else batched_vs_plain(mct, a.batch)
+    batched = None if (a.quick or a.plain_only) else batched_vs_plain(mct, a.batch)
     serve = None
-    if not a.quick and not a.no_serve:
+    if not a.quick and not a.no_serve and not a.plain_only:
         serve = served(mct, plain_request_capacity(mct))
     if a.copy_calibration:
         copy_calibration(a.copy_calibration)
