@@ -69,12 +69,12 @@ struct TreeNodeDev {
 };
 
 // The same tree with the bounds as floats (exact while every bound is an integer of magnitude
-// at most 2^24), one 64-byte record per node: first child | child count << 24, leaf index.
+// at most 2^23), one 64-byte record per node: first child | child count << 24, leaf index.
 struct TreeNodeF {
     float lo[7], hi[7];
     int32_t first_count, leaf;
 };
-constexpr double kTreeExact = 16777216.0;   // 2^24
+constexpr double kTreeExact = 8388608.0;   // 2^23
 
 // Seed-independent model tables (uploaded once).
 struct ModelDev {
@@ -106,7 +106,7 @@ inline std::vector<TreeNodeF> float_tree(const std::vector<TreeNodeDev>& tree) {
         for (int k = 0; k < 7; ++k) {
             const int64_t lo = tree[i].lo[k], hi = tree[i].hi[k];
             const int64_t lim = static_cast<int64_t>(kTreeExact);
-            if (lo < -lim || lo > lim || hi < -lim || hi > lim) return {};
+            if (lo < -lim || lo > lim || hi < -lim || hi > lim || lo > hi) return {};
             out[i].lo[k] = static_cast<float>(lo);
             out[i].hi[k] = static_cast<float>(hi);
         }
@@ -283,24 +283,24 @@ MCT_HD inline int tree_first_min(const TreeNodeDev* T, const int64_t* t, int64_t
     return closest >= 0 ? T[closest].leaf : -1;
 }
 
-// The search over the float tree with FP64 distances. With every target coordinate and bound
-// an integer of magnitude <= 2^24, each per-axis distance is <= 2^25, its square <= 2^50 and a
-// sum of seven < 2^53: all exact in FP64, so the distances, their order and the leaf found are
-// those of the int64 search above.
-MCT_HD MCT_FI double box_distance_f(const TreeNodeF& n, const double* t) {
+// The search over the float tree. With every target coordinate and bound an integer of
+// magnitude <= 2^23, each per-axis difference is exact in FP32 (<= 2^24), and as lo <= hi at
+// most one of above / below is positive, so max(above, below, 0) is the per-axis distance p;
+// p^2 <= 2^48 and a sum of seven < 2^51 are exact in FP64. The distances, their order and the
+// leaf found are those of the int64 search above.
+MCT_HD MCT_FI double box_distance_f(const TreeNodeF& n, const float* t) {
     double d = 0.0;
     for (int i = 0; i < 7; ++i) {
-        const double above = t[i] - static_cast<double>(n.hi[i]);
-        const double below = static_cast<double>(n.lo[i]) - t[i];
-        const double p = above > 0.0 ? above : (below > 0.0 ? below : 0.0);
-        d = d + p * p;
+        const float p = fmaxf(fmaxf(t[i] - n.hi[i], n.lo[i] - t[i]), 0.0f);
+        const double pd = static_cast<double>(p);
+        d = d + pd * pd;
     }
     return d;
 }
 
 // Nodes below `ntop` are read from `top` (the first nodes of the breadth-first flattening,
 // i.e. the upper levels, staged in LDS by the caller), the rest from T.
-MCT_HD inline int tree_first_min_f(const TreeNodeF* T, const TreeNodeF* top, int ntop, const double* t, double bound, double& out_d) {
+MCT_HD inline int tree_first_min_f(const TreeNodeF* T, const TreeNodeF* top, int ntop, const float* t, double bound, double& out_d) {
     int stack[12];   // node << 5 | next child (at most 19 children per node)
     int sp = 0;
     double best = bound;
@@ -374,10 +374,10 @@ MCT_HD inline void biome_search(const ModelDev& M, const TreeNodeF* top, int nto
         const size_t i = static_cast<size_t>(b) * kQPer + (qy * 4 + qz) * 4 + qx;
         const int64_t* t = targets + i * 7;
         bool small = M.treef != nullptr;
-        double td[7];
+        float td[7];
         for (int k = 0; k < 7; ++k) {
             small = small && t[k] >= -static_cast<int64_t>(kTreeExact) && t[k] <= static_cast<int64_t>(kTreeExact);
-            td[k] = static_cast<double>(t[k]);
+            td[k] = static_cast<float>(t[k]);
         }
         if (!small) {
             if (wide) {
@@ -861,6 +861,7 @@ struct Surf {
     // mx | my << 10 | mz << 20
     int zpx, zpz, zfx, zfz, zc_py;
     uint32_t zc[8];
+    int bc_py, bc_biome;   // quart row whose 8 zoom candidates were checked; their common biome or -1
     float prelim;
     int x, y, z;
     int grad_x, grad_z, surface_depth;
@@ -888,6 +889,17 @@ struct Surf {
     // the two quart rows kept while the walk moves down)
     MCT_HD MCT_FI int biome_y(int by) {
         const int ay = by - 2, py = ay >> 2, fy = ay & 3;
+        // The zoom picks one of the 8 quarts around (x-2, y-2, z-2) >> 2; they depend on the
+        // quart row py only. Where all 8 hold the same biome, that is the answer for the 4
+        // blocks of the row, whichever quart the fiddled distances would pick.
+        if (py != bc_py) {
+            bc_py = py;
+            const int b0 = quart_biome(zpx, py, zpz);
+            bool same = true;
+            for (int i = 1; i < 8; ++i) same = same && quart_biome(zpx + ((i >> 2) & 1), py + ((i >> 1) & 1), zpz + (i & 1)) == b0;
+            bc_biome = same ? b0 : -1;
+        }
+        if (bc_biome >= 0) return bc_biome;
         if (py != zc_py) {
             if (py == zc_py - 1) {
                 for (int k = 0; k < 4; ++k) zc[4 + k] = zc[k];
@@ -1014,6 +1026,8 @@ MCT_HD MCT_FI void surf_init(Surf& c, const SurfArgs& A, int ci, int lx, int lz,
     c.zfx = (c.x - 2) & 3;
     c.zfz = (c.z - 2) & 3;
     c.zc_py = INT32_MIN;
+    c.bc_py = INT32_MIN;
+    c.bc_biome = -1;
 }
 
 MCT_HD MCT_FI void surf_update_xz(Surf& c, int gx, int gz) {
