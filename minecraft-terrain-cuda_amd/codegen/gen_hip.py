@@ -734,6 +734,9 @@ class Gen:
             flags.append((1 if air else 0) | (2 if water else 0) | (4 if lava else 0) | (8 if (water or lava) else 0))
         o.append('constexpr int kPaletteSize = %d;' % len(pal))
         o.append('static const uint8_t kBlockFlags[64] = {%s};' % ', '.join(str(f) for f in flags + [0] * (64 - len(flags))))
+        # the same flags as bit masks over palette indices (palette <= 64): tested in registers
+        masks = [sum(1 << i for i, f in enumerate(flags) if f & b) for b in (1, 2, 4, 8)]
+        o.append('constexpr uint64_t kAirMask = %s, kWaterMask = %s, kLavaMask = %s, kFluidMask = %s;' % tuple(u64(m) for m in masks))
 
         def blk(name):
             if name not in pal:

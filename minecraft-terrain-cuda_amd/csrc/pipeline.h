@@ -749,7 +749,7 @@ MCT_HD MCT_FI void fill_cell(const FillArgs& A, int c, int t, float* density_out
 // Initial WORLD_SURFACE_WG height of a column (highest non-air block + 1, or minY).
 MCT_HD MCT_FI int column_height(const ModelDev& M, const uint8_t* blocks, int x, int z) {
     for (int yy = kHeight - 1; yy >= 0; --yy)
-        if (!(M.block_flags[blocks[(yy * 16 + z) * 16 + x]] & kFAir)) return kMinY + yy + 1;
+        if (!((mctgen::kAirMask >> blocks[(yy * 16 + z) * 16 + x]) & 1u)) return kMinY + yy + 1;
     return kMinY;
 }
 
@@ -980,8 +980,8 @@ struct Surf {
     // or a workgroup's staged slab in LDS)
     MCT_HD MCT_FI size_t bidx(int by) const { return static_cast<size_t>(by - kMinY) * static_cast<size_t>(stride); }
     MCT_HD MCT_FI int get(int by) const { return (by < kMinY || by >= kMinY + kHeight) ? mctgen::kAir : col[bidx(by)]; }
-    MCT_HD MCT_FI bool air(int s) const { return (A->M->block_flags[s] & kFAir) != 0; }
-    MCT_HD MCT_FI bool fluid(int s) const { return (A->M->block_flags[s] & kFFluid) != 0; }
+    MCT_HD MCT_FI bool air(int s) const { return ((mctgen::kAirMask >> s) & 1u) != 0; }
+    MCT_HD MCT_FI bool fluid(int s) const { return ((mctgen::kFluidMask >> s) & 1u) != 0; }
     // setBlockState + Heightmap.update(WORLD_SURFACE_WG); `write` false moves only the height
     MCT_HD MCT_FI void set(int by, int state, bool write) {
         if (by < kMinY || by >= kMinY + kHeight) return;
@@ -1054,7 +1054,7 @@ MCT_HD MCT_FI void eroded_badlands(Surf& c, int height) {
     for (int yy = start_y; yy >= kMinY; --yy) {
         const int old = c.get(yy);
         if (old == mctgen::kDefaultBlock) break;
-        if (c.A->M->block_flags[old] & kFWater) return;
+        if ((mctgen::kWaterMask >> old) & 1u) return;
     }
     for (int yy = start_y; yy >= kMinY && c.air(c.get(yy)); --yy) c.set(yy, mctgen::kDefaultBlock, true);
 }
