@@ -93,6 +93,19 @@ int mct_profile(mct_generator* g, int enable);
 int mct_kernel_times(mct_generator* g, double* ms, int64_t* launches, int cap, int* n);
 const char* mct_kernel_name(int i);
 
+/* Biome-only queries (no counterpart in the reference API): for block i at xyz[3i..3i+2] under
+ * seeds[i], the biome index the TERRAIN step stores for that block (BiomeManager zoom, then the
+ * quart's biome as left by the chunk's chain of climate R-tree searches), without generating
+ * chunks. biomes: count bytes. |x|, |z| <= 2^30 (MCT_EINVAL beyond); any y (quart y clamps to
+ * the world height). MCT_ECAPACITY if the model's climate bounds or a query's climate targets
+ * exceed the float tree's exact range (2^23). */
+int mct_biome_query(mct_generator* g, const int64_t* seeds, const int32_t* xyz, int count, uint8_t* biomes);
+
+/* zoom_selftest (terrain_gpu.h:57-59): the column-cached zoom that the surface pass uses against
+ * the plain FP64 BiomeManager zoom, on about `count` blocks (whole columns at pseudo-random
+ * x, z in +-29,999,984, every y). out: blocks checked, mismatches, quart-row refills. */
+int mct_zoom_selftest(int64_t world_seed, int64_t count, int64_t out[3]);
+
 /* ---- Serving (no counterpart in the reference, which serves one synchronous call at a time) ----
  * A server owns a generator with two pipeline slots, the per-seed tables of up to
  * seed_capacity seeds, and a result arena in HBM sized by hbm_budget_bytes (the budget minus

@@ -699,6 +699,11 @@ class Gen:
         o.append('constexpr int kNumLayers = %d, kNumNoises = %d, kNumBlended = %d;' % (P.count, len(P.keys), len(P.blended_ids)))
         o.append('constexpr int kNumChannels = %d, kNumOreChannels = %d, kNumColumns = %d, kNumClimateColumns = %d;' % (
             len(self.chans), len(self.ore_chans), len(self.col_a), len(self.col_b)))
+        # first interpolation channel the ore-vein evaluators read (channels below it are
+        # not interpolated for ore veins)
+        allch = self.chans + self.ore_chans
+        used = [allch.index(x) for x in self.reaches.get('OreBlock', []) if x in allch]
+        o.append('constexpr int kOreChanFirst = %d;' % (min(used) if used else len(allch)))
         o.append('constexpr int kNumOreRules = %d;' % len(self.ore_rules))
         o.append('constexpr int kFtsLowerBound = %d, kFtsCellHeight = %d;' % (self.fts['lower_bound'], self.fts['cell_height']))
         o.append('static const char* const kNoiseKeys[kNumNoises] = {%s};' % ', '.join('"%s"' % k for k in P.keys))

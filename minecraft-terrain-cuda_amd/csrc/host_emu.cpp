@@ -137,6 +137,36 @@ int mcte_seed_tables(void* p, int64_t seed, double* offsets, uint8_t* perms, int
     return mctgen::kNumLayers;
 }
 
+// Biome-only queries for n blocks (x, y, z triples) under one seed; -1 entries where the float
+// search cannot serve the query.
+int mcte_biome_query(void* p, int64_t seed, const int32_t* xyz, int n, int32_t* out) {
+    try {
+        Emu& e = *static_cast<Emu*>(p);
+        build_seed(e, seed);
+        for (int i = 0; i < n; ++i) out[i] = biome_query(*e.seed, e.M, xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2]);
+        return 0;
+    } catch (const std::exception& ex) {
+        g_err = ex.what();
+        return 1;
+    }
+}
+
+// The column-cached zoom against zoom_pick on n columns (x, z pairs), every y: mismatches.
+int64_t mcte_zoom_check(int64_t zoom_seed, const int32_t* xz, int n) {
+    int64_t bad = 0;
+    for (int i = 0; i < n; ++i) {
+        ZoomColumn zc;
+        zc.init(zoom_seed, xz[2 * i], xz[2 * i + 1]);
+        for (int y = kMinY + kHeight - 1; y >= kMinY; --y) {
+            int a, b, c, qa, qb, qc;
+            zc.pick(y, a, b, c);
+            zoom_pick(zoom_seed, xz[2 * i], y, xz[2 * i + 1], qa, qb, qc);
+            bad += (a != qa || b != qb || c != qc) ? 1 : 0;
+        }
+    }
+    return bad;
+}
+
 // Full pipeline for one seed and n chunks. Outputs (nullable): terrain and aquifer fill u8 per
 // block, biomes u8 per quart, density f32 per block (canonical layouts, chunks back to back).
 // stats[0] = chunks redone in exact order after the speculative surface pass.

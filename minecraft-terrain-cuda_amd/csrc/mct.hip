@@ -296,6 +296,41 @@ __global__ __launch_bounds__(256) void k_gather_biomes(int n, const int* nb, con
     out[idx] = bb[static_cast<size_t>(nb[c * 9 + 4]) * kQPer + q];
 }
 
+// Biome-only queries: one block per thread (seed slot, x, y, z); 255 where the float search
+// cannot serve it (counted in *unserved).
+__global__ __launch_bounds__(256) void k_biome_query(const SeedDev* seeds, const int32_t* slot, const int32_t* xyz, int n, uint8_t* out, int* unserved) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int b = biome_query(seeds[slot[i]], c_model, xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2]);
+    if (b < 0) atomicAdd(unserved, 1);
+    out[i] = static_cast<uint8_t>(b < 0 ? 255 : b);
+}
+
+// The column-cached zoom of the surface pass against zoom_pick: one pseudo-random column per
+// thread (x, z uniform in +-29,999,984), every y. out: checked, mismatches, row refills.
+__global__ __launch_bounds__(256) void k_zoom_selftest(int64_t zoom_seed, uint64_t salt, int columns, unsigned long long* out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= columns) return;
+    uint64_t h = salt + 0x9E3779B97F4A7C15ull * static_cast<uint64_t>(i + 1);
+    h = (h ^ (h >> 30)) * 0xBF58476D1CE4E5B9ull;
+    h = (h ^ (h >> 27)) * 0x94D049BB133111EBull;
+    h ^= h >> 31;
+    const int x = static_cast<int>((h & 0xFFFFFFFFull) % 59999969ull) - 29999984;
+    const int z = static_cast<int>((h >> 32) % 59999969ull) - 29999984;
+    ZoomColumn zc;
+    zc.init(zoom_seed, x, z);
+    unsigned long long bad = 0;
+    for (int y = kMinY + kHeight - 1; y >= kMinY; --y) {
+        int a, b, c, qa, qb, qc;
+        zc.pick(y, a, b, c);
+        zoom_pick(zoom_seed, x, y, z, qa, qb, qc);
+        bad += (a != qa || b != qb || c != qc) ? 1u : 0u;
+    }
+    atomicAdd(out, static_cast<unsigned long long>(kHeight));
+    if (bad) atomicAdd(out + 1, bad);
+    atomicAdd(out + 2, static_cast<unsigned long long>(zc.refills));
+}
+
 __global__ __launch_bounds__(256) void k_copy16(size_t n16, const uint4* src, uint4* dst) {
     const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i < n16) dst[i] = src[i];
@@ -747,6 +782,8 @@ struct Generator {
     ~Generator() {
         (void)hipDeviceSynchronize();
         slots.clear();
+        void* q[] = {d_qxyz, d_qslot, d_qbad, d_qout};
+        for (void* x : q) (void)hipFree(x);
         void* p[] = {d_tree, d_treef, d_leaf_biome, d_leaf_node, d_blended_first, d_seeds, d_jobs, d_stats};
         for (void* q : p) (void)hipFree(q);
         (void)hipHostFree(h_jobs);
@@ -933,6 +970,49 @@ struct Generator {
         S.pending = true;
     }
 
+    // Biome-only queries, in groups of at most kQueryGroup blocks and seed_cap distinct seeds.
+    static constexpr int kQueryGroup = 1 << 20;
+    int32_t *d_qxyz = nullptr, *d_qslot = nullptr, *d_qbad = nullptr;
+    uint8_t* d_qout = nullptr;
+    void biome_query(const int64_t* seeds, const int32_t* xyz, int count, uint8_t* out) {
+        if (!d_treef) throw Error(MCT_ECAPACITY, "biome queries need the float climate tree (a bound of this model exceeds 2^23)");
+        collect_all(nullptr);
+        Slot& S = *slots[0];
+        if (!d_qxyz) {
+            hip_check(hipMalloc(&d_qxyz, sizeof(int32_t) * 3 * kQueryGroup), "malloc");
+            hip_check(hipMalloc(&d_qslot, sizeof(int32_t) * kQueryGroup), "malloc");
+            hip_check(hipMalloc(&d_qout, kQueryGroup), "malloc");
+            hip_check(hipMalloc(&d_qbad, sizeof(int)), "malloc");
+        }
+        std::vector<int32_t> slot;
+        for (int off = 0; off < count;) {
+            int n = std::min(kQueryGroup, count - off);
+            std::vector<int64_t> distinct;
+            {
+                std::unordered_map<int64_t, int> seen;
+                for (int i = 0; i < n; ++i)
+                    if (seen.emplace(seeds[off + i], 1).second) {
+                        if (static_cast<int>(distinct.size()) == seed_cap) { n = i; break; }
+                        distinct.push_back(seeds[off + i]);
+                    }
+            }
+            ensure_seeds(distinct, S.stream);
+            slot.resize(static_cast<size_t>(n));
+            for (int i = 0; i < n; ++i) slot[i] = resident.at(seeds[off + i]);
+            hip_check(hipMemcpyAsync(d_qxyz, xyz + 3 * static_cast<size_t>(off), sizeof(int32_t) * 3 * n, hipMemcpyHostToDevice, S.stream), "upload");
+            hip_check(hipMemcpyAsync(d_qslot, slot.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice, S.stream), "upload");
+            hip_check(hipMemsetAsync(d_qbad, 0, sizeof(int), S.stream), "memset");
+            k_biome_query<<<grid_of(n, 256), 256, 0, S.stream>>>(d_seeds, d_qslot, d_qxyz, n, d_qout, d_qbad);
+            hip_check(hipGetLastError(), "query kernel");
+            int bad = 0;
+            hip_check(hipMemcpyAsync(out + off, d_qout, n, hipMemcpyDeviceToHost, S.stream), "download");
+            hip_check(hipMemcpyAsync(&bad, d_qbad, sizeof(int), hipMemcpyDeviceToHost, S.stream), "download");
+            hip_check(hipStreamSynchronize(S.stream), "sync");
+            if (bad) throw Error(MCT_ECAPACITY, std::to_string(bad) + " queries have climate targets beyond 2^23 (not served by the float tree)");
+            off += n;
+        }
+    }
+
     void generate(const int64_t* seeds, const int32_t* xz, int count, uint8_t* terrain, uint8_t* biomes, uint8_t* aquifer, mct_timing* tm,
                   bool to_device) {
         const auto w0 = std::chrono::steady_clock::now();
@@ -1049,6 +1129,40 @@ int mct_create(const char* ir_json, const char* terrain_json, int max_batch_chun
 
 int mct_destroy(mct_generator* g) {
     return guard([&] { delete g; });
+}
+
+static constexpr int64_t kQueryRange = int64_t{1} << 30;
+
+int mct_biome_query(mct_generator* g, const int64_t* seeds, const int32_t* xyz, int count, uint8_t* biomes) {
+    return guard([&] {
+        if (!g || count < 0 || (count > 0 && (!seeds || !xyz || !biomes))) throw Error(MCT_EINVAL, "bad arguments");
+        for (int i = 0; i < count; ++i)
+            if (std::abs(static_cast<int64_t>(xyz[3 * i])) > kQueryRange || std::abs(static_cast<int64_t>(xyz[3 * i + 2])) > kQueryRange)
+                throw Error(MCT_EINVAL, "query " + std::to_string(i) + ": |x| or |z| beyond 2^30");
+        g->g->biome_query(seeds, xyz, count, biomes);
+    });
+}
+
+int mct_zoom_selftest(int64_t world_seed, int64_t count, int64_t out[3]) {
+    return guard([&] {
+        if (!out || count < 0) throw Error(MCT_EINVAL, "bad arguments");
+        const int64_t zoom_seed = static_cast<int64_t>(sha256_long_le(world_seed));
+        unsigned long long* d = nullptr;
+        hip_check(hipMalloc(&d, sizeof(unsigned long long) * 3), "malloc");
+        hip_check(hipMemset(d, 0, sizeof(unsigned long long) * 3), "memset");
+        const int64_t columns = (count + kHeight - 1) / kHeight;
+        for (int64_t c0 = 0; c0 < columns; c0 += (1 << 24)) {
+            const int nc = static_cast<int>(std::min<int64_t>(1 << 24, columns - c0));
+            k_zoom_selftest<<<grid_of(nc, 256), 256>>>(zoom_seed, 0x5A4F4F4Dull + static_cast<uint64_t>(c0), nc, d);
+            hip_check(hipGetLastError(), "zoom selftest kernel");
+        }
+        unsigned long long h[3];
+        hip_check(hipMemcpy(h, d, sizeof h, hipMemcpyDeviceToHost), "download");
+        (void)hipFree(d);
+        out[0] = static_cast<int64_t>(h[0]);
+        out[1] = static_cast<int64_t>(h[1]);
+        out[2] = static_cast<int64_t>(h[2]);
+    });
 }
 
 int mct_server_create(const char* ir_json, const char* terrain_json, int max_batch_chunks, int seed_capacity, size_t hbm_budget_bytes, mct_server** out) {

@@ -300,7 +300,8 @@ MCT_HD MCT_FI double box_distance_f(const TreeNodeF& n, const float* t) {
 
 // Nodes below `ntop` are read from `top` (the first nodes of the breadth-first flattening,
 // i.e. the upper levels, staged in LDS by the caller), the rest from T.
-MCT_HD inline int tree_first_min_f(const TreeNodeF* T, const TreeNodeF* top, int ntop, const float* t, double bound, double& out_d) {
+MCT_HD inline int tree_first_min_f(const TreeNodeF* T, const TreeNodeF* top, int ntop, const float* t, double bound, double& out_d,
+                                    int exclude = -1) {
     int stack[12];   // node << 5 | next child (at most 19 children per node)
     int sp = 0;
     double best = bound;
@@ -324,8 +325,10 @@ MCT_HD inline int tree_first_min_f(const TreeNodeF* T, const TreeNodeF* top, int
         const double cd = box_distance_f(cn, t);
         if (best > cd) {
             if (cn.leaf >= 0) {
-                best = cd;
-                closest = c;
+                if (c != exclude) {
+                    best = cd;
+                    closest = c;
+                }
             } else if (sp < 12) {
                 stack[sp++] = c << 5;
             }
@@ -413,6 +416,76 @@ MCT_HD inline void biome_search_wide(const ModelDev& M, const int64_t* targets, 
 MCT_HD MCT_FI int chain_quart(int p) {
     const int z = p & 3, y = (p >> 2) & 3, x = (p >> 4) & 3, sy = p >> 6;
     return ((sy * 4 + y) * 4 + z) * 4 + x;
+}
+
+// ---- biome-only queries -------------------------------------------------------------------
+// What the TERRAIN step stores for block (x, y, z): the zoom picks a quart (y clamped to the
+// chunk's quart rows), and the answer is that quart's biome as the chunk's chain of R-tree
+// searches leaves it: result(p) = result(p-1) when it is also at the minimal distance at p,
+// else L*_p, with result(-1) = none. The query walks the chain back from p to the nearest
+// position whose minimum is unique (or the chain start), where the result cannot depend on
+// the history, then replays the rule forward.
+struct QuartSearch {
+    float t[7];
+    int leaf;        // L*
+    double d;        // its distance
+    bool unique;     // no other leaf at distance d
+};
+
+// Position p of the chain order of chunk (cx, cz): climate target, L*, uniqueness.
+MCT_HD inline bool quart_search(const SeedDev& S, const ModelDev& M, int cx, int cz, int p, QuartSearch& out) {
+    const int qi = chain_quart(p);
+    const int qx = qi & 3, qz = (qi >> 2) & 3, qy = qi >> 4;
+    float cv[mctgen::kNumClimateColumns > 0 ? mctgen::kNumClimateColumns : 1];
+    mctgen::eval_climate_columns(S.oct, cx * 16 + qx * 4, cz * 16 + qz * 4, cv);
+    float clim[6];
+    mctgen::eval_climate(S.oct, cx * 16 + qx * 4, kMinY + qy * 4, cz * 16 + qz * 4, cv, clim);
+    for (int k = 0; k < 7; ++k) {
+        const int64_t v = k < 6 ? quantize(clim[k]) : 0;
+        if (!M.treef || v < -static_cast<int64_t>(kTreeExact) || v > static_cast<int64_t>(kTreeExact)) return false;
+        out.t[k] = static_cast<float>(v);
+    }
+    const double inf = u2d(0x7FF0000000000000ull);
+    out.leaf = tree_first_min_f(M.treef, nullptr, 0, out.t, inf, out.d);
+    double d2 = 0.0;
+    const int other = tree_first_min_f(M.treef, nullptr, 0, out.t, out.d + 1.0, d2, M.leaf_node[out.leaf]);
+    out.unique = other < 0;
+    return true;
+}
+
+constexpr int kQueryWalk = 16;
+
+// Biome stored for block (x, y, z) under seed table S; -1 when the float search cannot be used
+// (the caller then generates the chunk's biomes the ordinary way).
+MCT_HD inline int biome_query(const SeedDev& S, const ModelDev& M, int32_t x, int32_t y, int32_t z) {
+    int32_t qx, qy, qz;
+    zoom_pick(S.zoom_seed, x, y, z, qx, qy, qz);
+    const int qmin = kMinY >> 2;
+    qy = clampi(qy, qmin, qmin + kQH - 1) - qmin;
+    const int cx = qx >> 2, cz = qz >> 2;
+    const int p = ((qy >> 2) << 6) | ((qx & 3) << 4) | ((qy & 3) << 2) | (qz & 3);   // chain position
+    QuartSearch w[kQueryWalk];
+    int n = 0, anchor = -1;
+    for (int k = p; k >= 0 && n < kQueryWalk; --k, ++n) {
+        if (!quart_search(S, M, cx, cz, k, w[n])) return -1;
+        if (w[n].unique || k == 0) { anchor = n; break; }
+    }
+    int r;
+    if (anchor >= 0) {
+        r = w[anchor].leaf;
+        for (int j = anchor - 1; j >= 0; --j) {
+            const double dp = box_distance_f(M.treef[M.leaf_node[r]], w[j].t);
+            if (!(dp == w[j].d)) r = w[j].leaf;
+        }
+    } else {   // no anchor within the walk: replay the chain from its start
+        r = -1;
+        QuartSearch q;
+        for (int k = 0; k <= p; ++k) {
+            if (!quart_search(S, M, cx, cz, k, q)) return -1;
+            r = (r >= 0 && box_distance_f(M.treef[M.leaf_node[r]], q.t) == q.d) ? r : q.leaf;
+        }
+    }
+    return M.leaf_biome[r];
 }
 
 // tie[p]: the previous chain position's L* differs from L* at p and is also at minimal
@@ -831,6 +904,65 @@ MCT_HD inline void surface_column(const SeedDev* seeds, const ModelDev& M, const
     r[kColPatch] = frozen_patch(M, x, z) ? 1.0f : 0.0f;
 }
 
+// ---- zoom down one block column -------------------------------------------------------------
+// BiomeManager.getBiome for the blocks of one column (fixed x, z): the fiddle indices of the
+// 8 corner quarts depend on the quart row only, so the column keeps those of the two rows the
+// block lies between (4 corners each, packed mx | my << 10 | mz << 20) and recomputes the LCG
+// chains of 4 corners when the walk moves down one quart row. The fiddled distances are the
+// plain FP64 ones of zoom_pick, in the same order.
+struct ZoomColumn {
+    int64_t seed;
+    int px, pz, fx, fz, py = INT32_MIN;
+    uint32_t zc[8];
+    int refills = 0;
+
+    MCT_HD MCT_FI void init(int64_t zoom_seed, int x, int z) {
+        seed = zoom_seed;
+        px = (x - 2) >> 2;
+        pz = (z - 2) >> 2;
+        fx = (x - 2) & 3;
+        fz = (z - 2) & 3;
+        py = INT32_MIN;
+    }
+    MCT_HD MCT_FI void layer(int cy, uint32_t* L) const {
+        for (int k = 0; k < 4; ++k) {
+            int mx, my, mz;
+            corner_fiddles(seed, px + (k >> 1), cy, pz + (k & 1), mx, my, mz);
+            L[k] = static_cast<uint32_t>(mx) | (static_cast<uint32_t>(my) << 10) | (static_cast<uint32_t>(mz) << 20);
+        }
+    }
+    // the quart picked for block y of the column (quart x, y, z)
+    MCT_HD MCT_FI void pick(int y, int& qx, int& qy, int& qz) {
+        const int ay = y - 2, ry = ay >> 2, fy = ay & 3;
+        if (ry != py) {
+            if (ry == py - 1) {
+                for (int k = 0; k < 4; ++k) zc[4 + k] = zc[k];
+                layer(ry, zc);
+            } else {
+                layer(ry, zc);
+                layer(ry + 1, zc + 4);
+            }
+            py = ry;
+            ++refills;
+        }
+        int best = 0;
+        double best_d = u2d(0x7FF0000000000000ull);
+        for (int i = 0; i < 8; ++i) {
+            const bool xf = (i & 4) != 0, yf = (i & 2) != 0, zf = (i & 1) != 0;
+            const uint32_t pk = zc[(yf ? 4 : 0) + (xf ? 2 : 0) + (zf ? 1 : 0)];
+            const int mx = static_cast<int>(pk & 1023u), my = static_cast<int>((pk >> 10) & 1023u), mz = static_cast<int>(pk >> 20);
+            const double d = (zoom_sq(fz, zf, mz) + zoom_sq(fy, yf, my)) + zoom_sq(fx, xf, mx);
+            if (best_d > d) {
+                best = i;
+                best_d = d;
+            }
+        }
+        qx = (best & 4) ? px + 1 : px;
+        qy = (best & 2) ? ry + 1 : ry;
+        qz = (best & 1) ? pz + 1 : pz;
+    }
+};
+
 // ---- material system: one column ----------------------------------------------------------
 struct SurfArgs {
     const SeedDev* seeds;
@@ -856,11 +988,7 @@ struct Surf {
     int* hm;
     const int* cmap;
     const float* rec;
-    // zoom: the column's fixed x / z parts and the fiddle indices of the corner quarts at
-    // quart rows zc_py (zc[0..3]) and zc_py + 1 (zc[4..7]), corner k = xf * 2 + zf, packed
-    // mx | my << 10 | mz << 20
-    int zpx, zpz, zfx, zfz, zc_py;
-    uint32_t zc[8];
+    ZoomColumn zoom;
     int bc_py, bc_biome;   // quart row whose 8 zoom candidates were checked; their common biome or -1
     float prelim;
     int x, y, z;
@@ -878,51 +1006,23 @@ struct Surf {
         const size_t bc = static_cast<size_t>(A->nb_index[ci * 9 + (dz + 1) * 3 + dx + 1]);
         return A->bbiomes[bc * kQPer + (cqy * 4 + (qz & 3)) * 4 + (qx & 3)];
     }
-    MCT_HD MCT_FI void zoom_layer(int cy, uint32_t* L) const {
-        for (int k = 0; k < 4; ++k) {
-            int mx, my, mz;
-            corner_fiddles(S->zoom_seed, zpx + (k >> 1), cy, zpz + (k & 1), mx, my, mz);
-            L[k] = static_cast<uint32_t>(mx) | (static_cast<uint32_t>(my) << 10) | (static_cast<uint32_t>(mz) << 20);
-        }
-    }
-    // BiomeManager.getBiome at (x, by, z) of this column (zoom_pick with the corner fiddles of
-    // the two quart rows kept while the walk moves down)
+    // BiomeManager.getBiome at (x, by, z) of this column. The zoom picks one of the 8 quarts
+    // around (x-2, y-2, z-2) >> 2, which depend on the quart row only: where all 8 hold the
+    // same biome, that is the answer for the 4 blocks of the row, whichever quart the fiddled
+    // distances would pick.
     MCT_HD MCT_FI int biome_y(int by) {
-        const int ay = by - 2, py = ay >> 2, fy = ay & 3;
-        // The zoom picks one of the 8 quarts around (x-2, y-2, z-2) >> 2; they depend on the
-        // quart row py only. Where all 8 hold the same biome, that is the answer for the 4
-        // blocks of the row, whichever quart the fiddled distances would pick.
+        const int py = (by - 2) >> 2;
         if (py != bc_py) {
             bc_py = py;
-            const int b0 = quart_biome(zpx, py, zpz);
+            const int b0 = quart_biome(zoom.px, py, zoom.pz);
             bool same = true;
-            for (int i = 1; i < 8; ++i) same = same && quart_biome(zpx + ((i >> 2) & 1), py + ((i >> 1) & 1), zpz + (i & 1)) == b0;
+            for (int i = 1; i < 8; ++i) same = same && quart_biome(zoom.px + ((i >> 2) & 1), py + ((i >> 1) & 1), zoom.pz + (i & 1)) == b0;
             bc_biome = same ? b0 : -1;
         }
         if (bc_biome >= 0) return bc_biome;
-        if (py != zc_py) {
-            if (py == zc_py - 1) {
-                for (int k = 0; k < 4; ++k) zc[4 + k] = zc[k];
-                zoom_layer(py, zc);
-            } else {
-                zoom_layer(py, zc);
-                zoom_layer(py + 1, zc + 4);
-            }
-            zc_py = py;
-        }
-        int best = 0;
-        double best_d = u2d(0x7FF0000000000000ull);
-        for (int i = 0; i < 8; ++i) {
-            const bool xf = (i & 4) != 0, yf = (i & 2) != 0, zf = (i & 1) != 0;
-            const uint32_t pk = zc[(yf ? 4 : 0) + (xf ? 2 : 0) + (zf ? 1 : 0)];
-            const int mx = static_cast<int>(pk & 1023u), my = static_cast<int>((pk >> 10) & 1023u), mz = static_cast<int>(pk >> 20);
-            const double d = (zoom_sq(zfz, zf, mz) + zoom_sq(fy, yf, my)) + zoom_sq(zfx, xf, mx);
-            if (best_d > d) {
-                best = i;
-                best_d = d;
-            }
-        }
-        return quart_biome((best & 4) ? zpx + 1 : zpx, (best & 2) ? py + 1 : py, (best & 1) ? zpz + 1 : zpz);
+        int qx, qy, qz;
+        zoom.pick(by, qx, qy, qz);
+        return quart_biome(qx, qy, qz);
     }
     MCT_HD MCT_FI int biome() {
         if (!biome_set) { biome_v = biome_y(y); biome_set = true; }
@@ -963,7 +1063,8 @@ struct Surf {
     }
     MCT_HD MCT_FI int ore_vein(int k) {
         if (!ore_set) {
-            interp_channels(A->corners, cmap, lx, y - kMinY, lz, 0, kCh, ore_ch);
+            // the ore evaluators read channels from kOreChanFirst on
+            interp_channels(A->corners, cmap, lx, y - kMinY, lz, mctgen::kOreChanFirst, kCh - mctgen::kOreChanFirst, ore_ch + mctgen::kOreChanFirst);
             ore_set = true;
         }
         const float density = mctgen::eval_ore(O, x, y, z, ore_ch, 2 * k);
@@ -1021,11 +1122,7 @@ MCT_HD MCT_FI void surf_init(Surf& c, const SurfArgs& A, int ci, int lx, int lz,
     c.x = cr.x * 16 + lx;
     c.z = cr.z * 16 + lz;
     c.y = 0;
-    c.zpx = (c.x - 2) >> 2;
-    c.zpz = (c.z - 2) >> 2;
-    c.zfx = (c.x - 2) & 3;
-    c.zfz = (c.z - 2) & 3;
-    c.zc_py = INT32_MIN;
+    c.zoom.init(c.S->zoom_seed, c.x, c.z);
     c.bc_py = INT32_MIN;
     c.bc_biome = -1;
 }

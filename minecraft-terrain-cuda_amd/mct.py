@@ -62,6 +62,8 @@ def lib():
         L.mct_kernel_times.argtypes = [P, P, P, I, ctypes.POINTER(I)]
         L.mct_kernel_name.argtypes = [I]
         L.mct_kernel_name.restype = ctypes.c_char_p
+        L.mct_biome_query.argtypes = [P, P, P, I, P]
+        L.mct_zoom_selftest.argtypes = [I64, I64, P]
         L.mct_server_create.argtypes = [ctypes.c_char_p, ctypes.c_char_p, I, I, SZ, ctypes.POINTER(P)]
         L.mct_server_submit.argtypes = [P, I64, P, I, P, P, ctypes.POINTER(I64)]
         L.mct_server_poll.argtypes = [P, P, P, I, ctypes.POINTER(I)]
@@ -136,6 +138,14 @@ class FullTerrainGenerator:
         s = np.ascontiguousarray(np.asarray(seeds, dtype=np.int64))
         _check(lib().mct_generate_device(self.h, _ptr(s), _ptr(xz), len(xz), d_terrain, d_biomes, ctypes.byref(timing) if timing is not None else None))
 
+    def biome_query(self, seeds, xyz):
+        """Biome stored by the TERRAIN step for each block (x, y, z) under its seed."""
+        p = np.ascontiguousarray(np.asarray(xyz, dtype=np.int32).reshape(-1, 3))
+        s = np.ascontiguousarray(np.broadcast_to(np.asarray(seeds, dtype=np.int64), (len(p),)))
+        out = np.empty(len(p), np.uint8)
+        _check(lib().mct_biome_query(self.h, _ptr(s), _ptr(p), len(p), _ptr(out)))
+        return out
+
     def profile(self, enable=True):
         _check(lib().mct_profile(self.h, 1 if enable else 0))
 
@@ -156,6 +166,13 @@ class FullTerrainGenerator:
         buf = np.zeros(1 << 16, np.float64)
         _check(lib().mct_last_batch_done_ms(self.h, _ptr(buf), len(buf), ctypes.byref(n)))
         return buf[:min(n.value, len(buf))].copy()
+
+
+def zoom_selftest(world_seed, count):
+    """(checked, mismatches, refills) of the column-cached zoom against the plain FP64 zoom."""
+    out = np.zeros(3, np.int64)
+    _check(lib().mct_zoom_selftest(int(world_seed), int(count), _ptr(out)))
+    return tuple(int(v) for v in out)
 
 
 def alloc_device(nbytes):

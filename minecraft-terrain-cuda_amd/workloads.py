@@ -11,6 +11,8 @@ inputs everywhere.
   requests           served stream: Poisson arrivals, seed from a 64-seed pool, W x H region
                      (W, H uniform 1..32) in spiral order around a random player chunk
   stress             1,024 seeds, 2^20 chunks uniform over +-117,187 (wrap chunks included)
+  queries            biome-only block queries: seed from the stress seeds, x / z uniform over
+                     the blocks of +-117,187 chunks, y uniform in -64..319
 """
 import math
 
@@ -123,13 +125,45 @@ def requests(rate_per_s, duration_s, pool=None, max_side=32):
         out.append((t, seed, spiral(px, pz, w, h)))
 
 
-def stress(n_seeds=1024, n_chunks=1 << 20):
-    """(seeds, chunk_seed_index, chunks): uniform chunks over +-117,187 with the wrap chunks
-    at the front, seed index uniform over the seed list."""
+def stress_seeds(n_seeds=1024):
     r = SplitMix64(STRESS_STREAM)
-    seeds = [r.signed() for _ in range(n_seeds)]
-    chunks = list(WRAP_CHUNKS)
-    while len(chunks) < n_chunks:
-        chunks.append((r.between(-CHUNK_RANGE, CHUNK_RANGE), r.between(-CHUNK_RANGE, CHUNK_RANGE)))
-    which = [r.below(n_seeds) for _ in range(n_chunks)]
-    return seeds, which, chunks
+    return [r.signed() for _ in range(n_seeds)]
+
+
+def splitmix_array(start, n):
+    """Outputs 1..n of SplitMix64 from `start`, vectorised (numpy uint64 wraps mod 2^64)."""
+    import numpy as np
+    with np.errstate(over='ignore'):
+        z = np.uint64(start & MASK) + np.arange(1, n + 1, dtype=np.uint64) * np.uint64(0x9E3779B97F4A7C15)
+        z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+        z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+        return z ^ (z >> np.uint64(31))
+
+
+def stress(n_seeds=1024, n_chunks=1 << 20):
+    """(seeds, chunk_seed_index, chunks) as numpy arrays: uniform chunks over +-117,187 with
+    the wrap chunks at the front, seed index uniform over the seed list."""
+    import numpy as np
+    seeds = np.array(stress_seeds(n_seeds), np.int64)
+    v = splitmix_array(STRESS_STREAM ^ 0x43484E4B, n_chunks)
+    span = np.uint64(2 * CHUNK_RANGE + 1)
+    xz = np.empty((n_chunks, 2), np.int32)
+    xz[:, 0] = ((v & np.uint64(0xFFFFFF)) % span).astype(np.int64) - CHUNK_RANGE
+    xz[:, 1] = (((v >> np.uint64(24)) & np.uint64(0xFFFFFF)) % span).astype(np.int64) - CHUNK_RANGE
+    which = ((v >> np.uint64(48)) % np.uint64(n_seeds)).astype(np.int32)
+    xz[:len(WRAP_CHUNKS)] = WRAP_CHUNKS
+    which[:len(WRAP_CHUNKS)] = 0
+    return seeds, which, xz
+
+
+def queries(n, seeds):
+    """n biome-only block queries: (seed per query, xyz int32 [n, 3])."""
+    import numpy as np
+    v = splitmix_array(STRESS_STREAM ^ 0x51554552, 2 * n).reshape(n, 2)
+    lo, span = -CHUNK_RANGE * 16, np.uint64((2 * CHUNK_RANGE + 1) * 16)
+    out = np.empty((n, 3), np.int32)
+    out[:, 0] = ((v[:, 0] & np.uint64(0xFFFFFFFF)) % span).astype(np.int64) + lo
+    out[:, 1] = ((v[:, 1] & np.uint64(0xFFFF)) % np.uint64(384)).astype(np.int64) - 64
+    out[:, 2] = ((v[:, 0] >> np.uint64(32)) % span).astype(np.int64) + lo
+    s = np.asarray(seeds, np.int64)[((v[:, 1] >> np.uint64(16)) % np.uint64(len(seeds))).astype(np.int64)]
+    return s, out

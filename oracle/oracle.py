@@ -120,6 +120,26 @@ class Oracle:
         self._check(lib().orc_generate(self.h, int(seed), xz.ctypes.data, n, int(threads), b.ctypes.data, a.ctypes.data, t.ctypes.data, _ptr(d)))
         return (t, b, a, d) if density else (t, b, a)
 
+    def biome_query(self, seeds, xyz):
+        """The biome the TERRAIN step stores for each block: BiomeManager's zoom picks the quart
+        (BiomeManager.getBiome), quart y clamps to the world height (Climate / noiseBiome), and
+        the answer is that quart's entry in its chunk's biome array."""
+        p = np.ascontiguousarray(np.asarray(xyz, dtype=np.int32).reshape(-1, 3))
+        s = np.broadcast_to(np.asarray(seeds, dtype=np.int64), (len(p),))
+        out = np.zeros(len(p), np.uint8)
+        cache = {}
+        for i in range(len(p)):
+            seed = int(s[i])
+            q = np.zeros((1, 3), np.int32)
+            lib().orc_zoom(lib().orc_zoom_seed(seed), p[i:i + 1].ctypes.data, 1, q.ctypes.data)
+            qx, qy, qz = (int(v) for v in q[0])
+            qy = min(max(qy, -16), -16 + HEIGHT // 4 - 1) + 16
+            key = (seed, qx >> 2, qz >> 2)
+            if key not in cache:
+                cache[key] = self.generate(seed, [key[1:]], threads=1)[1][0]
+            out[i] = cache[key][qy, qz & 3, qx & 3]
+        return out
+
     def root(self, name):
         r = lib().orc_root(self.h, name.encode())
         if r < 0:

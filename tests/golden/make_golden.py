@@ -3,7 +3,8 @@ its own sources by `make -C oracle ref` (oracle/_ref/libmcref.so). Runs in the b
 container only (the reference sources are not on the GPU machines); the fixtures it writes are
 data: inputs and the reference's outputs, on the synthetic overworld IR and terrain model.
 
-  python tests/golden/make_golden.py            fixtures.npz (+ plain_digests.npz with --plain)
+  python tests/golden/make_golden.py            fixtures.npz (+ plain_digests.npz with --plain,
+                                                stress_golden.npz with --stress)
 
 fixtures.npz holds, per part of the path:
   layers_<seed>_<key>   per-seed noise layers {frequency, amplitude, ox, oy, oz, smear} (f64),
@@ -17,6 +18,10 @@ fixtures.npz holds, per part of the path:
   c1_*                  the 8 x 8 region of the CPU-runnable configuration (digests)
 plain_digests.npz: 8-byte digests of terrain (u8) and biomes of every chunk of the plain
 three-region workload (12,296 chunks), region mode, for the full-size GPU parity check.
+stress_golden.npz: a 1,536-chunk sample of the 1,024-seed, 2^20-chunk stress list (its wrap
+chunks + evenly spaced indices; digests of terrain and biomes, one chunk per reference call) and
+the answers of the first 4,096 biome-only block queries (the biome the reference stores for the
+block's zoomed quart, quart y clamped to the world height).
 """
 import hashlib
 import json
@@ -172,5 +177,57 @@ def plain(R):
     print('plain digests: %.1fs' % (time.time() - t0), flush=True)
 
 
+def stress():
+    import threading
+    t0 = time.time()
+    seeds, which, xz = workloads.stress()
+    n = 1536
+    idx = np.unique(np.concatenate([np.arange(len(workloads.WRAP_CHUNKS)), np.linspace(0, len(xz) - 1, n - len(workloads.WRAP_CHUNKS)).astype(np.int64)]))
+    dt = np.zeros((len(idx), 8), np.uint8)
+    db = np.zeros((len(idx), 8), np.uint8)
+    qseed, qxyz = workloads.queries(4096, seeds)
+    qout = np.zeros(len(qxyz), np.uint8)
+    lock = threading.Lock()
+    jobs = [('c', k) for k in range(len(idx))] + [('q', k) for k in range(0, len(qxyz), 64)]
+    pos = [0]
+
+    def worker():
+        R = oracle.Reference()
+        zoom_seeds = {}
+        while True:
+            with lock:
+                if pos[0] >= len(jobs):
+                    return
+                kind, k = jobs[pos[0]]
+                pos[0] += 1
+            if kind == 'c':
+                i = idx[k]
+                t, b, _ = R.generate(int(seeds[which[i]]), [tuple(xz[i])])
+                dt[k], db[k] = digest(t[0].astype(np.uint8)), digest(b[0])
+                continue
+            for j in range(k, min(k + 64, len(qxyz))):
+                s = int(qseed[j])
+                if s not in zoom_seeds:
+                    zoom_seeds[s] = R.seed_misc(s)[0]
+                qx, qy, qz = R.zoom(zoom_seeds[s], qxyz[j:j + 1])[0]
+                qy = min(max(int(qy), -16), -16 + 95) + 16
+                b = R.chunk_biomes(s, int(qx) >> 2, int(qz) >> 2)
+                qout[j] = b[(qy * 4 + (int(qz) & 3)) * 4 + (int(qx) & 3)]
+
+    ths = [threading.Thread(target=worker) for _ in range(8)]
+    for th in ths:
+        th.start()
+    for th in ths:
+        th.join()
+    np.savez_compressed(os.path.join(HERE, 'stress_golden.npz'), index=idx, seed=seeds[which[idx]], xz=xz[idx], terrain=dt, biomes=db,
+                        query_seed=qseed, query_xyz=qxyz, query_biome=qout)
+    print('stress sample + queries: %.1fs' % (time.time() - t0), flush=True)
+
+
 if __name__ == '__main__':
-    main()
+    if '--stress' in sys.argv and '--plain' not in sys.argv and '--all' not in sys.argv:
+        stress()
+    else:
+        main()
+        if '--all' in sys.argv:
+            stress()

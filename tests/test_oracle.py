@@ -265,6 +265,30 @@ def test_c1_region_matches_reference(orc, golden):
         assert (digest(b[i]) == golden['c1_biomes'][i]).all(), i
 
 
+def stress_golden():
+    with np.load(os.path.join(GOLDEN, 'stress_golden.npz'), allow_pickle=False) as z:
+        return {k: z[k] for k in z.files}
+
+
+def test_stress_sample_matches_reference(orc):
+    """The wrap chunks and an even sample of the 1,024-seed stress list (one seed per chunk)."""
+    g = stress_golden()
+    seeds, which, xz = workloads.stress()
+    assert (seeds[which[g['index']]] == g['seed']).all() and (xz[g['index']] == g['xz']).all()
+    for k in list(range(4)) + list(range(4, len(g['index']), 97)):
+        t, b, _ = orc.generate(int(g['seed'][k]), [tuple(g['xz'][k])], threads=1)
+        assert (digest(t[0].astype(np.uint8)) == g['terrain'][k]).all(), k
+        assert (digest(b[0]) == g['biomes'][k]).all(), k
+
+
+def test_biome_queries_match_reference(orc):
+    g = stress_golden()
+    qs, q = workloads.queries(len(g['query_xyz']), workloads.stress_seeds())
+    assert (qs == g['query_seed']).all() and (q == g['query_xyz']).all()
+    sel = np.arange(0, len(q), 128)
+    assert (orc.biome_query(qs[sel], q[sel]) == g['query_biome'][sel]).all()
+
+
 def test_outputs_fit_the_u8_layout(orc):
     t, b, _ = orc.generate(0, [(3, -4)])
     import json
