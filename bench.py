@@ -8,6 +8,8 @@ is chunks/s of that step on one GPU. Alongside it the line carries:
 
   e2e_host_chunks_per_s   the same chunks with results copied to pinned host memory
   batched                 mixed-seed batch (256 uniform seeds, uneven chunk lists), chunks/s
+  served                  Poisson request streams: p50 / p99 latency and chunks/s per rate
+  stress                  1,024 seeds x 2^20 scattered chunks, chunks/s; 2^24 biome queries/s
   roofline                the dominant kernel's VALU issue rate against the MI355X VALU peak
   cpu_baseline            the reference cpu/ port (built from its sources) on the host cores
   stages_ms / kernels_ms  per-stage and per-kernel times of one step (HIP events)
@@ -175,6 +177,36 @@ def served(mct, plain_rate, seconds=4.0, fractions=(0.25, 0.5, 0.8, 3.0), max_ba
             'batches': st['batches'], 'mixed_batches': st['mixed_batches'], 'max_batch': max_batch}
 
 
+def stress(mct, batch, window=1 << 16, n_queries=1 << 24):
+    """The mixed stress workload: 2^20 chunks uniform over +-117,187 (wrap chunks included) of
+    1,024 seeds, streamed through a window of HBM output buffers (the whole output, ~105 GB,
+    is not kept); then 2^24 biome-only block queries (climate sampling, R-tree search chain
+    and zoom per query) from host arrays, answers back in host memory."""
+    import workloads
+    seeds, which, xz = workloads.stress()
+    cs = seeds[which]
+    n = len(xz)
+    d_t, d_b = mct.alloc_device(window * mct.BLOCKS_PER_CHUNK), mct.alloc_device(window * mct.QUARTS_PER_CHUNK)
+    g = mct.FullTerrainGenerator(max_batch_chunks=batch, pipeline_slots=2, seed_capacity=1024)
+    g.generate_device(cs[:window], xz[:window], d_t, d_b)   # every seed's tables resident, buffers sized
+    t0 = time.perf_counter()
+    for off in range(0, n, window):
+        g.generate_device(cs[off:off + window], xz[off:off + window], d_t, d_b)
+    tc = time.perf_counter() - t0
+    mct.free_device(d_t)
+    mct.free_device(d_b)
+    qs, q = workloads.queries(n_queries, seeds)
+    g.biome_query(qs[:1 << 16], q[:1 << 16])
+    t0 = time.perf_counter()
+    g.biome_query(qs, q)
+    tq = time.perf_counter() - t0
+    g.close()
+    return {'workload': '1,024 seeds, 2^20 chunks uniform over +-117,187 (wrap chunks first), %d-chunk HBM output window; '
+                        '2^24 biome-only block queries (x, z uniform over the same blocks, y uniform -64..319)' % window,
+            'chunks': n, 'chunks_per_s': round(n / tc, 1), 'seconds': round(tc, 3),
+            'queries': n_queries, 'queries_per_s': round(n_queries / tq, 1), 'query_seconds': round(tq, 3)}
+
+
 def plain_request_capacity(mct, n_requests=200):
     """Chunks/s of the plain generator on the served request mix: one request per call,
     results copied to pinned host memory (what a server without batching does)."""
@@ -226,6 +258,7 @@ def main():
     ap.add_argument('--quick', action='store_true', help='one 32x32 region instead of the full workload')
     ap.add_argument('--no-cpu', action='store_true')
     ap.add_argument('--no-serve', action='store_true', help='skip the served-request measurement')
+    ap.add_argument('--no-stress', action='store_true', help='skip the 1,024-seed stress leg')
     ap.add_argument('--plain-only', action='store_true', help='only the plain workload (no batched or served legs), e.g. under a profiler')
     ap.add_argument('--copy-calibration', type=int, default=0, metavar='BYTES',
                     help='also run one device-to-device copy of BYTES (the reference point for HBM counters)')
@@ -290,6 +323,7 @@ def main():
     serve = None
     if not a.quick and not a.no_serve and not a.plain_only:
         serve = served(mct, plain_request_capacity(mct))
+    mixed = None if (a.quick or a.plain_only or a.no_stress) else stress(mct, a.batch)
     if a.copy_calibration:
         copy_calibration(a.copy_calibration)
     cpu = None if a.no_cpu else cpu_baseline()
@@ -299,7 +333,7 @@ def main():
             'config': {'workload': 'plain TERRAIN, one seed (%d), %s, full height, outputs in HBM' % (
                 SEED, '32x32 region' if a.quick else '3 x 64x64-chunk regions (origin, (-2048,-2048), border chunk %d) + 8 wrap/border chunks' % BORDER_CHUNK),
                 'chunks': n, 'batch': a.batch, 'slots': a.slots},
-            'e2e_host_chunks_per_s': round(e2e, 1), 'batched': batched, 'served': serve, 'roofline': roof, 'cpu_baseline': cpu,
+            'e2e_host_chunks_per_s': round(e2e, 1), 'batched': batched, 'served': serve, 'stress': mixed, 'roofline': roof, 'cpu_baseline': cpu,
             'stages_ms': stages, 'kernels_ms': kernels}
     print(json.dumps(line), flush=True)
 

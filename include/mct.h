@@ -86,6 +86,13 @@ void* mct_alloc_device(size_t bytes);
 void mct_free_device(void* p);
 int mct_device_to_host(void* dst, const void* src, size_t bytes);
 
+/* Per-chunk 64-bit digests of count chunks of bytes_per_chunk bytes each at d_data (device
+ * memory, e.g. the terrain or biome output of mct_generate_device), computed on the device; no
+ * reference counterpart. digest = mix(sum over 8-byte little-endian words w_i, i = 0.., of
+ * mix(w_i + (i + 1) * 0x9E3779B97F4A7C15) mod 2^64, xor bytes_per_chunk), mix = SplitMix64's
+ * finaliser. bytes_per_chunk: a positive multiple of 8. digests: count values, host memory. */
+int mct_chunk_digests(const void* d_data, int64_t count, int64_t bytes_per_chunk, uint64_t* digests);
+
 /* Per-kernel timing (HIP events around each kernel group on its stream; adds a little
  * overhead, off by default). enable != 0 turns it on and clears the totals. */
 int mct_profile(mct_generator* g, int enable);

@@ -331,6 +331,29 @@ __global__ __launch_bounds__(256) void k_zoom_selftest(int64_t zoom_seed, uint64
     atomicAdd(out + 2, static_cast<unsigned long long>(zc.refills));
 }
 
+// Per-chunk 64-bit digest of output bytes in HBM, one workgroup per chunk: the sum (mod 2^64)
+// over 8-byte words w_i of mix(w_i + (i + 1) * golden), finalised with the byte count. Used to
+// compare whole runs without copying them to the host.
+__device__ inline uint64_t digest_mix(uint64_t z) {
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+__global__ __launch_bounds__(256) void k_chunk_digest(const uint64_t* data, int64_t words_per_chunk, uint64_t* out) {
+    const uint64_t* p = data + static_cast<size_t>(blockIdx.x) * static_cast<size_t>(words_per_chunk);
+    uint64_t h = 0;
+    for (int64_t i = threadIdx.x; i < words_per_chunk; i += 256) h += digest_mix(p[i] + 0x9E3779B97F4A7C15ull * static_cast<uint64_t>(i + 1));
+    __shared__ uint64_t part[256];
+    part[threadIdx.x] = h;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (static_cast<int>(threadIdx.x) < w) part[threadIdx.x] += part[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[blockIdx.x] = digest_mix(part[0] ^ static_cast<uint64_t>(words_per_chunk * 8));
+}
+
 __global__ __launch_bounds__(256) void k_copy16(size_t n16, const uint4* src, uint4* dst) {
     const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i < n16) dst[i] = src[i];
@@ -1289,6 +1312,30 @@ void mct_free_device(void* p) { (void)hipFree(p); }
 
 int mct_device_to_host(void* dst, const void* src, size_t bytes) {
     return guard([&] { hip_check(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost), "device to host"); });
+}
+
+int mct_chunk_digests(const void* d_data, int64_t count, int64_t bytes_per_chunk, uint64_t* digests) {
+    return guard([&] {
+        if (count < 0 || (count > 0 && (!d_data || !digests)) || bytes_per_chunk <= 0 || bytes_per_chunk % 8 != 0 ||
+            reinterpret_cast<uintptr_t>(d_data) % 8 != 0)
+            throw Error(MCT_EINVAL, "bad arguments (bytes_per_chunk must be a positive multiple of 8, data 8-byte aligned)");
+        if (count == 0) return;
+        constexpr int64_t kGroup = int64_t{1} << 20;
+        uint64_t* d = nullptr;
+        hip_check(hipMalloc(&d, sizeof(uint64_t) * static_cast<size_t>(std::min(count, kGroup))), "malloc");
+        try {
+            for (int64_t c0 = 0; c0 < count; c0 += kGroup) {
+                const int64_t nc = std::min(kGroup, count - c0);
+                k_chunk_digest<<<static_cast<unsigned>(nc), 256>>>(static_cast<const uint64_t*>(d_data) + c0 * (bytes_per_chunk / 8), bytes_per_chunk / 8, d);
+                hip_check(hipGetLastError(), "digest kernel");
+                hip_check(hipMemcpy(digests + c0, d, sizeof(uint64_t) * static_cast<size_t>(nc), hipMemcpyDeviceToHost), "download");
+            }
+        } catch (...) {
+            (void)hipFree(d);
+            throw;
+        }
+        hip_check(hipFree(d), "free");
+    });
 }
 
 int mct_profile(mct_generator* g, int enable) {

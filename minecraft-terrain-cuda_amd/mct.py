@@ -57,6 +57,7 @@ def lib():
         L.mct_alloc_device.restype = P
         L.mct_free_device.argtypes = [P]
         L.mct_device_to_host.argtypes = [P, P, SZ]
+        L.mct_chunk_digests.argtypes = [P, I64, I64, P]
         L.mct_info.argtypes = [P]
         L.mct_profile.argtypes = [P, I]
         L.mct_kernel_times.argtypes = [P, P, P, I, ctypes.POINTER(I)]
@@ -136,6 +137,8 @@ class FullTerrainGenerator:
         """Outputs stay in device memory (pointers from alloc_device)."""
         xz = _xz(chunks)
         s = np.ascontiguousarray(np.asarray(seeds, dtype=np.int64))
+        if len(s) != len(xz):
+            raise ValueError('one seed per chunk')
         _check(lib().mct_generate_device(self.h, _ptr(s), _ptr(xz), len(xz), d_terrain, d_biomes, ctypes.byref(timing) if timing is not None else None))
 
     def biome_query(self, seeds, xyz):
@@ -190,6 +193,27 @@ def device_to_host(p, nbytes):
     out = np.empty(nbytes, np.uint8)
     _check(lib().mct_device_to_host(_ptr(out), p, nbytes))
     return out
+
+
+def chunk_digests(p, count, bytes_per_chunk):
+    """Per-chunk 64-bit digests (uint64 array) of count chunks of device memory at p."""
+    out = np.zeros(int(count), np.uint64)
+    _check(lib().mct_chunk_digests(p, int(count), int(bytes_per_chunk), _ptr(out)))
+    return out
+
+
+def digest_host(chunks):
+    """The same digest computed on the host (numpy), for arrays of whole chunks (n, ...)."""
+    a = np.ascontiguousarray(chunks).reshape(len(chunks), -1)
+    w = a.view('<u8')
+    with np.errstate(over='ignore'):
+        def mix(z):
+            z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+            z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+            return z ^ (z >> np.uint64(31))
+        i = np.arange(1, w.shape[1] + 1, dtype=np.uint64) * np.uint64(0x9E3779B97F4A7C15)
+        h = mix(w + i[None, :]).sum(axis=1, dtype=np.uint64)
+        return mix(h ^ np.uint64(a.shape[1]))
 
 
 def info():
