@@ -131,6 +131,8 @@ class Gen:
         self.subs = {}
         self.splines = set()
         self.noise_fns = {}
+        self.corner_sites = []    # per octave sample site of the corner pass: (layer, xz scale)
+        self.corner_site_base = {}
         R = self.ir.roots
         N = self.ir.nodes
         self.final = R['final_density']
@@ -243,29 +245,110 @@ class Gen:
         # layer tables: inlined and unrolled, their octave temporaries pile up on the
         # evaluators' memoised values and force register spills and occupancy 1.
         loop = '#pragma clang loop unroll(disable)'
+        # On the device the octave table is read through a buffer resource of the wave's first
+        # lane's table; lanes of another seed's table take the next pass of the loop (one pass
+        # when the wave shares a seed).
         if kind in ('v', 'sv'):
             L.append('MCT_HD MCT_NI float %s(const mct::Octave* O, int32_t bx, int32_t by, int32_t bz, double xz, double ys, float amplitude) {' % name)
             L.append('    float acc = 0.0f;')
-            L.append(loop)
-            L.append('    for (int i = %d; i < %d; ++i) {' % (first, first + count))
             if kind == 'v':
-                L.append('        acc = acc + (amplitude * kLayerAmp[i]) * mct::perlin_volume(O[i], bx, by, bz, xz * kLayerFreq[i], ys * kLayerFreq[i]);')
+                term = 'acc = acc + (amplitude * kLayerAmp[i]) * mct::perlin_volume(T[i], %s, bx, by, bz, xz * kLayerFreq[i], ys * kLayerFreq[i]);'
             else:
-                L.append('        acc = acc + (amplitude * kLayerAmp[i]) * mct::smeared_volume(O[i], kLayerSmear[i], bx, by, bz, xz * kLayerFreq[i], ys * kLayerFreq[i]);')
-            L.append('    }')
-            L.append('    return acc;')
+                term = 'acc = acc + (amplitude * kLayerAmp[i]) * mct::smeared_volume(T[i], %s, kLayerSmear[i], bx, by, bz, xz * kLayerFreq[i], ys * kLayerFreq[i]);'
+            ret = 'acc'
         else:
             L.append('MCT_HD MCT_NI float %s(const mct::Octave* O, double x, double y, double z) {' % name)
             L.append('    float value = 0.0f;')
-            L.append(loop)
-            L.append('    for (int i = %d; i < %d; ++i) {' % (first, first + count))
             if kind == 'g':
-                L.append('        value += kLayerAmp[i] * mct::perlin_point(O[i], x * kLayerFreq[i], y * kLayerFreq[i], z * kLayerFreq[i]);')
+                term = 'value += kLayerAmp[i] * mct::perlin_point(T[i], %s, x * kLayerFreq[i], y * kLayerFreq[i], z * kLayerFreq[i]);'
             else:
-                L.append('        value += kLayerAmp[i] * mct::smeared_point(O[i], kLayerSmear[i], x * kLayerFreq[i], y * kLayerFreq[i], z * kLayerFreq[i]);')
-            L.append('    }')
-            L.append('    return value;')
+                term = 'value += kLayerAmp[i] * mct::smeared_point(T[i], %s, kLayerSmear[i], x * kLayerFreq[i], y * kLayerFreq[i], z * kLayerFreq[i]);'
+            ret = 'value'
+        L.append('#if defined(__HIP_DEVICE_COMPILE__)')
+        L.append('    for (;;) {')
+        L.append('        const mct::Octave* T = mct::wave_first(O);')
+        L.append('        if (T == O) {')
+        L.append('            const __amdgpu_buffer_rsrc_t R = mct::octave_rsrc(T, kNumLayers);')
+        L.append(loop)
+        L.append('            for (int i = %d; i < %d; ++i) {' % (first, first + count))
+        L.append('                ' + term % 'mct::PermBuf(R, i)')
+        L.append('            }')
+        L.append('            break;')
+        L.append('        }')
+        L.append('    }')
+        L.append('#else')
+        L.append('    const mct::Octave* T = O;')
+        L.append(loop)
+        L.append('    for (int i = %d; i < %d; ++i) {' % (first, first + count))
+        L.append('        ' + term % 'mct::PermRef{T[i].p}')
+        L.append('    }')
+        L.append('#endif')
+        L.append('    return %s;' % ret)
         L.append('}')
+        self.noise_fns[name] = L
+        return name
+
+    def corner_site(self, kind, key, xz):
+        """First site of a (noise stack, xz scale) call of the corner pass; its sites are one per
+        octave layer, whose x / z records the pass tabulates per block column."""
+        if isinstance(key, tuple):
+            bi, setname = key
+            first, count = self.plan.blended[bi][setname]
+        else:
+            first, count = self.plan.noise_range[self.plan.key_index[key]]
+        ck = (kind, key if not isinstance(key, tuple) else tuple(key), struct.pack('<d', xz))
+        if ck not in self.corner_site_base:
+            self.corner_site_base[ck] = len(self.corner_sites)
+            self.corner_sites += [(first + j, xz) for j in range(count)]
+        return self.corner_site_base[ck]
+
+    def noise_fn_tab(self, kind, key):
+        """Volume stack of the corner pass: x / z records from the column's LDS table (one per
+        layer of the stack, in layer order), y computed here. kind: 'v' plain, 'sv' smeared."""
+        name = {'v': 'nvc', 'sv': 'bvc'}[kind]
+        if isinstance(key, tuple):
+            bi, setname = key
+            first, count = self.plan.blended[bi][setname]
+            name += '_%d_%s' % (bi, setname)
+        else:
+            k = self.plan.key_index[key]
+            first, count = self.plan.noise_range[k]
+            name += '_%d' % k
+        if name in self.noise_fns:
+            return name
+        L = ['MCT_HD MCT_NI float %s(const mct::Octave* O, const MCT_LDS mct::AxisXZ* A, int32_t by, double ys, float amplitude) {' % name,
+             '    float acc = 0.0f;']
+        if kind == 'v':
+            yax = 'const mct::Axis Y = mct::axis_plain(by, ys * kLayerFreq[i], T[i].oy);'
+        else:
+            yax = 'const mct::Axis Y = mct::axis_smeared(by, ys * kLayerFreq[i], T[i].oy, kLayerSmear[i]);'
+        term = 'acc = acc + (amplitude * kLayerAmp[i]) * mct::volume_sample(%s, a.X, Y, a.Z);'
+        loop = '#pragma clang loop unroll(disable)'
+        L += ['#if defined(__HIP_DEVICE_COMPILE__)',
+              '    for (;;) {',
+              '        const mct::Octave* T = mct::wave_first(O);',
+              '        if (T == O) {',
+              '            const __amdgpu_buffer_rsrc_t R = mct::octave_rsrc(T, kNumLayers);',
+              loop,
+              '            for (int i = %d; i < %d; ++i) {' % (first, first + count),
+              '                const mct::AxisXZ a = A[i - %d];' % first,
+              '                ' + yax,
+              '                ' + term % 'mct::PermBuf(R, i)',
+              '            }',
+              '            break;',
+              '        }',
+              '    }',
+              '#else',
+              '    const mct::Octave* T = O;',
+              loop,
+              '    for (int i = %d; i < %d; ++i) {' % (first, first + count),
+              '        const mct::AxisXZ a = A[i - %d];' % first,
+              '        ' + yax,
+              '        ' + term % 'mct::PermRef{T[i].p}',
+              '    }',
+              '#endif',
+              '    return acc;',
+              '}']
         self.noise_fns[name] = L
         return name
 
@@ -382,7 +465,10 @@ class Gen:
             xz, ys = dl(f64h(n['xz_scale'])), dl(f64h(n['y_scale']))
             key = n['noise']
             if n['shift_x_zero'] and n['shift_y_zero'] and n['shift_z_zero']:
-                if vol:
+                if vol and ctx.name == 'Corner':
+                    site = self.corner_site('v', key, f64h(n['xz_scale']))
+                    L.append('const float v = %s(O, axz + %d, y, %s, 1.0f);' % (self.noise_fn_tab('v', key), site, ys))
+                elif vol:
                     L.append('const float v = %s(O, x, y, z, %s, %s, 1.0f);' % (self.noise_fn('v', key), xz, ys))
                 else:
                     L.append('const float v = %s(O, static_cast<double>(x) * %s, static_cast<double>(y) * %s, static_cast<double>(z) * %s);'
@@ -415,7 +501,11 @@ class Gen:
             ym = 684.412 * f64h(n['y_scale'])
             mxz = xzm / f64h(n['xz_factor'])
             my = ym / f64h(n['y_factor'])
-            if vol:
+            if vol and ctx.name == 'Corner':
+                def s(setname, a, b):
+                    site = self.corner_site('sv', (bi, setname), a)
+                    return '%s(O, axz + %d, y, %s, 1.0f)' % (self.noise_fn_tab('sv', (bi, setname)), site, dl(b))
+            elif vol:
                 def s(setname, a, b):
                     return '%s(O, x, y, z, %s, %s, 1.0f)' % (self.noise_fn('sv', (bi, setname)), dl(a), dl(b))
             else:
@@ -800,7 +890,8 @@ class Gen:
         main = []
         main += self.struct(Ctx('Columns', 'vol', 'ident', 'err'), [N[c]['in'] for c in self.col_a])
         main += self.struct(Ctx('ClimateColumns', 'vol', 'ident', 'err'), [N[c]['in'] for c in self.col_b])
-        main += self.struct(Ctx('Corner', 'vol', 'col', 'pass', cols=self.col_a), [N[c]['in'] for c in self.chans + self.ore_chans])
+        main += self.struct(Ctx('Corner', 'vol', 'col', 'pass', cols=self.col_a), [N[c]['in'] for c in self.chans + self.ore_chans],
+                            extra='const MCT_LDS mct::AxisXZ* axz;')
         main += self.struct(Ctx('Block', 'vol', 'subst', 'chan', chans=self.chans), [self.final])
         main += self.struct(Ctx('OreBlock', 'vol', 'subst', 'chan', chans=self.chans + self.ore_chans), self.ore_roots)
         main += self.struct(Ctx('Quart', 'vol', 'col', 'err', cols=self.col_b), self.climate)
@@ -822,7 +913,17 @@ class Gen:
               ['out[%d] = e.n%d();' % (k, N[c]['in']) for k, c in enumerate(self.col_a)])
         entry('void eval_climate_columns(const mct::Octave* O, int32_t x, int32_t z, float* out)', 'ClimateColumns e{O, x, 0, z};',
               ['out[%d] = e.n%d();' % (k, N[c]['in']) for k, c in enumerate(self.col_b)])
-        entry('void eval_corner(const mct::Octave* O, int32_t x, int32_t y, int32_t z, const float* col, float* out)', 'Corner e{O, x, y, z, col};',
+        # x / z records of the corner pass's octave sample sites, per block column
+        S = self.corner_sites
+        e.append('constexpr int kCornerSites = %d;' % len(S))
+        e.append('MCT_TABLE int32_t kCornerSiteLayer[%d] = {%s};' % (max(1, len(S)), ', '.join(str(l) for l, _ in S) or '0'))
+        e.append('MCT_TABLE double kCornerSiteScale[%d] = {%s};' % (max(1, len(S)), ', '.join(dl(v) for _, v in S) or '0.0'))
+        entry('mct::AxisXZ corner_axes(const mct::Octave* O, int32_t x, int32_t z, int site)',
+              'const int i = kCornerSiteLayer[site];', [
+                  'const double xs = kCornerSiteScale[site] * kLayerFreq[i];',
+                  'return mct::AxisXZ{mct::axis_plain(x, xs, O[i].ox), mct::axis_plain(z, xs, O[i].oz)};'])
+        entry('void eval_corner(const mct::Octave* O, int32_t x, int32_t y, int32_t z, const float* col, const MCT_LDS mct::AxisXZ* axz, float* out)',
+              'Corner e{O, x, y, z, col, axz};',
               ['out[%d] = e.n%d();' % (k, N[c]['in']) for k, c in enumerate(self.chans + self.ore_chans)])
         entry('float eval_block(const mct::Octave* O, int32_t x, int32_t y, int32_t z, const float* ch)', 'Block e{O, x, y, z, ch};',
               ['return e.n%d();' % self.final])
@@ -865,7 +966,9 @@ class Gen:
         dbg = ['// GENERATED debug dispatchers (bisection against the interpreter); not part of the library.', '#pragma once',
                '#include "evaluators.h"', 'namespace mctgen {']
         dbg.append('inline float debug_corner_node(const mct::Octave* O, int32_t x, int32_t y, int32_t z, const float* col, int id) {')
-        dbg.append('    Corner e{O, x, y, z, col};')
+        dbg.append('    mct::AxisXZ axz[kCornerSites > 0 ? kCornerSites : 1];')
+        dbg.append('    for (int k = 0; k < kCornerSites; ++k) axz[k] = corner_axes(O, x, z, k);')
+        dbg.append('    Corner e{O, x, y, z, col, axz};')
         dbg.append('    switch (id) {')
         dbg += ['    case %d: return e.n%d();' % (i, i) for i in self.reaches['Corner']]
         dbg.append('    }')

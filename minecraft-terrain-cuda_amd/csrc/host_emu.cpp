@@ -216,7 +216,11 @@ int mcte_generate(void* p, int64_t seed, const int32_t* xz, int n, uint8_t* terr
         std::vector<float> colv(static_cast<size_t>(U) * (mctgen::kNumColumns > 0 ? mctgen::kNumColumns : 1));
         std::vector<float> corners(static_cast<size_t>(U) * kCh * kRows);
         for (int u = 0; u < U; ++u) column_values(KC, seeds, ucols.data(), colv.data(), u);
-        for (int i = 0; i < U * kRows; ++i) corner_values(KC, seeds, ucols.data(), colv.data(), corners.data(), i);
+        std::vector<mct::AxisXZ> tab(static_cast<size_t>(mctgen::kCornerSites > 0 ? mctgen::kCornerSites : 1));
+        for (int u = 0; u < U; ++u) {
+            for (int k = 0; k < mctgen::kCornerSites; ++k) tab[k] = corner_site_axes(KC, seeds, ucols.data(), u, k);
+            for (int row = 0; row < kRows; ++row) corner_values(KC, seeds, ucols.data(), colv.data(), corners.data(), u, row, tab.data());
+        }
         // surface-level grid and aquifer
         std::vector<uint64_t> gkeys(static_cast<size_t>(n) * kGridKeys), ug;
         std::vector<int> gmap;

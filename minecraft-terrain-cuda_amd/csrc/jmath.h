@@ -19,6 +19,14 @@
 #define MCT_NI __attribute__((noinline))
 #endif
 
+// Pointers into LDS (workgroup memory) on the device, so that loads through them are ds_read
+// even across out-of-line calls; plain pointers on the host.
+#if defined(__HIP_DEVICE_COMPILE__)
+#define MCT_LDS __attribute__((address_space(3)))
+#else
+#define MCT_LDS
+#endif
+
 namespace mct {
 
 MCT_HD MCT_FI uint32_t f2u(float f) { uint32_t u; __builtin_memcpy(&u, &f, 4); return u; }

@@ -167,10 +167,27 @@ __global__ __launch_bounds__(256) void k_columns(KeyCodec K, const SeedDev* seed
     if (u < upper && u < *count) column_values(K, seeds, ucols, colv, u);
 }
 
+// Corner pass: each workgroup takes kCornerCols distinct columns x kRows rows (245 of 256
+// threads). The x / z records of every octave sample site of its columns are computed once
+// into LDS (the FP64 coordinate setup of two of the three axes, shared by the 49 rows), then
+// each thread evaluates its corner with only the y axis computed per sample.
+constexpr int kCornerCols = 256 / kRows;
+constexpr int kCornerTab = mctgen::kCornerSites > 0 ? mctgen::kCornerSites : 1;
 __global__ __launch_bounds__(256) void k_corners(KeyCodec K, const SeedDev* seeds, int upper, const uint64_t* ucols, const int* count, const float* colv,
                                                  float* corners) {
-    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx < upper * kRows && idx / kRows < *count) corner_values(K, seeds, ucols, colv, corners, idx);
+    __shared__ mct::AxisXZ tab[kCornerCols * kCornerTab];
+    const int c0 = blockIdx.x * kCornerCols;
+    const int ncol = min(min(kCornerCols, upper - c0), *count - c0);   // the same in every thread
+    if (ncol <= 0) return;
+    for (int k = threadIdx.x; k < ncol * mctgen::kCornerSites; k += blockDim.x) {
+        const int cl = k / mctgen::kCornerSites;
+        tab[k] = corner_site_axes(K, seeds, ucols, c0 + cl, k - cl * mctgen::kCornerSites);
+    }
+    __syncthreads();
+    const int cl = static_cast<int>(threadIdx.x) / kRows;
+    if (cl < ncol)
+        corner_values(K, seeds, ucols, colv, corners, c0 + cl, static_cast<int>(threadIdx.x) - cl * kRows,
+                      (const MCT_LDS mct::AxisXZ*)(tab + cl * kCornerTab));
 }
 
 // ---- surface-level grid / aquifer -----------------------------------------------------------
@@ -949,7 +966,7 @@ struct Generator {
         k_col_keys<<<grid_of(ck, 256), 256, 0, st>>>(KC, S.d_ch, n, S.d_ckeys);
         timed(S, kKDedupeCols, [&] { S.dedupe->run(S.d_ckeys, ck, KC.bits, S.d_ucols, S.d_colmap, S.d_counts, st); });
         timed(S, kKColumns, [&] { k_columns<<<grid_of(ck, 256), 256, 0, st>>>(KC, sd, ck, S.d_ucols, S.d_counts, S.d_colv); });
-        timed(S, kKCorners, [&] { k_corners<<<grid_of(static_cast<size_t>(ck) * kRows, 256), 256, 0, st>>>(KC, sd, ck, S.d_ucols, S.d_counts, S.d_colv, S.d_corners); });
+        timed(S, kKCorners, [&] { k_corners<<<grid_of(ck, kCornerCols), 256, 0, st>>>(KC, sd, ck, S.d_ucols, S.d_counts, S.d_colv, S.d_corners); });
         hip_check(hipGetLastError(), "noise kernels");
         hip_check(hipEventRecord(S.ev[3], st), "event");
         // surface-level grid and aquifer cells
@@ -1007,31 +1024,64 @@ struct Generator {
             hip_check(hipMalloc(&d_qout, kQueryGroup), "malloc");
             hip_check(hipMalloc(&d_qbad, sizeof(int)), "malloc");
         }
-        std::vector<int32_t> slot;
+        // Per group: local seed ids through a small open-addressing table, queries ordered by
+        // seed (counting sort) so that a wave's lanes share a seed's tables, answers scattered
+        // back to input order.
+        const int cap = seed_cap;
+        int tsize = 16;
+        while (tsize < 4 * cap) tsize <<= 1;
+        std::vector<int64_t> tkey(static_cast<size_t>(tsize));
+        std::vector<int32_t> tval(static_cast<size_t>(tsize));
+        std::vector<int32_t> lid, order, slot, cnt, sxyz;
+        std::vector<uint8_t> sout;
         for (int off = 0; off < count;) {
             int n = std::min(kQueryGroup, count - off);
+            std::fill(tval.begin(), tval.end(), -1);
             std::vector<int64_t> distinct;
-            {
-                std::unordered_map<int64_t, int> seen;
-                for (int i = 0; i < n; ++i)
-                    if (seen.emplace(seeds[off + i], 1).second) {
-                        if (static_cast<int>(distinct.size()) == seed_cap) { n = i; break; }
-                        distinct.push_back(seeds[off + i]);
-                    }
+            lid.resize(static_cast<size_t>(n));
+            for (int i = 0; i < n; ++i) {
+                const int64_t sd = seeds[off + i];
+                uint64_t h = static_cast<uint64_t>(sd) * 0x9E3779B97F4A7C15ull;
+                size_t at = static_cast<size_t>(h >> 40) & static_cast<size_t>(tsize - 1);
+                while (tval[at] >= 0 && tkey[at] != sd) at = (at + 1) & static_cast<size_t>(tsize - 1);
+                if (tval[at] < 0) {
+                    if (static_cast<int>(distinct.size()) == cap) { n = i; break; }
+                    tkey[at] = sd;
+                    tval[at] = static_cast<int32_t>(distinct.size());
+                    distinct.push_back(sd);
+                }
+                lid[i] = tval[at];
             }
             ensure_seeds(distinct, S.stream);
+            const int nd = static_cast<int>(distinct.size());
+            cnt.assign(static_cast<size_t>(nd) + 1, 0);
+            for (int i = 0; i < n; ++i) ++cnt[lid[i] + 1];
+            for (int d = 0; d < nd; ++d) cnt[d + 1] += cnt[d];
+            order.resize(static_cast<size_t>(n));
             slot.resize(static_cast<size_t>(n));
-            for (int i = 0; i < n; ++i) slot[i] = resident.at(seeds[off + i]);
-            hip_check(hipMemcpyAsync(d_qxyz, xyz + 3 * static_cast<size_t>(off), sizeof(int32_t) * 3 * n, hipMemcpyHostToDevice, S.stream), "upload");
+            sxyz.resize(static_cast<size_t>(n) * 3);
+            for (int i = 0; i < n; ++i) order[cnt[lid[i]]++] = i;
+            std::vector<int32_t> dslot(static_cast<size_t>(nd));
+            for (int d = 0; d < nd; ++d) dslot[d] = resident.at(distinct[d]);
+            for (int k = 0; k < n; ++k) {
+                const size_t i = static_cast<size_t>(off) + static_cast<size_t>(order[k]);
+                slot[k] = dslot[lid[order[k]]];
+                sxyz[3 * static_cast<size_t>(k)] = xyz[3 * i];
+                sxyz[3 * static_cast<size_t>(k) + 1] = xyz[3 * i + 1];
+                sxyz[3 * static_cast<size_t>(k) + 2] = xyz[3 * i + 2];
+            }
+            hip_check(hipMemcpyAsync(d_qxyz, sxyz.data(), sizeof(int32_t) * 3 * n, hipMemcpyHostToDevice, S.stream), "upload");
             hip_check(hipMemcpyAsync(d_qslot, slot.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice, S.stream), "upload");
             hip_check(hipMemsetAsync(d_qbad, 0, sizeof(int), S.stream), "memset");
             k_biome_query<<<grid_of(n, 256), 256, 0, S.stream>>>(d_seeds, d_qslot, d_qxyz, n, d_qout, d_qbad);
             hip_check(hipGetLastError(), "query kernel");
             int bad = 0;
-            hip_check(hipMemcpyAsync(out + off, d_qout, n, hipMemcpyDeviceToHost, S.stream), "download");
+            sout.resize(static_cast<size_t>(n));
+            hip_check(hipMemcpyAsync(sout.data(), d_qout, n, hipMemcpyDeviceToHost, S.stream), "download");
             hip_check(hipMemcpyAsync(&bad, d_qbad, sizeof(int), hipMemcpyDeviceToHost, S.stream), "download");
             hip_check(hipStreamSynchronize(S.stream), "sync");
             if (bad) throw Error(MCT_ECAPACITY, std::to_string(bad) + " queries have climate targets beyond 2^23 (not served by the float tree)");
+            for (int k = 0; k < n; ++k) out[off + order[k]] = sout[k];
             off += n;
         }
     }

@@ -9,6 +9,8 @@
 // Reference semantics: cpu/include/mccuda/noise_eval.h:21-304.
 #pragma once
 
+#include <stddef.h>
+
 #include "jmath.h"
 
 namespace mct {
@@ -39,26 +41,57 @@ MCT_HD MCT_FI double wrap(double x) {
     return x - q;
 }
 
-MCT_HD MCT_FI int P(const Octave& o, int32_t i) { return o.p[i & 0xFF]; }
+// Permutation lookups p[i & 255] of one octave. PermRef reads through the octave pointer.
+struct PermRef {
+    const uint8_t* p;
+    MCT_HD MCT_FI int operator()(int32_t i) const { return p[i & 0xFF]; }
+};
+
+#if defined(__HIP_DEVICE_COMPILE__)
+// The first active lane's octave-table pointer (wave-uniform, in SGPRs).
+__device__ MCT_FI const Octave* wave_first(const Octave* p) {
+    const uint64_t v = reinterpret_cast<uint64_t>(p);
+    const uint32_t lo = static_cast<uint32_t>(__builtin_amdgcn_readfirstlane(static_cast<int>(static_cast<uint32_t>(v))));
+    const uint32_t hi = static_cast<uint32_t>(__builtin_amdgcn_readfirstlane(static_cast<int>(static_cast<uint32_t>(v >> 32))));
+    return reinterpret_cast<const Octave*>((static_cast<uint64_t>(hi) << 32) | lo);
+}
+
+// Buffer resource over a wave-uniform table of n octaves (gfx9 raw buffer, stride 0,
+// bounds-checked to the table).
+__device__ MCT_FI __amdgpu_buffer_rsrc_t octave_rsrc(const Octave* table, int n) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<Octave*>(table), 0, n * static_cast<int>(sizeof(Octave)), 0x00020000);
+}
+
+// PermBuf: lookups of octave k of a wave-uniform table as raw buffer byte loads, the
+// octave's byte offset as scalar offset and the index as 32-bit vector offset (no per-lane
+// 64-bit address arithmetic).
+struct PermBuf {
+    __amdgpu_buffer_rsrc_t r;
+    int off;
+    __device__ MCT_FI PermBuf(__amdgpu_buffer_rsrc_t rsrc, int k) : r(rsrc), off(k * static_cast<int>(sizeof(Octave)) + static_cast<int>(offsetof(Octave, p))) {}
+    __device__ MCT_FI int operator()(int32_t i) const { return static_cast<int>(__builtin_amdgcn_raw_buffer_load_b8(r, i & 0xFF, off, 0)); }
+};
+#endif
 
 // Lattice hashes of the 8 cell corners, h[xyz] with bit 4 = x, 2 = y, 1 = z.
 struct Hash8 {
     int h000, h100, h010, h110, h001, h101, h011, h111;
 };
 
-MCT_HD MCT_FI Hash8 hash8(const Octave& o, int32_t fx, int32_t fy, int32_t fz) {
-    const int a = P(o, fx), b = P(o, iadd(fx, 1));
-    const int a0 = P(o, iadd(a, fy)), a1 = P(o, iadd(iadd(a, fy), 1));
-    const int b0 = P(o, iadd(b, fy)), b1 = P(o, iadd(iadd(b, fy), 1));
+template <class Perm>
+MCT_HD MCT_FI Hash8 hash8(const Perm& P, int32_t fx, int32_t fy, int32_t fz) {
+    const int a = P(fx), b = P(iadd(fx, 1));
+    const int a0 = P(iadd(a, fy)), a1 = P(iadd(iadd(a, fy), 1));
+    const int b0 = P(iadd(b, fy)), b1 = P(iadd(iadd(b, fy), 1));
     Hash8 h;
-    h.h000 = P(o, iadd(a0, fz));
-    h.h100 = P(o, iadd(b0, fz));
-    h.h010 = P(o, iadd(a1, fz));
-    h.h110 = P(o, iadd(b1, fz));
-    h.h001 = P(o, iadd(iadd(a0, fz), 1));
-    h.h101 = P(o, iadd(iadd(b0, fz), 1));
-    h.h011 = P(o, iadd(iadd(a1, fz), 1));
-    h.h111 = P(o, iadd(iadd(b1, fz), 1));
+    h.h000 = P(iadd(a0, fz));
+    h.h100 = P(iadd(b0, fz));
+    h.h010 = P(iadd(a1, fz));
+    h.h110 = P(iadd(b1, fz));
+    h.h001 = P(iadd(iadd(a0, fz), 1));
+    h.h101 = P(iadd(iadd(b0, fz), 1));
+    h.h011 = P(iadd(iadd(a1, fz), 1));
+    h.h111 = P(iadd(iadd(b1, fz), 1));
     return h;
 }
 
@@ -71,8 +104,9 @@ MCT_HD MCT_FI float dot3(int h, float x, float y, float z) {
 }
 
 // PerlinNoise.sampleAndLerp (point path); ty is the y fraction the smoothstep uses.
-MCT_HD MCT_FI float sample_lerp(const Octave& o, int32_t fx, int32_t fy, int32_t fz, float rx, float ry, float rz, float ty) {
-    const Hash8 h = hash8(o, fx, fy, fz);
+template <class Perm>
+MCT_HD MCT_FI float sample_lerp(const Perm& P, int32_t fx, int32_t fy, int32_t fz, float rx, float ry, float rz, float ty) {
+    const Hash8 h = hash8(P, fx, fy, fz);
     const float rx1 = rx - 1.0f, ry1 = ry - 1.0f, rz1 = rz - 1.0f;
     const float d000 = dot3(h.h000, rx, ry, rz);
     const float d100 = dot3(h.h100, rx1, ry, rz);
@@ -85,15 +119,18 @@ MCT_HD MCT_FI float sample_lerp(const Octave& o, int32_t fx, int32_t fy, int32_t
     return lerp3f(smooth(rx), smooth(ty), smooth(rz), d000, d100, d010, d110, d001, d101, d011, d111);
 }
 
-// PerlinNoise.get(x, y, z)
-MCT_HD MCT_FI float perlin_point(const Octave& o, double px, double py, double pz) {
+// PerlinNoise.get(x, y, z); P: the octave's permutation lookups.
+template <class Perm>
+MCT_HD MCT_FI float perlin_point(const Octave& o, const Perm& P, double px, double py, double pz) {
     const double x = wrap(px) + o.ox, y = wrap(py) + o.oy, z = wrap(pz) + o.oz;
     const int32_t fx = mfloor(x), fy = mfloor(y), fz = mfloor(z);
     const float rx = static_cast<float>(x - static_cast<double>(fx));
     const float ry = static_cast<float>(y - static_cast<double>(fy));
     const float rz = static_cast<float>(z - static_cast<double>(fz));
-    return sample_lerp(o, fx, fy, fz, rx, ry, rz, ry);
+    return sample_lerp(P, fx, fy, fz, rx, ry, rz, ry);
 }
+
+MCT_HD MCT_FI float perlin_point(const Octave& o, double px, double py, double pz) { return perlin_point(o, PermRef{o.p}, px, py, pz); }
 
 // SmearedPerlinNoise.computeFudgeY
 MCT_HD MCT_FI double fudge_y(double original_y, double rel_y, double smear) {
@@ -104,14 +141,19 @@ MCT_HD MCT_FI double fudge_y(double original_y, double rel_y, double smear) {
 }
 
 // SmearedPerlinNoise.get(x, y, z)
-MCT_HD MCT_FI float smeared_point(const Octave& o, double smear, double px, double py, double pz) {
+template <class Perm>
+MCT_HD MCT_FI float smeared_point(const Octave& o, const Perm& P, double smear, double px, double py, double pz) {
     const double x = wrap(px) + o.ox, y = wrap(py) + o.oy, z = wrap(pz) + o.oz;
     const int32_t fx = mfloor(x), fy = mfloor(y), fz = mfloor(z);
     const float rx = static_cast<float>(x - static_cast<double>(fx));
     const double rel_y = y - static_cast<double>(fy);
     const float rz = static_cast<float>(z - static_cast<double>(fz));
     const float fudged = static_cast<float>(rel_y - fudge_y(py, rel_y, smear));
-    return sample_lerp(o, fx, fy, fz, rx, fudged, rz, static_cast<float>(rel_y));
+    return sample_lerp(P, fx, fy, fz, rx, fudged, rz, static_cast<float>(rel_y));
+}
+
+MCT_HD MCT_FI float smeared_point(const Octave& o, double smear, double px, double py, double pz) {
+    return smeared_point(o, PermRef{o.p}, smear, px, py, pz);
 }
 
 // Coordinate setup of one axis of the volume path, the only FP64 work of a sample:
@@ -121,6 +163,11 @@ struct Axis {
     int32_t f;
     float r;
     float a;
+};
+
+// The x and z records of one octave sample site of a block column.
+struct AxisXZ {
+    Axis X, Z;
 };
 
 // x and z (and y of plain octaves): v = wrap(b * s) + offset
@@ -143,8 +190,9 @@ MCT_HD MCT_FI Axis axis_smeared(int32_t b, double s, double offset, double smear
 }
 
 // Volume-path sample from the three axis records: xz dots per corner, plus gy * ry.
-MCT_HD MCT_FI float volume_sample(const Octave& o, const Axis& X, const Axis& Y, const Axis& Z) {
-    const Hash8 h = hash8(o, X.f, Y.f, Z.f);
+template <class Perm>
+MCT_HD MCT_FI float volume_sample(const Perm& P, const Axis& X, const Axis& Y, const Axis& Z) {
+    const Hash8 h = hash8(P, X.f, Y.f, Z.f);
     const float rx = X.r, rz = Z.r, rx1 = rx - 1.0f, rz1 = rz - 1.0f;
     const float ry = Y.r, ry1 = ry - 1.0f;
 #define MCT_XZ(hh, xx, zz) ((gcomp(kGX, hh) * (xx)) + (gcomp(kGZ, hh) * (zz)))
@@ -161,19 +209,29 @@ MCT_HD MCT_FI float volume_sample(const Octave& o, const Axis& X, const Axis& Y,
 }
 
 // PerlinNoise.addToVolume, one sample (before the amplitude multiply).
-MCT_HD MCT_FI float perlin_volume(const Octave& o, int32_t bx, int32_t by, int32_t bz, double xs, double ys) {
+template <class Perm>
+MCT_HD MCT_FI float perlin_volume(const Octave& o, const Perm& P, int32_t bx, int32_t by, int32_t bz, double xs, double ys) {
     const Axis Z = axis_plain(bz, xs, o.oz);
     const Axis X = axis_plain(bx, xs, o.ox);
     const Axis Y = axis_plain(by, ys, o.oy);
-    return volume_sample(o, X, Y, Z);
+    return volume_sample(P, X, Y, Z);
+}
+
+MCT_HD MCT_FI float perlin_volume(const Octave& o, int32_t bx, int32_t by, int32_t bz, double xs, double ys) {
+    return perlin_volume(o, PermRef{o.p}, bx, by, bz, xs, ys);
 }
 
 // SmearedPerlinNoise.addToVolume, one sample.
-MCT_HD MCT_FI float smeared_volume(const Octave& o, double smear, int32_t bx, int32_t by, int32_t bz, double xs, double ys) {
+template <class Perm>
+MCT_HD MCT_FI float smeared_volume(const Octave& o, const Perm& P, double smear, int32_t bx, int32_t by, int32_t bz, double xs, double ys) {
     const Axis Z = axis_plain(bz, xs, o.oz);
     const Axis X = axis_plain(bx, xs, o.ox);
     const Axis Y = axis_smeared(by, ys, o.oy, smear);
-    return volume_sample(o, X, Y, Z);
+    return volume_sample(P, X, Y, Z);
+}
+
+MCT_HD MCT_FI float smeared_volume(const Octave& o, double smear, int32_t bx, int32_t by, int32_t bz, double xs, double ys) {
+    return smeared_volume(o, PermRef{o.p}, smear, bx, by, bz, xs, ys);
 }
 
 }  // namespace mct

@@ -526,14 +526,24 @@ MCT_HD MCT_FI void column_values(const KeyCodec& K, const SeedDev* seeds, const 
     mctgen::eval_columns(seeds[s].oct, qx * 4, qz * 4, colv + static_cast<size_t>(u) * (mctgen::kNumColumns > 0 ? mctgen::kNumColumns : 1));
 }
 
-// Channel inputs at one cell corner: corners[(u * kCh + ch) * kRows + row].
-MCT_HD MCT_FI void corner_values(const KeyCodec& K, const SeedDev* seeds, const uint64_t* ucols, const float* colv, float* corners, int idx) {
-    const int u = idx / kRows, row = idx % kRows;
+// x / z records of corner-pass octave sample site `site` of distinct column u.
+MCT_HD MCT_FI mct::AxisXZ corner_site_axes(const KeyCodec& K, const SeedDev* seeds, const uint64_t* ucols, int u, int site) {
+    int32_t s, qx, qz;
+    int f;
+    split_key(K, ucols[u], s, f, qx, qz);
+    return mctgen::corner_axes(seeds[s].oct, qx * 4, qz * 4, site);
+}
+
+// Channel inputs at one cell corner (column u, corner row): corners[(u * kCh + ch) * kRows + row].
+// axz: the column's site records (corner_site_axes for every site).
+MCT_HD MCT_FI void corner_values(const KeyCodec& K, const SeedDev* seeds, const uint64_t* ucols, const float* colv, float* corners, int u, int row,
+                                 const MCT_LDS mct::AxisXZ* axz) {
     int32_t s, qx, qz;
     int f;
     split_key(K, ucols[u], s, f, qx, qz);
     float out[kCh];
-    mctgen::eval_corner(seeds[s].oct, qx * 4, kMinY + row * 8, qz * 4, colv + static_cast<size_t>(u) * (mctgen::kNumColumns > 0 ? mctgen::kNumColumns : 1), out);
+    mctgen::eval_corner(seeds[s].oct, qx * 4, kMinY + row * 8, qz * 4, colv + static_cast<size_t>(u) * (mctgen::kNumColumns > 0 ? mctgen::kNumColumns : 1), axz,
+                        out);
     for (int ch = 0; ch < kCh; ++ch) corners[(static_cast<size_t>(u) * kCh + ch) * kRows + row] = out[ch];
 }
 
