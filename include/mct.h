@@ -69,6 +69,17 @@ int mct_generate_multi(mct_generator* g, const int64_t* seeds, const int32_t* ch
 int mct_generate_device(mct_generator* g, const int64_t* seeds, const int32_t* chunk_xz, int count, uint8_t* d_terrain, uint8_t* d_biomes,
                         mct_timing* timing);
 
+/* Core configuration: TerrainGenerator::generate(chunks, count, density_out, fill_out, timing)
+ * (cuda/include/mccuda_cuda/generator.h:30-45; CPU semantics cpu/src/volume_eval.cpp:676-690):
+ * final density (f32 per block, the same layout as terrain) and the core fill (u8 per block:
+ * 0 air, 1 stone, 2 water, 3 lava; no aquifers, no material rules, no biomes) of each chunk under
+ * seeds[i]. density (count * 98,304 floats) or fill (count * 98,304 B) may be NULL, not both.
+ * Host memory; blocks until all results are in place. */
+int mct_generate_core(mct_generator* g, const int64_t* seeds, const int32_t* chunk_xz, int count, float* density, uint8_t* fill, mct_timing* timing);
+/* The same with device-memory outputs (nothing copied to the host). */
+int mct_generate_core_device(mct_generator* g, const int64_t* seeds, const int32_t* chunk_xz, int count, float* d_density, uint8_t* d_fill,
+                             mct_timing* timing);
+
 /* FullTerrainGenerator::biome_tie_stats (terrain_gpu.h:47): out[0] quarts searched, out[1]
  * lookups whose leaf the R-tree's last-result rule changed, out[2] of those, a changed biome.
  * out[3]: chunks whose surface pass was redone in exact sequential column order. */

@@ -78,11 +78,56 @@ public:
         return v;
     }
 
+    // Biome stored for each block xyz[3i..3i+2] under seeds[i], without generating chunks;
+    // no counterpart in the reference class.
+    void biome_query(const int64_t* seeds, const int32_t* xyz, int count, uint8_t* biomes_out) {
+        check(mct_biome_query(g_, seeds, xyz, count, biomes_out));
+    }
+
     mct_generator* handle() const { return g_; }
 
 private:
     mct_generator* g_ = nullptr;
 };
+
+// Core configuration, the reference's TerrainGenerator (cuda/include/mccuda_cuda/generator.h:30-45):
+// final density and the core fill (0 air, 1 stone, 2 water, 3 lava) per block, no biomes,
+// aquifers or material rules. set_seed takes the world seed (the reference takes its SeedTables;
+// the tables are built on the device here).
+class TerrainGenerator {
+public:
+    TerrainGenerator(const std::string& ir_json, const std::string& terrain_json, int max_batch_chunks = 1024) {
+        check(mct_create(ir_json.c_str(), terrain_json.c_str(), max_batch_chunks, 2, 0, &g_));
+    }
+    ~TerrainGenerator() { mct_destroy(g_); }
+    TerrainGenerator(const TerrainGenerator&) = delete;
+    TerrainGenerator& operator=(const TerrainGenerator&) = delete;
+
+    void set_seed(int64_t seed) { seed_ = seed; }
+
+    // density_out (f32) may be null; outputs in the canonical layout, chunks back to back.
+    void generate(const ChunkXZ* chunks, int count, float* density_out, uint8_t* fill_out, TerrainTiming* timing = nullptr) {
+        std::vector<int64_t> seeds(static_cast<size_t>(count), seed_);
+        mct_timing t{};
+        check(mct_generate_core(g_, seeds.data(), reinterpret_cast<const int32_t*>(chunks), count, density_out, fill_out, timing ? &t : nullptr));
+        if (timing) *timing = TerrainTiming{t.h2d_ms, t.biome_ms, t.noise_ms, t.aquifer_ms, t.fill_ms, t.surface_ms, t.d2h_ms, t.wall_ms};
+    }
+
+private:
+    mct_generator* g_ = nullptr;
+    int64_t seed_ = 0;
+};
+
+struct ZoomSelfTest {     // terrain_gpu.h:56-58 (field names kept)
+    int64_t checked, mismatches, fallbacks;   // fallbacks: here the column cache's quart-row refills (no FP32 filter)
+};
+
+// The surface pass's column-cached zoom against the plain FP64 zoom on about `count` blocks.
+inline ZoomSelfTest zoom_selftest(int64_t world_seed, int64_t count) {
+    int64_t out[3];
+    check(mct_zoom_selftest(world_seed, count, out));
+    return ZoomSelfTest{out[0], out[1], out[2]};
+}
 
 inline uint8_t* alloc_pinned(size_t bytes) {    // terrain_gpu.h:61
     void* p = mct_alloc_pinned(bytes);

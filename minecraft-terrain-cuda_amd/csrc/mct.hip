@@ -224,6 +224,12 @@ __global__ __launch_bounds__(256) void k_fill(FillArgs A, int n) {
     fill_cell(A, idx / (256 * kCellsY), idx % (256 * kCellsY), nullptr);
 }
 
+__global__ __launch_bounds__(256) void k_fill_core(FillArgs A, int n) {
+    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= n * 256 * kCellsY) return;
+    fill_cell_core(A, idx / (256 * kCellsY), idx % (256 * kCellsY));
+}
+
 __global__ __launch_bounds__(256) void k_heights(int n, const uint8_t* blocks, int* h0, int* redo_count) {
     const int idx = blockIdx.x * blockDim.x + threadIdx.x;
     if (idx == 0) *redo_count = 0;   // list of chunks for the exact surface pass, filled by k_surface_check
@@ -632,6 +638,7 @@ struct Slot {
     int *d_colmap = nullptr, *d_gmap = nullptr, *d_counts = nullptr, *d_skip = nullptr;
     int *d_h0 = nullptr, *d_h1 = nullptr, *d_h2 = nullptr, *d_hf = nullptr, *d_flags = nullptr;
     float* d_colrec = nullptr;
+    float* d_density = nullptr;     // core configuration's density output (allocated on first use)
     std::unique_ptr<Dedupe> dedupe;
     hipEvent_t kev[kKCount][2] = {};
     bool kev_recorded = false;
@@ -681,7 +688,8 @@ struct Slot {
     ~Slot() {
         if (stream) (void)hipStreamSynchronize(stream);
         void* p[] = {d_ch, d_bch, d_nb, d_bcol, d_colv, d_corners, d_gv, d_prelim, d_targets, d_mind, d_lstar, d_grid, d_cloc, d_cfl, d_tie, d_bb,
-                     d_blocks, d_bout, d_aq, d_ckeys, d_ucols, d_gkeys, d_ug, d_colmap, d_gmap, d_counts, d_skip, d_h0, d_h1, d_h2, d_hf, d_flags, d_colrec};
+                     d_blocks, d_bout, d_aq, d_ckeys, d_ucols, d_gkeys, d_ug, d_colmap, d_gmap, d_counts, d_skip, d_h0, d_h1, d_h2, d_hf, d_flags, d_colrec,
+                     d_density};
         for (void* q : p) (void)hipFree(q);
         dedupe.reset();
         (void)hipHostFree(h_ch);
@@ -899,6 +907,49 @@ struct Generator {
         hip_check(hipStreamSynchronize(stream), "seed setup");
     }
 
+    // Core configuration (TerrainGenerator, no biomes / aquifers / material rules): the corner
+    // pass, then final density and the core fill per block. fill / density: destinations of this
+    // batch (device pointers when to_device, else host), either may be null.
+    void run_batch_core(Slot& S, const int64_t* seeds, const int32_t* xz, int n, uint8_t* fill, float* density, bool to_device) {
+        S.n = n;
+        std::vector<int64_t> distinct;
+        {
+            std::unordered_map<int64_t, int> seen;
+            for (int i = 0; i < n; ++i)
+                if (seen.emplace(seeds[i], 1).second) distinct.push_back(seeds[i]);
+        }
+        ensure_seeds(distinct, S.stream);
+        for (int i = 0; i < n; ++i) S.h_ch[i] = ChunkRef{xz[2 * i], xz[2 * i + 1], resident.at(seeds[i]), 0};
+        if (density && !to_device && !S.d_density) hip_check(hipMalloc(&S.d_density, sizeof(float) * static_cast<size_t>(S.cap) * kPer), "malloc");
+        hipStream_t st = S.stream;
+        hip_check(hipEventRecord(S.ev[0], st), "event");
+        hip_check(hipMemcpyAsync(S.d_ch, S.h_ch, sizeof(ChunkRef) * n, hipMemcpyHostToDevice, st), "upload chunks");
+        hip_check(hipEventRecord(S.ev[1], st), "event");
+        hip_check(hipEventRecord(S.ev[2], st), "event");
+        const SeedDev* sd = d_seeds;
+        const int ck = n * kCols;
+        const KeyCodec KC = codec_for(S.h_ch, n, 0, 4, 0, 4);
+        k_col_keys<<<grid_of(ck, 256), 256, 0, st>>>(KC, S.d_ch, n, S.d_ckeys);
+        timed(S, kKDedupeCols, [&] { S.dedupe->run(S.d_ckeys, ck, KC.bits, S.d_ucols, S.d_colmap, S.d_counts, st); });
+        timed(S, kKColumns, [&] { k_columns<<<grid_of(ck, 256), 256, 0, st>>>(KC, sd, ck, S.d_ucols, S.d_counts, S.d_colv); });
+        timed(S, kKCorners, [&] { k_corners<<<grid_of(ck, kCornerCols), 256, 0, st>>>(KC, sd, ck, S.d_ucols, S.d_counts, S.d_colv, S.d_corners); });
+        hip_check(hipGetLastError(), "noise kernels");
+        hip_check(hipEventRecord(S.ev[3], st), "event");
+        hip_check(hipEventRecord(S.ev[4], st), "event");
+        uint8_t* dfill = fill ? (to_device ? fill : S.d_blocks) : nullptr;
+        float* ddens = density ? (to_device ? density : S.d_density) : nullptr;
+        FillArgs F{sd, S.d_ch, S.d_corners, S.d_colmap, nullptr, nullptr, nullptr, dfill, ddens};
+        timed(S, kKFill, [&] { k_fill_core<<<grid_of(static_cast<size_t>(n) * 256 * kCellsY, 256), 256, 0, st>>>(F, n); });
+        hip_check(hipGetLastError(), "fill kernel");
+        for (int e = 5; e <= 7; ++e) hip_check(hipEventRecord(S.ev[e], st), "event");
+        if (!to_device) {
+            if (fill) hip_check(hipMemcpyAsync(fill, S.d_blocks, static_cast<size_t>(n) * kPer, hipMemcpyDeviceToHost, st), "download fill");
+            if (density) hip_check(hipMemcpyAsync(density, S.d_density, sizeof(float) * static_cast<size_t>(n) * kPer, hipMemcpyDeviceToHost, st), "download density");
+        }
+        hip_check(hipEventRecord(S.ev[8], st), "event");
+        S.pending = true;
+    }
+
     // Enqueues one batch on slot S (uploads, kernels, downloads); returns without waiting.
     void run_batch(Slot& S, const int64_t* seeds, const int32_t* xz, int n, uint8_t* terrain, uint8_t* biomes, uint8_t* aquifer, bool to_device) {
         S.n = n;
@@ -1087,7 +1138,7 @@ struct Generator {
     }
 
     void generate(const int64_t* seeds, const int32_t* xz, int count, uint8_t* terrain, uint8_t* biomes, uint8_t* aquifer, mct_timing* tm,
-                  bool to_device) {
+                  bool to_device, bool core = false, float* density = nullptr) {
         const auto w0 = std::chrono::steady_clock::now();
         collect_all(nullptr);
         done_ms.assign(static_cast<size_t>((count + max_batch - 1) / max_batch), 0.0);
@@ -1106,9 +1157,13 @@ struct Generator {
             next_slot = (next_slot + 1) % slots.size();
             collect(S, tm);
             S.batch = batch;
-            run_batch(S, seeds + off, xz + 2 * static_cast<size_t>(off), n, terrain + static_cast<size_t>(off) * kPer,
-                      biomes ? biomes + static_cast<size_t>(off) * kQPer : nullptr, aquifer ? aquifer + static_cast<size_t>(off) * kPer : nullptr,
-                      to_device);
+            if (core)
+                run_batch_core(S, seeds + off, xz + 2 * static_cast<size_t>(off), n, terrain ? terrain + static_cast<size_t>(off) * kPer : nullptr,
+                               density ? density + static_cast<size_t>(off) * kPer : nullptr, to_device);
+            else
+                run_batch(S, seeds + off, xz + 2 * static_cast<size_t>(off), n, terrain + static_cast<size_t>(off) * kPer,
+                          biomes ? biomes + static_cast<size_t>(off) * kQPer : nullptr, aquifer ? aquifer + static_cast<size_t>(off) * kPer : nullptr,
+                          to_device);
             off += n;
         }
         collect_all(tm);
@@ -1313,6 +1368,21 @@ int mct_generate_device(mct_generator* g, const int64_t* seeds, const int32_t* c
     return guard([&] {
         if (!g || count < 0 || (count > 0 && (!seeds || !chunk_xz || !d_terrain))) throw Error(MCT_EINVAL, "bad arguments");
         g->g->generate(seeds, chunk_xz, count, d_terrain, d_biomes, nullptr, timing, true);
+    });
+}
+
+int mct_generate_core(mct_generator* g, const int64_t* seeds, const int32_t* chunk_xz, int count, float* density, uint8_t* fill, mct_timing* timing) {
+    return guard([&] {
+        if (!g || count < 0 || (count > 0 && (!seeds || !chunk_xz || (!density && !fill)))) throw Error(MCT_EINVAL, "bad arguments");
+        g->g->generate(seeds, chunk_xz, count, fill, nullptr, nullptr, timing, false, true, density);
+    });
+}
+
+int mct_generate_core_device(mct_generator* g, const int64_t* seeds, const int32_t* chunk_xz, int count, float* d_density, uint8_t* d_fill,
+                             mct_timing* timing) {
+    return guard([&] {
+        if (!g || count < 0 || (count > 0 && (!seeds || !chunk_xz || (!d_density && !d_fill)))) throw Error(MCT_EINVAL, "bad arguments");
+        g->g->generate(seeds, chunk_xz, count, d_fill, nullptr, nullptr, timing, true, true, d_density);
     });
 }
 

@@ -787,19 +787,20 @@ struct FillArgs {
     const int32_t* cell_fluid;  // n x kAqCells x 2
     const int* skip;
     uint8_t* blocks;
+    float* density;             // core configuration: final density per block (nullable)
 };
 
 // The 8 blocks of one block column inside one 4x8 cell (t = (cy * 16 + z) * 16 + x): the z
 // and x lerps of every channel once, then y as the running sum of the step, which is the
 // same sequence of float additions the per-block form repeats from the cell's bottom.
 constexpr int kCellsY = kHeight / 8;
-MCT_HD MCT_FI void fill_cell(const FillArgs& A, int c, int t, float* density_out) {
-    const int x = t & 15, z = (t >> 4) & 15, cy = t >> 8;
+
+// Channel values at the bottom block of cell row cy of block column (x, z) and their per-block
+// y step: the z lerps, then the x lerps, of the cell's 8 corners.
+MCT_HD MCT_FI void cell_channels(const FillArgs& A, int c, int x, int z, int cy, float* val, float* step) {
     const int* cmap = A.colmap + static_cast<size_t>(c) * kCols;
     const int cx = x >> 2, xi = x & 3, cz = z >> 2, zi = z & 3;
     const size_t u00 = cmap[cz * 5 + cx], u10 = cmap[cz * 5 + cx + 1], u01 = cmap[(cz + 1) * 5 + cx], u11 = cmap[(cz + 1) * 5 + cx + 1];
-    constexpr int NC = mctgen::kNumChannels > 0 ? mctgen::kNumChannels : 1;
-    float val[NC], step[NC];
     const float az = static_cast<float>(zi) * 0.25f, ax = static_cast<float>(xi) * 0.25f;
     for (int k = 0; k < mctgen::kNumChannels; ++k) {
         const float* a = A.corners + (u00 * kCh + k) * kRows + cy;
@@ -812,6 +813,13 @@ MCT_HD MCT_FI void fill_cell(const FillArgs& A, int c, int t, float* density_out
         step[k] = (hi - lo) * 0.125f;
         val[k] = lo + step[k] * 0.0f;
     }
+}
+
+MCT_HD MCT_FI void fill_cell(const FillArgs& A, int c, int t, float* density_out) {
+    const int x = t & 15, z = (t >> 4) & 15, cy = t >> 8;
+    constexpr int NC = mctgen::kNumChannels > 0 ? mctgen::kNumChannels : 1;
+    float val[NC], step[NC];
+    cell_channels(A, c, x, z, cy, val, step);
     const ChunkRef cr = A.chunks[c];
     const Octave* O = A.seeds[cr.seed].oct;
     const int bx = cr.x * 16 + x, bz = cr.z * 16 + z;
@@ -825,6 +833,39 @@ MCT_HD MCT_FI void fill_cell(const FillArgs& A, int c, int t, float* density_out
         if (density_out) density_out[(cy * 8 + yi) * 256 + z * 16 + x] = d;
         const int s = aquifer_substance(O, cr.x, cr.z, loc, fl, skip, bx, by, bz, static_cast<double>(d));
         out[yi * 256] = static_cast<uint8_t>(s < 0 ? mctgen::kDefaultBlock : s);
+        for (int k = 0; k < mctgen::kNumChannels; ++k) val[k] += step[k];
+    }
+}
+
+// Block ids of the core configuration's fill (no aquifers, no material rules).
+enum CoreBlock : uint8_t { kCoreAir = 0, kCoreStone = 1, kCoreWater = 2, kCoreLava = 3 };
+
+// Vanilla doFill substance with aquifers disabled: the default block where density > 0, else
+// the global fluid picker (lava below min(-54, sea level) and y < -54, water below sea level).
+// Reference: cpu/include/mccuda/volume_eval.h:76-85.
+MCT_HD MCT_FI uint8_t core_fill(float density, int y) {
+    if (static_cast<double>(density) > 0.0) return kCoreStone;
+    const int lava_level = kSea < -54 ? kSea : -54;
+    if (y < lava_level) return y < -54 ? kCoreLava : kCoreAir;
+    return y < kSea ? kCoreWater : kCoreAir;
+}
+
+// Core configuration, one block column of one cell: final density (A.density) and core fill
+// (A.blocks), either may be null. Same channel interpolation and density as fill_cell.
+MCT_HD MCT_FI void fill_cell_core(const FillArgs& A, int c, int t) {
+    const int x = t & 15, z = (t >> 4) & 15, cy = t >> 8;
+    constexpr int NC = mctgen::kNumChannels > 0 ? mctgen::kNumChannels : 1;
+    float val[NC], step[NC];
+    cell_channels(A, c, x, z, cy, val, step);
+    const ChunkRef cr = A.chunks[c];
+    const Octave* O = A.seeds[cr.seed].oct;
+    const int bx = cr.x * 16 + x, bz = cr.z * 16 + z;
+    const size_t at = static_cast<size_t>(c) * kPer + static_cast<size_t>(cy * 8 * 256 + z * 16 + x);
+    for (int yi = 0; yi < 8; ++yi) {
+        const int by = kMinY + cy * 8 + yi;
+        const float d = mctgen::eval_block(O, bx, by, bz, val);
+        if (A.density) A.density[at + static_cast<size_t>(yi) * 256] = d;
+        if (A.blocks) A.blocks[at + static_cast<size_t>(yi) * 256] = core_fill(d, by);
         for (int k = 0; k < mctgen::kNumChannels; ++k) val[k] += step[k];
     }
 }

@@ -49,6 +49,8 @@ def lib():
         L.mct_generate_multi.argtypes = [P, P, P, I, P, P, P, P]
         L.mct_generate_device.argtypes = [P, P, P, I, P, P, P]
         L.mct_biome_tie_stats.argtypes = [P, P]
+        L.mct_generate_core.argtypes = [P, P, P, I, P, P, P]
+        L.mct_generate_core_device.argtypes = [P, P, P, I, P, P, P]
         L.mct_last_batch_done_ms.argtypes = [P, P, I, ctypes.POINTER(I)]
         L.mct_alloc_pinned.argtypes = [SZ]
         L.mct_alloc_pinned.restype = P
@@ -140,6 +142,22 @@ class FullTerrainGenerator:
         if len(s) != len(xz):
             raise ValueError('one seed per chunk')
         _check(lib().mct_generate_device(self.h, _ptr(s), _ptr(xz), len(xz), d_terrain, d_biomes, ctypes.byref(timing) if timing is not None else None))
+
+    def generate_core(self, seeds, chunks, density=True, fill=True, timing=None):
+        """Core configuration (TerrainGenerator): final density float32 (n, 384, 16, 16) and the
+        core fill uint8 (0 air, 1 stone, 2 water, 3 lava), either may be skipped (None)."""
+        xz = _xz(chunks)
+        n = len(xz)
+        s = np.ascontiguousarray(np.broadcast_to(np.asarray(seeds, dtype=np.int64), (n,)))
+        d = np.empty((n, 384, 16, 16), np.float32) if density else None
+        f = np.empty((n, 384, 16, 16), np.uint8) if fill else None
+        _check(lib().mct_generate_core(self.h, _ptr(s), _ptr(xz), n, _ptr(d), _ptr(f), ctypes.byref(timing) if timing is not None else None))
+        return d, f
+
+    def generate_core_device(self, seeds, chunks, d_density=None, d_fill=None, timing=None):
+        xz = _xz(chunks)
+        s = np.ascontiguousarray(np.broadcast_to(np.asarray(seeds, dtype=np.int64), (len(xz),)))
+        _check(lib().mct_generate_core_device(self.h, _ptr(s), _ptr(xz), len(xz), d_density, d_fill, ctypes.byref(timing) if timing is not None else None))
 
     def biome_query(self, seeds, xyz):
         """Biome stored by the TERRAIN step for each block (x, y, z) under its seed."""

@@ -81,6 +81,22 @@ def _ptr(a):
     return None if a is None else a.ctypes.data
 
 
+def core_fill(density, sea_level=None, min_y=None):
+    """Vanilla doFill with aquifers disabled, over (n, height, 16, 16) densities (y, z, x): 1 stone
+    where density > 0, else 3 lava below min(-54, sea level) for y < -54, 2 water below sea
+    level, 0 air (cpu/include/mccuda/volume_eval.h:76-85). Sea level and min y default to the
+    IR's."""
+    if sea_level is None or min_y is None:
+        import json
+        ir = json.load(open(IR_PATH))
+        sea_level = ir['sea_level'] if sea_level is None else sea_level
+        min_y = ir['min_y'] if min_y is None else min_y
+    y = (np.arange(density.shape[1]) + min_y)[None, :, None, None]
+    lava_level = min(sea_level, -54)
+    fluid = np.where(y < lava_level, np.where(y < -54, 3, 0), np.where(y < sea_level, 2, 0))
+    return np.where(density.astype(np.float64) > 0.0, 1, np.broadcast_to(fluid, density.shape)).astype(np.uint8)
+
+
 class Oracle:
     """The oracle over one IR + terrain model (the synthetic overworld by default)."""
 
@@ -119,6 +135,12 @@ class Oracle:
         d = np.zeros((n, HEIGHT, 16, 16), np.float32) if density else None
         self._check(lib().orc_generate(self.h, int(seed), xz.ctypes.data, n, int(threads), b.ctypes.data, a.ctypes.data, t.ctypes.data, _ptr(d)))
         return (t, b, a, d) if density else (t, b, a)
+
+    def generate_core(self, seed, chunks, threads=8):
+        """Core configuration: (final density f32, core fill u8), (n, 384, 16, 16) as (y, z, x).
+        The density is the TERRAIN step's (the same final-density volume sample of the chunk)."""
+        d = self.generate(seed, chunks, threads=threads, density=True)[3]
+        return d, core_fill(d)
 
     def biome_query(self, seeds, xyz):
         """The biome the TERRAIN step stores for each block: BiomeManager's zoom picks the quart
@@ -217,6 +239,7 @@ class Reference:
         L.ref_seed_layers.argtypes = [P, I64, P, P]
         L.ref_noise_ref.argtypes = [P, I64, ctypes.c_char_p, P, P]
         L.ref_chunk_biomes.argtypes = [P, I64, I, I, P]
+        L.ref_generate_core.argtypes = [P, I64, P, I, P, P]
         self.L = L
         self.h = L.ref_create(ir_path.encode(), terrain_path.encode())
         if not self.h:
@@ -235,6 +258,15 @@ class Reference:
         d = np.zeros((n, HEIGHT, 16, 16), np.float32) if density else None
         self._check(self.L.ref_generate_terrain(self.h, int(seed), xz.ctypes.data, n, b.ctypes.data, a.ctypes.data, t.ctypes.data, _ptr(d)))
         return (t, b, a, d) if density else (t, b, a)
+
+    def generate_core(self, seed, chunks):
+        """Core configuration (cpu_generate_chunk): (density f32, fill u8), (n, 384, 16, 16)."""
+        n = len(chunks)
+        xz = np.ascontiguousarray(np.array(chunks, dtype=np.int32).reshape(n, 2))
+        d = np.zeros((n, HEIGHT, 16, 16), np.float32)
+        f = np.zeros((n, HEIGHT, 16, 16), np.uint8)
+        self._check(self.L.ref_generate_core(self.h, int(seed), xz.ctypes.data, n, d.ctypes.data, f.ctypes.data))
+        return d, f
 
     def region(self, seed, x0, z0, w, d, threads=8):
         t = np.zeros((w * d, HEIGHT, 16, 16), np.uint16)

@@ -97,6 +97,23 @@ int ref_generate_terrain(void* p, int64_t seed, const int32_t* xz, int n, uint8_
     }
 }
 
+// Core configuration of independent chunks (cpu/src/volume_eval.cpp:676-690 cpu_generate_chunk):
+// final density f32 and core fill u8 per block (either may be null).
+int ref_generate_core(void* p, int64_t seed, const int32_t* xz, int n, float* density, uint8_t* fill) {
+    try {
+        Handle* h = static_cast<Handle*>(p);
+        TerrainSeed& ts = seed_of(h, seed);
+        const size_t per = static_cast<size_t>(16 * 16 * h->ir.height);
+        VolumeEvaluator ev(h->ir, ts.tables);
+        for (int i = 0; i < n; ++i)
+            cpu_generate_chunk(h->ir, ev, xz[2 * i], xz[2 * i + 1], density ? density + i * per : nullptr, fill ? fill + i * per : nullptr);
+        return 0;
+    } catch (const std::exception& e) {
+        g_err = e.what();
+        return 1;
+    }
+}
+
 // A w x d region at chunk (x0, z0), generated the way the reference bench does
 // (bench/bench_main.cpp:253-304): biomes of the region plus a one-chunk border once,
 // then fill + surface per chunk, on `threads` std::threads. Outputs row-major by chunk.

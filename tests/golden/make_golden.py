@@ -15,6 +15,7 @@ fixtures.npz holds, per part of the path:
   vol_<seed>_<root>_<k>, pts_xyz, pts_<seed>_<root>   volume-path and point-path samples
   edge_*                the edge-case chunks: per-chunk digests of terrain / biomes / aquifer /
                         density, full terrain + biomes of the first two chunks
+  core_density, core_fill   the core configuration (final density, core fill) of the edge chunks
   c1_*                  the 8 x 8 region of the CPU-runnable configuration (digests)
 plain_digests.npz: 8-byte digests of terrain (u8) and biomes of every chunk of the plain
 three-region workload (12,296 chunks), region mode, for the full-size GPU parity check.
@@ -139,6 +140,13 @@ def main():
         if len(full_t) < 2:
             full_t.append(t[0].astype(np.uint8))
             full_b.append(b[0])
+    # core configuration (final density + core fill, cpu_generate_chunk) of the same chunks
+    core_d, core_f = [], []
+    for s, c in zip(seeds, chunks):
+        d, f = R.generate_core(s, [c])
+        core_d.append(digest(d))
+        core_f.append(digest(f))
+    out['core_density'], out['core_fill'] = np.stack(core_d), np.stack(core_f)
     out['edge_seed'], out['edge_xz'] = np.array(seeds, np.int64), np.array(chunks, np.int32)
     for k, v in dig.items():
         out['edge_' + k] = np.stack(v)

@@ -289,6 +289,36 @@ def test_biome_queries_match_reference(orc):
     assert (orc.biome_query(qs[sel], q[sel]) == g['query_biome'][sel]).all()
 
 
+def test_core_configuration_matches_reference(orc, golden):
+    """Final density and the core fill (aquifers disabled) of the edge chunks."""
+    for i, (s, c) in enumerate(zip(golden['edge_seed'], golden['edge_xz'])):
+        d, f = orc.generate_core(int(s), [tuple(c)], threads=1)
+        assert (digest(d) == golden['core_density'][i]).all() and (digest(f) == golden['core_fill'][i]).all(), (s, c)
+
+
+def test_core_fill_closed_form():
+    """Stone exactly where density > 0 (so not at +-0 or NaN); below it the fluid picker: lava
+    under y = -54, air from -54 to the lava level's top (none with sea level 63), water to sea level."""
+    d = np.full((1, 384, 16, 16), -1.0, np.float32)
+    d[0, :, 0, 0] = 1e-30
+    d[0, :, 0, 1] = 0.0
+    d[0, :, 0, 2] = -0.0
+    d[0, :, 0, 3] = np.nan
+    f = oracle_module().core_fill(d, sea_level=63, min_y=-64)
+    y = np.arange(384) - 64
+    assert (f[0, :, 0, 0] == 1).all()
+    for col in (1, 2, 3, 4):
+        want = np.where(y < -54, 3, np.where(y < 63, 2, 0))
+        assert (f[0, :, 0, col] == want).all(), col
+    low = oracle_module().core_fill(d, sea_level=-60, min_y=-64)     # sea level below -54: lava level = sea level
+    assert (low[0, :, 0, 4] == np.where(y < -60, 3, 0)).all()
+
+
+def oracle_module():
+    import oracle
+    return oracle
+
+
 def test_outputs_fit_the_u8_layout(orc):
     t, b, _ = orc.generate(0, [(3, -4)])
     import json
