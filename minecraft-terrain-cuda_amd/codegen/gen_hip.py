@@ -695,7 +695,7 @@ class Gen:
                     if b >= 128:
                         raise SystemExit('biome index >= 128')
                     mask[b >> 6] |= 1 << (b & 63)
-                body = 'const int b = c.biome(); return ((b < 64 ? (%s >> b) : (%s >> (b - 64))) & 1ull) != 0;' % (u64(mask[0]), u64(mask[1]))
+                body = 'return c.biome_in(%s, %s);' % (u64(mask[0]), u64(mask[1]))
             elif t == 'noise_threshold':
                 if cn['noise'] not in key_index:
                     raise SystemExit('rule noise %s is not in the IR noise list' % cn['noise'])

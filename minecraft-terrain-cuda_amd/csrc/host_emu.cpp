@@ -253,7 +253,7 @@ int mcte_generate(void* p, int64_t seed, const int32_t* xz, int n, uint8_t* terr
         for (int c = 0; c < n; ++c)
             for (int col = 0; col < 256; ++col) surface_pass1(A, h0.data(), h1.data(), h2.data(), c, col);
         for (int c = 0; c < n; ++c)
-            for (int col = 0; col < 256; ++col) surface_pass2(A, h0.data(), h1.data(), h2.data(), hf.data(), c, col);
+            for (int col = 0; col < 256; ++col) surface_pass2<false>(A, h0.data(), h1.data(), h2.data(), hf.data(), c, col);
         int64_t redone = 0;
         for (int c = 0; c < n; ++c) {
             bool ok = true;

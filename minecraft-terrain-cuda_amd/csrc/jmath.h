@@ -150,17 +150,30 @@ MCT_HD MCT_FI float gradient_at(int mode, int32_t from, int32_t lo, int32_t hi, 
 // InterpolatedFunction cell fill for one block from its 8 cell-corner values (v[xyz],
 // x/z cell size 4 or 16, y cell size 8 or 1): lerp along z, then x, then y as the running
 // sum of `step`, exactly as the volume path accumulates it.
-MCT_HD MCT_FI float cell_fill(float v000, float v100, float v010, float v110, float v001, float v101, float v011, float v111, int xi, int yi,
-                              int zi, float inv_xz, float inv_y) {
+// The cell's z then x lerps: its value at the bottom (lo) and the per-block y step.
+MCT_HD MCT_FI void cell_lo_step(float v000, float v100, float v010, float v110, float v001, float v101, float v011, float v111, int xi, int zi,
+                                float inv_xz, float inv_y, float& lo, float& step) {
     const float az = static_cast<float>(zi) * inv_xz;
     const float v00 = lerpf(az, v000, v001), v01 = lerpf(az, v010, v011);
     const float v10 = lerpf(az, v100, v101), v11 = lerpf(az, v110, v111);
     const float ax = static_cast<float>(xi) * inv_xz;
-    const float lo = lerpf(ax, v00, v10), hi = lerpf(ax, v01, v11);
-    const float step = (hi - lo) * inv_y;
+    lo = lerpf(ax, v00, v10);
+    const float hi = lerpf(ax, v01, v11);
+    step = (hi - lo) * inv_y;
+}
+
+// The value yi blocks above the bottom: lo + step * 0, then yi additions of the step.
+MCT_HD MCT_FI float cell_at(float lo, float step, int yi) {
     float value = lo + step * 0.0f;
     for (int y = 0; y < yi; ++y) value += step;
     return value;
+}
+
+MCT_HD MCT_FI float cell_fill(float v000, float v100, float v010, float v110, float v001, float v101, float v011, float v111, int xi, int yi,
+                              int zi, float inv_xz, float inv_y) {
+    float lo, step;
+    cell_lo_step(v000, v100, v010, v110, v001, v101, v011, v111, xi, zi, inv_xz, inv_y, lo, step);
+    return cell_at(lo, step, yi);
 }
 
 }  // namespace mct

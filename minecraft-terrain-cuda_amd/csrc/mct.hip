@@ -255,21 +255,29 @@ __global__ __launch_bounds__(64) void k_surface1(SurfArgs A, int n, const int* h
     surface_pass1(A, h0, h1, h2, c, (t & 15) * 16 + (t >> 4));
 }
 
-// A workgroup is 64 columns of one chunk (x 0..15 by 4 z rows, lane = (z & 3) * 16 + x), whose
-// blocks are one contiguous 64-byte run per y: the 24 KB slab is staged in LDS, walked there
-// and written back.
-__global__ __launch_bounds__(64) void k_surface2(SurfArgs A, int n, const int* h0, const int* h1, const int* h2, int* hf) {
-    __shared__ uint4 slab[kHeight * 4];
-    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= n * 256) return;   // whole workgroups (256 columns per chunk)
+// A workgroup is 64 columns of one chunk (x 0..15 by 4 z rows, lane = (z & 3) * 16 + x). What
+// the walk reads of the blocks is their class (solid / air / water / other fluid): each lane
+// packs its column's classes into 24 LDS words (2 bits per block, word w of lane l at
+// w * 64 + l, so a y step of the wave reads 64 consecutive words), 6 KB per workgroup instead
+// of the 24 KB of the column bytes; rule results go straight to the chunk in global memory.
+static_assert(kHeight % 16 == 0, "class words hold 16 blocks");
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void k_surface2(SurfArgs A, int n, const int* h0, const int* h1, const int* h2, int* hf) {
+    __shared__ uint32_t cls[(kHeight / 16) * 64];
+    // the chunk index comes from the workgroup index alone, so everything derived from the chunk
+    // (seed tables, column records, neighbour biome chunks) is wave-uniform
+    const int c = static_cast<int>(blockIdx.x >> 2);
+    if (c >= n) return;   // whole workgroups (256 columns per chunk)
     A.M = &c_model;
-    const int c = idx >> 8, t = idx & 255;
-    uint8_t* base = A.blocks + static_cast<size_t>(c) * kPer + ((t >> 4) & ~3) * 16;
-    for (int k = threadIdx.x; k < kHeight * 4; k += 64) slab[k] = *reinterpret_cast<const uint4*>(base + (k >> 2) * 256 + (k & 3) * 16);
-    __syncthreads();
-    surface_pass2(A, h0, h1, h2, hf, c, (t & 15) * 16 + (t >> 4), reinterpret_cast<uint8_t*>(slab) + threadIdx.x, 64);
-    __syncthreads();
-    for (int k = threadIdx.x; k < kHeight * 4; k += 64) *reinterpret_cast<uint4*>(base + (k >> 2) * 256 + (k & 3) * 16) = slab[k];
+    const int t = static_cast<int>((blockIdx.x & 3) * 64 + threadIdx.x);
+    const int x = t & 15, z = t >> 4;
+    const uint8_t* colp = A.blocks + static_cast<size_t>(c) * kPer + z * 16 + x;
+#pragma clang loop unroll(disable)
+    for (int w = 0; w < kHeight / 16; ++w) {
+        uint32_t bits = 0;
+        for (int j = 0; j < 16; ++j) bits |= static_cast<uint32_t>(block_class(colp[static_cast<size_t>(w * 16 + j) * 256])) << (2 * j);
+        cls[w * 64 + threadIdx.x] = bits;
+    }
+    surface_pass2<true>(A, h0, h1, h2, hf, c, x * 16 + z, (uint32_t MCT_LDS*)(cls + threadIdx.x));
 }
 
 // A chunk needs the exact pass if any column's final height differs from the height the

@@ -1028,6 +1028,15 @@ struct SurfArgs {
     const float* colrec;        // n x 256 column records (kColF floats, column = lx * 16 + lz)
 };
 
+// Block classes the surface walk distinguishes: solid, air, water (the water state itself) and
+// other fluids.
+constexpr int kClsSolid = 0, kClsAir = 1, kClsWater = 2, kClsFluid = 3;
+MCT_HD MCT_FI int block_class(int s) {
+    if ((mctgen::kAirMask >> s) & 1u) return kClsAir;
+    if (s == mctgen::kWater) return kClsWater;
+    return ((mctgen::kFluidMask >> s) & 1u) ? kClsFluid : kClsSolid;
+}
+
 struct Surf {
     const SurfArgs* A;
     const Octave* O;
@@ -1036,11 +1045,15 @@ struct Surf {
     uint8_t* blocks;
     uint8_t* col;
     int stride;
+    uint32_t MCT_LDS* cls;   // the column's block classes in LDS (word w at cls[w * 64]), or null
     int* hm;
     const int* cmap;
     const float* rec;
     ZoomColumn zoom;
     int bc_py, bc_biome;   // quart row whose 8 zoom candidates were checked; their common biome or -1
+    uint64_t bc_any[2];    // biomes among that row's 8 candidates (bit b of word b >> 6)
+    int ore_cy;            // cell row of the cached ore-channel interpolation
+    float ore_lo[kCh], ore_step[kCh];
     float prelim;
     int x, y, z;
     int grad_x, grad_z, surface_depth;
@@ -1061,15 +1074,25 @@ struct Surf {
     // around (x-2, y-2, z-2) >> 2, which depend on the quart row only: where all 8 hold the
     // same biome, that is the answer for the 4 blocks of the row, whichever quart the fiddled
     // distances would pick.
-    MCT_HD MCT_FI int biome_y(int by) {
+    MCT_HD MCT_FI void zoom_row(int by) {
         const int py = (by - 2) >> 2;
-        if (py != bc_py) {
-            bc_py = py;
-            const int b0 = quart_biome(zoom.px, py, zoom.pz);
-            bool same = true;
-            for (int i = 1; i < 8; ++i) same = same && quart_biome(zoom.px + ((i >> 2) & 1), py + ((i >> 1) & 1), zoom.pz + (i & 1)) == b0;
-            bc_biome = same ? b0 : -1;
+        if (py == bc_py) return;
+        bc_py = py;
+        const int b0 = quart_biome(zoom.px, py, zoom.pz);
+        uint64_t any0 = 0, any1 = 0;
+        bool same = true;
+        for (int i = 0; i < 8; ++i) {
+            const int b = i == 0 ? b0 : quart_biome(zoom.px + ((i >> 2) & 1), py + ((i >> 1) & 1), zoom.pz + (i & 1));
+            same = same && b == b0;
+            if (b < 64) any0 |= 1ull << b;
+            else any1 |= 1ull << (b - 64);
         }
+        bc_biome = same ? b0 : -1;
+        bc_any[0] = any0;
+        bc_any[1] = any1;
+    }
+    MCT_HD MCT_FI int biome_y(int by) {
+        zoom_row(by);
         if (bc_biome >= 0) return bc_biome;
         int qx, qy, qz;
         zoom.pick(by, qx, qy, qz);
@@ -1078,6 +1101,18 @@ struct Surf {
     MCT_HD MCT_FI int biome() {
         if (!biome_set) { biome_v = biome_y(y); biome_set = true; }
         return biome_v;
+    }
+    // Whether the block's biome is in the set {b : bit b of (lo, hi)}. The zoom picks one of the
+    // row's 8 candidate quarts, so where the set holds all or none of them the answer needs
+    // no zoom.
+    MCT_HD MCT_FI bool biome_in(uint64_t lo, uint64_t hi) {
+        if (!biome_set) {
+            zoom_row(y);
+            if (((bc_any[0] & lo) | (bc_any[1] & hi)) == 0) return false;
+            if (((bc_any[0] & ~lo) | (bc_any[1] & ~hi)) == 0) return true;
+        }
+        const int b = biome();
+        return ((b < 64 ? (lo >> b) : (hi >> (b - 64))) & 1ull) != 0;
     }
     MCT_HD MCT_FI float col_noise(int k) const { return rec[k]; }
     MCT_HD MCT_FI float temperature(int b, int by) const { return biome_temperature_col(*A->M, b, by, rec[kColPatch] != 0.0f, rec[kColSnow]); }
@@ -1114,8 +1149,22 @@ struct Surf {
     }
     MCT_HD MCT_FI int ore_vein(int k) {
         if (!ore_set) {
-            // the ore evaluators read channels from kOreChanFirst on
-            interp_channels(A->corners, cmap, lx, y - kMinY, lz, mctgen::kOreChanFirst, kCh - mctgen::kOreChanFirst, ore_ch + mctgen::kOreChanFirst);
+            // the ore evaluators read channels from kOreChanFirst on; the cell's z / x lerps
+            // are kept while the walk stays in one cell row
+            const int yy = y - kMinY, cy = yy >> 3;
+            if (cy != ore_cy) {
+                ore_cy = cy;
+                const int cx = lx >> 2, xi = lx & 3, cz = lz >> 2, zi = lz & 3;
+                const size_t u00 = cmap[cz * 5 + cx], u10 = cmap[cz * 5 + cx + 1], u01 = cmap[(cz + 1) * 5 + cx], u11 = cmap[(cz + 1) * 5 + cx + 1];
+                for (int k = mctgen::kOreChanFirst; k < kCh; ++k) {
+                    const float* a = A->corners + (u00 * kCh + k) * kRows + cy;
+                    const float* b = A->corners + (u10 * kCh + k) * kRows + cy;
+                    const float* cc = A->corners + (u01 * kCh + k) * kRows + cy;
+                    const float* d = A->corners + (u11 * kCh + k) * kRows + cy;
+                    cell_lo_step(a[0], b[0], a[1], b[1], cc[0], d[0], cc[1], d[1], xi, zi, 0.25f, 0.125f, ore_lo[k], ore_step[k]);
+                }
+            }
+            for (int k = mctgen::kOreChanFirst; k < kCh; ++k) ore_ch[k] = cell_at(ore_lo[k], ore_step[k], yy & 7);
             ore_set = true;
         }
         const float density = mctgen::eval_ore(O, x, y, z, ore_ch, 2 * k);
@@ -1128,13 +1177,13 @@ struct Surf {
         return A->M->ore_state[k][2];
     }
 
-    // the column's blocks: col[(y - minY) * stride] (the chunk in global memory, stride 256,
-    // or a workgroup's staged slab in LDS)
+    // the column's blocks: col[(y - minY) * stride] (the chunk in global memory, stride 256)
     MCT_HD MCT_FI size_t bidx(int by) const { return static_cast<size_t>(by - kMinY) * static_cast<size_t>(stride); }
     MCT_HD MCT_FI int get(int by) const { return (by < kMinY || by >= kMinY + kHeight) ? mctgen::kAir : col[bidx(by)]; }
     MCT_HD MCT_FI bool air(int s) const { return ((mctgen::kAirMask >> s) & 1u) != 0; }
     MCT_HD MCT_FI bool fluid(int s) const { return ((mctgen::kFluidMask >> s) & 1u) != 0; }
-    // setBlockState + Heightmap.update(WORLD_SURFACE_WG); `write` false moves only the height
+    // setBlockState + Heightmap.update(WORLD_SURFACE_WG) on the column in memory; `write` false
+    // moves only the height
     MCT_HD MCT_FI void set(int by, int state, bool write) {
         if (by < kMinY || by >= kMinY + kHeight) return;
         if (write) col[bidx(by)] = static_cast<uint8_t>(state);
@@ -1162,6 +1211,7 @@ MCT_HD MCT_FI void surf_init(Surf& c, const SurfArgs& A, int ci, int lx, int lz,
     c.blocks = A.blocks + static_cast<size_t>(ci) * kPer;
     c.col = c.blocks + lz * 16 + lx;
     c.stride = 256;
+    c.cls = nullptr;
     c.hm = hm;
     c.ci = ci;
     c.rec = A.colrec + (static_cast<size_t>(ci) * 256 + static_cast<size_t>(lx * 16 + lz)) * kColF;
@@ -1176,6 +1226,7 @@ MCT_HD MCT_FI void surf_init(Surf& c, const SurfArgs& A, int ci, int lx, int lz,
     c.zoom.init(c.S->zoom_seed, c.x, c.z);
     c.bc_py = INT32_MIN;
     c.bc_biome = -1;
+    c.ore_cy = -1;
 }
 
 MCT_HD MCT_FI void surf_update_xz(Surf& c, int gx, int gz) {
@@ -1207,7 +1258,47 @@ MCT_HD MCT_FI void eroded_badlands(Surf& c, int height) {
     for (int yy = start_y; yy >= kMinY && c.air(c.get(yy)); --yy) c.set(yy, mctgen::kDefaultBlock, true);
 }
 
+// What the walk and the frozen-ocean extension test of a block: its class (kClsSolid / kClsAir
+// / kClsWater / kClsFluid). L: from the column's LDS class words (2 bits per block, 16 blocks
+// per word, Surf::cls), else from the block in memory.
+template <bool L>
+MCT_HD MCT_FI int block_at(const Surf& c, int by) {
+    if (by < kMinY || by >= kMinY + kHeight) return kClsAir;
+    if constexpr (L) {
+        const int r = by - kMinY;
+        return static_cast<int>((c.cls[(r >> 4) * 64] >> ((r & 15) * 2)) & 3u);
+    } else {
+        return block_class(c.col[c.bidx(by)]);
+    }
+}
+
+// Surf::set, keeping the LDS class words in step when L.
+template <bool L>
+MCT_HD MCT_FI void block_set(Surf& c, int by, int state, bool write) {
+    if constexpr (L) {
+        if (by < kMinY || by >= kMinY + kHeight) return;
+        if (write) {
+            c.col[c.bidx(by)] = static_cast<uint8_t>(state);
+            const int r = by - kMinY, sh = (r & 15) * 2;
+            uint32_t MCT_LDS* w = c.cls + (r >> 4) * 64;
+            *w = (*w & ~(3u << sh)) | (static_cast<uint32_t>(block_class(state)) << sh);
+        }
+        const int first = *c.hm;
+        if (by <= first - 2) return;
+        if (!c.air(state)) {
+            if (by >= first) *c.hm = by + 1;
+        } else if (first - 1 == by) {
+            for (int yy = by - 1; yy >= kMinY; --yy)
+                if (block_at<true>(c, yy) != kClsAir) { *c.hm = yy + 1; return; }
+            *c.hm = kMinY;
+        }
+    } else {
+        c.set(by, state, write);
+    }
+}
+
 // MaterialSystem frozen-ocean extension; with write == false only the height moves.
+template <bool L>
 MCT_HD MCT_FI void frozen_ocean(Surf& c, int min_surface, int biome, int height, bool write) {
     const double bx = c.x, bz = c.z;
     const double a = fabs(mctgen::surf_iceberg_surface(c.O, bx, 0.0, bz) * 8.25);
@@ -1226,37 +1317,37 @@ MCT_HD MCT_FI void frozen_ocean(Surf& c, int min_surface, int biome, int height,
     int snow = 0;
     const int itop = d2i(top), ibottom = d2i(bottom);
     for (int yy = jmaxi(height, itop + 1); yy >= min_surface; --yy) {
-        const int cur = c.get(yy);
+        const int cur = block_at<L>(c, yy);
         bool place = false;
-        if (c.air(cur) && yy < itop && next_double(r) > 0.01) place = true;
-        else if (cur == mctgen::kWater && yy > ibottom && yy < kSea && next_double(r) > 0.15) place = true;
+        if (cur == kClsAir && yy < itop && next_double(r) > 0.01) place = true;
+        else if (cur == kClsWater && yy > ibottom && yy < kSea && next_double(r) > 0.15) place = true;
         if (place) {
             if (snow <= max_snow && yy > min_snow_height) {
-                c.set(yy, mctgen::kSnowBlock, write);
+                block_set<L>(c, yy, mctgen::kSnowBlock, write);
                 ++snow;
             } else {
-                c.set(yy, mctgen::kPackedIce, write);
+                block_set<L>(c, yy, mctgen::kPackedIce, write);
             }
         }
     }
 }
 
 // The material-rule walk of one column from `height` down, then the frozen-ocean extension.
+template <bool L>
 MCT_HD MCT_FI void surface_walk(Surf& c, int height, int surface_biome, int starting) {
     int stone_above = 0, water_height = INT32_MIN, next_ceiling = INT32_MAX;
     for (int yy = height; yy >= kMinY; --yy) {
-        const int old = c.get(yy);
-        if (c.air(old)) {
+        const int old = block_at<L>(c, yy);
+        if (old == kClsAir) {
             stone_above = 0;
             water_height = INT32_MIN;
-        } else if (c.fluid(old)) {
+        } else if (old != kClsSolid) {
             if (water_height == INT32_MIN) water_height = yy + 1;
         } else {
             if (next_ceiling >= yy) {
                 next_ceiling = kWayBelow;
                 for (int la = yy - 1; la >= kMinY - 1; --la) {
-                    const int s = c.get(la);
-                    if (c.air(s) || c.fluid(s)) { next_ceiling = la + 1; break; }
+                    if (block_at<L>(c, la) != kClsSolid) { next_ceiling = la + 1; break; }
                 }
             }
             ++stone_above;
@@ -1267,11 +1358,11 @@ MCT_HD MCT_FI void surface_walk(Surf& c, int height, int surface_biome, int star
             c.biome_set = false;
             c.ore_set = false;
             const int state = mctgen::apply_rules(c);
-            if (state >= 0) c.set(yy, state, true);
+            if (state >= 0) block_set<L>(c, yy, state, true);
         }
     }
     if (surface_biome == mctgen::kBiomeFrozenOcean || surface_biome == mctgen::kBiomeDeepFrozenOcean)
-        frozen_ocean(c, c.min_surface_level(), surface_biome, starting, true);
+        frozen_ocean<L>(c, c.min_surface_level(), surface_biome, starting, true);
 }
 
 // Pass 1 of a column: eroded-badlands extension (real) -> h1, frozen-ocean dry run -> h2.
@@ -1287,16 +1378,17 @@ MCT_HD MCT_FI void surface_pass1(const SurfArgs& A, const int* h0, int* h1, int*
     h1[hidx] = hm;
     if (sb == mctgen::kBiomeFrozenOcean || sb == mctgen::kBiomeDeepFrozenOcean) {
         surf_update_xz(c, 0, 0);
-        frozen_ocean(c, c.min_surface_level(), sb, starting, false);
+        frozen_ocean<false>(c, c.min_surface_level(), sb, starting, false);
     }
     h2[hidx] = hm;
 }
 
 // Pass 2 of a column, assuming earlier columns (x-major, z-minor order) end at h2 and later
 // ones are still at h0; the final height goes to hf (checked against h2 afterwards).
-// slab: the column's blocks staged elsewhere (LDS) as slab[(y - minY) * stride], or null.
-MCT_HD MCT_FI void surface_pass2(const SurfArgs& A, const int* h0, const int* h1, const int* h2, int* hf, int ci, int col, uint8_t* slab = nullptr,
-                                 int stride = 0) {
+// L: the column's block classes are staged in LDS at cls (Surf::cls).
+template <bool L>
+MCT_HD MCT_FI void surface_pass2(const SurfArgs& A, const int* h0, const int* h1, const int* h2, int* hf, int ci, int col,
+                                 uint32_t MCT_LDS* cls = nullptr) {
     const int lx = col >> 4, lz = col & 15;
     const int hidx = ci * 256 + col;
     auto H = [&](int ox, int oz) {
@@ -1310,17 +1402,14 @@ MCT_HD MCT_FI void surface_pass2(const SurfArgs& A, const int* h0, const int* h1
     int hm = h1[hidx];
     Surf c;
     surf_init(c, A, ci, lx, lz, &hm);
-    if (slab) {
-        c.col = slab;
-        c.stride = stride;
-    }
+    c.cls = cls;
     const int starting = h0[hidx];
     const int sb = c.biome_y(starting);
     const int height = H(lx, lz) + 1;
     const int gx = H(lx + 1 < 15 ? lx + 1 : 15, lz) - H(lx - 1 > 0 ? lx - 1 : 0, lz);
     const int gz = H(lx, lz + 1 < 15 ? lz + 1 : 15) - H(lx, lz - 1 > 0 ? lz - 1 : 0);
     surf_update_xz(c, gx, gz);
-    surface_walk(c, height, sb, starting);
+    surface_walk<L>(c, height, sb, starting);
     hf[hidx] = hm;
 }
 
@@ -1338,7 +1427,7 @@ MCT_HD MCT_FI void surface_exact_column(const SurfArgs& A, int* hm, int ci, int 
     const int gx = H(lx + 1 < 15 ? lx + 1 : 15, lz) - H(lx - 1 > 0 ? lx - 1 : 0, lz);
     const int gz = H(lx, lz + 1 < 15 ? lz + 1 : 15) - H(lx, lz - 1 > 0 ? lz - 1 : 0);
     surf_update_xz(c, gx, gz);
-    surface_walk(c, height, sb, starting);
+    surface_walk<false>(c, height, sb, starting);
 }
 
 }  // namespace mct
