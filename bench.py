@@ -41,10 +41,10 @@ VALU_PEAK_TOPS = 256 * 64 * 2.4e9 / 1e12
 #   HBM bytes       FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 per thread, the factors from a plain
 #                   1 GiB device-to-device copy measured in the same runs
 WORK_PER_UNIT = {
-    'surface2': {'unit': 'block column', 'valu': 106741.6, 'hbm': 1036.64 + 392.00},
-    'corners': {'unit': 'corner', 'valu': 50108.0, 'hbm': 31.3},   # per corner evaluated (the grid has idle tail threads)
-    'biome_quarts': {'unit': 'run of 8 quarts', 'valu': 117936.2 + 8 * 149.0, 'hbm': 449.64 + 101.35 + 8 * (1.51 + 56.06)},   # targets + search
-    'fill': {'unit': 'block column of a cell (8 blocks)', 'valu': 1550.2, 'hbm': 12.65 + 8.00},
+    'surface2': {'unit': 'block column', 'valu': 64086.6, 'hbm': 1033.38 + 114.38},
+    'corners': {'unit': 'corner', 'valu': 36340.3, 'hbm': 30.0},   # per corner evaluated (whole idle workgroups past the distinct count are not units)
+    'biome_quarts': {'unit': 'run of 8 quarts', 'valu': 108288.5 + 8 * 150.0, 'hbm': 449.51 + 101.51 + 8 * (1.51 + 56.08)},   # search + targets
+    'fill': {'unit': 'block column of a cell (8 blocks)', 'valu': 1548.5, 'hbm': 12.65 + 8.00},
 }
 
 
