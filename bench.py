@@ -98,7 +98,7 @@ def cpu_baseline(seconds_target=15.0):
             'sample': '%dx%d-chunk region at (0,0), seed %d, reference cpu/ port region mode, %.1f s' % (w2, w2, SEED, dt)}
 
 
-def batched_vs_plain(mct, batch):
+def batched_vs_plain(mct, batch, tuning=None):
     """The 256-seed uneven workload in one mixed-seed call against the same chunks generated
     plainly, one seed's list per call (outputs in HBM both ways)."""
     import workloads
@@ -108,6 +108,8 @@ def batched_vs_plain(mct, batch):
     n = len(chunks)
     d_t, d_b = mct.alloc_device(n * mct.BLOCKS_PER_CHUNK), mct.alloc_device(n * mct.QUARTS_PER_CHUNK)
     g = mct.FullTerrainGenerator(max_batch_chunks=batch, pipeline_slots=2, seed_capacity=256)
+    if tuning is not None:
+        g.autotune(seeds[:batch], chunks[:batch])        # shapes for this mix, before the timed calls
     g.generate_device(seeds, chunks, d_t, d_b)          # tables resident, buffers sized
     t0 = time.perf_counter()
     g.generate_device(seeds, chunks, d_t, d_b)
@@ -259,6 +261,7 @@ def main():
     ap.add_argument('--no-cpu', action='store_true')
     ap.add_argument('--no-serve', action='store_true', help='skip the served-request measurement')
     ap.add_argument('--no-stress', action='store_true', help='skip the 1,024-seed stress leg')
+    ap.add_argument('--no-tune', action='store_true', help='default launch shapes (no autotuning)')
     ap.add_argument('--plain-only', action='store_true', help='only the plain workload (no batched or served legs), e.g. under a profiler')
     ap.add_argument('--copy-calibration', type=int, default=0, metavar='BYTES',
                     help='also run one device-to-device copy of BYTES (the reference point for HBM counters)')
@@ -274,6 +277,9 @@ def main():
     g = mct.FullTerrainGenerator(max_batch_chunks=a.batch, pipeline_slots=a.slots, seed_capacity=256)
     d_t = mct.alloc_device(n * mct.BLOCKS_PER_CHUNK)
     d_b = mct.alloc_device(n * mct.QUARTS_PER_CHUNK)
+    tuning = None
+    if not a.no_tune:   # launch shapes timed on the workload's first batch, before any timed step
+        tuning = g.autotune(seeds[:a.batch], chunks[:a.batch])
     for _ in range(a.warmup):
         g.generate_device(seeds, chunks, d_t, d_b)
     t0 = time.perf_counter()
@@ -284,6 +290,8 @@ def main():
     # one profiled step on a single-slot generator (no overlap between batches, so each
     # kernel's HIP-event time is its own): stage and kernel times
     gp = mct.FullTerrainGenerator(max_batch_chunks=a.batch, pipeline_slots=1, seed_capacity=256)
+    if tuning:
+        gp.set_tuning(tuning)
     gp.generate_device(seeds, chunks, d_t, d_b)
     tm = mct.Timing()
     gp.profile(True)
@@ -319,7 +327,7 @@ def main():
     mct.free_device(d_t)
     mct.free_device(d_b)
     g.close()
-    batched = None if (a.quick or a.plain_only) else batched_vs_plain(mct, a.batch)
+    batched = None if (a.quick or a.plain_only) else batched_vs_plain(mct, a.batch, None if a.no_tune else tuning)
     serve = None
     if not a.quick and not a.no_serve and not a.plain_only:
         serve = served(mct, plain_request_capacity(mct))
@@ -333,7 +341,7 @@ def main():
             'config': {'workload': 'plain TERRAIN, one seed (%d), %s, full height, outputs in HBM' % (
                 SEED, '32x32 region' if a.quick else '3 x 64x64-chunk regions (origin, (-2048,-2048), border chunk %d) + 8 wrap/border chunks' % BORDER_CHUNK),
                 'chunks': n, 'batch': a.batch, 'slots': a.slots},
-            'e2e_host_chunks_per_s': round(e2e, 1), 'batched': batched, 'served': serve, 'stress': mixed, 'roofline': roof, 'cpu_baseline': cpu,
+            'e2e_host_chunks_per_s': round(e2e, 1), 'tuning': tuning, 'batched': batched, 'served': serve, 'stress': mixed, 'roofline': roof, 'cpu_baseline': cpu,
             'stages_ms': stages, 'kernels_ms': kernels}
     print(json.dumps(line), flush=True)
 

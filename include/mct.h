@@ -104,6 +104,26 @@ int mct_device_to_host(void* dst, const void* src, size_t bytes);
  * finaliser. bytes_per_chunk: a positive multiple of 8. digests: count values, host memory. */
 int mct_chunk_digests(const void* d_data, int64_t count, int64_t bytes_per_chunk, uint64_t* digests);
 
+/* ---- Launch shapes and the autotuner (no counterpart in the reference API; the reference picks
+ * its tiles with a separate tuning program) ----
+ * workgroup[k]: workgroup size of kernel group k (mct_kernel_name(k)); 64, 128 or 256 where the
+ * group's kernels take any size, 0 where the kernel's layout fixes it (and 0 in mct_set_tuning
+ * also means the default, 256). run_len: quarts per R-tree search thread (1..96, default 8).
+ * ms[k]: from mct_autotune, group k's time on the sample with the chosen shape. Results never
+ * depend on the shapes. */
+typedef struct mct_tuning {
+    int32_t workgroup[24];
+    int32_t run_len;
+    int32_t reserved;
+    double ms[24];
+} mct_tuning;
+/* Times the shapes on a sample (its first max_batch_chunks chunks; chunk i under seeds[i]; outputs
+ * are discarded), keeps the fastest for each knob and writes them to out (may be NULL). */
+int mct_autotune(mct_generator* g, const int64_t* seeds, const int32_t* chunk_xz, int count, mct_tuning* out);
+int mct_get_tuning(mct_generator* g, mct_tuning* out);
+/* Applies a stored tuning (e.g. from an earlier mct_autotune on the same device and model). */
+int mct_set_tuning(mct_generator* g, const mct_tuning* t);
+
 /* Per-kernel timing (HIP events around each kernel group on its stream; adds a little
  * overhead, off by default). enable != 0 turns it on and clears the totals. */
 int mct_profile(mct_generator* g, int enable);

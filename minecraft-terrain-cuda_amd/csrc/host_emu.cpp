@@ -202,7 +202,7 @@ int mcte_generate(void* p, int64_t seed, const int32_t* xz, int n, uint8_t* terr
         for (int i = 0; i < nbk * 16; ++i) biome_column(seeds, bch.data(), bcol.data(), i);
         for (int i = 0; i < nbk * kQPer; ++i) biome_target(seeds, bch.data(), bcol.data(), targets.data(), i);
         const int ntop = M.treef ? std::min(mctgen::kTreeNodes, 384) : 0;
-        for (int i = 0; i < nbk * 16 * kBiomeRuns; ++i) biome_search(M, M.treef, ntop, targets.data(), lstar.data(), mind.data(), i, false);
+        for (int i = 0; i < nbk * 16 * biome_runs(kBiomeRunDefault); ++i) biome_search(M, M.treef, ntop, targets.data(), lstar.data(), mind.data(), i, false, kBiomeRunDefault);
         for (int i = 0; i < nbk * kQPer; ++i) biome_search_wide(M, targets.data(), lstar.data(), mind.data(), i);
         for (int i = 0; i < nbk * kQPer; ++i) biome_tie(M, targets.data(), lstar.data(), mind.data(), tie.data(), i);
         for (int b = 0; b < nbk; ++b) biome_chain_seq(M, targets.data(), lstar.data(), mind.data(), bb.data(), b);

@@ -82,14 +82,14 @@ __global__ __launch_bounds__(256) void k_biome_targets(const SeedDev* seeds, con
 // workgroup. Quarts with targets beyond the float tree's range are left to k_biome_search_wide.
 constexpr int kTreeTop = 384;
 constexpr int kSearchDepth = 12;
-__global__ __launch_bounds__(256) void k_biome_search(int nbk, const int64_t* targets, int32_t* lstar, int64_t* mind) {
+__global__ __launch_bounds__(256) void k_biome_search(int nbk, const int64_t* targets, int32_t* lstar, int64_t* mind, int run_len) {
     __shared__ TreeNodeF top[kTreeTop];
     const ModelDev& M = c_model;
     const int ntop = mctgen::kTreeNodes < kTreeTop ? mctgen::kTreeNodes : kTreeTop;
     for (int i = threadIdx.x; i < ntop * 16; i += blockDim.x) reinterpret_cast<uint32_t*>(top)[i] = reinterpret_cast<const uint32_t*>(M.treef)[i];
     __syncthreads();
     const int idx = blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx < nbk * 16 * kBiomeRuns) biome_search(M, top, ntop, targets, lstar, mind, idx, false);
+    if (idx < nbk * 16 * biome_runs(run_len)) biome_search(M, top, ntop, targets, lstar, mind, idx, false, run_len);
 }
 
 __global__ __launch_bounds__(256) void k_biome_search_wide(int nbk, const int64_t* targets, int32_t* lstar, int64_t* mind) {
@@ -98,9 +98,9 @@ __global__ __launch_bounds__(256) void k_biome_search_wide(int nbk, const int64_
 }
 
 // The search without the float tree (a model whose bounds it cannot hold exactly).
-__global__ __launch_bounds__(256) void k_biome_search_scalar(int nbk, const int64_t* targets, int32_t* lstar, int64_t* mind) {
+__global__ __launch_bounds__(256) void k_biome_search_scalar(int nbk, const int64_t* targets, int32_t* lstar, int64_t* mind, int run_len) {
     const int idx = blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx < nbk * 16 * kBiomeRuns) biome_search(c_model, nullptr, 0, targets, lstar, mind, idx, true);
+    if (idx < nbk * 16 * biome_runs(run_len)) biome_search(c_model, nullptr, 0, targets, lstar, mind, idx, true, run_len);
 }
 
 __global__ __launch_bounds__(256) void k_biome_ties(int nbk, const int64_t* targets, const int32_t* lstar, const int64_t* mind, uint8_t* tie) {
@@ -755,6 +755,11 @@ struct Generator {
     std::vector<double> done_ms;
     bool profile = false;
     double kernel_ms[kKCount] = {};
+    // Launch shapes (mct_autotune / mct_set_tuning): workgroup size per kernel group (0: fixed
+    // by the kernel's layout) and quarts per R-tree search thread. Results do not depend on them.
+    int wg[kKCount] = {};
+    int run_len = kBiomeRunDefault;
+    int W(int k) const { return wg[k] > 0 ? wg[k] : 256; }
     int64_t kernel_launches[kKCount] = {};
 
     // Records kernel-group events around `launch` on the slot stream when profiling.
@@ -939,7 +944,7 @@ struct Generator {
         const KeyCodec KC = codec_for(S.h_ch, n, 0, 4, 0, 4);
         k_col_keys<<<grid_of(ck, 256), 256, 0, st>>>(KC, S.d_ch, n, S.d_ckeys);
         timed(S, kKDedupeCols, [&] { S.dedupe->run(S.d_ckeys, ck, KC.bits, S.d_ucols, S.d_colmap, S.d_counts, st); });
-        timed(S, kKColumns, [&] { k_columns<<<grid_of(ck, 256), 256, 0, st>>>(KC, sd, ck, S.d_ucols, S.d_counts, S.d_colv); });
+        timed(S, kKColumns, [&] { k_columns<<<grid_of(ck, W(kKColumns)), W(kKColumns), 0, st>>>(KC, sd, ck, S.d_ucols, S.d_counts, S.d_colv); });
         timed(S, kKCorners, [&] { k_corners<<<grid_of(ck, kCornerCols), 256, 0, st>>>(KC, sd, ck, S.d_ucols, S.d_counts, S.d_colv, S.d_corners); });
         hip_check(hipGetLastError(), "noise kernels");
         hip_check(hipEventRecord(S.ev[3], st), "event");
@@ -947,7 +952,7 @@ struct Generator {
         uint8_t* dfill = fill ? (to_device ? fill : S.d_blocks) : nullptr;
         float* ddens = density ? (to_device ? density : S.d_density) : nullptr;
         FillArgs F{sd, S.d_ch, S.d_corners, S.d_colmap, nullptr, nullptr, nullptr, dfill, ddens};
-        timed(S, kKFill, [&] { k_fill_core<<<grid_of(static_cast<size_t>(n) * 256 * kCellsY, 256), 256, 0, st>>>(F, n); });
+        timed(S, kKFill, [&] { k_fill_core<<<grid_of(static_cast<size_t>(n) * 256 * kCellsY, W(kKFill)), W(kKFill), 0, st>>>(F, n); });
         hip_check(hipGetLastError(), "fill kernel");
         for (int e = 5; e <= 7; ++e) hip_check(hipEventRecord(S.ev[e], st), "event");
         if (!to_device) {
@@ -1003,18 +1008,20 @@ struct Generator {
         hip_check(hipEventRecord(S.ev[1], st), "event");
         const SeedDev* sd = d_seeds;
         // biomes
-        timed(S, kKBiomeColumns, [&] { k_biome_columns<<<grid_of(static_cast<size_t>(nbk) * 16, 256), 256, 0, st>>>(sd, S.d_bch, nbk, S.d_bcol); });
+        timed(S, kKBiomeColumns, [&] { k_biome_columns<<<grid_of(static_cast<size_t>(nbk) * 16, W(kKBiomeColumns)), W(kKBiomeColumns), 0, st>>>(sd, S.d_bch, nbk, S.d_bcol); });
         timed(S, kKBiomeQuarts, [&] {
             k_biome_targets<<<grid_of(static_cast<size_t>(nbk) * kQPer, 256), 256, 0, st>>>(sd, S.d_bch, nbk, S.d_bcol, S.d_targets);
             if (d_treef) {
-                k_biome_search<<<grid_of(static_cast<size_t>(nbk) * 16 * kBiomeRuns, 256), 256, 0, st>>>(nbk, S.d_targets, S.d_lstar, S.d_mind);
+                k_biome_search<<<grid_of(static_cast<size_t>(nbk) * 16 * biome_runs(run_len), W(kKBiomeQuarts)), W(kKBiomeQuarts), 0, st>>>(
+                    nbk, S.d_targets, S.d_lstar, S.d_mind, run_len);
                 k_biome_search_wide<<<grid_of(static_cast<size_t>(nbk) * kQPer, 256), 256, 0, st>>>(nbk, S.d_targets, S.d_lstar, S.d_mind);
             } else {
-                k_biome_search_scalar<<<grid_of(static_cast<size_t>(nbk) * 16 * kBiomeRuns, 256), 256, 0, st>>>(nbk, S.d_targets, S.d_lstar, S.d_mind);
+                k_biome_search_scalar<<<grid_of(static_cast<size_t>(nbk) * 16 * biome_runs(run_len), W(kKBiomeQuarts)), W(kKBiomeQuarts), 0, st>>>(
+                    nbk, S.d_targets, S.d_lstar, S.d_mind, run_len);
             }
         });
-        timed(S, kKBiomeTies, [&] { k_biome_ties<<<grid_of(static_cast<size_t>(nbk) * kQPer, 256), 256, 0, st>>>(nbk, S.d_targets, S.d_lstar, S.d_mind, S.d_tie); });
-        timed(S, kKBiomeChain, [&] { k_biome_chain<<<grid_of(static_cast<size_t>(nbk) * 64, 256), 256, 0, st>>>(nbk, S.d_targets, S.d_lstar, S.d_mind, S.d_tie, S.d_bb, d_stats); });
+        timed(S, kKBiomeTies, [&] { k_biome_ties<<<grid_of(static_cast<size_t>(nbk) * kQPer, W(kKBiomeTies)), W(kKBiomeTies), 0, st>>>(nbk, S.d_targets, S.d_lstar, S.d_mind, S.d_tie); });
+        timed(S, kKBiomeChain, [&] { k_biome_chain<<<grid_of(static_cast<size_t>(nbk) * 64, W(kKBiomeChain)), W(kKBiomeChain), 0, st>>>(nbk, S.d_targets, S.d_lstar, S.d_mind, S.d_tie, S.d_bb, d_stats); });
         quarts += static_cast<uint64_t>(nbk) * kQPer;
         hip_check(hipGetLastError(), "biome kernels");
         hip_check(hipEventRecord(S.ev[2], st), "event");
@@ -1024,7 +1031,7 @@ struct Generator {
         const KeyCodec KG = codec_for(S.h_ch, n, -16, 10, -8, 10);
         k_col_keys<<<grid_of(ck, 256), 256, 0, st>>>(KC, S.d_ch, n, S.d_ckeys);
         timed(S, kKDedupeCols, [&] { S.dedupe->run(S.d_ckeys, ck, KC.bits, S.d_ucols, S.d_colmap, S.d_counts, st); });
-        timed(S, kKColumns, [&] { k_columns<<<grid_of(ck, 256), 256, 0, st>>>(KC, sd, ck, S.d_ucols, S.d_counts, S.d_colv); });
+        timed(S, kKColumns, [&] { k_columns<<<grid_of(ck, W(kKColumns)), W(kKColumns), 0, st>>>(KC, sd, ck, S.d_ucols, S.d_counts, S.d_colv); });
         timed(S, kKCorners, [&] { k_corners<<<grid_of(ck, kCornerCols), 256, 0, st>>>(KC, sd, ck, S.d_ucols, S.d_counts, S.d_colv, S.d_corners); });
         hip_check(hipGetLastError(), "noise kernels");
         hip_check(hipEventRecord(S.ev[3], st), "event");
@@ -1032,16 +1039,16 @@ struct Generator {
         const int gk = n * kGridKeys;
         k_grid_keys<<<grid_of(gk, 256), 256, 0, st>>>(KG, S.d_ch, n, S.d_gkeys);
         timed(S, kKDedupeGrid, [&] { S.dedupe->run(S.d_gkeys, gk, KG.bits, S.d_ug, S.d_gmap, S.d_counts + 1, st); });
-        timed(S, kKFts, [&] { k_fts<<<grid_of(gk, 256), 256, 0, st>>>(KG, sd, gk, S.d_ug, S.d_counts + 1, S.d_gv); });
+        timed(S, kKFts, [&] { k_fts<<<grid_of(gk, W(kKFts)), W(kKFts), 0, st>>>(KG, sd, gk, S.d_ug, S.d_counts + 1, S.d_gv); });
         timed(S, kKGridScatter, [&] { k_grid_scatter<<<grid_of(gk, 256), 256, 0, st>>>(n, S.d_gmap, S.d_gv, S.d_grid, S.d_prelim); });
         k_aq_skip<<<grid_of(n, 256), 256, 0, st>>>(n, S.d_grid, S.d_skip);
-        timed(S, kKAqCells, [&] { k_aq_cells<<<grid_of(static_cast<size_t>(n) * kAqCells, 256), 256, 0, st>>>(sd, S.d_ch, n, S.d_grid, S.d_cloc, S.d_cfl); });
+        timed(S, kKAqCells, [&] { k_aq_cells<<<grid_of(static_cast<size_t>(n) * kAqCells, W(kKAqCells)), W(kKAqCells), 0, st>>>(sd, S.d_ch, n, S.d_grid, S.d_cloc, S.d_cfl); });
         hip_check(hipGetLastError(), "aquifer kernels");
         hip_check(hipEventRecord(S.ev[4], st), "event");
         // fill
         uint8_t* blocks = to_device ? terrain : S.d_blocks;
         FillArgs F{sd, S.d_ch, S.d_corners, S.d_colmap, S.d_cloc, S.d_cfl, S.d_skip, blocks};
-        timed(S, kKFill, [&] { k_fill<<<grid_of(static_cast<size_t>(n) * 256 * kCellsY, 256), 256, 0, st>>>(F, n); });
+        timed(S, kKFill, [&] { k_fill<<<grid_of(static_cast<size_t>(n) * 256 * kCellsY, W(kKFill)), W(kKFill), 0, st>>>(F, n); });
         hip_check(hipGetLastError(), "fill kernel");
         hip_check(hipEventRecord(S.ev[5], st), "event");
         if (aquifer) k_copy16<<<grid_of(static_cast<size_t>(n) * kPer / 16, 256), 256, 0, st>>>(static_cast<size_t>(n) * kPer / 16,
@@ -1050,8 +1057,8 @@ struct Generator {
         hip_check(hipEventRecord(S.ev[6], st), "event");
         // surface
         SurfArgs A{sd, nullptr, S.d_ch, blocks, S.d_nb, S.d_bb, S.d_corners, S.d_colmap, S.d_prelim, S.d_colrec};
-        timed(S, kKSurfaceCols, [&] { k_surface_cols<<<grid_of(static_cast<size_t>(n) * 256, 256), 256, 0, st>>>(sd, S.d_ch, n, S.d_colrec); });
-        timed(S, kKHeights, [&] { k_heights<<<grid_of(static_cast<size_t>(n) * 256, 256), 256, 0, st>>>(n, blocks, S.d_h0, S.d_counts + 2); });
+        timed(S, kKSurfaceCols, [&] { k_surface_cols<<<grid_of(static_cast<size_t>(n) * 256, W(kKSurfaceCols)), W(kKSurfaceCols), 0, st>>>(sd, S.d_ch, n, S.d_colrec); });
+        timed(S, kKHeights, [&] { k_heights<<<grid_of(static_cast<size_t>(n) * 256, W(kKHeights)), W(kKHeights), 0, st>>>(n, blocks, S.d_h0, S.d_counts + 2); });
         timed(S, kKSurface1, [&] { k_surface1<<<grid_of(static_cast<size_t>(n) * 256, 64), 64, 0, st>>>(A, n, S.d_h0, S.d_h1, S.d_h2); });
         timed(S, kKSurface2, [&] { k_surface2<<<grid_of(static_cast<size_t>(n) * 256, 64), 64, 0, st>>>(A, n, S.d_h0, S.d_h1, S.d_h2, S.d_hf); });
         k_surface_check<<<n, 256, 0, st>>>(n, S.d_h2, S.d_hf, S.d_flags, S.d_counts + 2, d_stats);
@@ -1143,6 +1150,94 @@ struct Generator {
             for (int k = 0; k < n; ++k) out[off + order[k]] = sout[k];
             off += n;
         }
+    }
+
+    // Kernel groups whose workgroup size is a launch parameter, and the sizes tried.
+    static bool tunable(int k) {
+        switch (k) {
+            case kKBiomeColumns: case kKBiomeQuarts: case kKBiomeTies: case kKBiomeChain: case kKColumns: case kKFts: case kKAqCells:
+            case kKFill: case kKHeights: case kKSurfaceCols: return true;
+            default: return false;
+        }
+    }
+
+    // Times each launch shape on a sample (the first max_batch chunks of it, outputs in HBM):
+    // per tunable kernel group, the workgroup sizes 64 / 128 / 256, and for the R-tree search the
+    // quarts per thread; keeps the fastest of each (coordinate descent, one knob at a time, the
+    // group's own HIP-event time, best of two timed passes after a warm pass).
+    void autotune(const int64_t* seeds, const int32_t* xz, int count, mct_tuning* out) {
+        count = std::min(count, max_batch);
+        collect_all(nullptr);
+        static_assert(kKCount <= 24, "mct_tuning holds 24 kernel groups");
+        uint8_t* d_t = nullptr;
+        hip_check(hipMalloc(&d_t, static_cast<size_t>(count) * kPer), "malloc");
+        const bool was = profile;
+        profile = true;
+        double best_ms[kKCount] = {};
+        auto group_ms = [&](int k) {
+            double best = 1e300;
+            generate(seeds, xz, count, d_t, nullptr, nullptr, nullptr, true);
+            for (int rep = 0; rep < 2; ++rep) {
+                for (int g = 0; g < kKCount; ++g) kernel_ms[g] = 0.0, kernel_launches[g] = 0;
+                generate(seeds, xz, count, d_t, nullptr, nullptr, nullptr, true);
+                best = std::min(best, kernel_ms[k]);
+            }
+            return best;
+        };
+        try {
+            const int sizes[] = {64, 128, 256};
+            for (int k = 0; k < kKCount; ++k) {
+                if (!tunable(k)) continue;
+                int pick = W(k);
+                double pick_ms = 1e300;
+                for (int v : sizes) {
+                    wg[k] = v;
+                    const double ms = group_ms(k);
+                    if (ms < pick_ms) { pick_ms = ms; pick = v; }
+                }
+                wg[k] = pick;
+                best_ms[k] = pick_ms;
+            }
+            const int runs[] = {4, 6, 8, 12, 16};
+            int pick = run_len;
+            double pick_ms = 1e300;
+            for (int v : runs) {
+                run_len = v;
+                const double ms = group_ms(kKBiomeQuarts);
+                if (ms < pick_ms) { pick_ms = ms; pick = v; }
+            }
+            run_len = pick;
+            best_ms[kKBiomeQuarts] = pick_ms;
+        } catch (...) {
+            profile = was;
+            (void)hipFree(d_t);
+            throw;
+        }
+        profile = was;
+        for (int g = 0; g < kKCount; ++g) kernel_ms[g] = 0.0, kernel_launches[g] = 0;
+        hip_check(hipFree(d_t), "free");
+        if (out) {
+            get_tuning(out);
+            for (int g = 0; g < kKCount; ++g) out->ms[g] = best_ms[g];
+        }
+    }
+
+    void get_tuning(mct_tuning* out) const {
+        *out = mct_tuning{};
+        for (int k = 0; k < kKCount; ++k) out->workgroup[k] = tunable(k) ? W(k) : 0;
+        out->run_len = run_len;
+    }
+
+    void set_tuning(const mct_tuning& t) {
+        for (int k = 0; k < kKCount; ++k) {
+            const int v = t.workgroup[k];
+            if (tunable(k) ? !(v == 0 || v == 64 || v == 128 || v == 256) : v != 0)
+                throw Error(MCT_EINVAL, std::string("workgroup size of ") + kKernelNames[k] + " must be " + (tunable(k) ? "0, 64, 128 or 256" : "0"));
+        }
+        if (t.run_len < 1 || t.run_len > kQH) throw Error(MCT_EINVAL, "run_len must be 1.." + std::to_string(kQH));
+        collect_all(nullptr);
+        for (int k = 0; k < kKCount; ++k) wg[k] = t.workgroup[k];
+        run_len = t.run_len;
     }
 
     void generate(const int64_t* seeds, const int32_t* xz, int count, uint8_t* terrain, uint8_t* biomes, uint8_t* aquifer, mct_timing* tm,
@@ -1391,6 +1486,27 @@ int mct_generate_core_device(mct_generator* g, const int64_t* seeds, const int32
     return guard([&] {
         if (!g || count < 0 || (count > 0 && (!seeds || !chunk_xz || (!d_density && !d_fill)))) throw Error(MCT_EINVAL, "bad arguments");
         g->g->generate(seeds, chunk_xz, count, d_fill, nullptr, nullptr, timing, true, true, d_density);
+    });
+}
+
+int mct_autotune(mct_generator* g, const int64_t* seeds, const int32_t* chunk_xz, int count, mct_tuning* out) {
+    return guard([&] {
+        if (!g || count < 1 || !seeds || !chunk_xz) throw Error(MCT_EINVAL, "bad arguments");
+        g->g->autotune(seeds, chunk_xz, count, out);
+    });
+}
+
+int mct_get_tuning(mct_generator* g, mct_tuning* out) {
+    return guard([&] {
+        if (!g || !out) throw Error(MCT_EINVAL, "bad arguments");
+        g->g->get_tuning(out);
+    });
+}
+
+int mct_set_tuning(mct_generator* g, const mct_tuning* t) {
+    return guard([&] {
+        if (!g || !t) throw Error(MCT_EINVAL, "bad arguments");
+        g->g->set_tuning(*t);
     });
 }
 

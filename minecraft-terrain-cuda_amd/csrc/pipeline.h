@@ -345,8 +345,10 @@ MCT_HD MCT_FI void biome_column(const SeedDev* seeds, const ChunkRef* bch, float
     mctgen::eval_climate_columns(seeds[c.seed].oct, c.x * 16 + (k & 3) * 4, c.z * 16 + (k >> 2) * 4, bcol + static_cast<size_t>(idx) * mctgen::kNumClimateColumns);
 }
 
-constexpr int kBiomeRun = 8;
-constexpr int kBiomeRuns = (kQH + kBiomeRun - 1) / kBiomeRun;
+// Quarts per R-tree search thread (a run down one quart column): a launch parameter, 8 unless
+// tuned.
+constexpr int kBiomeRunDefault = 8;
+MCT_HD MCT_FI int biome_runs(int run_len) { return (kQH + run_len - 1) / run_len; }
 
 // Climate target of one quart (idx = biome chunk * kQPer + (qy * 4 + qz) * 4 + qx).
 MCT_HD MCT_FI void biome_target(const SeedDev* seeds, const ChunkRef* bch, const float* bcol, int64_t* targets, int idx) {
@@ -361,19 +363,20 @@ MCT_HD MCT_FI void biome_target(const SeedDev* seeds, const ChunkRef* bch, const
     t[6] = 0;
 }
 
-// L* and its distance for a run of kBiomeRun quarts down one quart column; each search is
+// L* and its distance for a run of run_len quarts down one quart column; each search is
 // bounded by the previous quart's leaf distance + 1. Targets beyond the float tree's exact
 // range take the int64 search here when `wide` is set, else they are marked (L* = -2) for a
 // separate pass (biome_search_wide).
 MCT_HD inline void biome_search(const ModelDev& M, const TreeNodeF* top, int ntop, const int64_t* targets, int32_t* lstar, int64_t* mind, int idx,
-                                bool wide) {
-    const int b = idx / (16 * kBiomeRuns), rem = idx % (16 * kBiomeRuns);
+                                bool wide, int run_len) {
+    const int runs = biome_runs(run_len);
+    const int b = idx / (16 * runs), rem = idx % (16 * runs);
     const int colk = rem & 15, run = rem >> 4;
     const int qx = colk & 3, qz = colk >> 2;
     const double inf = u2d(0x7FF0000000000000ull);
     int prev = -1;
-    const int q1 = (run + 1) * kBiomeRun < kQH ? (run + 1) * kBiomeRun : kQH;
-    for (int qy = run * kBiomeRun; qy < q1; ++qy) {
+    const int q1 = (run + 1) * run_len < kQH ? (run + 1) * run_len : kQH;
+    for (int qy = run * run_len; qy < q1; ++qy) {
         const size_t i = static_cast<size_t>(b) * kQPer + (qy * 4 + qz) * 4 + qx;
         const int64_t* t = targets + i * 7;
         bool small = M.treef != nullptr;

@@ -27,6 +27,33 @@ class MctError(RuntimeError):
         self.code = code
 
 
+class Tuning(ctypes.Structure):
+    _fields_ = [('workgroup', ctypes.c_int32 * 24), ('run_len', ctypes.c_int32), ('reserved', ctypes.c_int32), ('ms', ctypes.c_double * 24)]
+
+    def as_dict(self):
+        n = kernel_group_count()
+        names = [lib().mct_kernel_name(i).decode() for i in range(n)]
+        return {'workgroup': {names[i]: int(self.workgroup[i]) for i in range(n) if self.workgroup[i]}, 'run_len': int(self.run_len),
+                'ms': {names[i]: round(float(self.ms[i]), 4) for i in range(n) if self.ms[i] > 0}}
+
+    @classmethod
+    def from_dict(cls, d):
+        t = cls()
+        n = kernel_group_count()
+        names = [lib().mct_kernel_name(i).decode() for i in range(n)]
+        for k, v in d.get('workgroup', {}).items():
+            t.workgroup[names.index(k)] = int(v)
+        t.run_len = int(d.get('run_len', 8))
+        return t
+
+
+def kernel_group_count():
+    n = 0
+    while lib().mct_kernel_name(n):
+        n += 1
+    return n
+
+
 class Timing(ctypes.Structure):
     _fields_ = [(k, ctypes.c_double) for k in ('h2d_ms', 'biome_ms', 'noise_ms', 'aquifer_ms', 'fill_ms', 'surface_ms', 'd2h_ms', 'wall_ms')]
 
@@ -50,6 +77,9 @@ def lib():
         L.mct_generate_device.argtypes = [P, P, P, I, P, P, P]
         L.mct_biome_tie_stats.argtypes = [P, P]
         L.mct_generate_core.argtypes = [P, P, P, I, P, P, P]
+        L.mct_autotune.argtypes = [P, P, P, I, P]
+        L.mct_get_tuning.argtypes = [P, P]
+        L.mct_set_tuning.argtypes = [P, P]
         L.mct_generate_core_device.argtypes = [P, P, P, I, P, P, P]
         L.mct_last_batch_done_ms.argtypes = [P, P, I, ctypes.POINTER(I)]
         L.mct_alloc_pinned.argtypes = [SZ]
@@ -142,6 +172,22 @@ class FullTerrainGenerator:
         if len(s) != len(xz):
             raise ValueError('one seed per chunk')
         _check(lib().mct_generate_device(self.h, _ptr(s), _ptr(xz), len(xz), d_terrain, d_biomes, ctypes.byref(timing) if timing is not None else None))
+
+    def autotune(self, seeds, chunks):
+        """Times the launch shapes on the sample and keeps the fastest; returns the tuning."""
+        xz = _xz(chunks)
+        s = np.ascontiguousarray(np.broadcast_to(np.asarray(seeds, dtype=np.int64), (len(xz),)))
+        t = Tuning()
+        _check(lib().mct_autotune(self.h, _ptr(s), _ptr(xz), len(xz), ctypes.byref(t)))
+        return t.as_dict()
+
+    def tuning(self):
+        t = Tuning()
+        _check(lib().mct_get_tuning(self.h, ctypes.byref(t)))
+        return t.as_dict()
+
+    def set_tuning(self, d):
+        _check(lib().mct_set_tuning(self.h, ctypes.byref(Tuning.from_dict(d))))
 
     def generate_core(self, seeds, chunks, density=True, fill=True, timing=None):
         """Core configuration (TerrainGenerator): final density float32 (n, 384, 16, 16) and the
