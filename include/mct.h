@@ -97,6 +97,13 @@ void* mct_alloc_device(size_t bytes);
 void mct_free_device(void* p);
 int mct_device_to_host(void* dst, const void* src, size_t bytes);
 
+/* Top-down view records of count chunks of terrain (device memory when on_device != 0, else
+ * host memory): per chunk 256 records in (z * 16 + x) order, 8 bytes each: top state (highest
+ * non-air block), floor state (first non-fluid block under it), min(255, top_y - floor_y), 0,
+ * int16 top y, int16 floor y (little-endian); the column record of the reference bench's MCTD
+ * file (bench/bench_main.cpp:132-166). records: count * 2,048 bytes, host memory. */
+int mct_topdown(const uint8_t* terrain, int count, int on_device, uint8_t* records);
+
 /* Per-chunk 64-bit digests of count chunks of bytes_per_chunk bytes each at d_data (device
  * memory, e.g. the terrain or biome output of mct_generate_device), computed on the device; no
  * reference counterpart. digest = mix(sum over 8-byte little-endian words w_i, i = 0.., of

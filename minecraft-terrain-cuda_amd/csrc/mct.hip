@@ -362,6 +362,25 @@ __global__ __launch_bounds__(256) void k_zoom_selftest(int64_t zoom_seed, uint64
     atomicAdd(out + 2, static_cast<unsigned long long>(zc.refills));
 }
 
+// Top-down record of each block column of n chunks (one thread per column, t = z * 16 + x):
+// the highest non-air block below the top, then under fluid the first non-fluid block.
+// 8 bytes: top state, floor state, min(255, top_y - floor_y), 0, int16 top y, int16 floor y
+// (little-endian), the record of the reference bench's MCTD view (bench/bench_main.cpp:132-166).
+__global__ __launch_bounds__(256) void k_topdown(const uint8_t* blocks, int n, uint8_t* out) {
+    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= n * 256) return;
+    const uint8_t* b = blocks + static_cast<size_t>(idx >> 8) * kPer + (idx & 255);
+    int y = kHeight - 1;
+    while (y > 0 && ((mctgen::kAirMask >> b[y * 256]) & 1u)) --y;
+    const int top = b[y * 256], top_y = y;
+    while (y > 0 && ((mctgen::kFluidMask >> b[y * 256]) & 1u)) --y;
+    const int floor_s = b[y * 256];
+    const int ty = kMinY + top_y, fy = kMinY + y;
+    const uint32_t lo = static_cast<uint32_t>(top) | (static_cast<uint32_t>(floor_s) << 8) | (static_cast<uint32_t>(min(255, top_y - y)) << 16);
+    const uint32_t hi = (static_cast<uint32_t>(ty) & 0xFFFFu) | ((static_cast<uint32_t>(fy) & 0xFFFFu) << 16);
+    reinterpret_cast<uint2*>(out)[idx] = make_uint2(lo, hi);
+}
+
 // Per-chunk 64-bit digest of output bytes in HBM, one workgroup per chunk: the sum (mod 2^64)
 // over 8-byte words w_i of mix(w_i + (i + 1) * golden), finalised with the byte count. Used to
 // compare whole runs without copying them to the host.
@@ -1556,6 +1575,36 @@ void mct_free_device(void* p) { (void)hipFree(p); }
 
 int mct_device_to_host(void* dst, const void* src, size_t bytes) {
     return guard([&] { hip_check(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost), "device to host"); });
+}
+
+int mct_topdown(const uint8_t* terrain, int count, int on_device, uint8_t* records) {
+    return guard([&] {
+        if (count < 0 || (count > 0 && (!terrain || !records))) throw Error(MCT_EINVAL, "bad arguments");
+        if (count == 0) return;
+        constexpr int kGroup = 1 << 14;
+        uint8_t *d_in = nullptr, *d_out = nullptr;
+        try {
+            if (!on_device) hip_check(hipMalloc(&d_in, static_cast<size_t>(std::min(count, kGroup)) * kPer), "malloc");
+            hip_check(hipMalloc(&d_out, static_cast<size_t>(std::min(count, kGroup)) * 256 * 8), "malloc");
+            for (int c0 = 0; c0 < count; c0 += kGroup) {
+                const int nc = std::min(kGroup, count - c0);
+                const uint8_t* src = terrain + static_cast<size_t>(c0) * kPer;
+                if (!on_device) {
+                    hip_check(hipMemcpy(d_in, src, static_cast<size_t>(nc) * kPer, hipMemcpyHostToDevice), "upload");
+                    src = d_in;
+                }
+                k_topdown<<<grid_of(static_cast<size_t>(nc) * 256, 256), 256>>>(src, nc, d_out);
+                hip_check(hipGetLastError(), "topdown kernel");
+                hip_check(hipMemcpy(records + static_cast<size_t>(c0) * 256 * 8, d_out, static_cast<size_t>(nc) * 256 * 8, hipMemcpyDeviceToHost), "download");
+            }
+        } catch (...) {
+            (void)hipFree(d_in);
+            (void)hipFree(d_out);
+            throw;
+        }
+        (void)hipFree(d_in);
+        hip_check(hipFree(d_out), "free");
+    });
 }
 
 int mct_chunk_digests(const void* d_data, int64_t count, int64_t bytes_per_chunk, uint64_t* digests) {

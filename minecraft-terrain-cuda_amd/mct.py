@@ -90,6 +90,7 @@ def lib():
         L.mct_free_device.argtypes = [P]
         L.mct_device_to_host.argtypes = [P, P, SZ]
         L.mct_chunk_digests.argtypes = [P, I64, I64, P]
+        L.mct_topdown.argtypes = [P, I, I, P]
         L.mct_info.argtypes = [P]
         L.mct_profile.argtypes = [P, I]
         L.mct_kernel_times.argtypes = [P, P, P, I, ctypes.POINTER(I)]
@@ -278,6 +279,19 @@ def digest_host(chunks):
         i = np.arange(1, w.shape[1] + 1, dtype=np.uint64) * np.uint64(0x9E3779B97F4A7C15)
         h = mix(w + i[None, :]).sum(axis=1, dtype=np.uint64)
         return mix(h ^ np.uint64(a.shape[1]))
+
+
+def topdown(terrain, count=None):
+    """Top-down records (uint8 [n, 256, 8], columns in z * 16 + x order) of n chunks of terrain:
+    a host array [n, 384, 16, 16], or a device pointer with count."""
+    if isinstance(terrain, np.ndarray):
+        t = np.ascontiguousarray(terrain, dtype=np.uint8).reshape(-1, BLOCKS_PER_CHUNK)
+        n, ptr, dev = len(t), _ptr(t), 0
+    else:
+        n, ptr, dev = int(count), terrain, 1
+    out = np.zeros((n, 256, 8), np.uint8)
+    _check(lib().mct_topdown(ptr, n, dev, _ptr(out)))
+    return out
 
 
 def info():

@@ -97,6 +97,42 @@ def core_fill(density, sea_level=None, min_y=None):
     return np.where(density.astype(np.float64) > 0.0, 1, np.broadcast_to(fluid, density.shape)).astype(np.uint8)
 
 
+def palette_flags():
+    """(air, fluid) boolean arrays over palette indices: air is minecraft:air itself, fluid any
+    water or lava state (cpu/src/terrain_model.cpp:120-127)."""
+    import json
+    pal = json.load(open(TERRAIN_PATH))['palette']
+    air = np.array([p == 'minecraft:air' for p in pal] + [False] * (256 - len(pal)))
+    fluid = np.array([p.startswith('minecraft:water') or p.startswith('minecraft:lava') for p in pal] + [False] * (256 - len(pal)))
+    return air, fluid
+
+
+def topdown_records(terrain):
+    """Top-down column records of terrain [n, height, 16, 16] (y, z, x) as [n, 256, 8] with
+    columns in z * 16 + x order (bench/bench_main.cpp:132-166): from the top down past air
+    (stopping at y index 0) -> top; then down past fluid (stopping at 0) -> floor; bytes top,
+    floor, min(255, top_y - floor_y), 0, int16 minY + top_y, int16 minY + floor_y."""
+    import json
+    min_y = json.load(open(IR_PATH))['min_y']
+    air, fluid = palette_flags()
+    t = np.asarray(terrain).reshape(len(terrain), terrain.shape[1], 256).astype(np.int64)
+    h = t.shape[1]
+    ys = np.arange(h)[None, :, None]
+    stop_top = ~air[t] | (ys == 0)
+    top_y = h - 1 - np.argmax(stop_top[:, ::-1, :], axis=1)                 # highest y that stops the air scan
+    stop_floor = (~fluid[t] | (ys == 0)) & (ys <= top_y[:, None, :])
+    floor_y = h - 1 - np.argmax(stop_floor[:, ::-1, :], axis=1)
+    n = len(t)
+    out = np.zeros((n, 256, 8), np.uint8)
+    cols = np.arange(256)[None, :]
+    out[:, :, 0] = t[np.arange(n)[:, None], top_y, cols]
+    out[:, :, 1] = t[np.arange(n)[:, None], floor_y, cols]
+    out[:, :, 2] = np.minimum(255, top_y - floor_y)
+    out[:, :, 4:6] = (top_y + min_y).astype('<i2')[..., None].view(np.uint8).reshape(n, 256, 2)
+    out[:, :, 6:8] = (floor_y + min_y).astype('<i2')[..., None].view(np.uint8).reshape(n, 256, 2)
+    return out
+
+
 class Oracle:
     """The oracle over one IR + terrain model (the synthetic overworld by default)."""
 

@@ -60,3 +60,42 @@ def test_core_rejects_no_output(mct):
         g.generate_core(1, [(0, 0)], density=False, fill=False)
     g.close()
     assert e.value.code == 1
+
+
+def test_topdown_records_match_oracle(mct):
+    """Device top-down records of the C1 region (host and device terrain) against the numpy
+    restatement of the reference bench's MCTD column walk."""
+    import oracle
+    seed, chunks = workloads.c1_region()
+    g = mct.FullTerrainGenerator(max_batch_chunks=64, pipeline_slots=2)
+    g.set_seed(seed)
+    t, _ = g.generate(chunks)
+    want = oracle.topdown_records(t)
+    assert (mct.topdown(t) == want).all()
+    n = len(chunks)
+    d_t = mct.alloc_device(n * mct.BLOCKS_PER_CHUNK)
+    g.generate_device(np.full(n, seed, np.int64), chunks, d_t)
+    assert (mct.topdown(d_t, n) == want).all()
+    mct.free_device(d_t)
+    g.close()
+
+
+def test_region_dump_tool_end_to_end(mct, tmp_path, monkeypatch):
+    """region_dump.py on a 3 x 3 region: the dump holds the generator's bytes in row-major chunk
+    order, the MCTD raster the device records placed by chunk."""
+    import sys
+    import oracle
+    import region_dump as rd
+    monkeypatch.setattr(sys, 'argv', ['region_dump.py', '--seed', '42', '--x0', '-2', '--z0', '7', '--region', '3',
+                                      '--dump', str(tmp_path / 'd'), '--topdown', str(tmp_path / 'v.mctd')])
+    rd.main()
+    meta, t, b = rd.read_dump(str(tmp_path / 'd'))
+    order = rd.region_order(-2, 7, 3)
+    g = mct.FullTerrainGenerator(max_batch_chunks=16, pipeline_slots=1)
+    g.set_seed(42)
+    wt, wb = g.generate(order)
+    g.close()
+    assert meta['seed'] == 42 and (t == wt).all() and (b == wb).all()
+    head, raster = rd.read_topdown(str(tmp_path / 'v.mctd'))
+    assert head == {'version': 1, 'region': 3, 'x0': -2, 'z0': 7, 'min_y': -64}
+    assert (raster == rd.topdown_raster(oracle.topdown_records(wt), order, -2, 7, 3)).all()

@@ -314,6 +314,58 @@ def test_core_fill_closed_form():
     assert (low[0, :, 0, 4] == np.where(y < -60, 3, 0)).all()
 
 
+def test_topdown_records_closed_form():
+    """Hand-built columns: all air (the scan stops at the bottom index), water over stone,
+    lava over air over stone, solid to the top; records and their int16 heights."""
+    om = oracle_module()
+    air, fluid = om.palette_flags()
+    A, S, W, L = [int(np.nonzero(m)[0][0]) for m in (air, ~air & ~fluid, fluid, fluid)]
+    W = next(i for i in range(len(fluid)) if fluid[i])
+    L = next(i for i in range(len(fluid)) if fluid[i] and i != W)
+    t = np.full((1, 384, 16, 16), A, np.uint8)
+    t[0, :100, 0, 1] = S                       # stone to y index 99
+    t[0, 100:120, 0, 1] = W                    # water 100..119
+    t[0, :50, 0, 2] = S
+    t[0, 60:70, 0, 2] = L                      # lava over air (the floor scan stops at air: not fluid)
+    t[0, :, 0, 3] = S                          # solid to the top
+    r = om.topdown_records(t)[0]
+    i16 = lambda rec, k: int(np.frombuffer(rec[k:k + 2].tobytes(), '<i2')[0])
+    assert list(r[0][:4]) == [A, A, 0, 0] and i16(r[0], 4) == -64 and i16(r[0], 6) == -64
+    assert list(r[1][:4]) == [W, S, 20, 0] and i16(r[1], 4) == 119 - 64 and i16(r[1], 6) == 99 - 64
+    assert list(r[2][:4]) == [L, A, 10, 0] and i16(r[2], 4) == 69 - 64 and i16(r[2], 6) == 59 - 64
+    assert list(r[3][:4]) == [S, S, 0, 0] and i16(r[3], 4) == 383 - 64
+
+
+def test_region_dump_and_mctd_formats(tmp_path):
+    """The writers' byte layouts: row-major chunk order whatever the input order, the meta
+    line, the MCTD header and raster placement; the readers give back what was written."""
+    import struct
+    import region_dump as rd
+    order = rd.region_order(5, -3, 2)[::-1]                  # reversed input order
+    t = np.stack([np.full((384, 16, 16), i, np.uint8) for i in range(4)])
+    b = np.stack([np.full((96, 4, 4), 10 + i, np.uint8) for i in range(4)])
+    rd.write_dump(str(tmp_path / 'd'), 7, 5, -3, 2, order, t, b)
+    meta, tt, bb = rd.read_dump(str(tmp_path / 'd'))
+    assert meta == {'seed': 7, 'x0': 5, 'z0': -3, 'region': 2, 'height': 384, 'min_y': -64, 'chunk_bytes': 98304, 'biome_bytes': 1536}
+    # file chunk k = row-major (cz - z0) * 2 + (cx - x0); input chunk i sat at order[i]
+    for i, (cx, cz) in enumerate(order):
+        k = (cz + 3) * 2 + (cx - 5)
+        assert (tt[k] == i).all() and (bb[k] == 10 + i).all()
+    rec = np.zeros((4, 256, 8), np.uint8)
+    for i in range(4):
+        rec[i, :, 0] = i
+        rec[i, 17, 1] = 100 + i                           # column z = 1, x = 1
+    raster = rd.topdown_raster(rec, order, 5, -3, 2)
+    rd.write_topdown(str(tmp_path / 'v.mctd'), 5, -3, 2, raster)
+    raw = open(tmp_path / 'v.mctd', 'rb').read()
+    assert raw[:4] == b'MCTD' and struct.unpack('<5i', raw[4:24]) == (1, 2, 5, -3, -64) and len(raw) == 24 + 32 * 32 * 8
+    head, back = rd.read_topdown(str(tmp_path / 'v.mctd'))
+    assert (back == raster).all()
+    for i, (cx, cz) in enumerate(order):
+        rx, rz = (cx - 5) * 16, (cz + 3) * 16
+        assert (back[rz:rz + 16, rx:rx + 16, 0] == i).all() and back[rz + 1, rx + 1, 1] == 100 + i
+
+
 def oracle_module():
     import oracle
     return oracle
