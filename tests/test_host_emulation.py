@@ -1,0 +1,49 @@
+"""The host-compiled twin of the HIP pipeline (libmct_host.so, the same generated evaluators
+and kernel bodies) against the reference's golden digests: the edge chunks, the C1 region,
+a stress sample and biome queries. CPU only."""
+import hashlib
+import os
+
+import numpy as np
+import pytest
+
+import workloads
+from conftest import GOLDEN
+
+gen_host = pytest.importorskip('gen_host')
+
+
+def digest(a, size=8):
+    return np.frombuffer(hashlib.blake2b(np.ascontiguousarray(a).tobytes(), digest_size=size).digest(), np.uint8)
+
+
+@pytest.fixture(scope='module')
+def host():
+    if not os.path.exists(gen_host.LIB_PATH):
+        pytest.skip('libmct_host.so not built')
+    return gen_host.HostGenerator()
+
+
+def test_edge_chunks(host, golden):
+    for i, (s, c) in enumerate(zip(golden['edge_seed'], golden['edge_xz'])):
+        t, b = host.generate(int(s), [tuple(c)], threads=1)
+        assert (digest(t[0]) == golden['edge_terrain'][i]).all() and (digest(b[0]) == golden['edge_biomes'][i]).all(), (s, c)
+
+
+def test_c1_region_in_parallel(host, golden):
+    seed, chunks = workloads.c1_region()
+    t, b = host.generate(seed, chunks, threads=8)
+    for i in range(len(chunks)):
+        assert (digest(t[i]) == golden['c1_terrain'][i]).all() and (digest(b[i]) == golden['c1_biomes'][i]).all(), i
+
+
+def test_stress_sample_and_queries(host):
+    with np.load(os.path.join(GOLDEN, 'stress_golden.npz'), allow_pickle=False) as z:
+        g = {k: z[k] for k in z.files}
+    for k in range(0, len(g['index']), 191):
+        t, b = host.generate(int(g['seed'][k]), [tuple(g['xz'][k])], threads=1)
+        assert (digest(t[0]) == g['terrain'][k]).all() and (digest(b[0]) == g['biomes'][k]).all(), k
+    sel = np.arange(0, len(g['query_xyz']), 97)
+    for i in sel:
+        got = host.biome_query(int(g['query_seed'][i]), g['query_xyz'][i:i + 1])
+        assert got[0] == g['query_biome'][i], i
