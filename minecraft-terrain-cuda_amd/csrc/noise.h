@@ -15,10 +15,12 @@
 
 namespace mct {
 
-// One gradient-noise octave of one seed: random origin and permutation.
+// One gradient-noise octave of one seed: random origin and permutation, the permutation stored
+// twice (p[256 + i] = p[i]) so that the lattice hashes index it without masking: with the
+// cell coordinates reduced mod 256 once, every lookup index is at most 511.
 struct Octave {
     double ox, oy, oz;
-    uint8_t p[256];
+    uint8_t p[512];
 };
 
 // GRADIENT[h & 15] components packed as 2-bit fields (value + 1), entry h at bits 2h..2h+1.
@@ -31,9 +33,14 @@ MCT_HD MCT_FI float gcomp(uint32_t packed, int h) { return static_cast<float>(st
 // Math.nextDown(2^24): GradientNoise.wrap leaves |x| below it untouched.
 constexpr double kWrapLimit = 0x1.fffffffffffffp+23;   // bits 0x416FFFFFFFFFFFFF
 
-// GradientNoise.wrap: x - floor(x / 2^25 + 1/2) * 2^25 outside [-limit, limit).
+// GradientNoise.wrap: x - floor(x / 2^25 + 1/2) * 2^25 outside [-limit, limit). On the device
+// the fold is skipped for the whole wave when no lane needs it (one vote).
 MCT_HD MCT_FI double wrap(double x) {
-    if (x >= -kWrapLimit && x < kWrapLimit) return x;
+    const bool in = x >= -kWrapLimit && x < kWrapLimit;
+#if defined(__HIP_DEVICE_COMPILE__)
+    if (__builtin_amdgcn_ballot_w64(!in) == 0) return x;
+#endif
+    if (in) return x;
     double q = x / 33554432.0;
     q = q + 0.5;
     q = floor(q);
@@ -41,10 +48,15 @@ MCT_HD MCT_FI double wrap(double x) {
     return x - q;
 }
 
-// Permutation lookups p[i & 255] of one octave. PermRef reads through the octave pointer.
+// Floor of a wrapped lattice coordinate (|v| < 2^24 + 256 after wrap plus an offset in
+// [0, 256), never NaN): Mth.floor without the saturation and NaN cases, which cannot occur.
+MCT_HD MCT_FI int32_t lattice_floor(double v) { return static_cast<int32_t>(floor(v)); }
+
+// Permutation lookups of one octave, index in [0, 511] (the doubled table). PermRef reads
+// through the octave pointer.
 struct PermRef {
     const uint8_t* p;
-    MCT_HD MCT_FI int operator()(int32_t i) const { return p[i & 0xFF]; }
+    MCT_HD MCT_FI int operator()(int32_t i) const { return p[i]; }
 };
 
 #if defined(__HIP_DEVICE_COMPILE__)
@@ -69,7 +81,7 @@ struct PermBuf {
     __amdgpu_buffer_rsrc_t r;
     int off;
     __device__ MCT_FI PermBuf(__amdgpu_buffer_rsrc_t rsrc, int k) : r(rsrc), off(k * static_cast<int>(sizeof(Octave)) + static_cast<int>(offsetof(Octave, p))) {}
-    __device__ MCT_FI int operator()(int32_t i) const { return static_cast<int>(__builtin_amdgcn_raw_buffer_load_b8(r, i & 0xFF, off, 0)); }
+    __device__ MCT_FI int operator()(int32_t i) const { return static_cast<int>(__builtin_amdgcn_raw_buffer_load_b8(r, i, off, 0)); }
 };
 #endif
 
@@ -78,20 +90,23 @@ struct Hash8 {
     int h000, h100, h010, h110, h001, h101, h011, h111;
 };
 
+// p[(a + f) & 255] for the Java int sums a + f of the reference equals p2[a + (f & 255)] with
+// the doubled table (a <= 255), so each coordinate is reduced once and no lookup is masked.
 template <class Perm>
 MCT_HD MCT_FI Hash8 hash8(const Perm& P, int32_t fx, int32_t fy, int32_t fz) {
-    const int a = P(fx), b = P(iadd(fx, 1));
-    const int a0 = P(iadd(a, fy)), a1 = P(iadd(iadd(a, fy), 1));
-    const int b0 = P(iadd(b, fy)), b1 = P(iadd(iadd(b, fy), 1));
+    const int X = fx & 0xFF, Y = fy & 0xFF, Z = fz & 0xFF;
+    const int a = P(X), b = P(X + 1);
+    const int a0 = P(a + Y), a1 = P(a + Y + 1);
+    const int b0 = P(b + Y), b1 = P(b + Y + 1);
     Hash8 h;
-    h.h000 = P(iadd(a0, fz));
-    h.h100 = P(iadd(b0, fz));
-    h.h010 = P(iadd(a1, fz));
-    h.h110 = P(iadd(b1, fz));
-    h.h001 = P(iadd(iadd(a0, fz), 1));
-    h.h101 = P(iadd(iadd(b0, fz), 1));
-    h.h011 = P(iadd(iadd(a1, fz), 1));
-    h.h111 = P(iadd(iadd(b1, fz), 1));
+    h.h000 = P(a0 + Z);
+    h.h100 = P(b0 + Z);
+    h.h010 = P(a1 + Z);
+    h.h110 = P(b1 + Z);
+    h.h001 = P(a0 + Z + 1);
+    h.h101 = P(b0 + Z + 1);
+    h.h011 = P(a1 + Z + 1);
+    h.h111 = P(b1 + Z + 1);
     return h;
 }
 
@@ -123,7 +138,7 @@ MCT_HD MCT_FI float sample_lerp(const Perm& P, int32_t fx, int32_t fy, int32_t f
 template <class Perm>
 MCT_HD MCT_FI float perlin_point(const Octave& o, const Perm& P, double px, double py, double pz) {
     const double x = wrap(px) + o.ox, y = wrap(py) + o.oy, z = wrap(pz) + o.oz;
-    const int32_t fx = mfloor(x), fy = mfloor(y), fz = mfloor(z);
+    const int32_t fx = lattice_floor(x), fy = lattice_floor(y), fz = lattice_floor(z);
     const float rx = static_cast<float>(x - static_cast<double>(fx));
     const float ry = static_cast<float>(y - static_cast<double>(fy));
     const float rz = static_cast<float>(z - static_cast<double>(fz));
@@ -144,7 +159,7 @@ MCT_HD MCT_FI double fudge_y(double original_y, double rel_y, double smear) {
 template <class Perm>
 MCT_HD MCT_FI float smeared_point(const Octave& o, const Perm& P, double smear, double px, double py, double pz) {
     const double x = wrap(px) + o.ox, y = wrap(py) + o.oy, z = wrap(pz) + o.oz;
-    const int32_t fx = mfloor(x), fy = mfloor(y), fz = mfloor(z);
+    const int32_t fx = lattice_floor(x), fy = lattice_floor(y), fz = lattice_floor(z);
     const float rx = static_cast<float>(x - static_cast<double>(fx));
     const double rel_y = y - static_cast<double>(fy);
     const float rz = static_cast<float>(z - static_cast<double>(fz));
@@ -173,7 +188,7 @@ struct AxisXZ {
 // x and z (and y of plain octaves): v = wrap(b * s) + offset
 MCT_HD MCT_FI Axis axis_plain(int32_t b, double s, double offset) {
     const double v = wrap(static_cast<double>(b) * s) + offset;
-    const int32_t f = mfloor(v);
+    const int32_t f = lattice_floor(v);
     const float r = static_cast<float>(v - static_cast<double>(f));
     return {f, r, smooth(r)};
 }
@@ -182,7 +197,7 @@ MCT_HD MCT_FI Axis axis_plain(int32_t b, double s, double offset) {
 MCT_HD MCT_FI Axis axis_smeared(int32_t b, double s, double offset, double smear) {
     const double orig = static_cast<double>(b) * s;
     const double v = wrap(orig) + offset;
-    const int32_t f = mfloor(v);
+    const int32_t f = lattice_floor(v);
     const double rel = v - static_cast<double>(f);
     const float a = smooth(static_cast<float>(rel));
     const float r = static_cast<float>(rel - fudge_y(orig, rel, smear));

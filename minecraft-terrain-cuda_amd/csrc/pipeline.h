@@ -151,7 +151,7 @@ MCT_HD MCT_FI void make_octave(Xoro& r, uint8_t* p, Octave& out) {
         p[i] = p[o + i];
         p[o + i] = t;
     }
-    for (int i = 0; i < 256; ++i) out.p[i] = p[i];
+    for (int i = 0; i < 256; ++i) out.p[i] = out.p[256 + i] = p[i];
 }
 
 MCT_HD MCT_FI Positional world_random(int64_t seed) {
