@@ -218,25 +218,26 @@ __global__ __launch_bounds__(256) void k_aq_cells(const SeedDev* seeds, const Ch
 
 // ---- fill --------------------------------------------------------------------------------
 // One thread per block column of a 4x8 cell (8 blocks); adjacent threads take adjacent x.
+// Cell rows are taken from the top down within a chunk, so the columns' heights are usually
+// final before the lower cells would raise them (their atomics are then skipped).
 __global__ __launch_bounds__(256) void k_fill(FillArgs A, int n) {
     const int idx = blockIdx.x * blockDim.x + threadIdx.x;
     if (idx >= n * 256 * kCellsY) return;
-    fill_cell(A, idx / (256 * kCellsY), idx % (256 * kCellsY), nullptr);
+    const int t = idx % (256 * kCellsY);
+    fill_cell(A, idx / (256 * kCellsY), ((kCellsY - 1 - (t >> 8)) << 8) | (t & 255), nullptr);
+}
+
+// Heights start at minY before the fill raises them; the exact-pass chunk list starts empty.
+__global__ __launch_bounds__(256) void k_heights_init(int n, int* h0, int* redo_count) {
+    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx == 0) *redo_count = 0;
+    if (idx < n * 256) h0[idx] = kMinY;
 }
 
 __global__ __launch_bounds__(256) void k_fill_core(FillArgs A, int n) {
     const int idx = blockIdx.x * blockDim.x + threadIdx.x;
     if (idx >= n * 256 * kCellsY) return;
     fill_cell_core(A, idx / (256 * kCellsY), idx % (256 * kCellsY));
-}
-
-__global__ __launch_bounds__(256) void k_heights(int n, const uint8_t* blocks, int* h0, int* redo_count) {
-    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx == 0) *redo_count = 0;   // list of chunks for the exact surface pass, filled by k_surface_check
-    if (idx >= n * 256) return;
-    const int c = idx >> 8, t = idx & 255;
-    const int x = t & 15, z = t >> 4;
-    h0[c * 256 + x * 16 + z] = column_height(c_model, blocks + static_cast<size_t>(c) * kPer, x, z);
 }
 
 // Column records of the surface pass (2D rule noises, snow-line noise, frozen-patch flag).
@@ -1066,8 +1067,11 @@ struct Generator {
         hip_check(hipEventRecord(S.ev[4], st), "event");
         // fill
         uint8_t* blocks = to_device ? terrain : S.d_blocks;
-        FillArgs F{sd, S.d_ch, S.d_corners, S.d_colmap, S.d_cloc, S.d_cfl, S.d_skip, blocks};
-        timed(S, kKFill, [&] { k_fill<<<grid_of(static_cast<size_t>(n) * 256 * kCellsY, W(kKFill)), W(kKFill), 0, st>>>(F, n); });
+        FillArgs F{sd, S.d_ch, S.d_corners, S.d_colmap, S.d_cloc, S.d_cfl, S.d_skip, blocks, nullptr, nullptr};
+        FillArgs FH = F;
+        FH.h0 = S.d_h0;
+        timed(S, kKHeights, [&] { k_heights_init<<<grid_of(static_cast<size_t>(n) * 256, W(kKHeights)), W(kKHeights), 0, st>>>(n, S.d_h0, S.d_counts + 2); });
+        timed(S, kKFill, [&] { k_fill<<<grid_of(static_cast<size_t>(n) * 256 * kCellsY, W(kKFill)), W(kKFill), 0, st>>>(FH, n); });
         hip_check(hipGetLastError(), "fill kernel");
         hip_check(hipEventRecord(S.ev[5], st), "event");
         if (aquifer) k_copy16<<<grid_of(static_cast<size_t>(n) * kPer / 16, 256), 256, 0, st>>>(static_cast<size_t>(n) * kPer / 16,
@@ -1077,7 +1081,6 @@ struct Generator {
         // surface
         SurfArgs A{sd, nullptr, S.d_ch, blocks, S.d_nb, S.d_bb, S.d_corners, S.d_colmap, S.d_prelim, S.d_colrec};
         timed(S, kKSurfaceCols, [&] { k_surface_cols<<<grid_of(static_cast<size_t>(n) * 256, W(kKSurfaceCols)), W(kKSurfaceCols), 0, st>>>(sd, S.d_ch, n, S.d_colrec); });
-        timed(S, kKHeights, [&] { k_heights<<<grid_of(static_cast<size_t>(n) * 256, W(kKHeights)), W(kKHeights), 0, st>>>(n, blocks, S.d_h0, S.d_counts + 2); });
         timed(S, kKSurface1, [&] { k_surface1<<<grid_of(static_cast<size_t>(n) * 256, 64), 64, 0, st>>>(A, n, S.d_h0, S.d_h1, S.d_h2); });
         timed(S, kKSurface2, [&] { k_surface2<<<grid_of(static_cast<size_t>(n) * 256, 64), 64, 0, st>>>(A, n, S.d_h0, S.d_h1, S.d_h2, S.d_hf); });
         k_surface_check<<<n, 256, 0, st>>>(n, S.d_h2, S.d_hf, S.d_flags, S.d_counts + 2, d_stats);

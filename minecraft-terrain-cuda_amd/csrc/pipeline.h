@@ -791,6 +791,7 @@ struct FillArgs {
     const int* skip;
     uint8_t* blocks;
     float* density;             // core configuration: final density per block (nullable)
+    int* h0;                    // n x 256 initial heights (column x * 16 + z), raised here; nullable
 };
 
 // The 8 blocks of one block column inside one 4x8 cell (t = (cy * 16 + z) * 16 + x): the z
@@ -830,14 +831,26 @@ MCT_HD MCT_FI void fill_cell(const FillArgs& A, int c, int t, float* density_out
     const int32_t* fl = A.cell_fluid + static_cast<size_t>(c) * kAqCells * 2;
     const int skip = A.skip[c];
     uint8_t* out = A.blocks + static_cast<size_t>(c) * kPer + static_cast<size_t>(cy * 8 * 256 + z * 16 + x);
+    int top = -1;
     for (int yi = 0; yi < 8; ++yi) {
         const int by = kMinY + cy * 8 + yi;
         const float d = mctgen::eval_block(O, bx, by, bz, val);
         if (density_out) density_out[(cy * 8 + yi) * 256 + z * 16 + x] = d;
         const int s = aquifer_substance(O, cr.x, cr.z, loc, fl, skip, bx, by, bz, static_cast<double>(d));
-        out[yi * 256] = static_cast<uint8_t>(s < 0 ? mctgen::kDefaultBlock : s);
+        const int state = s < 0 ? mctgen::kDefaultBlock : s;
+        out[yi * 256] = static_cast<uint8_t>(state);
+        if (!((mctgen::kAirMask >> state) & 1u)) top = yi;
         for (int k = 0; k < mctgen::kNumChannels; ++k) val[k] += step[k];
     }
+#if defined(__HIP_DEVICE_COMPILE__)
+    // WORLD_SURFACE_WG after the fill: the highest non-air block + 1 of the column, the max over
+    // its cells (the heights start at minY; a plain read skips most of the atomics)
+    if (top >= 0 && A.h0) {
+        int* h = A.h0 + static_cast<size_t>(c) * 256 + x * 16 + z;
+        const int v = kMinY + cy * 8 + top + 1;
+        if (v > *reinterpret_cast<volatile int*>(h)) atomicMax(h, v);
+    }
+#endif
 }
 
 // Block ids of the core configuration's fill (no aquifers, no material rules).
