@@ -452,6 +452,59 @@ __global__ __launch_bounds__(1024) void k_scan_small(int* v, int total, int* tot
     if (t == 1023 && total_out) *total_out = part[1023];
 }
 
+// Exclusive scan of long arrays (the radix histograms) in three launches: per-tile sums, a scan
+// of the tile sums, then each tile scanned in LDS from its offset. Integer sums: the same
+// result as any other order.
+constexpr int kScanTile = 2048;   // 256 threads x 8
+__global__ __launch_bounds__(256) void k_scan_tiles(const int* v, int total, int* sums) {
+    __shared__ int part[256];
+    const int base = blockIdx.x * kScanTile;
+    int a = 0;
+    for (int j = 0; j < 8; ++j) {
+        const int i = base + j * 256 + static_cast<int>(threadIdx.x);
+        if (i < total) a += v[i];
+    }
+    part[threadIdx.x] = a;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (static_cast<int>(threadIdx.x) < w) part[threadIdx.x] += part[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) sums[blockIdx.x] = part[0];
+}
+
+__global__ __launch_bounds__(256) void k_scan_apply(int* v, int total, const int* offsets) {
+    __shared__ int tile[kScanTile];
+    __shared__ int ts[256];
+    const int base = blockIdx.x * kScanTile, t = static_cast<int>(threadIdx.x);
+    for (int j = 0; j < 8; ++j) {
+        const int i = base + j * 256 + t;
+        tile[j * 256 + t] = i < total ? v[i] : 0;
+    }
+    __syncthreads();
+    int local = 0;
+    for (int k = 0; k < 8; ++k) local += tile[t * 8 + k];
+    ts[t] = local;
+    __syncthreads();
+    for (int off = 1; off < 256; off <<= 1) {
+        const int a = t >= off ? ts[t - off] : 0;
+        __syncthreads();
+        ts[t] += a;
+        __syncthreads();
+    }
+    int run = offsets[blockIdx.x] + ts[t] - local;
+    for (int k = 0; k < 8; ++k) {
+        const int x = tile[t * 8 + k];
+        tile[t * 8 + k] = run;
+        run += x;
+    }
+    __syncthreads();
+    for (int j = 0; j < 8; ++j) {
+        const int i = base + j * 256 + t;
+        if (i < total) v[i] = tile[j * 256 + t];
+    }
+}
+
 __device__ __forceinline__ unsigned long long match_digit(int d, bool valid) {
     unsigned long long m = __ballot(valid);
     for (int b = 0; b < 8; ++b) {
@@ -593,9 +646,10 @@ unsigned grid_of(size_t threads, int block) { return static_cast<unsigned>((thre
 struct Dedupe {
     int cap = 0;
     uint64_t *ka = nullptr, *kb = nullptr;
-    int *va = nullptr, *vb = nullptr, *hist = nullptr, *flag = nullptr, *tsum = nullptr, *total = nullptr;
+    int *va = nullptr, *vb = nullptr, *hist = nullptr, *flag = nullptr, *tsum = nullptr, *total = nullptr, *hsum = nullptr;
     explicit Dedupe(int capacity) : cap(capacity) {
         const int nb = (cap + kRsTile - 1) / kRsTile;
+        hip_check(hipMalloc(&hsum, sizeof(int) * ((256 * nb + kScanTile - 1) / kScanTile + 1)), "malloc");
         hip_check(hipMalloc(&ka, sizeof(uint64_t) * cap), "malloc");
         hip_check(hipMalloc(&kb, sizeof(uint64_t) * cap), "malloc");
         hip_check(hipMalloc(&va, sizeof(int) * cap), "malloc");
@@ -606,7 +660,7 @@ struct Dedupe {
         hip_check(hipMalloc(&total, sizeof(int)), "malloc");
     }
     ~Dedupe() {
-        void* p[] = {ka, kb, va, vb, flag, hist, tsum, total};
+        void* p[] = {ka, kb, va, vb, flag, hist, tsum, total, hsum};
         for (void* q : p) (void)hipFree(q);
     }
     void run(const uint64_t* keys, int n, int key_bits, uint64_t* unique, int* map, int* count, hipStream_t s) {
@@ -619,7 +673,14 @@ struct Dedupe {
         int* vout = va;
         for (int shift = 0; shift < key_bits; shift += 8) {
             k_rs_hist<<<nb, kRsThreads, 0, s>>>(kin, n, shift, nb, hist);
-            k_scan_small<<<1, 1024, 0, s>>>(hist, 256 * nb, nullptr);
+            const int tiles = (256 * nb + kScanTile - 1) / kScanTile;
+            if (tiles == 1) {
+                k_scan_small<<<1, 1024, 0, s>>>(hist, 256 * nb, nullptr);
+            } else {
+                k_scan_tiles<<<tiles, 256, 0, s>>>(hist, 256 * nb, hsum);
+                k_scan_small<<<1, 1024, 0, s>>>(hsum, tiles, nullptr);
+                k_scan_apply<<<tiles, 256, 0, s>>>(hist, 256 * nb, hsum);
+            }
             k_rs_scatter<<<nb, kRsThreads, 0, s>>>(kin, vin, n, shift, nb, hist, kout, vout);
             kin = kout;
             vin = vout;
