@@ -113,46 +113,60 @@ __global__ __launch_bounds__(256) void k_biome_ties(int nbk, const int64_t* targ
 // window is walked by lane 0 with the full rule.
 __global__ __launch_bounds__(256) void k_biome_chain(int nbk, const int64_t* targets, const int32_t* lstar, const int64_t* mind, const uint8_t* tie,
                                                      uint8_t* biomes, unsigned long long* stats) {
+    // One wave per biome chunk, 64 chain positions per step. While the chain is in sync (the
+    // previous result is the previous position's L*) a position's result is its L* unless its
+    // tie flag is set, where the previous L* wins; from there the chain is out of sync with that
+    // leaf X, which keeps winning while it is at minimal distance (all lanes test it at once)
+    // until it loses or a position's L* is X itself. Each switch costs one ballot.
     const int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     const int lane = threadIdx.x & 63;
     if (wave >= nbk) return;
     const ModelDev& M = c_model;
     const size_t base = static_cast<size_t>(wave) * kQPer;
-    int prev = -1, prev_l = -1;
+    int prev = -1;       // result at the previous chain position
+    bool sync = true;    // prev == L* of the previous position
+    unsigned long long changed = 0, changed_biome = 0;
     for (int p0 = 0; p0 < kQPer; p0 += 64) {
         const int p = p0 + lane;
         const size_t i = base + chain_quart(p);
         const int ls = lstar[i];
         const bool t = p > 0 && tie[base + p];
-        const unsigned long long any = __ballot(t);
-        if (any == 0 && prev == prev_l) {
-            biomes[i] = M.leaf_biome[ls];
-            prev = prev_l = __shfl(ls, 63);
-        } else {
-            if (lane == 0) {
-                unsigned long long changed = 0, changed_biome = 0;
-                for (int k = 0; k < 64; ++k) {
-                    const size_t ik = base + chain_quart(p0 + k);
-                    const int l = lstar[ik];
-                    int cur = l;
-                    if (prev >= 0 && !(prev == prev_l && !tie[base + p0 + k]))
-                        cur = box_distance(M.tree[M.leaf_node[prev]], targets + ik * 7) == mind[ik] ? prev : l;
-                    if (cur != l) {
-                        ++changed;
-                        if (M.leaf_biome[cur] != M.leaf_biome[l]) ++changed_biome;
-                    }
-                    biomes[ik] = M.leaf_biome[cur];
-                    prev = cur;
-                    prev_l = l;
+        int res = ls;
+        int k = 0;
+        while (k < 64) {
+            if (sync) {
+                const unsigned long long m = __ballot(t && lane >= k);
+                if (m == 0) break;
+                const int j = __builtin_ctzll(m);
+                const int before = __shfl(ls, j > 0 ? j - 1 : 0);
+                const int X = j > 0 ? before : prev;
+                if (lane == j) res = X;
+                prev = X;
+                sync = false;
+                k = j + 1;
+            } else {
+                const bool wins = lane >= k && box_distance(M.tree[M.leaf_node[prev]], targets + i * 7) == mind[i];
+                const unsigned long long m = __ballot(lane >= k && (!wins || ls == prev));
+                if (m == 0) {
+                    if (lane >= k) res = prev;
+                    k = 64;
+                    break;
                 }
-                if (changed) {
-                    atomicAdd(stats + 1, changed);
-                    atomicAdd(stats + 2, changed_biome);
-                }
+                const int j = __builtin_ctzll(m);
+                if (lane >= k && lane < j) res = prev;
+                prev = __shfl(ls, j);
+                sync = true;
+                k = j + 1;
             }
-            prev = __shfl(prev, 0);
-            prev_l = __shfl(prev_l, 0);
         }
+        if (sync) prev = __shfl(ls, 63);
+        biomes[i] = M.leaf_biome[res];
+        changed += static_cast<unsigned long long>(__popcll(__ballot(res != ls)));
+        changed_biome += static_cast<unsigned long long>(__popcll(__ballot(res != ls && M.leaf_biome[res] != M.leaf_biome[ls])));
+    }
+    if (lane == 0 && changed) {
+        atomicAdd(stats + 1, changed);
+        atomicAdd(stats + 2, changed_biome);
     }
 }
 

@@ -161,5 +161,14 @@ def test_tie_statistics_are_counted(mct):
     g.generate(workloads.region(0, 0, 8))
     st = g.biome_tie_stats()
     g.close()
-    assert st['quarts'] >= 100 * 1536          # (8 + 2)^2 biome chunks
+    assert st['quarts'] == 100 * 1536          # (8 + 2)^2 biome chunks
     assert 0 < st['overrides'] < st['quarts'] // 100
+    # the counts the sequential lane-0 walk of the chain gave on the same inputs (its biomes
+    # matched the reference's on every chunk of the plain and stress checks)
+    assert (st['overrides'], st['biome_changes']) == (78, 0)
+    g = mct.FullTerrainGenerator(max_batch_chunks=4096, pipeline_slots=1, seed_capacity=1024)
+    s, w, xz = workloads.stress()
+    g.generate_multi(s[w[:20000]], xz[:20000])
+    st = g.biome_tie_stats()
+    g.close()
+    assert (st['quarts'], st['overrides'], st['biome_changes']) == (276480000, 164237, 91449)
