@@ -255,7 +255,7 @@ def main():
     ap.add_argument('--gpus', type=int, default=1)
     ap.add_argument('--steps', type=int, default=3)
     ap.add_argument('--warmup', type=int, default=1)
-    ap.add_argument('--batch', type=int, default=4096)
+    ap.add_argument('--batch', type=int, default=0, help='chunks per device batch (0: the fastest of 4096 / 8192 / 16384 on this workload)')
     ap.add_argument('--slots', type=int, default=2)
     ap.add_argument('--quick', action='store_true', help='one 32x32 region instead of the full workload')
     ap.add_argument('--no-cpu', action='store_true')
@@ -274,9 +274,20 @@ def main():
     chunks = region(0, 0, 32) if a.quick else plain_region_set()
     n = len(chunks)
     seeds = np.full(n, SEED, np.int64)
-    g = mct.FullTerrainGenerator(max_batch_chunks=a.batch, pipeline_slots=a.slots, seed_capacity=256)
     d_t = mct.alloc_device(n * mct.BLOCKS_PER_CHUNK)
     d_b = mct.alloc_device(n * mct.QUARTS_PER_CHUNK)
+    batch_trials = None
+    if a.batch == 0:   # batch size: one untimed pass per candidate, before any timed step
+        batch_trials = {}
+        for b in (4096, 8192, 16384):
+            gt = mct.FullTerrainGenerator(max_batch_chunks=b, pipeline_slots=a.slots, seed_capacity=256)
+            gt.generate_device(seeds, chunks, d_t, d_b)
+            t0 = time.perf_counter()
+            gt.generate_device(seeds, chunks, d_t, d_b)
+            batch_trials[b] = round(n / (time.perf_counter() - t0), 1)
+            gt.close()
+        a.batch = max(batch_trials, key=batch_trials.get)
+    g = mct.FullTerrainGenerator(max_batch_chunks=a.batch, pipeline_slots=a.slots, seed_capacity=256)
     tuning = None
     if not a.no_tune:   # launch shapes timed on the workload's first batch, before any timed step
         tuning = g.autotune(seeds[:a.batch], chunks[:a.batch])
@@ -341,7 +352,7 @@ def main():
             'config': {'workload': 'plain TERRAIN, one seed (%d), %s, full height, outputs in HBM' % (
                 SEED, '32x32 region' if a.quick else '3 x 64x64-chunk regions (origin, (-2048,-2048), border chunk %d) + 8 wrap/border chunks' % BORDER_CHUNK),
                 'chunks': n, 'batch': a.batch, 'slots': a.slots},
-            'e2e_host_chunks_per_s': round(e2e, 1), 'tuning': tuning, 'batched': batched, 'served': serve, 'stress': mixed, 'roofline': roof, 'cpu_baseline': cpu,
+            'e2e_host_chunks_per_s': round(e2e, 1), 'batch_trials': batch_trials, 'tuning': tuning, 'batched': batched, 'served': serve, 'stress': mixed, 'roofline': roof, 'cpu_baseline': cpu,
             'stages_ms': stages, 'kernels_ms': kernels}
     print(json.dumps(line), flush=True)
 
