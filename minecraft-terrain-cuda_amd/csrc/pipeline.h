@@ -298,6 +298,26 @@ MCT_HD MCT_FI double box_distance_f(const TreeNodeF& n, const float* t) {
     return d;
 }
 
+// Distance to the leaf reached by always descending into the nearest child box: an upper
+// bound of the minimal leaf distance, to start a search with (+1, so that leaves at exactly the
+// minimum are still taken).
+MCT_HD inline double tree_greedy_f(const TreeNodeF* T, const TreeNodeF* top, int ntop, const float* t) {
+    auto node = [&](int i) -> const TreeNodeF& { return i < ntop ? top[i] : T[i]; };
+    int n = 0;
+    for (int depth = 0; depth < 16 && node(n).leaf < 0; ++depth) {
+        const int fc = node(n).first_count;
+        const int first = fc & 0xFFFFFF, count = fc >> 24;
+        int pick = first;
+        double pd = box_distance_f(node(first), t);
+        for (int k = 1; k < count; ++k) {
+            const double d = box_distance_f(node(first + k), t);
+            if (d < pd) { pd = d; pick = first + k; }
+        }
+        n = pick;
+    }
+    return node(n).leaf >= 0 ? box_distance_f(node(n), t) : u2d(0x7FF0000000000000ull);
+}
+
 // Nodes below `ntop` are read from `top` (the first nodes of the breadth-first flattening,
 // i.e. the upper levels, staged in LDS by the caller), the rest from T.
 MCT_HD inline int tree_first_min_f(const TreeNodeF* T, const TreeNodeF* top, int ntop, const float* t, double bound, double& out_d,
@@ -398,7 +418,7 @@ MCT_HD inline void biome_search(const ModelDev& M, const TreeNodeF* top, int nto
             continue;
         }
         double dd = 0.0;
-        const double bound = prev >= 0 ? box_distance_f(M.treef[M.leaf_node[prev]], td) + 1.0 : inf;
+        const double bound = (prev >= 0 ? box_distance_f(M.treef[M.leaf_node[prev]], td) : tree_greedy_f(M.treef, top, ntop, td)) + 1.0;
         int leaf = tree_first_min_f(M.treef, top, ntop, td, bound, dd);
         if (leaf < 0) leaf = tree_first_min_f(M.treef, top, ntop, td, inf, dd);
         lstar[i] = leaf;
@@ -1067,6 +1087,7 @@ struct Surf {
     const float* rec;
     ZoomColumn zoom;
     int bc_py, bc_biome;   // quart row whose 8 zoom candidates were checked; their common biome or -1
+    int bc_first;          // the first candidate's biome (a valid placeholder)
     uint64_t bc_any[2];    // biomes among that row's 8 candidates (bit b of word b >> 6)
     int ore_cy;            // cell row of the cached ore-channel interpolation
     float ore_lo[kCh], ore_step[kCh];
@@ -1075,6 +1096,7 @@ struct Surf {
     int grad_x, grad_z, surface_depth;
     int stone_above, stone_below, water_height;
     bool sec_set, biome_set, min_set, ore_set;
+    bool need_biome;       // a rule needed the exact biome before it was known (see surface_walk)
     double secondary;
     int biome_v, min_surface;
     float ore_ch[kCh];
@@ -1104,6 +1126,7 @@ struct Surf {
             else any1 |= 1ull << (b - 64);
         }
         bc_biome = same ? b0 : -1;
+        bc_first = b0;
         bc_any[0] = any0;
         bc_any[1] = any1;
     }
@@ -1114,16 +1137,22 @@ struct Surf {
         zoom.pick(by, qx, qy, qz);
         return quart_biome(qx, qy, qz);
     }
+    // The block's biome inside the rules. The zoom is not run here (its code would be inlined
+    // at every biome test of the rule tree): where it is needed and not yet known, the rule
+    // evaluation flags it, carries on with a placeholder, and the walk runs the zoom once and
+    // evaluates the rules again (they have no effects beyond per-block memos).
     MCT_HD MCT_FI int biome() {
-        if (!biome_set) { biome_v = biome_y(y); biome_set = true; }
+        if (!biome_set) {
+            need_biome = true;
+            return bc_first;
+        }
         return biome_v;
     }
     // Whether the block's biome is in the set {b : bit b of (lo, hi)}. The zoom picks one of the
-    // row's 8 candidate quarts, so where the set holds all or none of them the answer needs
-    // no zoom.
+    // row's 8 candidate quarts (zoom_row, set up by the walk), so where the set holds all or
+    // none of them the answer needs no zoom.
     MCT_HD MCT_FI bool biome_in(uint64_t lo, uint64_t hi) {
         if (!biome_set) {
-            zoom_row(y);
             if (((bc_any[0] & lo) | (bc_any[1] & hi)) == 0) return false;
             if (((bc_any[0] & ~lo) | (bc_any[1] & ~hi)) == 0) return true;
         }
@@ -1235,7 +1264,7 @@ MCT_HD MCT_FI void surf_init(Surf& c, const SurfArgs& A, int ci, int lx, int lz,
     const float* pc = A.prelim + static_cast<size_t>(ci) * 4;
     // chunk_surface_level: InterpolatedFunction 16 x 1 over the chunk's 4 corners
     c.prelim = cell_fill(pc[0], pc[1], pc[0], pc[1], pc[2], pc[3], pc[2], pc[3], lx, 0, lz, 1.0f / 16.0f, 1.0f / 1.0f);
-    c.sec_set = c.biome_set = c.min_set = c.ore_set = false;
+    c.sec_set = c.biome_set = c.min_set = c.ore_set = c.need_biome = false;
     c.x = cr.x * 16 + lx;
     c.z = cr.z * 16 + lz;
     c.y = 0;
@@ -1373,7 +1402,14 @@ MCT_HD MCT_FI void surface_walk(Surf& c, int height, int surface_biome, int star
             c.y = yy;
             c.biome_set = false;
             c.ore_set = false;
-            const int state = mctgen::apply_rules(c);
+            c.need_biome = false;
+            c.zoom_row(yy);
+            int state = mctgen::apply_rules(c);
+            if (c.need_biome) {
+                c.biome_v = c.biome_y(yy);
+                c.biome_set = true;
+                state = mctgen::apply_rules(c);
+            }
             if (state >= 0) block_set<L>(c, yy, state, true);
         }
     }
