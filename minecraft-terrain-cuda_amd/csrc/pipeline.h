@@ -16,6 +16,7 @@
 #pragma once
 
 #include <algorithm>
+#include <type_traits>
 #include <vector>
 
 #include "jmath.h"
@@ -1073,6 +1074,9 @@ MCT_HD MCT_FI int block_class(int s) {
     return ((mctgen::kFluidMask >> s) & 1u) ? kClsFluid : kClsSolid;
 }
 
+// Marker byte of an ore-vein block whose filler-gap test is pending (palette indices are < 64).
+constexpr int kGapMarker = 0x80;
+
 struct Surf {
     const SurfArgs* A;
     const Octave* O;
@@ -1192,7 +1196,13 @@ struct Surf {
         const int offset = jroundf(rec[mctgen::kColClay] * 4.0f);
         return S->clay[(y + offset + 192) % 192];
     }
-    MCT_HD MCT_FI int ore_vein(int k) {
+    static constexpr bool kDeferGap = false;
+    MCT_HD MCT_FI int ore_vein(int k) { return ore_vein_impl<false>(k); }
+    // Ore veins (OreVeinifier as the model's roots express it). D: the filler-gap noise is not
+    // evaluated here; the block gets a marker (kGapMarker | raw bit << 4 | k) that the walk
+    // resolves after the column, with every lane of the wave evaluating its markers together.
+    template <bool D>
+    MCT_HD MCT_FI int ore_vein_impl(int k) {
         if (!ore_set) {
             // the ore evaluators read channels from kOreChanFirst on; the cell's z / x lerps
             // are kept while the walk stays in one cell row
@@ -1217,8 +1227,16 @@ struct Surf {
         Xoro r = at(S->ore, x, y, z);
         if (next_float(r) > density) return -1;
         const float richness = mctgen::eval_ore(O, x, y, z, ore_ch, 2 * k + 1);
-        if (next_float(r) < richness && mctgen::eval_point(O, x, y, z, 5 + k) < 0.0f)
-            return next_float(r) < A->M->ore_raw[k] ? A->M->ore_state[k][1] : A->M->ore_state[k][0];
+        if (next_float(r) < richness) {
+            if constexpr (D) {
+                // the random draw that follows a passed gap test, taken now (r is this block's own)
+                const bool raw = next_float(r) < A->M->ore_raw[k];
+                return kGapMarker | (raw ? 0x10 : 0) | k;
+            } else {
+                if (mctgen::eval_point(O, x, y, z, 5 + k) < 0.0f)
+                    return next_float(r) < A->M->ore_raw[k] ? A->M->ore_state[k][1] : A->M->ore_state[k][0];
+            }
+        }
         return A->M->ore_state[k][2];
     }
 
@@ -1243,6 +1261,13 @@ struct Surf {
         }
     }
 };
+
+// The surface state with the filler-gap noise of ore veins deferred (Surf::ore_vein_impl).
+struct SurfGap : Surf {
+    static constexpr bool kDeferGap = true;
+    MCT_HD MCT_FI int ore_vein(int k) { return ore_vein_impl<true>(k); }
+};
+static_assert(mctgen::kPaletteSize <= kGapMarker, "palette indices must stay below the gap marker");
 
 MCT_HD MCT_FI void surf_init(Surf& c, const SurfArgs& A, int ci, int lx, int lz, int* hm) {
     c.A = &A;
@@ -1378,9 +1403,10 @@ MCT_HD MCT_FI void frozen_ocean(Surf& c, int min_surface, int biome, int height,
 }
 
 // The material-rule walk of one column from `height` down, then the frozen-ocean extension.
-template <bool L>
-MCT_HD MCT_FI void surface_walk(Surf& c, int height, int surface_biome, int starting) {
+template <bool L, class S>
+MCT_HD MCT_FI void surface_walk(S& c, int height, int surface_biome, int starting) {
     int stone_above = 0, water_height = INT32_MIN, next_ceiling = INT32_MAX;
+    int marker_top = kMinY - 1;   // highest block given a gap marker (S::kDeferGap)
     for (int yy = height; yy >= kMinY; --yy) {
         const int old = block_at<L>(c, yy);
         if (old == kClsAir) {
@@ -1410,7 +1436,39 @@ MCT_HD MCT_FI void surface_walk(Surf& c, int height, int surface_biome, int star
                 c.biome_set = true;
                 state = mctgen::apply_rules(c);
             }
-            if (state >= 0) block_set<L>(c, yy, state, true);
+            if (S::kDeferGap && state >= kGapMarker) {
+                // the filler's class and height; the byte carries the pending test
+                block_set<L>(c, yy, c.A->M->ore_state[state & 15][2], true);
+                c.col[c.bidx(yy)] = static_cast<uint8_t>(state);
+                if (marker_top < kMinY) marker_top = yy;
+            } else if (state >= 0) {
+                block_set<L>(c, yy, state, true);
+            }
+        }
+    }
+    if constexpr (S::kDeferGap) {
+        // Pending filler-gap tests: each lane takes its next marked block (top down), and all lanes
+        // that have one evaluate the gap noise together, until no lane of the wave has one left.
+        int scan = marker_top;
+        for (;;) {
+            int found = kMinY - 1;
+            while (scan >= kMinY) {
+                const int by = scan--;
+                if (c.col[c.bidx(by)] >= kGapMarker) { found = by; break; }
+            }
+            const bool have = found >= kMinY;
+#if defined(__HIP_DEVICE_COMPILE__)
+            if (__builtin_amdgcn_ballot_w64(have) == 0) break;
+#else
+            if (!have) break;
+#endif
+            if (have) {
+                const int m = c.col[c.bidx(found)];
+                const int k = m & 15;
+                const float gap = mctgen::eval_point(c.O, c.x, found, c.z, 5 + k);
+                const int st = gap < 0.0f ? ((m & 0x10) ? c.A->M->ore_state[k][1] : c.A->M->ore_state[k][0]) : c.A->M->ore_state[k][2];
+                c.col[c.bidx(found)] = static_cast<uint8_t>(st);
+            }
         }
     }
     if (surface_biome == mctgen::kBiomeFrozenOcean || surface_biome == mctgen::kBiomeDeepFrozenOcean)
@@ -1452,7 +1510,7 @@ MCT_HD MCT_FI void surface_pass2(const SurfArgs& A, const int* h0, const int* h1
         return v - 1;
     };
     int hm = h1[hidx];
-    Surf c;
+    typename std::conditional<L, SurfGap, Surf>::type c;
     surf_init(c, A, ci, lx, lz, &hm);
     c.cls = cls;
     const int starting = h0[hidx];
