@@ -1092,6 +1092,8 @@ struct Surf {
     ZoomColumn zoom;
     int bc_py, bc_biome;   // quart row whose 8 zoom candidates were checked; their common biome or -1
     int bc_first;          // the first candidate's biome (a valid placeholder)
+    int qoff[4];           // zoom_columns
+    uint32_t row_lo, row_hi;   // candidate biomes of quart rows bc_py and bc_py + 1
     uint64_t bc_any[2];    // biomes among that row's 8 candidates (bit b of word b >> 6)
     int ore_cy;            // cell row of the cached ore-channel interpolation
     float ore_lo[kCh], ore_step[kCh];
@@ -1116,15 +1118,43 @@ struct Surf {
     // around (x-2, y-2, z-2) >> 2, which depend on the quart row only: where all 8 hold the
     // same biome, that is the answer for the 4 blocks of the row, whichever quart the fiddled
     // distances would pick.
+    // The 4 quart columns the zoom candidates lie in ((px, pz) + (dx, dz), j = dx * 2 + dz) as
+    // offsets into the biome chunks' quarts, set once per column; a quart row's 4 biomes packed
+    // in a word, byte j.
+    MCT_HD MCT_FI void zoom_columns() {
+        for (int j = 0; j < 4; ++j) {
+            const int qx = zoom.px + (j >> 1), qz = zoom.pz + (j & 1);
+            const int dx = (qx >> 2) - cx, dz = (qz >> 2) - cz;
+            const int bc = A->nb_index[ci * 9 + (dz + 1) * 3 + dx + 1];
+            qoff[j] = bc * kQPer + (qz & 3) * 4 + (qx & 3);
+        }
+    }
+    MCT_HD MCT_FI uint32_t zoom_quart_row(int qy) const {
+        const int qmin = kMinY >> 2;
+        const int o = (clampi(qy, qmin, qmin + kQH - 1) - qmin) * 16;
+        uint32_t w = 0;
+        for (int j = 0; j < 4; ++j) w |= static_cast<uint32_t>(A->bbiomes[qoff[j] + o]) << (8 * j);
+        return w;
+    }
+    // The 8 zoom candidates of block row `by` (quart rows py and py + 1 of the 4 columns): walking
+    // down, each new row reuses the previous lower row.
     MCT_HD MCT_FI void zoom_row(int by) {
         const int py = (by - 2) >> 2;
         if (py == bc_py) return;
+        if (py == bc_py - 1) {
+            row_hi = row_lo;
+            row_lo = zoom_quart_row(py);
+        } else {
+            row_lo = zoom_quart_row(py);
+            row_hi = zoom_quart_row(py + 1);
+        }
         bc_py = py;
-        const int b0 = quart_biome(zoom.px, py, zoom.pz);
+        const int b0 = static_cast<int>(row_lo & 0xFFu);
         uint64_t any0 = 0, any1 = 0;
         bool same = true;
         for (int i = 0; i < 8; ++i) {
-            const int b = i == 0 ? b0 : quart_biome(zoom.px + ((i >> 2) & 1), py + ((i >> 1) & 1), zoom.pz + (i & 1));
+            const uint32_t row = ((i >> 1) & 1) ? row_hi : row_lo;
+            const int b = static_cast<int>((row >> (8 * (((i >> 2) & 1) * 2 + (i & 1)))) & 0xFFu);
             same = same && b == b0;
             if (b < 64) any0 |= 1ull << b;
             else any1 |= 1ull << (b - 64);
@@ -1294,6 +1324,7 @@ MCT_HD MCT_FI void surf_init(Surf& c, const SurfArgs& A, int ci, int lx, int lz,
     c.z = cr.z * 16 + lz;
     c.y = 0;
     c.zoom.init(c.S->zoom_seed, c.x, c.z);
+    c.zoom_columns();
     c.bc_py = INT32_MIN;
     c.bc_biome = -1;
     c.ore_cy = -1;
@@ -1407,8 +1438,24 @@ template <bool L, class S>
 MCT_HD MCT_FI void surface_walk(S& c, int height, int surface_biome, int starting) {
     int stone_above = 0, water_height = INT32_MIN, next_ceiling = INT32_MAX;
     int marker_top = kMinY - 1;   // highest block given a gap marker (S::kDeferGap)
+    uint32_t word = 0;   // the LDS class word of the current 16 blocks (L)
+    int word_r = -1;
     for (int yy = height; yy >= kMinY; --yy) {
-        const int old = block_at<L>(c, yy);
+        int old;
+        if constexpr (L) {
+            const int r = yy - kMinY;
+            if (r >= kHeight) {
+                old = kClsAir;
+            } else {
+                if ((r >> 4) != word_r) {
+                    word_r = r >> 4;
+                    word = c.cls[word_r * 64];
+                }
+                old = static_cast<int>((word >> ((r & 15) * 2)) & 3u);
+            }
+        } else {
+            old = block_at<L>(c, yy);
+        }
         if (old == kClsAir) {
             stone_above = 0;
             water_height = INT32_MIN;
