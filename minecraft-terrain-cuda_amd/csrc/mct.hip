@@ -278,6 +278,7 @@ __global__ __launch_bounds__(64) void k_surface1(SurfArgs A, int n, const int* h
 static_assert(kHeight % 16 == 0, "class words hold 16 blocks");
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void k_surface2(SurfArgs A, int n, const int* h0, const int* h1, const int* h2, int* hf) {
     __shared__ uint32_t cls[(kHeight / 16) * 64];
+    __shared__ float recs[kColF * 64];
     // the chunk index comes from the workgroup index alone, so everything derived from the chunk
     // (seed tables, column records, neighbour biome chunks) is wave-uniform
     const int c = static_cast<int>(blockIdx.x >> 2);
@@ -292,7 +293,9 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void k_
         for (int j = 0; j < 16; ++j) bits |= static_cast<uint32_t>(block_class(colp[static_cast<size_t>(w * 16 + j) * 256])) << (2 * j);
         cls[w * 64 + threadIdx.x] = bits;
     }
-    surface_pass2<true>(A, h0, h1, h2, hf, c, x * 16 + z, (uint32_t MCT_LDS*)(cls + threadIdx.x));
+    const float* rec = A.colrec + (static_cast<size_t>(c) * 256 + static_cast<size_t>(x * 16 + z)) * kColF;
+    for (int k = 0; k < kColF; ++k) recs[k * 64 + threadIdx.x] = rec[k];
+    surface_pass2<true>(A, h0, h1, h2, hf, c, x * 16 + z, (uint32_t MCT_LDS*)(cls + threadIdx.x), (const float MCT_LDS*)(recs + threadIdx.x));
 }
 
 // A chunk needs the exact pass if any column's final height differs from the height the

@@ -1295,7 +1295,13 @@ struct Surf {
 // The surface state with the filler-gap noise of ore veins deferred (Surf::ore_vein_impl).
 struct SurfGap : Surf {
     static constexpr bool kDeferGap = true;
+    const float MCT_LDS* lrec;   // the column record staged in LDS (value k at lrec[k * 64])
     MCT_HD MCT_FI int ore_vein(int k) { return ore_vein_impl<true>(k); }
+    MCT_HD MCT_FI float col_noise(int k) const { return lrec[k * 64]; }
+    MCT_HD MCT_FI int band() const {
+        const int offset = jroundf(lrec[mctgen::kColClay * 64] * 4.0f);
+        return S->clay[(y + offset + 192) % 192];
+    }
 };
 static_assert(mctgen::kPaletteSize <= kGapMarker, "palette indices must stay below the gap marker");
 
@@ -1542,10 +1548,10 @@ MCT_HD MCT_FI void surface_pass1(const SurfArgs& A, const int* h0, int* h1, int*
 
 // Pass 2 of a column, assuming earlier columns (x-major, z-minor order) end at h2 and later
 // ones are still at h0; the final height goes to hf (checked against h2 afterwards).
-// L: the column's block classes are staged in LDS at cls (Surf::cls).
+// L: the column's block classes are staged in LDS at cls (Surf::cls), its record at lrec.
 template <bool L>
 MCT_HD MCT_FI void surface_pass2(const SurfArgs& A, const int* h0, const int* h1, const int* h2, int* hf, int ci, int col,
-                                 uint32_t MCT_LDS* cls = nullptr) {
+                                 uint32_t MCT_LDS* cls = nullptr, const float MCT_LDS* lrec = nullptr) {
     const int lx = col >> 4, lz = col & 15;
     const int hidx = ci * 256 + col;
     auto H = [&](int ox, int oz) {
@@ -1560,6 +1566,7 @@ MCT_HD MCT_FI void surface_pass2(const SurfArgs& A, const int* h0, const int* h1
     typename std::conditional<L, SurfGap, Surf>::type c;
     surf_init(c, A, ci, lx, lz, &hm);
     c.cls = cls;
+    if constexpr (L) c.lrec = lrec;
     const int starting = h0[hidx];
     const int sb = c.biome_y(starting);
     const int height = H(lx, lz) + 1;
