@@ -1470,8 +1470,24 @@ MCT_HD MCT_FI void surface_walk(S& c, int height, int surface_biome, int startin
         } else {
             if (next_ceiling >= yy) {
                 next_ceiling = kWayBelow;
-                for (int la = yy - 1; la >= kMinY - 1; --la) {
-                    if (block_at<L>(c, la) != kClsSolid) { next_ceiling = la + 1; break; }
+                if constexpr (L) {
+                    // the highest non-solid block below yy (below the world: air), 16 blocks per
+                    // class word: bit 2i of (w | w >> 1) & 0x55555555 marks block i non-solid
+                    next_ceiling = kMinY;
+                    const int r = yy - 1 - kMinY;
+                    for (int wi = r >> 4; r >= 0 && wi >= 0; --wi) {
+                        const uint32_t w = c.cls[wi * 64];
+                        uint32_t ns = (w | (w >> 1)) & 0x55555555u;
+                        if (wi == (r >> 4)) ns &= (2u << (2 * (r & 15))) - 1u;
+                        if (ns) {
+                            next_ceiling = kMinY + wi * 16 + (31 - __builtin_clz(ns)) / 2 + 1;
+                            break;
+                        }
+                    }
+                } else {
+                    for (int la = yy - 1; la >= kMinY - 1; --la) {
+                        if (block_at<L>(c, la) != kClsSolid) { next_ceiling = la + 1; break; }
+                    }
                 }
             }
             ++stone_above;
