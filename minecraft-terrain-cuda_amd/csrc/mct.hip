@@ -231,14 +231,117 @@ __global__ __launch_bounds__(256) void k_aq_cells(const SeedDev* seeds, const Ch
 }
 
 // ---- fill --------------------------------------------------------------------------------
-// One thread per block column of a 4x8 cell (8 blocks); adjacent threads take adjacent x.
-// Cell rows are taken from the top down within a chunk, so the columns' heights are usually
-// final before the lower cells would raise them (their atomics are then skipped).
+// A workgroup fills one cell row of one chunk (W threads: W / 16 z rows of 16 x, one block
+// column of the row's 8 blocks each); rows are taken from the top down within a chunk, so the
+// columns' heights are usually final before the lower rows would raise them (their atomics are
+// then skipped). The chunk's aquifer cells the row can read are staged in LDS. The blocks
+// whose aquifer barrier / pressure part decides (a few percent of the blocks, but spread over
+// most waves that reach the aquifer) are queued in LDS and resolved after the row, densely, by
+// all threads of the workgroup.
+constexpr int kFillStage = 36;   // 3 x 3 aquifer cells by at most 4 grid rows (8 block rows +-1)
+constexpr int kFillHead = kFillStage * 6 + 4;   // ints before the per-thread arrays
+static_assert(kFillHead * 4 % 16 == 0, "per-thread LDS arrays stay 16-byte aligned");
+constexpr size_t fill_lds_bytes(int w) { return static_cast<size_t>(kFillHead) * 4 + 52 * static_cast<size_t>(w); }
+
+struct AqCellsLds {
+    const MCT_LDS int* loc;   // 4 ints per staged cell
+    const MCT_LDS int* fl;    // 2 ints per staged cell
+    int base;                 // chunk cell index of staged cell 0
+    __device__ MCT_FI void at(int i, int& x, int& y, int& z) const {
+        const MCT_LDS int* p = loc + (i - base) * 4;
+        x = p[0]; y = p[1]; z = p[2];
+    }
+    __device__ MCT_FI Fluid fluid(int i) const { return Fluid{fl[(i - base) * 2], fl[(i - base) * 2 + 1]}; }
+};
+
 __global__ __launch_bounds__(256) void k_fill(FillArgs A, int n) {
-    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= n * 256 * kCellsY) return;
-    const int t = idx % (256 * kCellsY);
-    fill_cell(A, idx / (256 * kCellsY), ((kCellsY - 1 - (t >> 8)) << 8) | (t & 255), nullptr);
+    extern __shared__ int fill_lds[];
+    const int W = static_cast<int>(blockDim.x), tid = static_cast<int>(threadIdx.x);
+    const size_t idx0 = static_cast<size_t>(blockIdx.x) * static_cast<size_t>(W);
+    if (idx0 >= static_cast<size_t>(n) * 256 * kCellsY) return;   // whole workgroups only
+    const int c = static_cast<int>(idx0 / (256 * kCellsY));
+    const int t0 = static_cast<int>(idx0 % (256 * kCellsY));
+    const int cy = kCellsY - 1 - (t0 >> 8);
+    const int col0 = t0 & 255;
+    int* sloc = fill_lds;
+    int* sfl = fill_lds + kFillStage * 4;
+    int* qn = fill_lds + kFillStage * 6;
+    int* hm = fill_lds + kFillHead;
+    float* qd = reinterpret_cast<float*>(fill_lds + kFillHead + W);
+    uint16_t* qk = reinterpret_cast<uint16_t*>(fill_lds + kFillHead + 9 * W);
+    const ChunkRef cr = A.chunks[c];
+    const int ylo = kMinY + cy * 8;
+    const int skip = A.skip[c];
+    const int gy_lo = max(grid_yf(ylo + 1) - 1, kAqMinGY), gy_hi = min(grid_yf(ylo + 8) + 1, kAqMaxGY);
+    const int base = (gy_lo - kAqMinGY) * 9;
+    if (ylo <= skip) {   // some block of the row can reach the aquifer cells
+        const int32_t* loc = A.cell_loc + static_cast<size_t>(c) * kAqCells * 4;
+        const int32_t* fl = A.cell_fluid + static_cast<size_t>(c) * kAqCells * 2;
+        for (int k = tid; k < (gy_hi - gy_lo + 1) * 9; k += W) {
+            for (int j = 0; j < 3; ++j) sloc[k * 4 + j] = loc[(base + k) * 4 + j];
+            sfl[k * 2] = fl[(base + k) * 2];
+            sfl[k * 2 + 1] = fl[(base + k) * 2 + 1];
+        }
+    }
+    if (tid == 0) *qn = 0;
+    __syncthreads();
+    const AqCellsLds C{(const MCT_LDS int*)sloc, (const MCT_LDS int*)sfl, base};
+    const Octave* O = A.seeds[cr.seed].oct;
+    const int x = (col0 + tid) & 15, z = (col0 + tid) >> 4;
+    constexpr int NC = mctgen::kNumChannels > 0 ? mctgen::kNumChannels : 1;
+    float val[NC], step[NC];
+    cell_channels(A, c, x, z, cy, val, step);
+    const int bx = cr.x * 16 + x, bz = cr.z * 16 + z;
+    uint8_t* out = A.blocks + static_cast<size_t>(c) * kPer + static_cast<size_t>(cy * 8 * 256 + z * 16 + x);
+    const uint32_t below = __builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u));   // lane id
+    int top = -1;
+    for (int yi = 0; yi < 8; ++yi) {
+        const int by = ylo + yi;
+        const float d = mctgen::eval_block(O, bx, by, bz, val);
+        AqNearest nr;
+        const int s = aquifer_front(C, cr.x, cr.z, skip, bx, by, bz, static_cast<double>(d), nr);
+        const bool pend = s == kAqPending;
+        const uint64_t m = __builtin_amdgcn_ballot_w64(pend);
+        if (m) {
+            const int lead = __builtin_ctzll(m);
+            int slot = 0;
+            if (static_cast<int>(below) == lead) slot = atomicAdd(qn, __builtin_popcountll(m));
+            slot = __builtin_amdgcn_readlane(slot, lead);
+            if (pend) {
+                const int k = slot + static_cast<int>(__builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(m >> 32), __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(m), 0u)));
+                qd[k] = d;
+                qk[k] = static_cast<uint16_t>((tid << 3) | yi);
+            }
+        }
+        if (!pend) {
+            const int state = s < 0 ? mctgen::kDefaultBlock : s;
+            out[yi * 256] = static_cast<uint8_t>(state);
+            if (!((mctgen::kAirMask >> state) & 1u)) top = yi;
+        }
+        for (int k = 0; k < mctgen::kNumChannels; ++k) val[k] += step[k];
+    }
+    hm[tid] = top;
+    __syncthreads();
+    const int q = *qn;
+    for (int e = tid; e < q; e += W) {
+        const int key = qk[e], lt = key >> 3, yi = key & 7;
+        const int ex = (col0 + lt) & 15, ez = (col0 + lt) >> 4;
+        const int ebx = cr.x * 16 + ex, eby = ylo + yi, ebz = cr.z * 16 + ez;
+        const AqNearest nr = aquifer_nearest(C, cr.x, cr.z, ebx, eby, ebz);
+        const int s = aquifer_back(O, C, ebx, eby, ebz, static_cast<double>(qd[e]), nr);
+        const int state = s < 0 ? mctgen::kDefaultBlock : s;
+        A.blocks[static_cast<size_t>(c) * kPer + static_cast<size_t>((cy * 8 + yi) * 256 + ez * 16 + ex)] = static_cast<uint8_t>(state);
+        if (!((mctgen::kAirMask >> state) & 1u)) atomicMax(&hm[lt], yi);
+    }
+    __syncthreads();
+    top = hm[tid];
+    // WORLD_SURFACE_WG after the fill: the highest non-air block + 1 of the column, the max over
+    // its cells (the heights start at minY; a plain read skips most of the atomics)
+    if (top >= 0 && A.h0) {
+        int* h = A.h0 + static_cast<size_t>(c) * 256 + x * 16 + z;
+        const int v = ylo + top + 1;
+        if (v > *reinterpret_cast<volatile int*>(h)) atomicMax(h, v);
+    }
 }
 
 // Heights start at minY before the fill raises them; the exact-pass chunk list starts empty.
@@ -1149,7 +1252,7 @@ struct Generator {
         FillArgs FH = F;
         FH.h0 = S.d_h0;
         timed(S, kKHeights, [&] { k_heights_init<<<grid_of(static_cast<size_t>(n) * 256, W(kKHeights)), W(kKHeights), 0, st>>>(n, S.d_h0, S.d_counts + 2); });
-        timed(S, kKFill, [&] { k_fill<<<grid_of(static_cast<size_t>(n) * 256 * kCellsY, W(kKFill)), W(kKFill), 0, st>>>(FH, n); });
+        timed(S, kKFill, [&] { k_fill<<<grid_of(static_cast<size_t>(n) * 256 * kCellsY, W(kKFill)), W(kKFill), fill_lds_bytes(W(kKFill)), st>>>(FH, n); });
         hip_check(hipGetLastError(), "fill kernel");
         hip_check(hipEventRecord(S.ev[5], st), "event");
         if (aquifer) k_copy16<<<grid_of(static_cast<size_t>(n) * kPer / 16, 256), 256, 0, st>>>(static_cast<size_t>(n) * kPer / 16,

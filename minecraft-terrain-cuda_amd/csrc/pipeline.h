@@ -755,13 +755,22 @@ MCT_HD MCT_FI double aquifer_pressure(const Octave* O, int x, int y, int z, doub
     return 2.0 * (noise + gradient);
 }
 
-// NoiseBasedAquifer.computeSubstance; -1 means solid (the default block).
-MCT_HD inline int aquifer_substance(const Octave* O, int cx, int cz, const int32_t* loc, const int32_t* fl, int skip_above, int x, int y, int z,
-                                    double density) {
-    if (density > 0.0) return -1;
-    const Fluid global = global_fluid(y);
-    if (y > skip_above) return fluid_at(global, y);
-    if (fluid_at(global, y) == mctgen::kLava) return mctgen::kLava;
+// The aquifer cells of a chunk as computeSubstance reads them: cell i's location (x, y, z)
+// and its fluid status. Cell index i = ((gy - kAqMinGY) * 3 + (gz - (cz - 1))) * 3 + (gx - (cx - 1)).
+struct AqCellsGlobal {
+    const int32_t* loc;   // 4 ints per cell (x, y, z, 0)
+    const int32_t* fl;    // 2 ints per cell (level, fluid)
+    MCT_HD MCT_FI void at(int i, int& x, int& y, int& z) const { x = loc[i * 4]; y = loc[i * 4 + 1]; z = loc[i * 4 + 2]; }
+    MCT_HD MCT_FI Fluid fluid(int i) const { return Fluid{fl[i * 2], fl[i * 2 + 1]}; }
+};
+
+// The three nearest aquifer cells of a block (computeSubstance's 2 x 3 x 2 scan, ties to the
+// later cell of the scan) and the fourth distance.
+struct AqNearest {
+    int i1, i2, i3, d1, d2, d3;
+};
+template <class Cells>
+MCT_HD MCT_FI AqNearest aquifer_nearest(const Cells& C, int cx, int cz, int x, int y, int z) {
     const int xa = (x - 5) >> 4, ya = grid_yf(y + 1), za = (z - 5) >> 4;
     int d1 = INT32_MAX, d2 = INT32_MAX, d3 = INT32_MAX, d4 = INT32_MAX;
     int i1 = 0, i2 = 0, i3 = 0;
@@ -770,35 +779,69 @@ MCT_HD inline int aquifer_substance(const Octave* O, int cx, int cz, const int32
             for (int oz = 0; oz <= 1; ++oz) {
                 const int gx = xa + ox, gy = ya + oy, gz = za + oz;
                 const int index = ((gy - kAqMinGY) * 3 + (gz - (cz - 1))) * 3 + (gx - (cx - 1));
-                const int dx = loc[index * 4] - x, dy = loc[index * 4 + 1] - y, dz = loc[index * 4 + 2] - z;
+                int lx, ly, lz;
+                C.at(index, lx, ly, lz);
+                const int dx = lx - x, dy = ly - y, dz = lz - z;
                 const int nd = dx * dx + dy * dy + dz * dz;
                 if (d1 >= nd) { i3 = i2; i2 = i1; i1 = index; d4 = d3; d3 = d2; d2 = d1; d1 = nd; }
                 else if (d2 >= nd) { i3 = i2; i2 = index; d4 = d3; d3 = d2; d2 = nd; }
                 else if (d3 >= nd) { i3 = index; d4 = d3; d3 = nd; }
                 else if (d4 >= nd) { d4 = nd; }
             }
-    const Fluid s1{fl[i1 * 2], fl[i1 * 2 + 1]};
-    const double sim12 = 1.0 - static_cast<double>(d2 - d1) / 25.0;
-    const int fluid = fluid_at(s1, y);
-    if (sim12 <= 0.0) return fluid;
+    return AqNearest{i1, i2, i3, d1, d2, d3};
+}
+
+// NoiseBasedAquifer.computeSubstance up to the barrier: the result, or kAqPending where the
+// barrier / pressure part decides (aquifer_back). -1 means solid (the default block).
+// sim12 = 1 - (d2 - d1) / 25 is <= 0 exactly when d2 - d1 >= 25 (the quotient of integers
+// 0..24 by 25 rounds below 1, of 25 and up to 1 or more), so the front compares integers.
+constexpr int kAqPending = -2;
+template <class Cells>
+MCT_HD MCT_FI int aquifer_front(const Cells& C, int cx, int cz, int skip_above, int x, int y, int z, double density, AqNearest& nr) {
+    if (density > 0.0) return -1;
+    const Fluid global = global_fluid(y);
+    if (y > skip_above) return fluid_at(global, y);
+    if (fluid_at(global, y) == mctgen::kLava) return mctgen::kLava;
+    nr = aquifer_nearest(C, cx, cz, x, y, z);
+    const int fluid = fluid_at(C.fluid(nr.i1), y);
+    if (nr.d2 - nr.d1 >= 25) return fluid;
     if (fluid == mctgen::kWater && fluid_at(global_fluid(y - 1), y - 1) == mctgen::kLava) return fluid;
+    return kAqPending;
+}
+
+// The rest of computeSubstance for a block the front left pending (sim12 > 0, not water over
+// the lava level): the barrier-weighted pressures of the cell pairs 1-2, 1-3, 2-3.
+template <class Cells>
+MCT_HD inline int aquifer_back(const Octave* O, const Cells& C, int x, int y, int z, double density, const AqNearest& nr) {
+    const Fluid s1 = C.fluid(nr.i1);
+    const int fluid = fluid_at(s1, y);
+    const double sim12 = 1.0 - static_cast<double>(nr.d2 - nr.d1) / 25.0;
     double barrier = 0.0;
     bool barrier_set = false;
-    const Fluid s2{fl[i2 * 2], fl[i2 * 2 + 1]};
+    const Fluid s2 = C.fluid(nr.i2);
     const double b12 = sim12 * aquifer_pressure(O, x, y, z, barrier, barrier_set, s1, s2);
     if (density + b12 > 0.0) return -1;
-    const Fluid s3{fl[i3 * 2], fl[i3 * 2 + 1]};
-    const double sim13 = 1.0 - static_cast<double>(d3 - d1) / 25.0;
+    const Fluid s3 = C.fluid(nr.i3);
+    const double sim13 = 1.0 - static_cast<double>(nr.d3 - nr.d1) / 25.0;
     if (sim13 > 0.0) {
         const double b13 = sim12 * sim13 * aquifer_pressure(O, x, y, z, barrier, barrier_set, s1, s3);
         if (density + b13 > 0.0) return -1;
     }
-    const double sim23 = 1.0 - static_cast<double>(d3 - d2) / 25.0;
+    const double sim23 = 1.0 - static_cast<double>(nr.d3 - nr.d2) / 25.0;
     if (sim23 > 0.0) {
         const double b23 = sim12 * sim23 * aquifer_pressure(O, x, y, z, barrier, barrier_set, s2, s3);
         if (density + b23 > 0.0) return -1;
     }
     return fluid;
+}
+
+// NoiseBasedAquifer.computeSubstance; -1 means solid (the default block).
+MCT_HD inline int aquifer_substance(const Octave* O, int cx, int cz, const int32_t* loc, const int32_t* fl, int skip_above, int x, int y, int z,
+                                    double density) {
+    const AqCellsGlobal C{loc, fl};
+    AqNearest nr;
+    const int s = aquifer_front(C, cx, cz, skip_above, x, y, z, density, nr);
+    return s == kAqPending ? aquifer_back(O, C, x, y, z, density, nr) : s;
 }
 
 // ---- fill: one block -------------------------------------------------------------------
