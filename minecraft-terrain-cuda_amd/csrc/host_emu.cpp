@@ -25,6 +25,7 @@ struct Emu {
     std::vector<TreeNodeF> treef;
     std::vector<uint8_t> leaf_biome;
     std::vector<int32_t> leaf_node;
+    std::vector<int32_t> leaf_rank;
     std::unique_ptr<SeedDev> seed;
     int64_t seed_value = 0;
     bool has_seed = false;
@@ -49,6 +50,9 @@ void fill_model(Emu& e) {
     M.treef = e.treef.empty() ? nullptr : e.treef.data();
     M.leaf_biome = e.leaf_biome.data();
     M.leaf_node = e.leaf_node.data();
+    e.leaf_rank = leaf_dfs_rank(e.tree, mctgen::kTreeLeaves);
+    M.leaf_rank = e.leaf_rank.data();
+    if (!e.treef.empty()) column_groups(e.treef, M);
     std::memcpy(M.block_flags, mctgen::kBlockFlags, 64);
     std::memcpy(M.biome_temp, mctgen::kBiomeTemp, sizeof M.biome_temp);
     std::memcpy(M.biome_frozen, mctgen::kBiomeFrozen, sizeof M.biome_frozen);
@@ -202,7 +206,7 @@ int mcte_generate(void* p, int64_t seed, const int32_t* xz, int n, uint8_t* terr
         for (int i = 0; i < nbk * 16; ++i) biome_column(seeds, bch.data(), bcol.data(), i);
         for (int i = 0; i < nbk * kQPer; ++i) biome_target(seeds, bch.data(), bcol.data(), targets.data(), i);
         const int ntop = M.treef ? std::min(mctgen::kTreeNodes, 384) : 0;
-        for (int i = 0; i < nbk * 16 * biome_runs(kBiomeRunDefault); ++i) biome_search(M, M.treef, ntop, targets.data(), lstar.data(), mind.data(), i, false, kBiomeRunDefault);
+        for (int i = 0; i < nbk * 16; ++i) biome_search_column(M, M.treef, ntop, targets.data(), lstar.data(), mind.data(), i);
         for (int i = 0; i < nbk * kQPer; ++i) biome_search_wide(M, targets.data(), lstar.data(), mind.data(), i);
         for (int i = 0; i < nbk * kQPer; ++i) biome_tie(M, targets.data(), lstar.data(), mind.data(), tie.data(), i);
         for (int b = 0; b < nbk; ++b) biome_chain_seq(M, targets.data(), lstar.data(), mind.data(), bb.data(), b);

@@ -92,6 +92,18 @@ __global__ __launch_bounds__(256) void k_biome_search(int nbk, const int64_t* ta
     if (idx < nbk * 16 * biome_runs(run_len)) biome_search(M, top, ntop, targets, lstar, mind, idx, false, run_len);
 }
 
+// L* for whole quart columns (one thread per column): a restricted search per group of the
+// column's varying coordinate instead of one search per quart.
+__global__ __launch_bounds__(256) void k_biome_search_cols(int nbk, const int64_t* targets, int32_t* lstar, int64_t* mind) {
+    __shared__ TreeNodeF top[kTreeTop];
+    const ModelDev& M = c_model;
+    const int ntop = mctgen::kTreeNodes < kTreeTop ? mctgen::kTreeNodes : kTreeTop;
+    for (int i = threadIdx.x; i < ntop * 16; i += blockDim.x) reinterpret_cast<uint32_t*>(top)[i] = reinterpret_cast<const uint32_t*>(M.treef)[i];
+    __syncthreads();
+    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx < nbk * 16) biome_search_column(M, top, ntop, targets, lstar, mind, idx);
+}
+
 __global__ __launch_bounds__(256) void k_biome_search_wide(int nbk, const int64_t* targets, int32_t* lstar, int64_t* mind) {
     const int idx = blockIdx.x * blockDim.x + threadIdx.x;
     if (idx < nbk * kQPer) biome_search_wide(c_model, targets, lstar, mind, idx);
@@ -939,6 +951,7 @@ struct Generator {
     TreeNodeF* d_treef = nullptr;
     uint8_t* d_leaf_biome = nullptr;
     int32_t* d_leaf_node = nullptr;
+    int32_t* d_leaf_rank = nullptr;
     int* d_blended_first = nullptr;
     int nnormal = 0;
     // resident seeds
@@ -998,11 +1011,16 @@ struct Generator {
         hip_check(hipMemcpy(d_leaf_biome, mctgen::kLeafBiome, mctgen::kTreeLeaves, hipMemcpyHostToDevice), "upload");
         hip_check(hipMalloc(&d_leaf_node, sizeof(int32_t) * mctgen::kTreeLeaves), "malloc");
         hip_check(hipMemcpy(d_leaf_node, mctgen::kLeafNode, sizeof(int32_t) * mctgen::kTreeLeaves, hipMemcpyHostToDevice), "upload");
+        const std::vector<int32_t> rank = leaf_dfs_rank(tree, mctgen::kTreeLeaves);
+        hip_check(hipMalloc(&d_leaf_rank, sizeof(int32_t) * rank.size()), "malloc");
+        hip_check(hipMemcpy(d_leaf_rank, rank.data(), sizeof(int32_t) * rank.size(), hipMemcpyHostToDevice), "upload");
         ModelDev M{};
         M.tree = d_tree;
         M.treef = d_treef;
         M.leaf_biome = d_leaf_biome;
         M.leaf_node = d_leaf_node;
+        M.leaf_rank = d_leaf_rank;
+        if (d_treef) column_groups(treef, M);
         std::memcpy(M.block_flags, mctgen::kBlockFlags, 64);
         std::memcpy(M.biome_temp, mctgen::kBiomeTemp, sizeof M.biome_temp);
         std::memcpy(M.biome_frozen, mctgen::kBiomeFrozen, sizeof M.biome_frozen);
@@ -1046,7 +1064,7 @@ struct Generator {
         slots.clear();
         void* q[] = {d_qxyz, d_qslot, d_qbad, d_qout};
         for (void* x : q) (void)hipFree(x);
-        void* p[] = {d_tree, d_treef, d_leaf_biome, d_leaf_node, d_blended_first, d_seeds, d_jobs, d_stats};
+        void* p[] = {d_tree, d_treef, d_leaf_biome, d_leaf_node, d_leaf_rank, d_blended_first, d_seeds, d_jobs, d_stats};
         for (void* q : p) (void)hipFree(q);
         (void)hipHostFree(h_jobs);
         if (t0) (void)hipEventDestroy(t0);
@@ -1213,8 +1231,8 @@ struct Generator {
         timed(S, kKBiomeQuarts, [&] {
             k_biome_targets<<<grid_of(static_cast<size_t>(nbk) * kQPer, 256), 256, 0, st>>>(sd, S.d_bch, nbk, S.d_bcol, S.d_targets);
             if (d_treef) {
-                k_biome_search<<<grid_of(static_cast<size_t>(nbk) * 16 * biome_runs(run_len), W(kKBiomeQuarts)), W(kKBiomeQuarts), 0, st>>>(
-                    nbk, S.d_targets, S.d_lstar, S.d_mind, run_len);
+                k_biome_search_cols<<<grid_of(static_cast<size_t>(nbk) * 16, W(kKBiomeQuarts)), W(kKBiomeQuarts), 0, st>>>(
+                    nbk, S.d_targets, S.d_lstar, S.d_mind);
                 k_biome_search_wide<<<grid_of(static_cast<size_t>(nbk) * kQPer, 256), 256, 0, st>>>(nbk, S.d_targets, S.d_lstar, S.d_mind);
             } else {
                 k_biome_search_scalar<<<grid_of(static_cast<size_t>(nbk) * 16 * biome_runs(run_len), W(kKBiomeQuarts)), W(kKBiomeQuarts), 0, st>>>(

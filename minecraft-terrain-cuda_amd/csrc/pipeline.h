@@ -77,12 +77,19 @@ struct TreeNodeF {
 };
 constexpr double kTreeExact = 8388608.0;   // 2^23
 
+// Column search groups: for climate coordinate k, the distinct intervals the leaves have on
+// it (col_groups[k] of them), or 0 when there are more than kColGroups.
+constexpr int kColGroups = 8;
+
 // Seed-independent model tables (uploaded once).
 struct ModelDev {
     const TreeNodeDev* tree;
     const TreeNodeF* treef;         // null when some bound is beyond 2^24 (int64 search only)
     const uint8_t* leaf_biome;
     const int32_t* leaf_node;
+    const int32_t* leaf_rank;       // position of each leaf in the tree's depth-first order
+    int32_t col_groups[7];
+    float col_lo[7][kColGroups], col_hi[7][kColGroups];
     uint8_t block_flags[64];
     float biome_temp[128];
     uint8_t biome_frozen[128];
@@ -116,6 +123,42 @@ inline std::vector<TreeNodeF> float_tree(const std::vector<TreeNodeDev>& tree) {
         out[i].leaf = tree[i].leaf;
     }
     return out;
+}
+
+// Depth-first rank of every leaf (children in order), by leaf index.
+inline std::vector<int32_t> leaf_dfs_rank(const std::vector<TreeNodeDev>& tree, int nleaves) {
+    std::vector<int32_t> rank(static_cast<size_t>(nleaves), -1);
+    if (tree.empty()) return rank;
+    std::vector<std::pair<int, int>> stack{{0, 0}};
+    int next = 0;
+    if (tree[0].leaf >= 0) { rank[static_cast<size_t>(tree[0].leaf)] = 0; return rank; }
+    while (!stack.empty()) {
+        auto& top = stack.back();
+        const TreeNodeDev& n = tree[static_cast<size_t>(top.first)];
+        if (top.second >= n.count) { stack.pop_back(); continue; }
+        const int c = n.first + top.second++;
+        if (tree[static_cast<size_t>(c)].leaf >= 0) rank[static_cast<size_t>(tree[static_cast<size_t>(c)].leaf)] = next++;
+        else stack.push_back({c, 0});
+    }
+    return rank;
+}
+
+// The column search groups of every coordinate (float bounds; only used with the float tree).
+inline void column_groups(const std::vector<TreeNodeF>& treef, ModelDev& M) {
+    for (int k = 0; k < 7; ++k) {
+        std::vector<std::pair<float, float>> g;
+        for (const TreeNodeF& n : treef) {
+            if (n.leaf < 0) continue;
+            const std::pair<float, float> iv{n.lo[k], n.hi[k]};
+            if (std::find(g.begin(), g.end(), iv) == g.end()) g.push_back(iv);
+            if (g.size() > static_cast<size_t>(kColGroups)) break;
+        }
+        M.col_groups[k] = (g.empty() || g.size() > static_cast<size_t>(kColGroups)) ? 0 : static_cast<int32_t>(g.size());
+        for (int j = 0; j < kColGroups; ++j) {
+            M.col_lo[k][j] = j < M.col_groups[k] ? g[static_cast<size_t>(j)].first : 0.0f;
+            M.col_hi[k][j] = j < M.col_groups[k] ? g[static_cast<size_t>(j)].second : 0.0f;
+        }
+    }
 }
 
 // ---- keys for the position dedupe -------------------------------------------------------
@@ -359,6 +402,48 @@ MCT_HD inline int tree_first_min_f(const TreeNodeF* T, const TreeNodeF* top, int
     return closest >= 0 ? node(closest).leaf : -1;
 }
 
+// tree_first_min_f restricted to the leaves whose interval on coordinate k is [glo, ghi]
+// (t[k] = glo): subtrees whose interval on k does not cover [glo, ghi] hold none of them and
+// are skipped, which leaves the depth-first order of the group's leaves as it is. Returns the
+// node index of the first leaf at minimal distance below `bound`, or -1.
+MCT_HD inline int tree_first_min_group_f(const TreeNodeF* T, const TreeNodeF* top, int ntop, const float* t, double bound, double& out_d, int k,
+                                         float glo, float ghi) {
+    int stack[12];
+    int sp = 0;
+    double best = bound;
+    int closest = -1;
+    auto node = [&](int i) -> const TreeNodeF& { return i < ntop ? top[i] : T[i]; };
+    if (node(0).leaf >= 0) {
+        const double d = box_distance_f(node(0), t);
+        if (best > d && node(0).lo[k] == glo && node(0).hi[k] == ghi) { best = d; closest = 0; }
+    } else {
+        stack[0] = 0;
+        sp = 1;
+    }
+    while (sp > 0) {
+        const int e = stack[sp - 1];
+        const int fc = node(e >> 5).first_count;
+        const int j = e & 31;
+        if (j >= (fc >> 24)) { --sp; continue; }
+        stack[sp - 1] = e + 1;
+        const int c = (fc & 0xFFFFFF) + j;
+        const TreeNodeF& cn = node(c);
+        const bool leaf = cn.leaf >= 0;
+        if (leaf ? (cn.lo[k] != glo || cn.hi[k] != ghi) : (cn.lo[k] > glo || cn.hi[k] < ghi)) continue;
+        const double cd = box_distance_f(cn, t);
+        if (best > cd) {
+            if (leaf) {
+                best = cd;
+                closest = c;
+            } else if (sp < 12) {
+                stack[sp++] = c << 5;
+            }
+        }
+    }
+    out_d = best;
+    return closest;
+}
+
 // Climate columns of one biome chunk column (y-independent parts at y = 0).
 MCT_HD MCT_FI void biome_column(const SeedDev* seeds, const ChunkRef* bch, float* bcol, int idx) {
     const int b = idx >> 4, k = idx & 15;
@@ -434,6 +519,92 @@ MCT_HD inline void biome_search_wide(const ModelDev& M, const int64_t* targets, 
     int64_t d = 0;
     lstar[idx] = tree_first_min(M.tree, targets + static_cast<size_t>(idx) * 7, INT64_MAX, d);
     mind[idx] = d;
+}
+
+// L* and its distance for all quarts of one quart column (idx = biome chunk * 16 + qz * 4 + qx).
+// Along a column usually one climate coordinate varies (the depth) and the leaves take a few
+// distinct intervals on it (groups). A leaf's distance is then A (over the other coordinates,
+// fixed for the column) plus the squared distance of the column's value v on that coordinate
+// to the leaf's interval, the same for its whole group. One search per group restricted to
+// the group (at v = the group's lower bound, where that term is 0) gives the group's minimal
+// A and its first leaf in depth-first order; a quart's L* is the leaf of the group minimising
+// A_g + d_g(v)^2, ties to the leaf first in depth-first order, which is the leaf and distance
+// the unrestricted search gives. Columns that vary in more than one coordinate, or in one with
+// too many groups, or whose targets leave the float tree's exact range, take the per-quart
+// search.
+MCT_HD inline void biome_search_column(const ModelDev& M, const TreeNodeF* top, int ntop, const int64_t* targets, int32_t* lstar, int64_t* mind, int idx) {
+    const int b = idx >> 4, qx = idx & 3, qz = (idx >> 2) & 3;
+    const size_t col = static_cast<size_t>(b) * kQPer + qz * 4 + qx;   // + qy * 16: the column's quarts
+    const int64_t* t0 = targets + col * 7;
+    int vary = 0;
+    bool small = M.treef != nullptr;
+    for (int qy = 0; qy < kQH; ++qy) {
+        const int64_t* t = targets + (col + static_cast<size_t>(qy) * 16) * 7;
+        for (int k = 0; k < 7; ++k) {
+            if (t[k] != t0[k]) vary |= 1 << k;
+            small = small && t[k] >= -static_cast<int64_t>(kTreeExact) && t[k] <= static_cast<int64_t>(kTreeExact);
+        }
+    }
+    int k = -1;
+    if (small) {
+        if (vary == 0) {
+            for (int j = 6; j >= 0; --j)
+                if (M.col_groups[j] > 0) k = j;
+        } else if ((vary & (vary - 1)) == 0) {
+            const int j = __builtin_ctz(static_cast<unsigned>(vary));
+            if (M.col_groups[j] > 0) k = j;
+        }
+    }
+    if (k < 0) {
+        biome_search(M, top, ntop, targets, lstar, mind, idx, false, kQH);
+        return;
+    }
+    auto node = [&](int i) -> const TreeNodeF& { return i < ntop ? top[i] : M.treef[i]; };
+    const double inf = u2d(0x7FF0000000000000ull);
+    float tf[7];
+    for (int j = 0; j < 7; ++j) tf[j] = static_cast<float>(t0[j]);
+    const int G = M.col_groups[k];
+    double A[kColGroups];
+    int leaf[kColGroups], rank[kColGroups];
+    float lo[kColGroups], hi[kColGroups];
+#pragma unroll
+    for (int g = 0; g < kColGroups; ++g) {
+        leaf[g] = -1;
+        rank[g] = INT32_MAX;
+        A[g] = inf;
+        lo[g] = M.col_lo[k][g];
+        hi[g] = M.col_hi[k][g];
+        if (g < G) {
+            float tg[7];
+            for (int j = 0; j < 7; ++j) tg[j] = j == k ? lo[g] : tf[j];
+            double d = 0.0;
+            int n = tree_first_min_group_f(M.treef, top, ntop, tg, tree_greedy_f(M.treef, top, ntop, tg) + 1.0, d, k, lo[g], hi[g]);
+            if (n < 0) n = tree_first_min_group_f(M.treef, top, ntop, tg, inf, d, k, lo[g], hi[g]);
+            if (n >= 0) {
+                A[g] = d;
+                leaf[g] = node(n).leaf;
+                rank[g] = M.leaf_rank[leaf[g]];
+            }
+        }
+    }
+    for (int qy = 0; qy < kQH; ++qy) {
+        const size_t i = col + static_cast<size_t>(qy) * 16;
+        const float v = static_cast<float>(targets[i * 7 + k]);
+        double best = inf;
+        int bl = -1, br = INT32_MAX;
+#pragma unroll
+        for (int g = 0; g < kColGroups; ++g) {
+            const float p = fmaxf(fmaxf(v - hi[g], lo[g] - v), 0.0f);
+            const double d = A[g] + static_cast<double>(p) * static_cast<double>(p);
+            if (leaf[g] >= 0 && (d < best || (d == best && rank[g] < br))) {
+                best = d;
+                bl = leaf[g];
+                br = rank[g];
+            }
+        }
+        lstar[i] = bl;
+        mind[i] = static_cast<int64_t>(best);
+    }
 }
 
 // Chain order of doCreateBiomes inside a chunk: 4-quart sections, then x, y, z.
