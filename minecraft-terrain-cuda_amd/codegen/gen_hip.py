@@ -729,10 +729,32 @@ class Gen:
             conds.append((name, body))
             return name
 
-        def rule(r):
+        # y range of the blocks each ore-vein rule can be reached at: the hull over the rule's
+        # paths of the plain y thresholds on them (y_above without stone or surface depth, and
+        # its negation); other conditions only narrow it further
+        ymin, yend = self.ir.doc['min_y'], self.ir.doc['min_y'] + self.ir.doc['height']
+        ore_range = {}
+
+        def deref(cn):
+            while cn['t'] == 'ref':
+                cn = cn['cond']
+            return cn
+
+        def plain_y(cn):
+            return cn['t'] == 'y_above' and not cn['add_stone_depth'] and cn['surface_depth_multiplier'] == 0
+
+        def narrow(cn, yr):
+            cn = deref(cn)
+            if plain_y(cn):
+                return (max(yr[0], cn['anchor']), yr[1])
+            if cn['t'] == 'not' and plain_y(deref(cn['cond'])):
+                return (yr[0], min(yr[1], deref(cn['cond'])['anchor']))
+            return yr
+
+        def rule(r, yr=(ymin, yend)):
             t = r['t']
             if t == 'ref':
-                return rule(r['rule'])
+                return rule(r['rule'], yr)
             name = 'rule%d' % cnt['r']
             cnt['r'] += 1
             if t == 'block':
@@ -740,13 +762,17 @@ class Gen:
             elif t == 'bandlands':
                 body = 'return c.band();'
             elif t == 'ore_vein':
-                body = 'return c.ore_vein(%d);' % self.ore_rules.index(r)
+                k = self.ore_rules.index(r)
+                if yr[0] < yr[1]:
+                    lo, hi = ore_range.get(k, yr)
+                    ore_range[k] = (min(lo, yr[0]), max(hi, yr[1]))
+                body = 'return c.ore_vein(%d);' % k
             elif t == 'condition':
                 cn = cond(r['if'])
-                th = rule(r['then'])
+                th = rule(r['then'], narrow(r['if'], yr))
                 body = 'return %s(c) ? %s(c) : -1;' % (cn, th)
             elif t == 'sequence':
-                subs = [rule(x) for x in r['rules']]
+                subs = [rule(x, yr) for x in r['rules']]
                 body = 'int s;\n' + '\n'.join('    s = %s(c); if (s >= 0) return s;' % x for x in subs) + '\n    return -1;'
             else:
                 raise SystemExit('unsupported rule ' + t)
@@ -764,6 +790,11 @@ class Gen:
         for name, body in rules:
             o.append('template <class C> MCT_HD MCT_FI int %s(C& c) {\n    %s\n}' % (name, body))
         o.append('template <class C> MCT_HD MCT_FI int apply_rules(C& c) { return %s(c); }' % root)
+        # per ore-vein rule: the y range its blocks can be in ([lo, hi); empty if unreachable)
+        nore = max(len(self.ore_rules), 1)
+        rng = [ore_range.get(k, (0, 0)) for k in range(nore)]
+        o.append('constexpr int kOreYLo[%d] = {%s};' % (nore, ', '.join(str(a) for a, _ in rng)))
+        o.append('constexpr int kOreYHi[%d] = {%s};' % (nore, ', '.join(str(b) for _, b in rng)))
         o.append('constexpr int kColNoises = %d;   // per-column noise slots' % len(self.col_noise))
         o.append('constexpr int kColSurface = 0, kColSecondary = 1, kColClay = 2;')
         o.append('MCT_HD MCT_FI float surf_column_noise(const mct::Octave* O, double x, double z, int k) {')

@@ -376,6 +376,15 @@ __global__ __launch_bounds__(256) void k_surface_cols(const SeedDev* seeds, cons
     surface_column(seeds, c_model, chunks, rec, idx);
 }
 
+// Ore-vein outcomes of the blocks the ore rules can reach, ahead of the surface walk: one thread
+// per block column, adjacent threads adjacent x.
+__global__ __launch_bounds__(256) void k_ore(SurfArgs A, int n) {
+    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= n * 256) return;
+    A.M = &c_model;
+    ore_precompute(A, idx);
+}
+
 // Threads of a wave take adjacent x so the column walks read adjacent bytes of a y-layer.
 __global__ __launch_bounds__(64) void k_surface1(SurfArgs A, int n, const int* h0, int* h1, int* h2) {
     const int idx = blockIdx.x * blockDim.x + threadIdx.x;
@@ -831,11 +840,11 @@ struct Dedupe {
 // Kernel groups timed with HIP events on the batch stream when profiling is enabled.
 enum KernelId {
     kKBiomeColumns, kKBiomeQuarts, kKBiomeTies, kKBiomeChain, kKDedupeCols, kKColumns, kKCorners, kKDedupeGrid, kKFts, kKGridScatter,
-    kKAqCells, kKFill, kKHeights, kKSurface1, kKSurface2, kKSurfaceExact, kKGather, kKSurfaceCols, kKCount
+    kKAqCells, kKFill, kKHeights, kKSurface1, kKSurface2, kKSurfaceExact, kKGather, kKSurfaceCols, kKOre, kKCount
 };
 const char* const kKernelNames[kKCount] = {"biome_columns", "biome_quarts", "biome_ties", "biome_chain", "dedupe_columns", "columns", "corners",
                                            "dedupe_grid", "fts", "grid_scatter+aquifer_skip", "aquifer_cells", "fill", "heights", "surface1",
-                                           "surface2", "surface_check+exact", "gather_biomes", "surface_columns"};
+                                           "surface2", "surface_check+exact", "gather_biomes", "surface_columns", "ore_veins"};
 
 struct Slot {
     hipStream_t stream = nullptr;
@@ -859,6 +868,7 @@ struct Slot {
     int *d_colmap = nullptr, *d_gmap = nullptr, *d_counts = nullptr, *d_skip = nullptr;
     int *d_h0 = nullptr, *d_h1 = nullptr, *d_h2 = nullptr, *d_hf = nullptr, *d_flags = nullptr;
     float* d_colrec = nullptr;
+    uint8_t* d_ore = nullptr;
     float* d_density = nullptr;     // core configuration's density output (allocated on first use)
     std::unique_ptr<Dedupe> dedupe;
     hipEvent_t kev[kKCount][2] = {};
@@ -902,6 +912,7 @@ struct Slot {
         M(&d_h2, sizeof(int) * mb * 256);
         M(&d_hf, sizeof(int) * mb * 256);
         M(&d_colrec, sizeof(float) * mb * 256 * kColF);
+        M(&d_ore, static_cast<size_t>(mb) * 256 * kOreRowsPad);
         M(&d_flags, sizeof(int) * mb);
         M(&d_counts, sizeof(int) * 4);
         dedupe = std::make_unique<Dedupe>(mb * kGridKeys);
@@ -1278,7 +1289,9 @@ struct Generator {
                                                                                                reinterpret_cast<uint4*>(S.d_aq));
         hip_check(hipEventRecord(S.ev[6], st), "event");
         // surface
-        SurfArgs A{sd, nullptr, S.d_ch, blocks, S.d_nb, S.d_bb, S.d_corners, S.d_colmap, S.d_prelim, S.d_colrec};
+        SurfArgs A{sd, nullptr, S.d_ch, blocks, S.d_nb, S.d_bb, S.d_corners, S.d_colmap, S.d_prelim, S.d_colrec, S.d_ore};
+        if (kOreRows > 0)
+            timed(S, kKOre, [&] { k_ore<<<grid_of(static_cast<size_t>(n) * 256, W(kKOre)), W(kKOre), 0, st>>>(A, n); });
         timed(S, kKSurfaceCols, [&] { k_surface_cols<<<grid_of(static_cast<size_t>(n) * 256, W(kKSurfaceCols)), W(kKSurfaceCols), 0, st>>>(sd, S.d_ch, n, S.d_colrec); });
         timed(S, kKSurface1, [&] { k_surface1<<<grid_of(static_cast<size_t>(n) * 256, 64), 64, 0, st>>>(A, n, S.d_h0, S.d_h1, S.d_h2); });
         timed(S, kKSurface2, [&] { k_surface2<<<grid_of(static_cast<size_t>(n) * 256, 64), 64, 0, st>>>(A, n, S.d_h0, S.d_h1, S.d_h2, S.d_hf); });
@@ -1377,7 +1390,7 @@ struct Generator {
     static bool tunable(int k) {
         switch (k) {
             case kKBiomeColumns: case kKBiomeQuarts: case kKBiomeTies: case kKBiomeChain: case kKColumns: case kKFts: case kKAqCells:
-            case kKFill: case kKHeights: case kKSurfaceCols: return true;
+            case kKFill: case kKHeights: case kKSurfaceCols: case kKOre: return true;
             default: return false;
         }
     }

@@ -1277,7 +1277,29 @@ struct SurfArgs {
     const int* colmap;
     const float* prelim;        // n x 4 chunk-corner preliminary surface values
     const float* colrec;        // n x 256 column records (kColF floats, column = lx * 16 + lz)
+    uint8_t* ore;               // n x 256 x kOreRowsPad ore-vein outcomes (ore_precompute), or null
 };
+
+// ---- ore veins ahead of the surface rules -------------------------------------------------
+// An ore-vein rule k is only reached for blocks with y in [kOreYLo[k], kOreYHi[k]) (the plain
+// y thresholds on its paths through the rules). Its outcome at a block depends on the block
+// alone (position, the interpolated ore channels, the seed's ore random), so it is computed
+// for every block of that range in a dense pass before the surface walk, one byte per block:
+// 0xFF (none), the state, or a gap marker (kGapMarker | raw << 4 | k) that the walk resolves.
+// Per block column (chunk * 256 + lz * 16 + lx) kOreRowsPad bytes: the rows of rule 0, then
+// rule 1, ..., so that the walk down a column reads them 16 at a time.
+constexpr int kOreRules = mctgen::kNumOreRules;
+constexpr int ore_rows(int k) { return mctgen::kOreYHi[k] > mctgen::kOreYLo[k] ? mctgen::kOreYHi[k] - mctgen::kOreYLo[k] : 0; }
+constexpr int ore_cells(int k) {
+    return ore_rows(k) > 0 ? ((mctgen::kOreYHi[k] - 1 - kMinY) >> 3) - ((mctgen::kOreYLo[k] - kMinY) >> 3) + 1 : 0;
+}
+constexpr int ore_row_offset(int k) {
+    int s = 0;
+    for (int j = 0; j < k; ++j) s += ore_rows(j);
+    return s;
+}
+constexpr int kOreRows = ore_row_offset(kOreRules);
+constexpr int kOreRowsPad = (kOreRows + 15) & ~15;
 
 // Block classes the surface walk distinguishes: solid, air, water (the water state itself) and
 // other fluids.
@@ -1510,7 +1532,23 @@ struct Surf {
 struct SurfGap : Surf {
     static constexpr bool kDeferGap = true;
     const float MCT_LDS* lrec;   // the column record staged in LDS (value k at lrec[k * 64])
-    MCT_HD MCT_FI int ore_vein(int k) { return ore_vein_impl<true>(k); }
+    // the ore outcomes of 16 rows of the column (rows ore_wb .. ore_wb + 15), kept while the
+    // walk moves down through them
+    int ore_wb;
+    uint32_t ore_w[4];
+    MCT_HD MCT_FI int ore_vein(int k) {
+        const int r = ore_row_offset(k) + (y - mctgen::kOreYLo[k]);
+        if ((r & ~15) != ore_wb) {
+            ore_wb = r & ~15;
+            struct alignas(16) Row16 { uint32_t w[4]; };
+            const Row16 w = *reinterpret_cast<const Row16*>(A->ore + (static_cast<size_t>(ci) * 256 + lz * 16 + lx) * kOreRowsPad + ore_wb);
+            for (int j = 0; j < 4; ++j) ore_w[j] = w.w[j];
+        }
+        const int q = (r >> 2) & 3;
+        const uint32_t w = q == 0 ? ore_w[0] : q == 1 ? ore_w[1] : q == 2 ? ore_w[2] : ore_w[3];
+        const int v = static_cast<int>((w >> ((r & 3) * 8)) & 0xFFu);
+        return v == 0xFF ? -1 : v;
+    }
     MCT_HD MCT_FI float col_noise(int k) const { return lrec[k * 64]; }
     MCT_HD MCT_FI int band() const {
         const int offset = jroundf(lrec[mctgen::kColClay * 64] * 4.0f);
@@ -1518,6 +1556,66 @@ struct SurfGap : Surf {
     }
 };
 static_assert(mctgen::kPaletteSize <= kGapMarker, "palette indices must stay below the gap marker");
+
+// Ore-vein outcomes of one block column (idx = chunk * 256 + lz * 16 + lx), rows in order, as
+// Surf::ore_vein_impl<true> gives them: per cell row the z / x lerps of the ore channels, then
+// y as the running sum of the step from the cell's bottom. The bytes are stored 16 at a time.
+MCT_HD inline void ore_precompute(const SurfArgs& A, int idx) {
+    const int col = idx & 255, lx = col & 15, lz = col >> 4, c = idx >> 8;
+    const ChunkRef cr = A.chunks[c];
+    const SeedDev& S = A.seeds[cr.seed];
+    const ModelDev& M = *A.M;
+    const int* cmap = A.colmap + static_cast<size_t>(c) * kCols;
+    const int cx = lx >> 2, xi = lx & 3, cz = lz >> 2, zi = lz & 3;
+    const size_t u00 = cmap[cz * 5 + cx], u10 = cmap[cz * 5 + cx + 1], u01 = cmap[(cz + 1) * 5 + cx], u11 = cmap[(cz + 1) * 5 + cx + 1];
+    const int x = cr.x * 16 + lx, z = cr.z * 16 + lz;
+    uint8_t* out = A.ore + static_cast<size_t>(idx) * kOreRowsPad;
+    struct alignas(16) Row16 { uint32_t w[4]; };
+    uint32_t w0 = 0, w1 = 0, w2 = 0, w3 = 0, acc = 0;   // the last 4 complete words, the open one
+    int r = 0;
+    for (int k = 0; k < kOreRules; ++k) {
+        if (ore_rows(k) == 0) continue;
+        const int cy0 = (mctgen::kOreYLo[k] - kMinY) >> 3;
+        for (int cy = cy0; cy < cy0 + ore_cells(k); ++cy) {
+            float ch[kCh], step[kCh];
+            for (int q = 0; q < kCh; ++q) ch[q] = step[q] = 0.0f;
+            for (int q = mctgen::kOreChanFirst; q < kCh; ++q) {
+                const float* a = A.corners + (u00 * kCh + q) * kRows + cy;
+                const float* b = A.corners + (u10 * kCh + q) * kRows + cy;
+                const float* cc = A.corners + (u01 * kCh + q) * kRows + cy;
+                const float* d = A.corners + (u11 * kCh + q) * kRows + cy;
+                float lo;
+                cell_lo_step(a[0], b[0], a[1], b[1], cc[0], d[0], cc[1], d[1], xi, zi, 0.25f, 0.125f, lo, step[q]);
+                ch[q] = lo + step[q] * 0.0f;
+            }
+            for (int yi = 0; yi < 8; ++yi) {
+                const int y = kMinY + cy * 8 + yi;
+                if (y >= mctgen::kOreYLo[k] && y < mctgen::kOreYHi[k]) {
+                    uint32_t v = 0xFF;
+                    const float density = mctgen::eval_ore(S.oct, x, y, z, ch, 2 * k);
+                    if (density > 0.0f) {
+                        Xoro rnd = at(S.ore, x, y, z);
+                        if (!(next_float(rnd) > density)) {
+                            const float richness = mctgen::eval_ore(S.oct, x, y, z, ch, 2 * k + 1);
+                            if (next_float(rnd) < richness) v = static_cast<uint32_t>(kGapMarker | (next_float(rnd) < M.ore_raw[k] ? 0x10 : 0) | k);
+                            else v = static_cast<uint32_t>(M.ore_state[k][2]);
+                        }
+                    }
+                    acc |= v << ((r & 3) * 8);
+                    if ((r & 3) == 3) { w0 = w1; w1 = w2; w2 = w3; w3 = acc; acc = 0; }
+                    if ((r & 15) == 15) *reinterpret_cast<Row16*>(out + (r - 15)) = Row16{{w0, w1, w2, w3}};
+                    ++r;
+                }
+                for (int q = mctgen::kOreChanFirst; q < kCh; ++q) ch[q] += step[q];
+            }
+        }
+    }
+    if (r & 15) {   // the last, partial 16 rows
+        if (r & 3) { w0 = w1; w1 = w2; w2 = w3; w3 = acc; }
+        for (int j = ((r & 15) + 3) >> 2; j < 4; ++j) { w0 = w1; w1 = w2; w2 = w3; w3 = 0; }
+        *reinterpret_cast<Row16*>(out + (r & ~15)) = Row16{{w0, w1, w2, w3}};
+    }
+}
 
 MCT_HD MCT_FI void surf_init(Surf& c, const SurfArgs& A, int ci, int lx, int lz, int* hm) {
     c.A = &A;
@@ -1796,7 +1894,10 @@ MCT_HD MCT_FI void surface_pass2(const SurfArgs& A, const int* h0, const int* h1
     typename std::conditional<L, SurfGap, Surf>::type c;
     surf_init(c, A, ci, lx, lz, &hm);
     c.cls = cls;
-    if constexpr (L) c.lrec = lrec;
+    if constexpr (L) {
+        c.lrec = lrec;
+        c.ore_wb = -16;
+    }
     const int starting = h0[hidx];
     const int sb = c.biome_y(starting);
     const int height = H(lx, lz) + 1;
