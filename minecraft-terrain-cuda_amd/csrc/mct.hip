@@ -196,7 +196,10 @@ __global__ __launch_bounds__(256) void k_columns(KeyCodec K, const SeedDev* seed
 // Corner pass: each workgroup takes kCornerCols distinct columns x kRows rows (245 of 256
 // threads). The x / z records of every octave sample site of its columns are computed once
 // into LDS (the FP64 coordinate setup of two of the three axes, shared by the 49 rows), then
-// each thread evaluates its corner with only the y axis computed per sample.
+// each thread evaluates its corner with only the y axis computed per sample. Threads take the
+// corners row by row (thread = row * kCornerCols + column), so a wave holds about 13 adjacent
+// rows: the terrain's y-dependent branches (cave noises below the surface, entrances above,
+// the ore channels' y range) then mostly agree across a wave.
 constexpr int kCornerCols = 256 / kRows;
 constexpr int kCornerTab = mctgen::kCornerSites > 0 ? mctgen::kCornerSites : 1;
 __global__ __launch_bounds__(256) void k_corners(KeyCodec K, const SeedDev* seeds, int upper, const uint64_t* ucols, const int* count, const float* colv,
@@ -210,10 +213,9 @@ __global__ __launch_bounds__(256) void k_corners(KeyCodec K, const SeedDev* seed
         tab[k] = corner_site_axes(K, seeds, ucols, c0 + cl, k - cl * mctgen::kCornerSites);
     }
     __syncthreads();
-    const int cl = static_cast<int>(threadIdx.x) / kRows;
-    if (cl < ncol)
-        corner_values(K, seeds, ucols, colv, corners, c0 + cl, static_cast<int>(threadIdx.x) - cl * kRows,
-                      (const MCT_LDS mct::AxisXZ*)(tab + cl * kCornerTab));
+    const int row = static_cast<int>(threadIdx.x) / kCornerCols, cl = static_cast<int>(threadIdx.x) - row * kCornerCols;
+    if (row < kRows && cl < ncol)
+        corner_values(K, seeds, ucols, colv, corners, c0 + cl, row, (const MCT_LDS mct::AxisXZ*)(tab + cl * kCornerTab));
 }
 
 // ---- surface-level grid / aquifer -----------------------------------------------------------
