@@ -1285,7 +1285,7 @@ struct SurfArgs {
 // y thresholds on its paths through the rules). Its outcome at a block depends on the block
 // alone (position, the interpolated ore channels, the seed's ore random), so it is computed
 // for every block of that range in a dense pass before the surface walk, one byte per block:
-// 0xFF (none), the state, or a gap marker (kGapMarker | raw << 4 | k) that the walk resolves.
+// 0xFF (none) or the state.
 // Per block column (chunk * 256 + lz * 16 + lx) kOreRowsPad bytes: the rows of rule 0, then
 // rule 1, ..., so that the walk down a column reads them 16 at a time.
 constexpr int kOreRules = mctgen::kNumOreRules;
@@ -1310,8 +1310,6 @@ MCT_HD MCT_FI int block_class(int s) {
     return ((mctgen::kFluidMask >> s) & 1u) ? kClsFluid : kClsSolid;
 }
 
-// Marker byte of an ore-vein block whose filler-gap test is pending (palette indices are < 64).
-constexpr int kGapMarker = 0x80;
 
 struct Surf {
     const SurfArgs* A;
@@ -1462,13 +1460,8 @@ struct Surf {
         const int offset = jroundf(rec[mctgen::kColClay] * 4.0f);
         return S->clay[(y + offset + 192) % 192];
     }
-    static constexpr bool kDeferGap = false;
-    MCT_HD MCT_FI int ore_vein(int k) { return ore_vein_impl<false>(k); }
-    // Ore veins (OreVeinifier as the model's roots express it). D: the filler-gap noise is not
-    // evaluated here; the block gets a marker (kGapMarker | raw bit << 4 | k) that the walk
-    // resolves after the column, with every lane of the wave evaluating its markers together.
-    template <bool D>
-    MCT_HD MCT_FI int ore_vein_impl(int k) {
+    // Ore veins (OreVeinifier as the model's roots express it).
+    MCT_HD MCT_FI int ore_vein(int k) {
         if (!ore_set) {
             // the ore evaluators read channels from kOreChanFirst on; the cell's z / x lerps
             // are kept while the walk stays in one cell row
@@ -1493,16 +1486,8 @@ struct Surf {
         Xoro r = at(S->ore, x, y, z);
         if (next_float(r) > density) return -1;
         const float richness = mctgen::eval_ore(O, x, y, z, ore_ch, 2 * k + 1);
-        if (next_float(r) < richness) {
-            if constexpr (D) {
-                // the random draw that follows a passed gap test, taken now (r is this block's own)
-                const bool raw = next_float(r) < A->M->ore_raw[k];
-                return kGapMarker | (raw ? 0x10 : 0) | k;
-            } else {
-                if (mctgen::eval_point(O, x, y, z, 5 + k) < 0.0f)
-                    return next_float(r) < A->M->ore_raw[k] ? A->M->ore_state[k][1] : A->M->ore_state[k][0];
-            }
-        }
+        if (next_float(r) < richness && mctgen::eval_point(O, x, y, z, 5 + k) < 0.0f)
+            return next_float(r) < A->M->ore_raw[k] ? A->M->ore_state[k][1] : A->M->ore_state[k][0];
         return A->M->ore_state[k][2];
     }
 
@@ -1528,9 +1513,9 @@ struct Surf {
     }
 };
 
-// The surface state with the filler-gap noise of ore veins deferred (Surf::ore_vein_impl).
+// The surface state of the LDS walk: the column record in LDS, ore veins read from the dense
+// precompute (ore_precompute).
 struct SurfGap : Surf {
-    static constexpr bool kDeferGap = true;
     const float MCT_LDS* lrec;   // the column record staged in LDS (value k at lrec[k * 64])
     // the ore outcomes of 16 rows of the column (rows ore_wb .. ore_wb + 15), kept while the
     // walk moves down through them
@@ -1555,11 +1540,15 @@ struct SurfGap : Surf {
         return S->clay[(y + offset + 192) % 192];
     }
 };
-static_assert(mctgen::kPaletteSize <= kGapMarker, "palette indices must stay below the gap marker");
+static_assert(mctgen::kPaletteSize < 0xFF, "0xFF marks no ore vein in the precomputed bytes");
 
 // Ore-vein outcomes of one block column (idx = chunk * 256 + lz * 16 + lx), rows in order, as
-// Surf::ore_vein_impl<true> gives them: per cell row the z / x lerps of the ore channels, then
-// y as the running sum of the step from the cell's bottom. The bytes are stored 16 at a time.
+// Surf::ore_vein gives them: per cell row the z / x lerps of the ore channels, then y as the
+// running sum of the step from the cell's bottom. The bytes are stored 16 at a time. Blocks
+// that pass the richness test need the filler-gap noise: they are marked in a row mask (with
+// the raw-ore draw, taken at once from the block's own random) and evaluated afterwards, every
+// lane of the wave taking its next marked row together, until none is left.
+constexpr int kOreMaskWords = (kOreRowsPad + 31) / 32;
 MCT_HD inline void ore_precompute(const SurfArgs& A, int idx) {
     const int col = idx & 255, lx = col & 15, lz = col >> 4, c = idx >> 8;
     const ChunkRef cr = A.chunks[c];
@@ -1572,6 +1561,8 @@ MCT_HD inline void ore_precompute(const SurfArgs& A, int idx) {
     uint8_t* out = A.ore + static_cast<size_t>(idx) * kOreRowsPad;
     struct alignas(16) Row16 { uint32_t w[4]; };
     uint32_t w0 = 0, w1 = 0, w2 = 0, w3 = 0, acc = 0;   // the last 4 complete words, the open one
+    uint32_t gap[kOreMaskWords], raw[kOreMaskWords];   // rows waiting for the gap noise; their raw draw
+    for (int j = 0; j < kOreMaskWords; ++j) gap[j] = raw[j] = 0;
     int r = 0;
     for (int k = 0; k < kOreRules; ++k) {
         if (ore_rows(k) == 0) continue;
@@ -1597,8 +1588,16 @@ MCT_HD inline void ore_precompute(const SurfArgs& A, int idx) {
                         Xoro rnd = at(S.ore, x, y, z);
                         if (!(next_float(rnd) > density)) {
                             const float richness = mctgen::eval_ore(S.oct, x, y, z, ch, 2 * k + 1);
-                            if (next_float(rnd) < richness) v = static_cast<uint32_t>(kGapMarker | (next_float(rnd) < M.ore_raw[k] ? 0x10 : 0) | k);
-                            else v = static_cast<uint32_t>(M.ore_state[k][2]);
+                            v = static_cast<uint32_t>(M.ore_state[k][2]);
+                            if (next_float(rnd) < richness) {
+                                const bool rw = next_float(rnd) < M.ore_raw[k];
+                                for (int j = 0; j < kOreMaskWords; ++j) {
+                                    if ((r >> 5) == j) {
+                                        gap[j] |= 1u << (r & 31);
+                                        if (rw) raw[j] |= 1u << (r & 31);
+                                    }
+                                }
+                            }
                         }
                     }
                     acc |= v << ((r & 3) * 8);
@@ -1614,6 +1613,29 @@ MCT_HD inline void ore_precompute(const SurfArgs& A, int idx) {
         if (r & 3) { w0 = w1; w1 = w2; w2 = w3; w3 = acc; }
         for (int j = ((r & 15) + 3) >> 2; j < 4; ++j) { w0 = w1; w1 = w2; w2 = w3; w3 = 0; }
         *reinterpret_cast<Row16*>(out + (r & ~15)) = Row16{{w0, w1, w2, w3}};
+    }
+    for (;;) {
+        int row = -1;
+        for (int j = kOreMaskWords - 1; j >= 0 && row < 0; --j)
+            if (gap[j]) row = j * 32 + 31 - __builtin_clz(gap[j]);
+#if defined(__HIP_DEVICE_COMPILE__)
+        if (__builtin_amdgcn_ballot_w64(row >= 0) == 0) break;
+#else
+        if (row < 0) break;
+#endif
+        if (row >= 0) {
+            int k = 0;
+            while (k < kOreRules - 1 && row >= ore_row_offset(k + 1)) ++k;
+            const int y = mctgen::kOreYLo[k] + row - ore_row_offset(k);
+            bool rw = false;
+            for (int j = 0; j < kOreMaskWords; ++j) {
+                if ((row >> 5) == j) {
+                    gap[j] &= ~(1u << (row & 31));
+                    rw = ((raw[j] >> (row & 31)) & 1u) != 0;
+                }
+            }
+            if (mctgen::eval_point(S.oct, x, y, z, 5 + k) < 0.0f) out[row] = static_cast<uint8_t>(rw ? M.ore_state[k][1] : M.ore_state[k][0]);
+        }
     }
 }
 
@@ -1755,7 +1777,6 @@ MCT_HD MCT_FI void frozen_ocean(Surf& c, int min_surface, int biome, int height,
 template <bool L, class S>
 MCT_HD MCT_FI void surface_walk(S& c, int height, int surface_biome, int starting) {
     int stone_above = 0, water_height = INT32_MIN, next_ceiling = INT32_MAX;
-    int marker_top = kMinY - 1;   // highest block given a gap marker (S::kDeferGap)
     uint32_t word = 0;   // the LDS class word of the current 16 blocks (L)
     int word_r = -1;
     for (int yy = height; yy >= kMinY; --yy) {
@@ -1817,39 +1838,7 @@ MCT_HD MCT_FI void surface_walk(S& c, int height, int surface_biome, int startin
                 c.biome_set = true;
                 state = mctgen::apply_rules(c);
             }
-            if (S::kDeferGap && state >= kGapMarker) {
-                // the filler's class and height; the byte carries the pending test
-                block_set<L>(c, yy, c.A->M->ore_state[state & 15][2], true);
-                c.col[c.bidx(yy)] = static_cast<uint8_t>(state);
-                if (marker_top < kMinY) marker_top = yy;
-            } else if (state >= 0) {
-                block_set<L>(c, yy, state, true);
-            }
-        }
-    }
-    if constexpr (S::kDeferGap) {
-        // Pending filler-gap tests: each lane takes its next marked block (top down), and all lanes
-        // that have one evaluate the gap noise together, until no lane of the wave has one left.
-        int scan = marker_top;
-        for (;;) {
-            int found = kMinY - 1;
-            while (scan >= kMinY) {
-                const int by = scan--;
-                if (c.col[c.bidx(by)] >= kGapMarker) { found = by; break; }
-            }
-            const bool have = found >= kMinY;
-#if defined(__HIP_DEVICE_COMPILE__)
-            if (__builtin_amdgcn_ballot_w64(have) == 0) break;
-#else
-            if (!have) break;
-#endif
-            if (have) {
-                const int m = c.col[c.bidx(found)];
-                const int k = m & 15;
-                const float gap = mctgen::eval_point(c.O, c.x, found, c.z, 5 + k);
-                const int st = gap < 0.0f ? ((m & 0x10) ? c.A->M->ore_state[k][1] : c.A->M->ore_state[k][0]) : c.A->M->ore_state[k][2];
-                c.col[c.bidx(found)] = static_cast<uint8_t>(st);
-            }
+            if (state >= 0) block_set<L>(c, yy, state, true);
         }
     }
     if (surface_biome == mctgen::kBiomeFrozenOcean || surface_biome == mctgen::kBiomeDeepFrozenOcean)
