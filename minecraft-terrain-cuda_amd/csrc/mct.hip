@@ -182,6 +182,22 @@ __global__ __launch_bounds__(256) void k_biome_chain(int nbk, const int64_t* tar
     }
 }
 
+// The biomes of each quart column of a biome chunk (a superset of any block's zoom candidates
+// taken from it), so the surface rules can skip the zoom where no candidate can matter.
+__global__ __launch_bounds__(256) void k_biome_masks(int nbk, const uint8_t* bb, uint64_t* mask) {
+    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= nbk * 16) return;
+    const uint8_t* q = bb + static_cast<size_t>(idx >> 4) * kQPer + (idx & 15);
+    uint64_t m0 = 0, m1 = 0;
+    for (int qy = 0; qy < kQH; ++qy) {
+        const int b = q[qy * 16];
+        if (b < 64) m0 |= 1ull << b;
+        else m1 |= 1ull << (b - 64);
+    }
+    mask[idx * 2] = m0;
+    mask[idx * 2 + 1] = m1;
+}
+
 // ---- density ------------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_col_keys(KeyCodec K, const ChunkRef* ch, int n, uint64_t* keys) {
     const int idx = blockIdx.x * blockDim.x + threadIdx.x;
@@ -866,6 +882,7 @@ struct Slot {
     int64_t *d_targets = nullptr, *d_mind = nullptr;
     int32_t *d_lstar = nullptr, *d_grid = nullptr, *d_cloc = nullptr, *d_cfl = nullptr;
     uint8_t *d_tie = nullptr, *d_bb = nullptr, *d_blocks = nullptr, *d_bout = nullptr, *d_aq = nullptr;
+    uint64_t* d_bmask = nullptr;    // biome chunks x 16 quart columns x 2: the biomes of each quart column
     uint64_t *d_ckeys = nullptr, *d_ucols = nullptr, *d_gkeys = nullptr, *d_ug = nullptr;
     int *d_colmap = nullptr, *d_gmap = nullptr, *d_counts = nullptr, *d_skip = nullptr;
     int *d_h0 = nullptr, *d_h1 = nullptr, *d_h2 = nullptr, *d_hf = nullptr, *d_flags = nullptr;
@@ -921,7 +938,7 @@ struct Slot {
     }
     ~Slot() {
         if (stream) (void)hipStreamSynchronize(stream);
-        void* p[] = {d_ch, d_bch, d_nb, d_bcol, d_colv, d_corners, d_gv, d_prelim, d_targets, d_mind, d_lstar, d_grid, d_cloc, d_cfl, d_tie, d_bb,
+        void* p[] = {d_ch, d_bch, d_nb, d_bcol, d_colv, d_corners, d_gv, d_prelim, d_targets, d_mind, d_lstar, d_grid, d_cloc, d_cfl, d_tie, d_bb, d_bmask,
                      d_blocks, d_bout, d_aq, d_ckeys, d_ucols, d_gkeys, d_ug, d_colmap, d_gmap, d_counts, d_skip, d_h0, d_h1, d_h2, d_hf, d_flags, d_colrec,
                      d_density};
         for (void* q : p) (void)hipFree(q);
@@ -937,7 +954,7 @@ struct Slot {
     }
     void ensure_biome_capacity(int nbk) {
         if (nbk <= bcap) return;
-        void* old[] = {d_bch, d_bcol, d_targets, d_mind, d_lstar, d_tie, d_bb};
+        void* old[] = {d_bch, d_bcol, d_targets, d_mind, d_lstar, d_tie, d_bb, d_bmask};
         for (void* q : old) (void)hipFree(q);
         const size_t q = static_cast<size_t>(nbk) * kQPer;
         hip_check(hipMalloc(&d_bch, sizeof(ChunkRef) * nbk), "malloc");
@@ -947,11 +964,13 @@ struct Slot {
         hip_check(hipMalloc(&d_lstar, sizeof(int32_t) * q), "malloc");
         hip_check(hipMalloc(&d_tie, q), "malloc");
         hip_check(hipMalloc(&d_bb, q), "malloc");
+        hip_check(hipMalloc(&d_bmask, sizeof(uint64_t) * nbk * 16 * 2), "malloc");
         bcap = nbk;
     }
     size_t biome_bytes() const {
         const size_t q = static_cast<size_t>(bcap) * kQPer;
-        return sizeof(ChunkRef) * bcap + sizeof(float) * bcap * 16 * std::max(mctgen::kNumClimateColumns, 1) + q * (7 * 8 + 8 + 4 + 1 + 1);
+        return sizeof(ChunkRef) * bcap + sizeof(float) * bcap * 16 * std::max(mctgen::kNumClimateColumns, 1) + q * (7 * 8 + 8 + 4 + 1 + 1) +
+               sizeof(uint64_t) * bcap * 16 * 2;
     }
 };
 
@@ -1254,6 +1273,7 @@ struct Generator {
         });
         timed(S, kKBiomeTies, [&] { k_biome_ties<<<grid_of(static_cast<size_t>(nbk) * kQPer, W(kKBiomeTies)), W(kKBiomeTies), 0, st>>>(nbk, S.d_targets, S.d_lstar, S.d_mind, S.d_tie); });
         timed(S, kKBiomeChain, [&] { k_biome_chain<<<grid_of(static_cast<size_t>(nbk) * 64, W(kKBiomeChain)), W(kKBiomeChain), 0, st>>>(nbk, S.d_targets, S.d_lstar, S.d_mind, S.d_tie, S.d_bb, d_stats); });
+        k_biome_masks<<<grid_of(static_cast<size_t>(nbk) * 16, 256), 256, 0, st>>>(nbk, S.d_bb, S.d_bmask);
         quarts += static_cast<uint64_t>(nbk) * kQPer;
         hip_check(hipGetLastError(), "biome kernels");
         hip_check(hipEventRecord(S.ev[2], st), "event");
@@ -1291,7 +1311,7 @@ struct Generator {
                                                                                                reinterpret_cast<uint4*>(S.d_aq));
         hip_check(hipEventRecord(S.ev[6], st), "event");
         // surface
-        SurfArgs A{sd, nullptr, S.d_ch, blocks, S.d_nb, S.d_bb, S.d_corners, S.d_colmap, S.d_prelim, S.d_colrec, S.d_ore};
+        SurfArgs A{sd, nullptr, S.d_ch, blocks, S.d_nb, S.d_bb, S.d_corners, S.d_colmap, S.d_prelim, S.d_colrec, S.d_ore, S.d_bmask};
         if (kOreRows > 0)
             timed(S, kKOre, [&] { k_ore<<<grid_of(static_cast<size_t>(n) * 256, W(kKOre)), W(kKOre), 0, st>>>(A, n); });
         timed(S, kKSurfaceCols, [&] { k_surface_cols<<<grid_of(static_cast<size_t>(n) * 256, W(kKSurfaceCols)), W(kKSurfaceCols), 0, st>>>(sd, S.d_ch, n, S.d_colrec); });

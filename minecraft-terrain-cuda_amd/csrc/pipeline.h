@@ -1278,6 +1278,7 @@ struct SurfArgs {
     const float* prelim;        // n x 4 chunk-corner preliminary surface values
     const float* colrec;        // n x 256 column records (kColF floats, column = lx * 16 + lz)
     uint8_t* ore;               // n x 256 x kOreRowsPad ore-vein outcomes (ore_precompute), or null
+    const uint64_t* bmask;      // biome chunks x 16 quart columns x 2: biomes of each quart column, or null
 };
 
 // ---- ore veins ahead of the surface rules -------------------------------------------------
@@ -1329,6 +1330,7 @@ struct Surf {
     int qoff[4];           // zoom_columns
     uint32_t row_lo, row_hi;   // candidate biomes of quart rows bc_py and bc_py + 1
     uint64_t bc_any[2];    // biomes among that row's 8 candidates (bit b of word b >> 6)
+    uint64_t col_any[2];   // biomes of the 4 candidate quart columns at any height (all bits: unknown)
     int ore_cy;            // cell row of the cached ore-channel interpolation
     float ore_lo[kCh], ore_step[kCh];
     float prelim;
@@ -1356,11 +1358,17 @@ struct Surf {
     // offsets into the biome chunks' quarts, set once per column; a quart row's 4 biomes packed
     // in a word, byte j.
     MCT_HD MCT_FI void zoom_columns() {
+        col_any[0] = col_any[1] = A->bmask ? 0 : ~0ull;
         for (int j = 0; j < 4; ++j) {
             const int qx = zoom.px + (j >> 1), qz = zoom.pz + (j & 1);
             const int dx = (qx >> 2) - cx, dz = (qz >> 2) - cz;
             const int bc = A->nb_index[ci * 9 + (dz + 1) * 3 + dx + 1];
             qoff[j] = bc * kQPer + (qz & 3) * 4 + (qx & 3);
+            if (A->bmask) {
+                const uint64_t* m = A->bmask + (static_cast<size_t>(bc) * 16 + (qz & 3) * 4 + (qx & 3)) * 2;
+                col_any[0] |= m[0];
+                col_any[1] |= m[1];
+            }
         }
     }
     MCT_HD MCT_FI uint32_t zoom_quart_row(int qy) const {
@@ -1411,16 +1419,21 @@ struct Surf {
     // evaluates the rules again (they have no effects beyond per-block memos).
     MCT_HD MCT_FI int biome() {
         if (!biome_set) {
+            zoom_row(y);
             need_biome = true;
             return bc_first;
         }
         return biome_v;
     }
     // Whether the block's biome is in the set {b : bit b of (lo, hi)}. The zoom picks one of the
-    // row's 8 candidate quarts (zoom_row, set up by the walk), so where the set holds all or
-    // none of them the answer needs no zoom.
+    // row's 8 candidate quarts (zoom_row, loaded here when first needed), so where the set holds
+    // all or none of them the answer needs no zoom; where it holds all or none of the biomes of
+    // the 4 candidate columns at any height, not even the row.
     MCT_HD MCT_FI bool biome_in(uint64_t lo, uint64_t hi) {
         if (!biome_set) {
+            if (((col_any[0] & lo) | (col_any[1] & hi)) == 0) return false;
+            if (((col_any[0] & ~lo) | (col_any[1] & ~hi)) == 0) return true;
+            zoom_row(y);
             if (((bc_any[0] & lo) | (bc_any[1] & hi)) == 0) return false;
             if (((bc_any[0] & ~lo) | (bc_any[1] & ~hi)) == 0) return true;
         }
@@ -1831,7 +1844,6 @@ MCT_HD MCT_FI void surface_walk(S& c, int height, int surface_biome, int startin
             c.biome_set = false;
             c.ore_set = false;
             c.need_biome = false;
-            c.zoom_row(yy);
             int state = mctgen::apply_rules(c);
             if (c.need_biome) {
                 c.biome_v = c.biome_y(yy);
