@@ -30,21 +30,28 @@ sys.path.insert(0, os.path.join(ROOT, 'minecraft-terrain-cuda_amd'))
 
 from workloads import SEED, BORDER_CHUNK, region, plain_region_set  # noqa: E402
 
-# MI355X VALU issue rate: 256 CUs x 64 lanes x 2.4 GHz (non-packed, one op per lane per
-# cycle; FP32 and FP64 both full rate on CDNA4; no FMA is used, so one op = one flop).
-VALU_PEAK_TOPS = 256 * 64 * 2.4e9 / 1e12
+# MI355X VALU peak: the published FP32 vector peak, 157.3 TFLOP/s, is 256 CUs x 128 lanes x
+# 2.4 GHz x 2 flops (FMA). In instructions: 32-bit VALU operations issue at two wave64
+# instructions per CU per cycle (128 lane-ops), FP64 ones at half that (the published FP64
+# vector peak, 78.6 TFLOP/s). The roofline counts VALU lane-ops against the 32-bit rate; a
+# kernel with FP64 work cannot reach 1.0.
+VALU_PEAK_TOPS = 256 * 128 * 2.4e9 / 1e12
 
 # Work per unit of the main kernels, measured with rocprofv3 on this workload (the summaries
-# are in profiles/valu_plain3x64.md and profiles/hbm_plain3x64.md; the unit is what one thread
-# of the kernel processes):
-#   VALU lane-ops   SQ_INSTS_VALU x 64 per thread (the instruction stream the kernel issues)
-#   HBM bytes       FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 per thread, the factors from a plain
-#                   1 GiB device-to-device copy measured in the same runs
+# are in profiles/valu_plain3x64.md and profiles/hbm_plain3x64.md; per thread there, per unit
+# here):
+#   VALU lane-ops   SQ_INSTS_VALU x 64 (the instruction stream the kernel issues)
+#   HBM bytes       FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0, the factors from a plain 1 GiB
+#                   device-to-device copy measured in the same runs
 WORK_PER_UNIT = {
-    'surface2': {'unit': 'block column', 'valu': 64086.6, 'hbm': 1033.38 + 114.38},
-    'corners': {'unit': 'corner', 'valu': 36340.3, 'hbm': 30.0},   # per corner evaluated (whole idle workgroups past the distinct count are not units)
-    'biome_quarts': {'unit': 'run of 8 quarts', 'valu': 108288.5 + 8 * 150.0, 'hbm': 449.51 + 101.51 + 8 * (1.51 + 56.08)},   # search + targets
-    'fill': {'unit': 'block column of a cell (8 blocks)', 'valu': 1548.5, 'hbm': 12.65 + 8.00},
+    'surface2': {'unit': 'block column', 'valu': 14853.9, 'hbm': 1014.23 + 75.27},
+    # per corner evaluated: the launch covers every corner column slot of the batch, 1.62
+    # threads per distinct corner on this workload, and the idle workgroups exit at once
+    'corners': {'unit': 'corner', 'valu': 23401.2, 'hbm': 31.19},
+    # search per quart column (2 restricted searches) + 96 quart targets
+    'biome_quarts': {'unit': 'quart column of a biome chunk', 'valu': 42348.2 + 96 * 150.0, 'hbm': 10763.87 + 1151.11 + 96 * (1.51 + 56.08)},
+    'fill': {'unit': 'block column of a cell (8 blocks)', 'valu': 1344.4, 'hbm': 14.18 + 8.63},
+    'ore_veins': {'unit': 'block column', 'valu': 11352.4, 'hbm': 35.76 + 221.00},
 }
 
 
@@ -59,11 +66,11 @@ def distinct_columns(chunks):
 def units_per_step(chunks, batch):
     """Threads (units) each main kernel runs over the whole workload, batch by batch (corner
     columns and biome chunks are deduplicated within a batch)."""
-    u = {'surface2': 256 * len(chunks), 'fill': 256 * 48 * len(chunks), 'corners': 0, 'biome_quarts': 0}
+    u = {'surface2': 256 * len(chunks), 'fill': 256 * 48 * len(chunks), 'ore_veins': 256 * len(chunks), 'corners': 0, 'biome_quarts': 0}
     for k in range(0, len(chunks), batch):
         part = chunks[k:k + batch]
         u['corners'] += distinct_columns(part) * 49
-        u['biome_quarts'] += len({(x + dx, z + dz) for x, z in part for dz in (-1, 0, 1) for dx in (-1, 0, 1)}) * 16 * 12
+        u['biome_quarts'] += len({(x + dx, z + dz) for x, z in part for dz in (-1, 0, 1) for dx in (-1, 0, 1)}) * 16
     return u
 
 
@@ -262,6 +269,7 @@ def main():
     ap.add_argument('--no-serve', action='store_true', help='skip the served-request measurement')
     ap.add_argument('--no-stress', action='store_true', help='skip the 1,024-seed stress leg')
     ap.add_argument('--no-tune', action='store_true', help='default launch shapes (no autotuning)')
+    ap.add_argument('--tuning-json', metavar='JSON', help='launch shapes to use instead of autotuning, as the "tuning" object of a bench line')
     ap.add_argument('--plain-only', action='store_true', help='only the plain workload (no batched or served legs), e.g. under a profiler')
     ap.add_argument('--copy-calibration', type=int, default=0, metavar='BYTES',
                     help='also run one device-to-device copy of BYTES (the reference point for HBM counters)')
@@ -289,7 +297,10 @@ def main():
         a.batch = max(batch_trials, key=batch_trials.get)
     g = mct.FullTerrainGenerator(max_batch_chunks=a.batch, pipeline_slots=a.slots, seed_capacity=256)
     tuning = None
-    if not a.no_tune:   # launch shapes timed on the workload's first batch, before any timed step
+    if a.tuning_json:
+        tuning = json.loads(a.tuning_json)
+        g.set_tuning(tuning)
+    elif not a.no_tune:   # launch shapes timed on the workload's first batch, before any timed step
         tuning = g.autotune(seeds[:a.batch], chunks[:a.batch])
     for _ in range(a.warmup):
         g.generate_device(seeds, chunks, d_t, d_b)
