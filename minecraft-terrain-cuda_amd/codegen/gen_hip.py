@@ -962,6 +962,15 @@ class Gen:
               ['switch (which) {'] + ['    case %d: return e.n%d();' % (k, r) for k, r in enumerate(self.ore_roots)] + ['}', 'return 0.0f;'])
         entry('void eval_climate(const mct::Octave* O, int32_t x, int32_t y, int32_t z, const float* col, float* out)', 'Quart e{O, x, y, z, col};',
               ['out[%d] = e.n%d();' % (k, r) for k, r in enumerate(self.climate)])
+        # climate values that are a column-record read in the Quart context (a y = 0 slice of the
+        # climate columns): the same for every quart of a quart column
+        qctx = Ctx('Quart', 'vol', 'col', 'err', cols=self.col_b)
+        colconst = 0
+        for k, r in enumerate(self.climate):
+            n = N[r]
+            if n['t'] == 'slice' and self._slice_mode(n, qctx) == 'col':
+                colconst |= 1 << k
+        e.append('constexpr int kClimateColumnConst = 0x%x;   // bit k: climate value k is the same for all quarts of a quart column' % colconst)
         dens, ub = self.fts['density'], self.fts['upper_bound']
         entry('float fts_probe(const mct::Octave* O, int32_t x, int32_t z, float upper, PointY0& sub)',
               'const float q = upper / static_cast<float>(kFtsCellHeight);', [

@@ -536,16 +536,29 @@ MCT_HD inline void biome_search_column(const ModelDev& M, const TreeNodeF* top, 
     const int b = idx >> 4, qx = idx & 3, qz = (idx >> 2) & 3;
     const size_t col = static_cast<size_t>(b) * kQPer + qz * 4 + qx;   // + qy * 16: the column's quarts
     const int64_t* t0 = targets + col * 7;
-    int vary = 0;
-    bool small = M.treef != nullptr;
-    for (int qy = 0; qy < kQH; ++qy) {
-        const int64_t* t = targets + (col + static_cast<size_t>(qy) * 16) * 7;
-        for (int k = 0; k < 7; ++k) {
-            if (t[k] != t0[k]) vary |= 1 << k;
-            small = small && t[k] >= -static_cast<int64_t>(kTreeExact) && t[k] <= static_cast<int64_t>(kTreeExact);
+    auto in_range = [](int64_t v) { return v >= -static_cast<int64_t>(kTreeExact) && v <= static_cast<int64_t>(kTreeExact); };
+    // The code generator marks the climate values that are column-record reads (the same for
+    // every quart of the column); the target's offset coordinate is always 0. When at most one
+    // coordinate is left, the column varies in that one alone and needs no scan of its targets;
+    // a quart whose varying value leaves the float tree's exact range is marked for the int64
+    // search (L* = -2) as in the per-quart search.
+    constexpr int kMayVary = 0x3F & ~mctgen::kClimateColumnConst;
+    int vary = 0, k = -1;
+    bool small = M.treef != nullptr, scanned = false;
+    if constexpr ((kMayVary & (kMayVary - 1)) == 0) {
+        for (int j = 0; j < 7; ++j)
+            if (!((kMayVary >> j) & 1)) small = small && in_range(t0[j]);
+        vary = kMayVary;
+    } else {
+        scanned = true;
+        for (int qy = 0; qy < kQH; ++qy) {
+            const int64_t* t = targets + (col + static_cast<size_t>(qy) * 16) * 7;
+            for (int j = 0; j < 7; ++j) {
+                if (t[j] != t0[j]) vary |= 1 << j;
+                small = small && in_range(t[j]);
+            }
         }
     }
-    int k = -1;
     if (small) {
         if (vary == 0) {
             for (int j = 6; j >= 0; --j)
@@ -589,7 +602,13 @@ MCT_HD inline void biome_search_column(const ModelDev& M, const TreeNodeF* top, 
     }
     for (int qy = 0; qy < kQH; ++qy) {
         const size_t i = col + static_cast<size_t>(qy) * 16;
-        const float v = static_cast<float>(targets[i * 7 + k]);
+        const int64_t tv = targets[i * 7 + k];
+        if (!scanned && !in_range(tv)) {
+            lstar[i] = -2;
+            mind[i] = -1;
+            continue;
+        }
+        const float v = static_cast<float>(tv);
         double best = inf;
         int bl = -1, br = INT32_MAX;
 #pragma unroll
