@@ -267,11 +267,12 @@ __global__ __launch_bounds__(256) void k_aq_cells(const SeedDev* seeds, const Ch
 // then skipped). The chunk's aquifer cells the row can read are staged in LDS. The blocks
 // whose aquifer barrier / pressure part decides (a few percent of the blocks, but spread over
 // most waves that reach the aquifer) are queued in LDS and resolved after the row, densely, by
-// all threads of the workgroup.
+// all threads of the workgroup. Each column's 8 block classes (2 bits each) go to the class
+// words the surface walk reads (SurfArgs::clsw), half a word per cell row.
 constexpr int kFillStage = 36;   // 3 x 3 aquifer cells by at most 4 grid rows (8 block rows +-1)
 constexpr int kFillHead = kFillStage * 6 + 4;   // ints before the per-thread arrays
 static_assert(kFillHead * 4 % 16 == 0, "per-thread LDS arrays stay 16-byte aligned");
-constexpr size_t fill_lds_bytes(int w) { return static_cast<size_t>(kFillHead) * 4 + 52 * static_cast<size_t>(w); }
+constexpr size_t fill_lds_bytes(int w) { return static_cast<size_t>(kFillHead) * 4 + 56 * static_cast<size_t>(w); }
 
 struct AqCellsLds {
     const MCT_LDS int* loc;   // 4 ints per staged cell
@@ -297,8 +298,9 @@ __global__ __launch_bounds__(256) void k_fill(FillArgs A, int n) {
     int* sfl = fill_lds + kFillStage * 4;
     int* qn = fill_lds + kFillStage * 6;
     int* hm = fill_lds + kFillHead;
-    float* qd = reinterpret_cast<float*>(fill_lds + kFillHead + W);
-    uint16_t* qk = reinterpret_cast<uint16_t*>(fill_lds + kFillHead + 9 * W);
+    uint32_t* fr = reinterpret_cast<uint32_t*>(fill_lds + kFillHead + W);   // class bits of the column's 8 blocks
+    float* qd = reinterpret_cast<float*>(fill_lds + kFillHead + 2 * W);
+    uint16_t* qk = reinterpret_cast<uint16_t*>(fill_lds + kFillHead + 10 * W);
     const ChunkRef cr = A.chunks[c];
     const int ylo = kMinY + cy * 8;
     const int skip = A.skip[c];
@@ -325,6 +327,7 @@ __global__ __launch_bounds__(256) void k_fill(FillArgs A, int n) {
     uint8_t* out = A.blocks + static_cast<size_t>(c) * kPer + static_cast<size_t>(cy * 8 * 256 + z * 16 + x);
     const uint32_t below = __builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u));   // lane id
     int top = -1;
+    uint32_t frag = 0;
     for (int yi = 0; yi < 8; ++yi) {
         const int by = ylo + yi;
         const float d = mctgen::eval_block(O, bx, by, bz, val);
@@ -347,10 +350,12 @@ __global__ __launch_bounds__(256) void k_fill(FillArgs A, int n) {
             const int state = s < 0 ? mctgen::kDefaultBlock : s;
             out[yi * 256] = static_cast<uint8_t>(state);
             if (!((mctgen::kAirMask >> state) & 1u)) top = yi;
+            frag |= static_cast<uint32_t>(block_class(state)) << (2 * yi);
         }
         for (int k = 0; k < mctgen::kNumChannels; ++k) val[k] += step[k];
     }
     hm[tid] = top;
+    fr[tid] = frag;   // queued blocks still at 0, kClsSolid
     __syncthreads();
     const int q = *qn;
     for (int e = tid; e < q; e += W) {
@@ -362,9 +367,13 @@ __global__ __launch_bounds__(256) void k_fill(FillArgs A, int n) {
         const int state = s < 0 ? mctgen::kDefaultBlock : s;
         A.blocks[static_cast<size_t>(c) * kPer + static_cast<size_t>((cy * 8 + yi) * 256 + ez * 16 + ex)] = static_cast<uint8_t>(state);
         if (!((mctgen::kAirMask >> state) & 1u)) atomicMax(&hm[lt], yi);
+        const uint32_t cl = static_cast<uint32_t>(block_class(state));
+        if (cl != kClsSolid) atomicOr(&fr[lt], cl << (2 * yi));
     }
     __syncthreads();
     top = hm[tid];
+    if (A.clsw)
+        reinterpret_cast<uint16_t*>(A.clsw + (static_cast<size_t>(c) * (kHeight / 16) + (cy >> 1)) * 256 + col0 + tid)[cy & 1] = static_cast<uint16_t>(fr[tid]);
     // WORLD_SURFACE_WG after the fill: the highest non-air block + 1 of the column, the max over
     // its cells (the heights start at minY; a plain read skips most of the atomics)
     if (top >= 0 && A.h0) {
@@ -428,13 +437,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void k_
     A.M = &c_model;
     const int t = static_cast<int>((blockIdx.x & 3) * 64 + threadIdx.x);
     const int x = t & 15, z = t >> 4;
-    const uint8_t* colp = A.blocks + static_cast<size_t>(c) * kPer + z * 16 + x;
-#pragma clang loop unroll(disable)
-    for (int w = 0; w < kHeight / 16; ++w) {
-        uint32_t bits = 0;
-        for (int j = 0; j < 16; ++j) bits |= static_cast<uint32_t>(block_class(colp[static_cast<size_t>(w * 16 + j) * 256])) << (2 * j);
-        cls[w * 64 + threadIdx.x] = bits;
-    }
+    const uint32_t* cw = A.clsw + static_cast<size_t>(c) * (kHeight / 16) * 256 + t;
+    for (int w = 0; w < kHeight / 16; ++w) cls[w * 64 + threadIdx.x] = cw[w * 256];
     const float* rec = A.colrec + (static_cast<size_t>(c) * 256 + static_cast<size_t>(x * 16 + z)) * kColF;
     for (int k = 0; k < kColF; ++k) recs[k * 64 + threadIdx.x] = rec[k];
     surface_pass2<true>(A, h0, h1, h2, hf, c, x * 16 + z, (uint32_t MCT_LDS*)(cls + threadIdx.x), (const float MCT_LDS*)(recs + threadIdx.x));
@@ -888,6 +892,7 @@ struct Slot {
     int *d_h0 = nullptr, *d_h1 = nullptr, *d_h2 = nullptr, *d_hf = nullptr, *d_flags = nullptr;
     float* d_colrec = nullptr;
     uint8_t* d_ore = nullptr;
+    uint32_t* d_clsw = nullptr;     // block-class words (SurfArgs::clsw)
     float* d_density = nullptr;     // core configuration's density output (allocated on first use)
     std::unique_ptr<Dedupe> dedupe;
     hipEvent_t kev[kKCount][2] = {};
@@ -932,6 +937,7 @@ struct Slot {
         M(&d_hf, sizeof(int) * mb * 256);
         M(&d_colrec, sizeof(float) * mb * 256 * kColF);
         M(&d_ore, static_cast<size_t>(mb) * 256 * kOreRowsPad);
+        M(&d_clsw, sizeof(uint32_t) * mb * (kHeight / 16) * 256);
         M(&d_flags, sizeof(int) * mb);
         M(&d_counts, sizeof(int) * 4);
         dedupe = std::make_unique<Dedupe>(mb * kGridKeys);
@@ -940,7 +946,7 @@ struct Slot {
         if (stream) (void)hipStreamSynchronize(stream);
         void* p[] = {d_ch, d_bch, d_nb, d_bcol, d_colv, d_corners, d_gv, d_prelim, d_targets, d_mind, d_lstar, d_grid, d_cloc, d_cfl, d_tie, d_bb, d_bmask,
                      d_blocks, d_bout, d_aq, d_ckeys, d_ucols, d_gkeys, d_ug, d_colmap, d_gmap, d_counts, d_skip, d_h0, d_h1, d_h2, d_hf, d_flags, d_colrec,
-                     d_density};
+                     d_ore, d_clsw, d_density};
         for (void* q : p) (void)hipFree(q);
         dedupe.reset();
         (void)hipHostFree(h_ch);
@@ -1302,6 +1308,7 @@ struct Generator {
         FillArgs F{sd, S.d_ch, S.d_corners, S.d_colmap, S.d_cloc, S.d_cfl, S.d_skip, blocks, nullptr, nullptr};
         FillArgs FH = F;
         FH.h0 = S.d_h0;
+        FH.clsw = S.d_clsw;
         timed(S, kKHeights, [&] { k_heights_init<<<grid_of(static_cast<size_t>(n) * 256, W(kKHeights)), W(kKHeights), 0, st>>>(n, S.d_h0, S.d_counts + 2); });
         timed(S, kKFill, [&] { k_fill<<<grid_of(static_cast<size_t>(n) * 256 * kCellsY, W(kKFill)), W(kKFill), fill_lds_bytes(W(kKFill)), st>>>(FH, n); });
         hip_check(hipGetLastError(), "fill kernel");
@@ -1311,7 +1318,7 @@ struct Generator {
                                                                                                reinterpret_cast<uint4*>(S.d_aq));
         hip_check(hipEventRecord(S.ev[6], st), "event");
         // surface
-        SurfArgs A{sd, nullptr, S.d_ch, blocks, S.d_nb, S.d_bb, S.d_corners, S.d_colmap, S.d_prelim, S.d_colrec, S.d_ore, S.d_bmask};
+        SurfArgs A{sd, nullptr, S.d_ch, blocks, S.d_nb, S.d_bb, S.d_corners, S.d_colmap, S.d_prelim, S.d_colrec, S.d_ore, S.d_bmask, S.d_clsw};
         if (kOreRows > 0)
             timed(S, kKOre, [&] { k_ore<<<grid_of(static_cast<size_t>(n) * 256, W(kKOre)), W(kKOre), 0, st>>>(A, n); });
         timed(S, kKSurfaceCols, [&] { k_surface_cols<<<grid_of(static_cast<size_t>(n) * 256, W(kKSurfaceCols)), W(kKSurfaceCols), 0, st>>>(sd, S.d_ch, n, S.d_colrec); });

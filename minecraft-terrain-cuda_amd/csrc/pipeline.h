@@ -1046,6 +1046,7 @@ struct FillArgs {
     uint8_t* blocks;
     float* density;             // core configuration: final density per block (nullable)
     int* h0;                    // n x 256 initial heights (column x * 16 + z), raised here; nullable
+    uint32_t* clsw;             // n x 24 x 256 block-class words (see SurfArgs::clsw); nullable
 };
 
 // The 8 blocks of one block column inside one 4x8 cell (t = (cy * 16 + z) * 16 + x): the z
@@ -1298,6 +1299,10 @@ struct SurfArgs {
     const float* colrec;        // n x 256 column records (kColF floats, column = lx * 16 + lz)
     uint8_t* ore;               // n x 256 x kOreRowsPad ore-vein outcomes (ore_precompute), or null
     const uint64_t* bmask;      // biome chunks x 16 quart columns x 2: biomes of each quart column, or null
+    // n x 24 x 256 block-class words: word w of column t = z * 16 + x holds the classes
+    // (block_class, 2 bits) of blocks w * 16 .. w * 16 + 15 above minY, the lowest at bit 0.
+    // Written by the fill, kept current by the badlands extension; read by the LDS walk.
+    uint32_t* clsw;
 };
 
 // ---- ore veins ahead of the surface rules -------------------------------------------------
@@ -1713,22 +1718,27 @@ MCT_HD MCT_FI void surf_update_xz(Surf& c, int gx, int gz) {
 }
 
 // MaterialSystem eroded-badlands extension (writes blocks and moves the height).
-MCT_HD MCT_FI void eroded_badlands(Surf& c, int height) {
+MCT_HD MCT_FI bool eroded_badlands(Surf& c, int height) {
     const double bx = c.x, bz = c.z;
     const double a = fabs(mctgen::surf_badlands_surface(c.O, bx, 0.0, bz) * 8.25);
     const double b = static_cast<double>(static_cast<float>(mctgen::surf_badlands_pillar(c.O, bx * 0.2, 0.0, bz * 0.2)) * 15.0f);
     const double pillar = jmind(a, b);
-    if (pillar <= 0.0) return;
+    if (pillar <= 0.0) return false;
     const double floor_v = fabs(mctgen::surf_badlands_pillar_roof(c.O, bx * 0.75, 0.0, bz * 0.75) * 1.5);
     const double top = 64.0 + jmind(pillar * pillar * 2.5, ceil(floor_v * 50.0) + 24.0);
     const int start_y = mfloor(top);
-    if (height > start_y) return;
+    if (height > start_y) return false;
     for (int yy = start_y; yy >= kMinY; --yy) {
         const int old = c.get(yy);
         if (old == mctgen::kDefaultBlock) break;
-        if ((mctgen::kWaterMask >> old) & 1u) return;
+        if ((mctgen::kWaterMask >> old) & 1u) return false;
     }
-    for (int yy = start_y; yy >= kMinY && c.air(c.get(yy)); --yy) c.set(yy, mctgen::kDefaultBlock, true);
+    bool wrote = false;
+    for (int yy = start_y; yy >= kMinY && c.air(c.get(yy)); --yy) {
+        c.set(yy, mctgen::kDefaultBlock, true);
+        wrote = true;
+    }
+    return wrote;
 }
 
 // What the walk and the frozen-ocean extension test of a block: its class (kClsSolid / kClsAir
@@ -1885,7 +1895,14 @@ MCT_HD MCT_FI void surface_pass1(const SurfArgs& A, const int* h0, int* h1, int*
     surf_init(c, A, ci, lx, lz, &hm);
     const int starting = hm;
     const int sb = c.biome_y(starting);
-    if (sb == mctgen::kBiomeErodedBadlands) eroded_badlands(c, starting);
+    if (sb == mctgen::kBiomeErodedBadlands && eroded_badlands(c, starting) && A.clsw) {
+        // the column's class words again (the extension turned air into stone)
+        for (int w = 0; w < kHeight / 16; ++w) {
+            uint32_t bits = 0;
+            for (int j = 0; j < 16; ++j) bits |= static_cast<uint32_t>(block_class(c.col[c.bidx(kMinY + w * 16 + j)])) << (2 * j);
+            A.clsw[(static_cast<size_t>(ci) * (kHeight / 16) + w) * 256 + lz * 16 + lx] = bits;
+        }
+    }
     h1[hidx] = hm;
     if (sb == mctgen::kBiomeFrozenOcean || sb == mctgen::kBiomeDeepFrozenOcean) {
         surf_update_xz(c, 0, 0);
