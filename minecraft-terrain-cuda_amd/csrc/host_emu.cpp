@@ -206,7 +206,14 @@ int mcte_generate(void* p, int64_t seed, const int32_t* xz, int n, uint8_t* terr
         for (int i = 0; i < nbk * 16; ++i) biome_column(seeds, bch.data(), bcol.data(), i);
         for (int i = 0; i < nbk * kQPer; ++i) biome_target(seeds, bch.data(), bcol.data(), targets.data(), i);
         const int ntop = M.treef ? std::min(mctgen::kTreeNodes, 384) : 0;
-        for (int i = 0; i < nbk * 16; ++i) biome_search_column(M, M.treef, ntop, targets.data(), lstar.data(), mind.data(), i);
+        const int G = (M.treef && kOneVarying) ? M.col_groups[kVaryCoord] : 0;
+        if (G > 0) {
+            std::vector<GroupRes> gres(static_cast<size_t>(nbk) * 16 * G);
+            for (int i = 0; i < nbk * 16 * G; ++i) biome_group_search(M, M.treef, ntop, targets.data(), lstar.data(), mind.data(), gres.data(), G, i);
+            for (int i = 0; i < nbk * kQPer; ++i) biome_quart_final(M, targets.data(), gres.data(), lstar.data(), mind.data(), G, i);
+        } else {
+            for (int i = 0; i < nbk * 16; ++i) biome_search_column(M, M.treef, ntop, targets.data(), lstar.data(), mind.data(), i);
+        }
         for (int i = 0; i < nbk * kQPer; ++i) biome_search_wide(M, targets.data(), lstar.data(), mind.data(), i);
         for (int i = 0; i < nbk * kQPer; ++i) biome_tie(M, targets.data(), lstar.data(), mind.data(), tie.data(), i);
         for (int b = 0; b < nbk; ++b) biome_chain_seq(M, targets.data(), lstar.data(), mind.data(), bb.data(), b);

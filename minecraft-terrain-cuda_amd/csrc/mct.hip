@@ -104,6 +104,23 @@ __global__ __launch_bounds__(256) void k_biome_search_cols(int nbk, const int64_
     if (idx < nbk * 16) biome_search_column(M, top, ntop, targets, lstar, mind, idx);
 }
 
+// The column search split in two: group searches (thread per column and group), then each
+// quart's L* (thread per quart).
+__global__ __launch_bounds__(256) void k_biome_groups(int nbk, int G, const int64_t* targets, int32_t* lstar, int64_t* mind, GroupRes* res) {
+    __shared__ TreeNodeF top[kTreeTop];
+    const ModelDev& M = c_model;
+    const int ntop = mctgen::kTreeNodes < kTreeTop ? mctgen::kTreeNodes : kTreeTop;
+    for (int i = threadIdx.x; i < ntop * 16; i += blockDim.x) reinterpret_cast<uint32_t*>(top)[i] = reinterpret_cast<const uint32_t*>(M.treef)[i];
+    __syncthreads();
+    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx < nbk * 16 * G) biome_group_search(M, top, ntop, targets, lstar, mind, res, G, idx);
+}
+
+__global__ __launch_bounds__(256) void k_biome_quart_final(int nbk, int G, const int64_t* targets, const GroupRes* res, int32_t* lstar, int64_t* mind) {
+    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx < nbk * kQPer) biome_quart_final(c_model, targets, res, lstar, mind, G, idx);
+}
+
 __global__ __launch_bounds__(256) void k_biome_search_wide(int nbk, const int64_t* targets, int32_t* lstar, int64_t* mind) {
     const int idx = blockIdx.x * blockDim.x + threadIdx.x;
     if (idx < nbk * kQPer) biome_search_wide(c_model, targets, lstar, mind, idx);
@@ -887,6 +904,7 @@ struct Slot {
     int32_t *d_lstar = nullptr, *d_grid = nullptr, *d_cloc = nullptr, *d_cfl = nullptr;
     uint8_t *d_tie = nullptr, *d_bb = nullptr, *d_blocks = nullptr, *d_bout = nullptr, *d_aq = nullptr;
     uint64_t* d_bmask = nullptr;    // biome chunks x 16 quart columns x 2: the biomes of each quart column
+    GroupRes* d_gres = nullptr;     // biome chunks x 16 quart columns x groups: column search results
     uint64_t *d_ckeys = nullptr, *d_ucols = nullptr, *d_gkeys = nullptr, *d_ug = nullptr;
     int *d_colmap = nullptr, *d_gmap = nullptr, *d_counts = nullptr, *d_skip = nullptr;
     int *d_h0 = nullptr, *d_h1 = nullptr, *d_h2 = nullptr, *d_hf = nullptr, *d_flags = nullptr;
@@ -944,7 +962,7 @@ struct Slot {
     }
     ~Slot() {
         if (stream) (void)hipStreamSynchronize(stream);
-        void* p[] = {d_ch, d_bch, d_nb, d_bcol, d_colv, d_corners, d_gv, d_prelim, d_targets, d_mind, d_lstar, d_grid, d_cloc, d_cfl, d_tie, d_bb, d_bmask,
+        void* p[] = {d_ch, d_bch, d_nb, d_bcol, d_colv, d_corners, d_gv, d_prelim, d_targets, d_mind, d_lstar, d_grid, d_cloc, d_cfl, d_tie, d_bb, d_bmask, d_gres,
                      d_blocks, d_bout, d_aq, d_ckeys, d_ucols, d_gkeys, d_ug, d_colmap, d_gmap, d_counts, d_skip, d_h0, d_h1, d_h2, d_hf, d_flags, d_colrec,
                      d_ore, d_clsw, d_density};
         for (void* q : p) (void)hipFree(q);
@@ -960,7 +978,7 @@ struct Slot {
     }
     void ensure_biome_capacity(int nbk) {
         if (nbk <= bcap) return;
-        void* old[] = {d_bch, d_bcol, d_targets, d_mind, d_lstar, d_tie, d_bb, d_bmask};
+        void* old[] = {d_bch, d_bcol, d_targets, d_mind, d_lstar, d_tie, d_bb, d_bmask, d_gres};
         for (void* q : old) (void)hipFree(q);
         const size_t q = static_cast<size_t>(nbk) * kQPer;
         hip_check(hipMalloc(&d_bch, sizeof(ChunkRef) * nbk), "malloc");
@@ -971,12 +989,13 @@ struct Slot {
         hip_check(hipMalloc(&d_tie, q), "malloc");
         hip_check(hipMalloc(&d_bb, q), "malloc");
         hip_check(hipMalloc(&d_bmask, sizeof(uint64_t) * nbk * 16 * 2), "malloc");
+        hip_check(hipMalloc(&d_gres, sizeof(GroupRes) * nbk * 16 * kColGroups), "malloc");
         bcap = nbk;
     }
     size_t biome_bytes() const {
         const size_t q = static_cast<size_t>(bcap) * kQPer;
         return sizeof(ChunkRef) * bcap + sizeof(float) * bcap * 16 * std::max(mctgen::kNumClimateColumns, 1) + q * (7 * 8 + 8 + 4 + 1 + 1) +
-               sizeof(uint64_t) * bcap * 16 * 2;
+               sizeof(uint64_t) * bcap * 16 * 2 + sizeof(GroupRes) * bcap * 16 * kColGroups;
     }
 };
 
@@ -990,6 +1009,7 @@ struct Generator {
     uint8_t* d_leaf_biome = nullptr;
     int32_t* d_leaf_node = nullptr;
     int32_t* d_leaf_rank = nullptr;
+    int vary_groups = 0;            // column search groups of the one varying climate coordinate (0: none)
     int* d_blended_first = nullptr;
     int nnormal = 0;
     // resident seeds
@@ -1059,6 +1079,7 @@ struct Generator {
         M.leaf_node = d_leaf_node;
         M.leaf_rank = d_leaf_rank;
         if (d_treef) column_groups(treef, M);
+        vary_groups = (d_treef && kOneVarying) ? M.col_groups[kVaryCoord] : 0;
         std::memcpy(M.block_flags, mctgen::kBlockFlags, 64);
         std::memcpy(M.biome_temp, mctgen::kBiomeTemp, sizeof M.biome_temp);
         std::memcpy(M.biome_frozen, mctgen::kBiomeFrozen, sizeof M.biome_frozen);
@@ -1269,8 +1290,15 @@ struct Generator {
         timed(S, kKBiomeQuarts, [&] {
             k_biome_targets<<<grid_of(static_cast<size_t>(nbk) * kQPer, 256), 256, 0, st>>>(sd, S.d_bch, nbk, S.d_bcol, S.d_targets);
             if (d_treef) {
-                k_biome_search_cols<<<grid_of(static_cast<size_t>(nbk) * 16, W(kKBiomeQuarts)), W(kKBiomeQuarts), 0, st>>>(
-                    nbk, S.d_targets, S.d_lstar, S.d_mind);
+                if (vary_groups > 0) {
+                    k_biome_groups<<<grid_of(static_cast<size_t>(nbk) * 16 * vary_groups, W(kKBiomeQuarts)), W(kKBiomeQuarts), 0, st>>>(
+                        nbk, vary_groups, S.d_targets, S.d_lstar, S.d_mind, S.d_gres);
+                    k_biome_quart_final<<<grid_of(static_cast<size_t>(nbk) * kQPer, 256), 256, 0, st>>>(nbk, vary_groups, S.d_targets, S.d_gres, S.d_lstar,
+                                                                                                         S.d_mind);
+                } else {
+                    k_biome_search_cols<<<grid_of(static_cast<size_t>(nbk) * 16, W(kKBiomeQuarts)), W(kKBiomeQuarts), 0, st>>>(
+                        nbk, S.d_targets, S.d_lstar, S.d_mind);
+                }
                 k_biome_search_wide<<<grid_of(static_cast<size_t>(nbk) * kQPer, 256), 256, 0, st>>>(nbk, S.d_targets, S.d_lstar, S.d_mind);
             } else {
                 k_biome_search_scalar<<<grid_of(static_cast<size_t>(nbk) * 16 * biome_runs(run_len), W(kKBiomeQuarts)), W(kKBiomeQuarts), 0, st>>>(

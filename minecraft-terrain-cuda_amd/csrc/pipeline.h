@@ -521,6 +521,12 @@ MCT_HD inline void biome_search_wide(const ModelDev& M, const int64_t* targets, 
     mind[idx] = d;
 }
 
+// Climate target coordinates that may vary along a quart column: all but the column-record reads
+// the code generator marks and the offset coordinate (always 0).
+constexpr int kMayVary = 0x3F & ~mctgen::kClimateColumnConst;
+constexpr bool kOneVarying = (kMayVary & (kMayVary - 1)) == 0;
+constexpr int kVaryCoord = kMayVary == 0 ? 6 : (kMayVary & 1 ? 0 : kMayVary & 2 ? 1 : kMayVary & 4 ? 2 : kMayVary & 8 ? 3 : kMayVary & 16 ? 4 : 5);
+
 // L* and its distance for all quarts of one quart column (idx = biome chunk * 16 + qz * 4 + qx).
 // Along a column usually one climate coordinate varies (the depth) and the leaves take a few
 // distinct intervals on it (groups). A leaf's distance is then A (over the other coordinates,
@@ -542,7 +548,6 @@ MCT_HD inline void biome_search_column(const ModelDev& M, const TreeNodeF* top, 
     // coordinate is left, the column varies in that one alone and needs no scan of its targets;
     // a quart whose varying value leaves the float tree's exact range is marked for the int64
     // search (L* = -2) as in the per-quart search.
-    constexpr int kMayVary = 0x3F & ~mctgen::kClimateColumnConst;
     int vary = 0, k = -1;
     bool small = M.treef != nullptr, scanned = false;
     if constexpr ((kMayVary & (kMayVary - 1)) == 0) {
@@ -624,6 +629,74 @@ MCT_HD inline void biome_search_column(const ModelDev& M, const TreeNodeF* top, 
         lstar[i] = bl;
         mind[i] = static_cast<int64_t>(best);
     }
+}
+
+// The column search in two launches with more threads: the group searches of all columns
+// (one thread per column and group), then every quart's L* from its column's group results
+// (one thread per quart). Used when the code generator shows that at most one climate value
+// varies along a quart column (kVaryCoord) and that coordinate has at most kColGroups groups.
+struct GroupRes {
+    double a;        // minimal distance over the other coordinates within the group
+    int32_t leaf;    // its first leaf in depth-first order; -1: none; -3: the column took the per-quart search
+    int32_t rank;    // that leaf's depth-first rank
+};
+
+// Group g of quart column c (idx = c * G + g).
+MCT_HD inline void biome_group_search(const ModelDev& M, const TreeNodeF* top, int ntop, const int64_t* targets, int32_t* lstar, int64_t* mind,
+                                      GroupRes* out, int G, int idx) {
+    const int colidx = idx / G, g = idx - colidx * G, k = kVaryCoord;
+    const int b = colidx >> 4, qx = colidx & 3, qz = (colidx >> 2) & 3;
+    const size_t col = static_cast<size_t>(b) * kQPer + qz * 4 + qx;
+    const int64_t* t0 = targets + col * 7;
+    bool small = M.treef != nullptr;
+    for (int j = 0; j < 7; ++j)
+        if (j != k) small = small && t0[j] >= -static_cast<int64_t>(kTreeExact) && t0[j] <= static_cast<int64_t>(kTreeExact);
+    if (!small) {
+        if (g == 0) biome_search(M, top, ntop, targets, lstar, mind, colidx, false, kQH);
+        out[idx] = GroupRes{0.0, -3, 0};
+        return;
+    }
+    const double inf = u2d(0x7FF0000000000000ull);
+    const float lo = M.col_lo[k][g], hi = M.col_hi[k][g];
+    float tg[7];
+    for (int j = 0; j < 7; ++j) tg[j] = j == k ? lo : static_cast<float>(t0[j]);
+    double d = 0.0;
+    int n = tree_first_min_group_f(M.treef, top, ntop, tg, tree_greedy_f(M.treef, top, ntop, tg) + 1.0, d, k, lo, hi);
+    if (n < 0) n = tree_first_min_group_f(M.treef, top, ntop, tg, inf, d, k, lo, hi);
+    if (n < 0) {
+        out[idx] = GroupRes{inf, -1, INT32_MAX};
+    } else {
+        const int leaf = n < ntop ? top[n].leaf : M.treef[n].leaf;
+        out[idx] = GroupRes{d, leaf, M.leaf_rank[leaf]};
+    }
+}
+
+// L* and its distance of quart idx (biome chunk * kQPer + quart) from its column's G groups.
+MCT_HD inline void biome_quart_final(const ModelDev& M, const int64_t* targets, const GroupRes* res, int32_t* lstar, int64_t* mind, int G, int idx) {
+    const int b = idx / kQPer, q = idx - b * kQPer;
+    const GroupRes* r = res + static_cast<size_t>(b * 16 + (q & 15)) * G;
+    if (r[0].leaf == -3) return;
+    const int k = kVaryCoord;
+    const int64_t tv = targets[static_cast<size_t>(idx) * 7 + k];
+    if (tv < -static_cast<int64_t>(kTreeExact) || tv > static_cast<int64_t>(kTreeExact)) {
+        lstar[idx] = -2;
+        mind[idx] = -1;
+        return;
+    }
+    const float v = static_cast<float>(tv);
+    double best = u2d(0x7FF0000000000000ull);
+    int bl = -1, br = INT32_MAX;
+    for (int g = 0; g < G; ++g) {
+        const float p = fmaxf(fmaxf(v - M.col_hi[k][g], M.col_lo[k][g] - v), 0.0f);
+        const double d = r[g].a + static_cast<double>(p) * static_cast<double>(p);
+        if (r[g].leaf >= 0 && (d < best || (d == best && r[g].rank < br))) {
+            best = d;
+            bl = r[g].leaf;
+            br = r[g].rank;
+        }
+    }
+    lstar[idx] = bl;
+    mind[idx] = static_cast<int64_t>(best);
 }
 
 // Chain order of doCreateBiomes inside a chunk: 4-quart sections, then x, y, z.
