@@ -677,16 +677,43 @@ class Gen:
         for k in ('surface', 'surface_secondary', 'clay_bands_offset'):
             col_slot('minecraft:' + k)
 
-        def noise_call(key, is3d):
+        # biomes a column-noise slot can be read under: the union over the rule paths to its
+        # conditions of the biome sets those paths require (biome conditions and their
+        # negations on the path); slots 0..2 are read outside the rules too
+        ALL = (1 << 128) - 1
+        slot_biomes = {0: ALL, 1: ALL, 2: ALL}
+
+        def noise_call(key, is3d, bset):
             if not is3d:
-                return 'c.col_noise(%d)' % col_slot(key)
+                k = col_slot(key)
+                slot_biomes[k] = slot_biomes.get(k, 0) | bset
+                return 'c.col_noise(%d)' % k
             g = self.noise_fn('g', key)
             return '%s(c.O, static_cast<double>(c.x), static_cast<double>(c.y), static_cast<double>(c.z))' % g
 
-        def cond(cn):
+        def biome_set(cn):
+            m = 0
+            for b in cn['biomes']:
+                m |= 1 << b
+            return m
+
+        def narrow_biomes(cn, bset):
+            while cn['t'] == 'ref':
+                cn = cn['cond']
+            if cn['t'] == 'biome':
+                return bset & biome_set(cn)
+            if cn['t'] == 'not':
+                inner = cn['cond']
+                while inner['t'] == 'ref':
+                    inner = inner['cond']
+                if inner['t'] == 'biome':
+                    return bset & ~biome_set(inner) & ALL
+            return bset
+
+        def cond(cn, bset=ALL):
             t = cn['t']
             if t == 'ref':
-                return cond(cn['cond'])
+                return cond(cn['cond'], bset)
             name = 'cond%d' % cnt['c']
             cnt['c'] += 1
             if t == 'biome':
@@ -699,7 +726,7 @@ class Gen:
             elif t == 'noise_threshold':
                 if cn['noise'] not in key_index:
                     raise SystemExit('rule noise %s is not in the IR noise list' % cn['noise'])
-                body = 'const double v = static_cast<double>(%s); return v >= %s && v <= %s;' % (noise_call(cn['noise'], cn['is_3d']), dl(f64h(cn['min'])),
+                body = 'const double v = static_cast<double>(%s); return v >= %s && v <= %s;' % (noise_call(cn['noise'], cn['is_3d'], bset), dl(f64h(cn['min'])),
                                                                                                    dl(f64h(cn['max'])))
             elif t == 'vertical_gradient':
                 if cn['random_name'] not in self.rand_names:
@@ -712,7 +739,7 @@ class Gen:
                 body = 'return c.water_height == INT32_MIN || c.y + %s >= c.water_height + %d + c.surface_depth * %d;' % (
                     'c.stone_above' if cn['add_stone_depth'] else '0', cn['offset'], cn['surface_depth_multiplier'])
             elif t == 'not':
-                body = 'return !%s(c);' % cond(cn['cond'])
+                body = 'return !%s(c);' % cond(cn['cond'], bset)
             elif t == 'stone_depth':
                 body = 'return c.stone_depth(%d, %s, %d, %s);' % (cn['offset'], 'true' if cn['add_surface_depth'] else 'false',
                                                                  cn['secondary_depth_range'], 'true' if cn['ceiling'] else 'false')
@@ -751,10 +778,10 @@ class Gen:
                 return (yr[0], min(yr[1], deref(cn['cond'])['anchor']))
             return yr
 
-        def rule(r, yr=(ymin, yend)):
+        def rule(r, yr=(ymin, yend), bset=ALL):
             t = r['t']
             if t == 'ref':
-                return rule(r['rule'], yr)
+                return rule(r['rule'], yr, bset)
             name = 'rule%d' % cnt['r']
             cnt['r'] += 1
             if t == 'block':
@@ -768,11 +795,11 @@ class Gen:
                     ore_range[k] = (min(lo, yr[0]), max(hi, yr[1]))
                 body = 'return c.ore_vein(%d);' % k
             elif t == 'condition':
-                cn = cond(r['if'])
-                th = rule(r['then'], narrow(r['if'], yr))
+                cn = cond(r['if'], bset)
+                th = rule(r['then'], narrow(r['if'], yr), narrow_biomes(r['if'], bset))
                 body = 'return %s(c) ? %s(c) : -1;' % (cn, th)
             elif t == 'sequence':
-                subs = [rule(x, yr) for x in r['rules']]
+                subs = [rule(x, yr, bset) for x in r['rules']]
                 body = 'int s;\n' + '\n'.join('    s = %s(c); if (s >= 0) return s;' % x for x in subs) + '\n    return -1;'
             else:
                 raise SystemExit('unsupported rule ' + t)
@@ -797,6 +824,10 @@ class Gen:
         o.append('constexpr int kOreYHi[%d] = {%s};' % (nore, ', '.join(str(b) for _, b in rng)))
         o.append('constexpr int kColNoises = %d;   // per-column noise slots' % len(self.col_noise))
         o.append('constexpr int kColSurface = 0, kColSecondary = 1, kColClay = 2;')
+        # per column-noise slot: the biomes under which the rules can read it (bit b of word
+        # b >> 6); a column none of whose candidate biomes is among them never reads the slot
+        o.append('constexpr uint64_t kColNoiseBiomes[kColNoises][2] = {%s};' % ', '.join(
+            '{%s, %s}' % (u64(slot_biomes.get(k, 0) & ((1 << 64) - 1)), u64(slot_biomes.get(k, 0) >> 64)) for k in range(len(self.col_noise))))
         o.append('MCT_HD MCT_FI float surf_column_noise(const mct::Octave* O, double x, double z, int k) {')
         o.append('    switch (k) {')
         for k, g in enumerate(self.col_noise):

@@ -414,10 +414,10 @@ __global__ __launch_bounds__(256) void k_fill_core(FillArgs A, int n) {
 }
 
 // Column records of the surface pass (2D rule noises, snow-line noise, frozen-patch flag).
-__global__ __launch_bounds__(256) void k_surface_cols(const SeedDev* seeds, const ChunkRef* chunks, int n, float* rec) {
+__global__ __launch_bounds__(256) void k_surface_cols(const SeedDev* seeds, const ChunkRef* chunks, int n, float* rec, const int* nb, const uint64_t* bmask) {
     const int idx = blockIdx.x * blockDim.x + threadIdx.x;
     if (idx >= n * 256) return;
-    surface_column(seeds, c_model, chunks, rec, idx);
+    surface_column(seeds, c_model, chunks, rec, idx, nb, bmask);
 }
 
 // Ore-vein outcomes of the blocks the ore rules can reach, ahead of the surface walk: one thread
@@ -1349,7 +1349,8 @@ struct Generator {
         SurfArgs A{sd, nullptr, S.d_ch, blocks, S.d_nb, S.d_bb, S.d_corners, S.d_colmap, S.d_prelim, S.d_colrec, S.d_ore, S.d_bmask, S.d_clsw};
         if (kOreRows > 0)
             timed(S, kKOre, [&] { k_ore<<<grid_of(static_cast<size_t>(n) * 256, W(kKOre)), W(kKOre), 0, st>>>(A, n); });
-        timed(S, kKSurfaceCols, [&] { k_surface_cols<<<grid_of(static_cast<size_t>(n) * 256, W(kKSurfaceCols)), W(kKSurfaceCols), 0, st>>>(sd, S.d_ch, n, S.d_colrec); });
+        timed(S, kKSurfaceCols, [&] { k_surface_cols<<<grid_of(static_cast<size_t>(n) * 256, W(kKSurfaceCols)), W(kKSurfaceCols), 0, st>>>(sd, S.d_ch, n, S.d_colrec, S.d_nb,
+                                                                                                                                       S.d_bmask); });
         timed(S, kKSurface1, [&] { k_surface1<<<grid_of(static_cast<size_t>(n) * 256, 64), 64, 0, st>>>(A, n, S.d_h0, S.d_h1, S.d_h2); });
         timed(S, kKSurface2, [&] { k_surface2<<<grid_of(static_cast<size_t>(n) * 256, 64), 64, 0, st>>>(A, n, S.d_h0, S.d_h1, S.d_h2, S.d_hf); });
         k_surface_check<<<n, 256, 0, st>>>(n, S.d_h2, S.d_hf, S.d_flags, S.d_counts + 2, d_stats);

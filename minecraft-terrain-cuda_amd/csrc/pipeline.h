@@ -1289,12 +1289,36 @@ constexpr int kColF = mctgen::kColNoises + 2;
 constexpr int kColSnow = mctgen::kColNoises, kColPatch = mctgen::kColNoises + 1;
 
 // The record of one column (idx = chunk * 256 + lx * 16 + lz).
-MCT_HD inline void surface_column(const SeedDev* seeds, const ModelDev& M, const ChunkRef* chunks, float* rec, int idx) {
+// Biomes of the 4 quart columns the zoom of block column (x, z) of chunk ci picks from, at any
+// height (per quart column: bmask; all bits without it).
+MCT_HD MCT_FI void column_candidates(const int* nb_index, const uint64_t* bmask, const ChunkRef& cr, int ci, int x, int z, uint64_t* out) {
+    out[0] = out[1] = bmask ? 0 : ~0ull;
+    if (!bmask) return;
+    const int px = (x - 2) >> 2, pz = (z - 2) >> 2;
+    for (int j = 0; j < 4; ++j) {
+        const int qx = px + (j >> 1), qz = pz + (j & 1);
+        const int bc = nb_index[ci * 9 + ((qz >> 2) - cr.z + 1) * 3 + (qx >> 2) - cr.x + 1];
+        const uint64_t* m = bmask + (static_cast<size_t>(bc) * 16 + (qz & 3) * 4 + (qx & 3)) * 2;
+        out[0] |= m[0];
+        out[1] |= m[1];
+    }
+}
+
+// The column record: the rule tree's 2D noises (a slot the rules can only read under biomes
+// none of the column's zoom candidates has is left 0: it is never read), the snow-line noise
+// and the frozen-patch test.
+MCT_HD inline void surface_column(const SeedDev* seeds, const ModelDev& M, const ChunkRef* chunks, float* rec, int idx, const int* nb_index = nullptr,
+                                  const uint64_t* bmask = nullptr) {
     const int ci = idx >> 8, col = idx & 255;
     const ChunkRef cr = chunks[ci];
     const int x = cr.x * 16 + (col >> 4), z = cr.z * 16 + (col & 15);
     float* r = rec + static_cast<size_t>(idx) * kColF;
-    for (int k = 0; k < mctgen::kColNoises; ++k) r[k] = mctgen::surf_column_noise(seeds[cr.seed].oct, static_cast<double>(x), static_cast<double>(z), k);
+    uint64_t cand[2];
+    column_candidates(nb_index, bmask, cr, ci, x, z, cand);
+    for (int k = 0; k < mctgen::kColNoises; ++k)
+        r[k] = ((cand[0] & mctgen::kColNoiseBiomes[k][0]) | (cand[1] & mctgen::kColNoiseBiomes[k][1])) != 0
+                   ? mctgen::surf_column_noise(seeds[cr.seed].oct, static_cast<double>(x), static_cast<double>(z), k)
+                   : 0.0f;
     r[kColSnow] = snow_noise(M, x, z);
     r[kColPatch] = frozen_patch(M, x, z) ? 1.0f : 0.0f;
 }
