@@ -44,14 +44,15 @@ VALU_PEAK_TOPS = 256 * 128 * 2.4e9 / 1e12
 #   HBM bytes       FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0, the factors from a plain 1 GiB
 #                   device-to-device copy measured in the same runs
 WORK_PER_UNIT = {
-    'surface2': {'unit': 'block column', 'valu': 14853.9, 'hbm': 1014.23 + 75.27},
+    'surface2': {'unit': 'block column', 'valu': 12505.6, 'hbm': 360.31 + 76.36},
     # per corner evaluated: the launch covers every corner column slot of the batch, 1.62
     # threads per distinct corner on this workload, and the idle workgroups exit at once
     'corners': {'unit': 'corner', 'valu': 23401.2, 'hbm': 31.19},
-    # search per quart column (2 restricted searches) + 96 quart targets
-    'biome_quarts': {'unit': 'quart column of a biome chunk', 'valu': 42348.2 + 96 * 150.0, 'hbm': 10763.87 + 1151.11 + 96 * (1.51 + 56.08)},
-    'fill': {'unit': 'block column of a cell (8 blocks)', 'valu': 1344.4, 'hbm': 14.18 + 8.63},
-    'ore_veins': {'unit': 'block column', 'valu': 11352.4, 'hbm': 35.76 + 221.00},
+    # 2 restricted group searches per quart column + per quart its final minimum and target
+    'biome_quarts': {'unit': 'quart column of a biome chunk', 'valu': 2 * 23538.4 + 96 * (69.2 + 150.0),
+                     'hbm': 2 * (31.95 + 16.00) + 96 * (58.05 + 12.00 + 1.51 + 56.05 + 4.00)},
+    'fill': {'unit': 'block column of a cell (8 blocks)', 'valu': 1455.6, 'hbm': 11.12 + 13.22},
+    'ore_veins': {'unit': 'block column', 'valu': 11352.4, 'hbm': 125.28 + 259.67},
 }
 
 
