@@ -322,7 +322,7 @@ class Gen:
             yax = 'const mct::Axis Y = mct::axis_plain(by, ys * kLayerFreq[i], T[i].oy);'
         else:
             yax = 'const mct::Axis Y = mct::axis_smeared(by, ys * kLayerFreq[i], T[i].oy, kLayerSmear[i]);'
-        term = 'acc = acc + (amplitude * kLayerAmp[i]) * mct::volume_sample(%s, a.X, Y, a.Z);'
+        term = 'acc = acc + (amplitude * kLayerAmp[i]) * mct::volume_sample_xz(%s, a, Y);'
         loop = '#pragma clang loop unroll(disable)'
         L += ['#if defined(__HIP_DEVICE_COMPILE__)',
               '    for (;;) {',
@@ -983,7 +983,10 @@ class Gen:
         entry('mct::AxisXZ corner_axes(const mct::Octave* O, int32_t x, int32_t z, int site)',
               'const int i = kCornerSiteLayer[site];', [
                   'const double xs = kCornerSiteScale[site] * kLayerFreq[i];',
-                  'return mct::AxisXZ{mct::axis_plain(x, xs, O[i].ox), mct::axis_plain(z, xs, O[i].oz)};'])
+                  'mct::Axis X = mct::axis_plain(x, xs, O[i].ox);',
+                  'const int xi = X.f & 0xFF;',
+                  'X.f = static_cast<int32_t>(O[i].p[xi]) | (static_cast<int32_t>(O[i].p[xi + 1]) << 16);   // perm_x',
+                  'return mct::AxisXZ{X, mct::axis_plain(z, xs, O[i].oz)};'])
         entry('void eval_corner(const mct::Octave* O, int32_t x, int32_t y, int32_t z, const float* col, const MCT_LDS mct::AxisXZ* axz, float* out)',
               'Corner e{O, x, y, z, col, axz};',
               ['out[%d] = e.n%d();' % (k, N[c]['in']) for k, c in enumerate(self.chans + self.ore_chans)])

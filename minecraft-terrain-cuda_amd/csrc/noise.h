@@ -110,6 +110,24 @@ MCT_HD MCT_FI Hash8 hash8(const Perm& P, int32_t fx, int32_t fy, int32_t fz) {
     return h;
 }
 
+// hash8 from the first-level entries a = p[x & 255], b = p[(x & 255) + 1].
+template <class Perm>
+MCT_HD MCT_FI Hash8 hash8_ab(const Perm& P, int a, int b, int32_t fy, int32_t fz) {
+    const int Y = fy & 0xFF, Z = fz & 0xFF;
+    const int a0 = P(a + Y), a1 = P(a + Y + 1);
+    const int b0 = P(b + Y), b1 = P(b + Y + 1);
+    Hash8 h;
+    h.h000 = P(a0 + Z);
+    h.h100 = P(b0 + Z);
+    h.h010 = P(a1 + Z);
+    h.h110 = P(b1 + Z);
+    h.h001 = P(a0 + Z + 1);
+    h.h101 = P(b0 + Z + 1);
+    h.h011 = P(a1 + Z + 1);
+    h.h111 = P(b1 + Z + 1);
+    return h;
+}
+
 // Gradient dot for the point path: (gx*x + gy*y) + gz*z.
 MCT_HD MCT_FI float dot3(int h, float x, float y, float z) {
     const float a = gcomp(kGX, h) * x;
@@ -180,7 +198,9 @@ struct Axis {
     float a;
 };
 
-// The x and z records of one octave sample site of a block column.
+// The x and z records of one octave sample site of a block column. X.f is not the x lattice
+// cell but the two first-level permutation entries of it, p[x & 255] | p[(x & 255) + 1] << 16
+// (perm_x): the hash of every sample of the column starts from them.
 struct AxisXZ {
     Axis X, Z;
 };
@@ -204,10 +224,8 @@ MCT_HD MCT_FI Axis axis_smeared(int32_t b, double s, double offset, double smear
     return {f, r, a};
 }
 
-// Volume-path sample from the three axis records: xz dots per corner, plus gy * ry.
-template <class Perm>
-MCT_HD MCT_FI float volume_sample(const Perm& P, const Axis& X, const Axis& Y, const Axis& Z) {
-    const Hash8 h = hash8(P, X.f, Y.f, Z.f);
+// Volume-path sample from the hashes and the three axis records: xz dots per corner, plus gy * ry.
+MCT_HD MCT_FI float volume_corners(const Hash8& h, const Axis& X, const Axis& Y, const Axis& Z) {
     const float rx = X.r, rz = Z.r, rx1 = rx - 1.0f, rz1 = rz - 1.0f;
     const float ry = Y.r, ry1 = ry - 1.0f;
 #define MCT_XZ(hh, xx, zz) ((gcomp(kGX, hh) * (xx)) + (gcomp(kGZ, hh) * (zz)))
@@ -221,6 +239,17 @@ MCT_HD MCT_FI float volume_sample(const Perm& P, const Axis& X, const Axis& Y, c
     const float c111 = MCT_XZ(h.h111, rx1, rz1) + gcomp(kGY, h.h111) * ry1;
 #undef MCT_XZ
     return lerp3f(X.a, Y.a, Z.a, c000, c100, c010, c110, c001, c101, c011, c111);
+}
+
+template <class Perm>
+MCT_HD MCT_FI float volume_sample(const Perm& P, const Axis& X, const Axis& Y, const Axis& Z) {
+    return volume_corners(hash8(P, X.f, Y.f, Z.f), X, Y, Z);
+}
+
+// The same from a column table record (AxisXZ, X.f holding perm_x).
+template <class Perm>
+MCT_HD MCT_FI float volume_sample_xz(const Perm& P, const AxisXZ& A, const Axis& Y) {
+    return volume_corners(hash8_ab(P, A.X.f & 0xFFFF, A.X.f >> 16, Y.f, A.Z.f), A.X, Y, A.Z);
 }
 
 // PerlinNoise.addToVolume, one sample (before the amplitude multiply).
