@@ -514,10 +514,12 @@ class Gen:
                         self.noise_fn('sg', (bi, setname)), dl(a), dl(b), dl(a))
             L.append('float a = %s;' % s('main', mxz, my))
             L.append('a = mct::clampf(a + 0.5f, 0.0f, 1.0f);')
-            L.append('float v;')
-            L.append('if (a == 0.0f) v = %s;' % s('min_limit', xzm, ym))
-            L.append('else if (a == 1.0f) v = %s;' % s('max_limit', xzm, ym))
-            L.append('else { const float lo = %s; const float hi = %s; v = mct::lerpf(a, lo, hi); }' % (s('min_limit', xzm, ym), s('max_limit', xzm, ym)))
+            # each limit stack at most once per wave: the lanes that need it call it together
+            # (as if / else-if / else, lanes of the three cases would call the stacks up to 4 times)
+            L.append('float lo = 0.0f, hi = 0.0f;')
+            L.append('if (a != 1.0f) lo = %s;' % s('min_limit', xzm, ym))
+            L.append('if (a != 0.0f) hi = %s;' % s('max_limit', xzm, ym))
+            L.append('const float v = a == 0.0f ? lo : (a == 1.0f ? hi : mct::lerpf(a, lo, hi));')
         elif t == 'binary':
             kind, a, b = self.plan_binary(n)
             if kind == 'pass':
