@@ -47,7 +47,7 @@ WORK_PER_UNIT = {
     'surface2': {'unit': 'block column', 'valu': 12505.6, 'hbm': 360.31 + 76.36},
     # per corner evaluated: the launch covers every corner column slot of the batch, 1.62
     # threads per distinct corner on this workload, and the idle workgroups exit at once
-    'corners': {'unit': 'corner', 'valu': 23401.2, 'hbm': 31.19},
+    'corners': {'unit': 'corner', 'valu': 17988.6, 'hbm': 31.29},
     # 2 restricted group searches per quart column + per quart its final minimum and target
     'biome_quarts': {'unit': 'quart column of a biome chunk', 'valu': 2 * 23538.4 + 96 * (69.2 + 150.0),
                      'hbm': 2 * (31.95 + 16.00) + 96 * (58.05 + 12.00 + 1.51 + 56.05 + 4.00)},
