@@ -1027,8 +1027,20 @@ struct AqCellsGlobal {
     MCT_HD MCT_FI Fluid fluid(int i) const { return Fluid{fl[i * 2], fl[i * 2 + 1]}; }
 };
 
+// dx^2 + dy^2 + dz^2 of offsets within +-2^23: the products are exact in 24-bit signed
+// multiplies, which issue at the full VALU rate (the 32-bit multiply is quarter rate).
+MCT_HD MCT_FI int sq3_small(int dx, int dy, int dz) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __mul24(dx, dx) + __mul24(dy, dy) + __mul24(dz, dz);
+#else
+    return dx * dx + dy * dy + dz * dz;
+#endif
+}
+
 // The three nearest aquifer cells of a block (computeSubstance's 2 x 3 x 2 scan, ties to the
-// later cell of the scan) and the fourth distance.
+// later cell of the scan). A cell of the scan lies within
+// [16 * gx, 16 * gx + 9] x [12 * gy, 12 * gy + 8] x [16 * gz, 16 * gz + 9] (aquifer_cell) and
+// gx, gy, gz are within one cell of the block's, so every offset is at most 22 in size.
 struct AqNearest {
     int i1, i2, i3, d1, d2, d3;
 };
@@ -1045,7 +1057,7 @@ MCT_HD MCT_FI AqNearest aquifer_nearest(const Cells& C, int cx, int cz, int x, i
                 int lx, ly, lz;
                 C.at(index, lx, ly, lz);
                 const int dx = lx - x, dy = ly - y, dz = lz - z;
-                const int nd = dx * dx + dy * dy + dz * dz;
+                const int nd = sq3_small(dx, dy, dz);
                 if (d1 >= nd) { i3 = i2; i2 = i1; i1 = index; d4 = d3; d3 = d2; d2 = d1; d1 = nd; }
                 else if (d2 >= nd) { i3 = i2; i2 = index; d4 = d3; d3 = d2; d2 = nd; }
                 else if (d3 >= nd) { i3 = index; d4 = d3; d3 = nd; }
