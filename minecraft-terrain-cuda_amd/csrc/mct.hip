@@ -291,19 +291,59 @@ constexpr int kFillHead = kFillStage * 6 + 4;   // ints before the per-thread ar
 static_assert(kFillHead * 4 % 16 == 0, "per-thread LDS arrays stay 16-byte aligned");
 constexpr size_t fill_lds_bytes(int w) { return static_cast<size_t>(kFillHead) * 4 + 56 * static_cast<size_t>(w); }
 
+// A staged aquifer cell in chunk-local coordinates (l = location - 16 * chunk in x and z):
+// s = |l|^2 and the factors -2 l of the cross terms, so that |l - b|^2 - |b|^2 of a block b of
+// the chunk is s + ax * bx + ay * by + az * bz.
+struct alignas(16) AqCellTerm {
+    int s, ax, ay, az;
+};
+
 struct AqCellsLds {
-    const MCT_LDS int* loc;   // 4 ints per staged cell
-    const MCT_LDS int* fl;    // 2 ints per staged cell
-    int base;                 // chunk cell index of staged cell 0
-    __device__ MCT_FI void at(int i, int& x, int& y, int& z) const {
-        const MCT_LDS int* p = loc + (i - base) * 4;
-        x = p[0]; y = p[1]; z = p[2];
-    }
+    const MCT_LDS AqCellTerm* loc;   // one per staged cell
+    const MCT_LDS int* fl;           // 2 ints per staged cell
+    int base;                        // chunk cell index of staged cell 0
     __device__ MCT_FI Fluid fluid(int i) const { return Fluid{fl[(i - base) * 2], fl[(i - base) * 2 + 1]}; }
 };
 
+// a * b + c for a, b within +-2^23, as one full-rate v_mad_i32_i24 (the compiler falls back to
+// the quarter-rate 32-bit multiply where one factor is wave-uniform).
+__device__ MCT_FI int mad_i24(int a, int b, int c) {
+    int r;
+    asm("v_mad_i32_i24 %0, %1, %2, %3" : "=v"(r) : "v"(a), "v"(b), "v"(c));
+    return r;
+}
+
+// aquifer_nearest over staged cells. Each distance is |l - b|^2 - |b|^2: the block's own |b|^2
+// is common to all 12 cells, so the order, the ties and every difference of two distances are
+// those of the plain scan, and only differences are read (d2 - d1, d3 - d1, d3 - d2). Every
+// factor is within +-2^10 (cells at chunk-local x, z in -16..25 and y in -84..332), so the products are
+// exact 24-bit multiply-adds.
+__device__ MCT_FI AqNearest aquifer_nearest(const AqCellsLds& C, int cx, int cz, int x, int y, int z) {
+    const int xa = (x - 5) >> 4, ya = grid_yf(y + 1), za = (z - 5) >> 4;
+    const int i0 = ((ya - 1 - kAqMinGY) * 3 + (za - (cz - 1))) * 3 + (xa - (cx - 1));
+    const int bx = x - cx * 16, bz = z - cz * 16;
+    const MCT_LDS AqCellTerm* p = C.loc + (i0 - C.base);
+    int d1 = INT32_MAX, d2 = INT32_MAX, d3 = INT32_MAX, d4 = INT32_MAX;
+    int o1 = 0, o2 = 0, o3 = 0;
+#pragma unroll
+    for (int ox = 0; ox <= 1; ++ox)
+#pragma unroll
+        for (int oy = 0; oy <= 2; ++oy)
+#pragma unroll
+            for (int oz = 0; oz <= 1; ++oz) {
+                const int o = oy * 9 + oz * 3 + ox;
+                const AqCellTerm t = p[o];
+                const int nd = mad_i24(t.az, bz, mad_i24(t.ay, y, mad_i24(t.ax, bx, t.s)));
+                if (d1 >= nd) { o3 = o2; o2 = o1; o1 = o; d4 = d3; d3 = d2; d2 = d1; d1 = nd; }
+                else if (d2 >= nd) { o3 = o2; o2 = o; d4 = d3; d3 = d2; d2 = nd; }
+                else if (d3 >= nd) { o3 = o; d4 = d3; d3 = nd; }
+                else if (d4 >= nd) { d4 = nd; }
+            }
+    return AqNearest{i0 + o1, i0 + o2, i0 + o3, d1, d2, d3};
+}
+
 __global__ __launch_bounds__(256) void k_fill(FillArgs A, int n) {
-    extern __shared__ int fill_lds[];
+    extern __shared__ __attribute__((aligned(16))) int fill_lds[];
     const int W = static_cast<int>(blockDim.x), tid = static_cast<int>(threadIdx.x);
     const size_t idx0 = static_cast<size_t>(blockIdx.x) * static_cast<size_t>(W);
     if (idx0 >= static_cast<size_t>(n) * 256 * kCellsY) return;   // whole workgroups only
@@ -327,14 +367,18 @@ __global__ __launch_bounds__(256) void k_fill(FillArgs A, int n) {
         const int32_t* loc = A.cell_loc + static_cast<size_t>(c) * kAqCells * 4;
         const int32_t* fl = A.cell_fluid + static_cast<size_t>(c) * kAqCells * 2;
         for (int k = tid; k < (gy_hi - gy_lo + 1) * 9; k += W) {
-            for (int j = 0; j < 3; ++j) sloc[k * 4 + j] = loc[(base + k) * 4 + j];
+            const int lx = loc[(base + k) * 4] - cr.x * 16, ly = loc[(base + k) * 4 + 1], lz = loc[(base + k) * 4 + 2] - cr.z * 16;
+            sloc[k * 4] = lx * lx + ly * ly + lz * lz;
+            sloc[k * 4 + 1] = -2 * lx;
+            sloc[k * 4 + 2] = -2 * ly;
+            sloc[k * 4 + 3] = -2 * lz;
             sfl[k * 2] = fl[(base + k) * 2];
             sfl[k * 2 + 1] = fl[(base + k) * 2 + 1];
         }
     }
     if (tid == 0) *qn = 0;
     __syncthreads();
-    const AqCellsLds C{(const MCT_LDS int*)sloc, (const MCT_LDS int*)sfl, base};
+    const AqCellsLds C{(const MCT_LDS AqCellTerm*)sloc, (const MCT_LDS int*)sfl, base};
     const Octave* O = A.seeds[cr.seed].oct;
     const int x = (col0 + tid) & 15, z = (col0 + tid) >> 4;
     constexpr int NC = mctgen::kNumChannels > 0 ? mctgen::kNumChannels : 1;
