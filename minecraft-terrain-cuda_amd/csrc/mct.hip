@@ -389,6 +389,7 @@ __global__ __launch_bounds__(256) void k_fill(FillArgs A, int n) {
     const uint32_t below = __builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u));   // lane id
     int top = -1;
     uint32_t frag = 0;
+#pragma unroll
     for (int yi = 0; yi < 8; ++yi) {
         const int by = ylo + yi;
         const float d = mctgen::eval_block(O, bx, by, bz, val);
