@@ -342,6 +342,47 @@ __device__ MCT_FI AqNearest aquifer_nearest(const AqCellsLds& C, int cx, int cz,
     return AqNearest{i0 + o1, i0 + o2, i0 + o3, d1, d2, d3};
 }
 
+// The 12 cells of the nearest-3 scan for the blocks of one column that share grid row ya
+// (y + 1 in [12 ya, 12 ya + 11]): per cell the column part p = s + ax * bx + az * bz of the
+// expanded distance and ay, so that a block's distance is one multiply-add, ay * y + p. ya is
+// the same for all threads of a workgroup (one cell row of one chunk), so reloads are uniform.
+struct AqColumnRow {
+    AqCellsLds C;
+    int i0 = 0, ya = INT32_MIN;
+    int p[12], ay[12];
+    __device__ MCT_FI void load(int cx, int cz, int x, int z, int y) {
+        ya = grid_yf(y + 1);
+        const int xa = (x - 5) >> 4, za = (z - 5) >> 4;
+        i0 = ((ya - 1 - kAqMinGY) * 3 + (za - (cz - 1))) * 3 + (xa - (cx - 1));
+        const int bx = x - cx * 16, bz = z - cz * 16;
+        const MCT_LDS AqCellTerm* q = C.loc + (i0 - C.base);
+#pragma unroll
+        for (int k = 0; k < 12; ++k) {
+            const AqCellTerm t = q[(k % 6 >> 1) * 9 + (k & 1) * 3 + k / 6];
+            p[k] = mad_i24(t.az, bz, mad_i24(t.ax, bx, t.s));
+            ay[k] = t.ay;
+        }
+    }
+    __device__ MCT_FI Fluid fluid(int i) const { return C.fluid(i); }
+};
+
+// aquifer_nearest for a block of the loaded column row: the scan of the staged-cell form
+// (same distances, same order k = ox * 6 + oy * 2 + oz).
+__device__ MCT_FI AqNearest aquifer_nearest(const AqColumnRow& R, int, int, int, int y, int) {
+    int d1 = INT32_MAX, d2 = INT32_MAX, d3 = INT32_MAX, d4 = INT32_MAX;
+    int o1 = 0, o2 = 0, o3 = 0;
+#pragma unroll
+    for (int k = 0; k < 12; ++k) {
+        const int o = (k % 6 >> 1) * 9 + (k & 1) * 3 + k / 6;
+        const int nd = mad_i24(R.ay[k], y, R.p[k]);
+        if (d1 >= nd) { o3 = o2; o2 = o1; o1 = o; d4 = d3; d3 = d2; d2 = d1; d1 = nd; }
+        else if (d2 >= nd) { o3 = o2; o2 = o; d4 = d3; d3 = d2; d2 = nd; }
+        else if (d3 >= nd) { o3 = o; d4 = d3; d3 = nd; }
+        else if (d4 >= nd) { d4 = nd; }
+    }
+    return AqNearest{R.i0 + o1, R.i0 + o2, R.i0 + o3, d1, d2, d3};
+}
+
 __global__ __launch_bounds__(256) void k_fill(FillArgs A, int n) {
     extern __shared__ __attribute__((aligned(16))) int fill_lds[];
     const int W = static_cast<int>(blockDim.x), tid = static_cast<int>(threadIdx.x);
@@ -379,6 +420,7 @@ __global__ __launch_bounds__(256) void k_fill(FillArgs A, int n) {
     if (tid == 0) *qn = 0;
     __syncthreads();
     const AqCellsLds C{(const MCT_LDS AqCellTerm*)sloc, (const MCT_LDS int*)sfl, base};
+    AqColumnRow R{C};
     const Octave* O = A.seeds[cr.seed].oct;
     const int x = (col0 + tid) & 15, z = (col0 + tid) >> 4;
     constexpr int NC = mctgen::kNumChannels > 0 ? mctgen::kNumChannels : 1;
@@ -393,8 +435,9 @@ __global__ __launch_bounds__(256) void k_fill(FillArgs A, int n) {
     for (int yi = 0; yi < 8; ++yi) {
         const int by = ylo + yi;
         const float d = mctgen::eval_block(O, bx, by, bz, val);
+        if (ylo <= skip && grid_yf(by + 1) != R.ya) R.load(cr.x, cr.z, bx, bz, by);
         AqNearest nr;
-        const int s = aquifer_front(C, cr.x, cr.z, skip, bx, by, bz, static_cast<double>(d), nr);
+        const int s = aquifer_front(R, cr.x, cr.z, skip, bx, by, bz, static_cast<double>(d), nr);
         const bool pend = s == kAqPending;
         const uint64_t m = __builtin_amdgcn_ballot_w64(pend);
         if (m) {
