@@ -343,9 +343,10 @@ __device__ MCT_FI AqNearest aquifer_nearest(const AqCellsLds& C, int cx, int cz,
 }
 
 // The 12 cells of the nearest-3 scan for the blocks of one column that share grid row ya
-// (y + 1 in [12 ya, 12 ya + 11]): per cell the column part p = s + ax * bx + az * bz of the
-// expanded distance and ay, so that a block's distance is one multiply-add, ay * y + p. ya is
-// the same for all threads of a workgroup (one cell row of one chunk), so reloads are uniform.
+// (y + 1 in [12 ya, 12 ya + 11]), as terms of the scan key (below): per cell 32 times the
+// column part s + ax * bx + az * bz of the expanded distance, plus 31 - k, and 32 ay, so that
+// a block's key is one multiply-add, ay * y + p (|32 ay| < 2^15, |p| < 2^23). ya is the same
+// for all threads of a workgroup (one cell row of one chunk), so reloads are uniform.
 struct AqColumnRow {
     AqCellsLds C;
     int i0 = 0, ya = INT32_MIN;
@@ -359,28 +360,39 @@ struct AqColumnRow {
 #pragma unroll
         for (int k = 0; k < 12; ++k) {
             const AqCellTerm t = q[(k % 6 >> 1) * 9 + (k & 1) * 3 + k / 6];
-            p[k] = mad_i24(t.az, bz, mad_i24(t.ax, bx, t.s));
-            ay[k] = t.ay;
+            p[k] = mad_i24(t.az, bz, mad_i24(t.ax, bx, t.s)) * 32 + (31 - k);
+            ay[k] = t.ay * 32;
         }
     }
     __device__ MCT_FI Fluid fluid(int i) const { return C.fluid(i); }
 };
 
-// aquifer_nearest for a block of the loaded column row: the scan of the staged-cell form
-// (same distances, same order k = ox * 6 + oy * 2 + oz).
+// aquifer_nearest for a block of the loaded column row: the scan of the staged-cell form in
+// the same order k = ox * 6 + oy * 2 + oz, each cell ranked by its key 32 * distance + 31 - k.
+// The keys are distinct and ordered by distance and, among equal distances, later cells first:
+// the order the scan's insertion (d >= nd puts the new cell first) builds. The three smallest
+// keys are kept with one min and two med3; distances and cell offsets are decoded at the end.
+constexpr uint64_t kAqScanOffsets = [] {
+    uint64_t t = 0;
+    for (int k = 0; k < 12; ++k) t |= static_cast<uint64_t>((k % 6 >> 1) * 9 + (k & 1) * 3 + k / 6) << (5 * k);
+    return t;
+}();
+__device__ MCT_FI int aq_key_offset(int key) { return static_cast<int>((kAqScanOffsets >> (5 * (31 - (key & 31)))) & 31u); }
+__device__ MCT_FI int med3_i32(int a, int b, int c) {
+    int r;
+    asm("v_med3_i32 %0, %1, %2, %3" : "=v"(r) : "v"(a), "v"(b), "v"(c));
+    return r;
+}
 __device__ MCT_FI AqNearest aquifer_nearest(const AqColumnRow& R, int, int, int, int y, int) {
-    int d1 = INT32_MAX, d2 = INT32_MAX, d3 = INT32_MAX, d4 = INT32_MAX;
-    int o1 = 0, o2 = 0, o3 = 0;
+    int k1 = INT32_MAX, k2 = INT32_MAX, k3 = INT32_MAX;
 #pragma unroll
     for (int k = 0; k < 12; ++k) {
-        const int o = (k % 6 >> 1) * 9 + (k & 1) * 3 + k / 6;
-        const int nd = mad_i24(R.ay[k], y, R.p[k]);
-        if (d1 >= nd) { o3 = o2; o2 = o1; o1 = o; d4 = d3; d3 = d2; d2 = d1; d1 = nd; }
-        else if (d2 >= nd) { o3 = o2; o2 = o; d4 = d3; d3 = d2; d2 = nd; }
-        else if (d3 >= nd) { o3 = o; d4 = d3; d3 = nd; }
-        else if (d4 >= nd) { d4 = nd; }
+        const int key = mad_i24(R.ay[k], y, R.p[k]);
+        k3 = med3_i32(k2, key, k3);   // k1 < k2 < k3: the new third is the median of k2, key, k3
+        k2 = med3_i32(k1, key, k2);
+        k1 = min(k1, key);
     }
-    return AqNearest{R.i0 + o1, R.i0 + o2, R.i0 + o3, d1, d2, d3};
+    return AqNearest{R.i0 + aq_key_offset(k1), R.i0 + aq_key_offset(k2), R.i0 + aq_key_offset(k3), k1 >> 5, k2 >> 5, k3 >> 5};
 }
 
 __global__ __launch_bounds__(256) void k_fill(FillArgs A, int n) {
