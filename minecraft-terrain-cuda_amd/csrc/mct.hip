@@ -443,6 +443,8 @@ __global__ __launch_bounds__(256) void k_fill(FillArgs A, int n) {
     const uint32_t below = __builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u));   // lane id
     int top = -1;
     uint32_t frag = 0;
+    int qwave = 0;
+    const bool one_wave = W <= 64;   // then the queue needs no LDS atomic
 #pragma unroll
     for (int yi = 0; yi < 8; ++yi) {
         const int by = ylo + yi;
@@ -454,9 +456,12 @@ __global__ __launch_bounds__(256) void k_fill(FillArgs A, int n) {
         const uint64_t m = __builtin_amdgcn_ballot_w64(pend);
         if (m) {
             const int lead = __builtin_ctzll(m);
-            int slot = 0;
-            if (static_cast<int>(below) == lead) slot = atomicAdd(qn, __builtin_popcountll(m));
-            slot = __builtin_amdgcn_readlane(slot, lead);
+            int slot = qwave;   // one wave per workgroup: the queue length is wave-uniform
+            if (!one_wave) {
+                if (static_cast<int>(below) == lead) slot = atomicAdd(qn, __builtin_popcountll(m));
+                slot = __builtin_amdgcn_readlane(slot, lead);
+            }
+            qwave += __builtin_popcountll(m);
             if (pend) {
                 const int k = slot + static_cast<int>(__builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(m >> 32), __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(m), 0u)));
                 qd[k] = d;
@@ -473,6 +478,7 @@ __global__ __launch_bounds__(256) void k_fill(FillArgs A, int n) {
     }
     hm[tid] = top;
     fr[tid] = frag;   // queued blocks still at 0, kClsSolid
+    if (one_wave && tid == 0) *qn = qwave;
     __syncthreads();
     const int q = *qn;
     for (int e = tid; e < q; e += W) {
