@@ -51,7 +51,7 @@ WORK_PER_UNIT = {
     # 2 restricted group searches per quart column + per quart its final minimum and target
     'biome_quarts': {'unit': 'quart column of a biome chunk', 'valu': 2 * 23538.4 + 96 * (69.2 + 150.0),
                      'hbm': 2 * (31.95 + 16.00) + 96 * (58.05 + 12.00 + 1.51 + 56.05 + 4.00)},
-    'fill': {'unit': 'block column of a cell (8 blocks)', 'valu': 1161.2, 'hbm': 11.27 + 13.16},
+    'fill': {'unit': 'block column of a cell (8 blocks)', 'valu': 1141.7, 'hbm': 11.29 + 13.15},
     'ore_veins': {'unit': 'block column', 'valu': 11352.4, 'hbm': 125.28 + 259.67},
 }
 
