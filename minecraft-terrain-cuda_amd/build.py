@@ -43,7 +43,7 @@ def run(cmd, log=None):
 
 
 def codegen(ir=IR, terrain=TERRAIN):
-    run([sys.executable, os.path.join(HERE, 'codegen', 'gen_hip.py'), ir, terrain, os.path.join(CSRC, 'gen')], 'codegen')
+    run([sys.executable, os.path.join(HERE, 'codegen', 'compile_ir.py'), ir, terrain, os.path.join(CSRC, 'gen')], 'codegen')
 
 
 def build_device(out=LIB, extra=(), remarks=None):

@@ -19,6 +19,22 @@
 #define MCT_NI __attribute__((noinline))
 #endif
 
+// Closures the generated evaluators define for memoised values: always inlined.
+#define MCT_CLOSURE __attribute__((always_inline))
+
+// Operation-count hooks. The host build compiled with MCT_COUNT_OPS adds each hook's counts
+// (FP64, FP32, INT32 operations, memory reads) to a per-thread tally each time it runs; every
+// other build, the device's included, compiles them to nothing.
+#if defined(MCT_COUNT_OPS) && !defined(__HIP_DEVICE_COMPILE__)
+namespace mct {
+struct OpTally { unsigned long long f64, f32, i32, mem; };
+inline thread_local OpTally g_tally{};
+}  // namespace mct
+#define MCT_OPS(a, b, c, d) ((void)(mct::g_tally.f64 += (a), mct::g_tally.f32 += (b), mct::g_tally.i32 += (c), mct::g_tally.mem += (d)))
+#else
+#define MCT_OPS(a, b, c, d) ((void)0)
+#endif
+
 // Pointers into LDS (workgroup memory) on the device, so that loads through them are ds_read
 // even across out-of-line calls; plain pointers on the host.
 #if defined(__HIP_DEVICE_COMPILE__)
@@ -135,6 +151,25 @@ MCT_HD MCT_FI float vmin(float first, float second) { return second < first ? se
 MCT_HD MCT_FI float vmax(float first, float second) { return second > first ? second : first; }
 
 MCT_HD MCT_FI bool in_range(float v, float lo_incl, float hi_excl) { return v >= lo_incl && v < hi_excl; }
+
+// CubicSpline.Multipoint between points (x1, y1) and (x2, y2) with derivatives d1, d2:
+// t = (in - x1) / (x2 - x1); lerp(t, y1, y2) + t * (1 - t) * lerp(t, d1 * dx - dy, -d2 * dx + dy).
+MCT_HD MCT_FI float spline_segment(float in, float x1, float x2, float d1, float d2, float y1, float y2) {
+    const float t = (in - x1) / (x2 - x1);
+    const float dx = x2 - x1, dy = y2 - y1;
+    const float a = d1 * dx - dy;
+    const float b = -d2 * dx + dy;
+    const float l1 = lerpf(t, y1, y2);
+    const float w = t * (1.0f - t);
+    const float l2 = lerpf(t, a, b);
+    return l1 + w * l2;
+}
+
+// Past an end point (x, y) with a non-zero derivative d: y + d * (in - x).
+MCT_HD MCT_FI float spline_extend(float in, float x, float d, float y) {
+    const float dx = in - x;
+    return y + d * dx;
+}
 
 // GradientFunction (axis gradients with tiling). mode 0 clamp_to_edge, 1 repeat,
 // 2 mirrored_repeat; factor = (toValue - fromValue) / (to - from) in float.

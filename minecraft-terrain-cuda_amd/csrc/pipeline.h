@@ -1602,6 +1602,22 @@ struct Surf {
         if (!min_set) { min_surface = mfloor(static_cast<double>(prelim)) + surface_depth - 8; min_set = true; }
         return min_surface;
     }
+    // The remaining SurfaceRules conditions (cpu/src/terrain.cpp:454-746, Surface::test).
+    // NoiseThreshold: the noise value, widened to double, in [lo, hi].
+    MCT_HD MCT_FI bool noise_in(float v, double lo, double hi) const {
+        const double d = static_cast<double>(v);
+        return d >= lo && d <= hi;
+    }
+    // YConditionSource: y (+ the stone depth above) at or above anchor + surface depth * m.
+    MCT_HD MCT_FI bool y_above(int anchor, int m, bool add_stone) const { return y + (add_stone ? stone_above : 0) >= anchor + surface_depth * m; }
+    // WaterConditionSource: no water above, or y (+ stone depth) at or above the water height
+    // + offset + surface depth * m.
+    MCT_HD MCT_FI bool water_above(int offset, int m, bool add_stone) const {
+        return water_height == INT32_MIN || y + (add_stone ? stone_above : 0) >= water_height + offset + surface_depth * m;
+    }
+    MCT_HD MCT_FI bool steep() const { return grad_x <= -4 || grad_z >= 4; }
+    MCT_HD MCT_FI bool hole() const { return surface_depth <= 0; }
+    MCT_HD MCT_FI bool above_preliminary_surface() { return y >= min_surface_level(); }
     MCT_HD MCT_FI int band() const {
         const int offset = jroundf(rec[mctgen::kColClay] * 4.0f);
         return S->clay[(y + offset + 192) % 192];
