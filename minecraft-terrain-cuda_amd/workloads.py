@@ -31,6 +31,12 @@ C1_STREAM = 0x43315F5345454453     # SplitMix64 start for the C1 seed
 BATCH_STREAM = 0x4241544348454421  # 256-seed batch
 SERVE_STREAM = 0x5345525645212121  # request stream
 STRESS_STREAM = 0x5354524553532121
+# The served configuration's three fixed Poisson rates (requests/s): 25 / 50 / 80% of the
+# plain generator's request capacity on this mix (about 400 requests/s, 108,808 chunks/s,
+# when they were fixed). Each rate's stream is prefix-stable, so the checked requests are the
+# first ones the bench serves.
+SERVE_RATES = (100.0, 200.0, 320.0)
+SERVE_CHECKED = 40    # requests per rate whose every chunk is checked against reference digests
 
 
 class SplitMix64:

@@ -276,6 +276,8 @@ class Reference:
         L.ref_noise_ref.argtypes = [P, I64, ctypes.c_char_p, P, P]
         L.ref_chunk_biomes.argtypes = [P, I64, I, I, P]
         L.ref_generate_core.argtypes = [P, I64, P, I, P, P]
+        L.ref_digest_terrain.argtypes = [P, P, P, I64, I, I, P, P]
+        L.ref_query_biomes.argtypes = [P, P, P, I64, I, I, P]
         self.L = L
         self.h = L.ref_create(ir_path.encode(), terrain_path.encode())
         if not self.h:
@@ -303,6 +305,29 @@ class Reference:
         f = np.zeros((n, HEIGHT, 16, 16), np.uint8)
         self._check(self.L.ref_generate_core(self.h, int(seed), xz.ctypes.data, n, d.ctypes.data, f.ctypes.data))
         return d, f
+
+    def digest_terrain(self, seeds, xz, threads=8, run=16):
+        """Per-chunk 64-bit digests (terrain u8, biomes) of the full TERRAIN step of chunks
+        xz [n, 2] under seeds [n], in the form of the library's device digest."""
+        seeds = np.ascontiguousarray(seeds, np.int64)
+        xz = np.ascontiguousarray(xz, np.int32).reshape(-1, 2)
+        n = len(xz)
+        if len(seeds) != n:
+            raise ValueError('one seed per chunk')
+        dt, db = np.zeros(n, np.uint64), np.zeros(n, np.uint64)
+        self._check(self.L.ref_digest_terrain(self.h, seeds.ctypes.data, xz.ctypes.data, n, threads, run, dt.ctypes.data, db.ctypes.data))
+        return dt, db
+
+    def query_biomes(self, seeds, xyz, threads=8, run=64):
+        """Answers of biome-only block queries: the biome stored for each block's zoomed quart."""
+        seeds = np.ascontiguousarray(seeds, np.int64)
+        xyz = np.ascontiguousarray(xyz, np.int32).reshape(-1, 3)
+        n = len(xyz)
+        if len(seeds) != n:
+            raise ValueError('one seed per query')
+        out = np.zeros(n, np.uint8)
+        self._check(self.L.ref_query_biomes(self.h, seeds.ctypes.data, xyz.ctypes.data, n, threads, run, out.ctypes.data))
+        return out
 
     def region(self, seed, x0, z0, w, d, threads=8):
         t = np.zeros((w * d, HEIGHT, 16, 16), np.uint16)

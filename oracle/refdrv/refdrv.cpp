@@ -292,6 +292,128 @@ int ref_tree_search(void* p, const int64_t* targets, const int32_t* candidates, 
     }
 }
 
+// The 64-bit chunk digest of the library's k_chunk_digest: the sum (mod 2^64) over the 8-byte
+// little-endian words w_i of mix(w_i + (i + 1) * 0x9E3779B97F4A7C15), finalised as
+// mix(sum ^ byte count), with mix the SplitMix64 finaliser.
+static uint64_t digest_mix(uint64_t z) {
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+static uint64_t chunk_digest(const uint8_t* p, size_t bytes) {
+    uint64_t h = 0;
+    for (size_t i = 0; i < bytes / 8; ++i) {
+        uint64_t w;
+        std::memcpy(&w, p + 8 * i, 8);
+        h += digest_mix(w + 0x9E3779B97F4A7C15ull * static_cast<uint64_t>(i + 1));
+    }
+    return digest_mix(h ^ static_cast<uint64_t>(bytes));
+}
+
+// Digests of the full TERRAIN step (cpu_generate_terrain, each chunk with its own 3x3 biome
+// neighbourhood) of n chunks, chunk i under seeds[i], on `threads` std::threads: terrain as u8
+// palette indices (98,304 B) and biomes (1,536 B), each with chunk_digest. Work is handed out
+// in runs of `run` consecutive chunks; each thread keeps the TerrainSeed of the last seed it
+// met, so a list sorted by seed builds each seed's tables about once per run.
+int ref_digest_terrain(void* p, const int64_t* seeds, const int32_t* xz, int64_t n, int threads, int run, uint64_t* dig_terrain, uint64_t* dig_biomes) {
+    try {
+        Handle* h = static_cast<Handle*>(p);
+        const size_t per = static_cast<size_t>(16 * 16 * h->ir.height), qper = static_cast<size_t>(16 * (h->ir.height / 4));
+        std::atomic<int64_t> next{0};
+        std::atomic<int> failed{0};
+        std::vector<std::string> errs(static_cast<size_t>(threads));
+        std::vector<std::thread> pool;
+        for (int t = 0; t < threads; ++t)
+            pool.emplace_back([&, t] {
+                try {
+                    std::unique_ptr<TerrainSeed> ts;
+                    int64_t have = 0;
+                    std::vector<uint8_t> t8(per);
+                    int64_t i0;
+                    while (!failed.load() && (i0 = next.fetch_add(run)) < n) {
+                        for (int64_t i = i0; i < std::min(n, i0 + run); ++i) {
+                            if (!ts || have != seeds[i]) {
+                                ts = std::make_unique<TerrainSeed>(h->ir, h->tm, seeds[i]);
+                                have = seeds[i];
+                            }
+                            ChunkTerrain c;
+                            cpu_generate_terrain(*ts, xz[2 * i], xz[2 * i + 1], c);
+                            for (size_t k = 0; k < per; ++k) t8[k] = static_cast<uint8_t>(c.terrain[k]);
+                            dig_terrain[i] = chunk_digest(t8.data(), per);
+                            dig_biomes[i] = chunk_digest(c.biomes.data(), qper);
+                        }
+                    }
+                } catch (const std::exception& e) {
+                    errs[static_cast<size_t>(t)] = e.what();
+                    failed = 1;
+                }
+            });
+        for (auto& th : pool) th.join();
+        for (auto& e : errs)
+            if (!e.empty()) {
+                g_err = e;
+                return 1;
+            }
+        return 0;
+    } catch (const std::exception& e) {
+        g_err = e.what();
+        return 1;
+    }
+}
+
+// Answers of n biome-only block queries on `threads` threads: the biome the TERRAIN step stores
+// for the block's zoomed quart (quart y clamped to the world height), i.e. the chunk biomes
+// (cpu_chunk_biomes, fresh R-tree candidate per chunk) at the quart BiomeManager's zoom picks.
+int ref_query_biomes(void* p, const int64_t* seeds, const int32_t* xyz, int64_t n, int threads, int run, uint8_t* out) {
+    try {
+        Handle* h = static_cast<Handle*>(p);
+        const int qh = h->ir.height / 4, qmin = h->ir.min_y >> 2;
+        std::atomic<int64_t> next{0};
+        std::atomic<int> failed{0};
+        std::vector<std::string> errs(static_cast<size_t>(threads));
+        std::vector<std::thread> pool;
+        for (int t = 0; t < threads; ++t)
+            pool.emplace_back([&, t] {
+                try {
+                    std::unique_ptr<TerrainSeed> ts;
+                    std::unique_ptr<VolumeEvaluator> ev;
+                    int64_t have = 0;
+                    std::vector<uint8_t> b(static_cast<size_t>(16 * qh));
+                    int64_t i0;
+                    while (!failed.load() && (i0 = next.fetch_add(run)) < n) {
+                        for (int64_t i = i0; i < std::min(n, i0 + run); ++i) {
+                            if (!ts || have != seeds[i]) {
+                                ev.reset();
+                                ts = std::make_unique<TerrainSeed>(h->ir, h->tm, seeds[i]);
+                                ev = std::make_unique<VolumeEvaluator>(h->ir, ts->tables);
+                                have = seeds[i];
+                            }
+                            int qx = 0, qy = 0, qz = 0;
+                            zoomed_biome(ts->zoom_seed, xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2], [&](int a, int bb, int c) { qx = a; qy = bb; qz = c; return 0; });
+                            cpu_chunk_biomes(*ts, *ev, qx >> 2, qz >> 2, b.data());
+                            const int cqy = std::min(std::max(qy, qmin), qmin + qh - 1) - qmin;
+                            out[i] = b[static_cast<size_t>((cqy * 4 + (qz & 3)) * 4 + (qx & 3))];
+                        }
+                    }
+                } catch (const std::exception& e) {
+                    errs[static_cast<size_t>(t)] = e.what();
+                    failed = 1;
+                }
+            });
+        for (auto& th : pool) th.join();
+        for (auto& e : errs)
+            if (!e.empty()) {
+                g_err = e;
+                return 1;
+            }
+        return 0;
+    } catch (const std::exception& e) {
+        g_err = e.what();
+        return 1;
+    }
+}
+
 // BiomeManager zoom: the quart (qx, qy, qz) picked for each block.
 int ref_zoom(int64_t zoom_seed, const int32_t* xyz, int n, int32_t* out_q) {
     for (int i = 0; i < n; ++i) {
