@@ -136,6 +136,10 @@ int mct_set_tuning(mct_generator* g, const mct_tuning* t);
 int mct_profile(mct_generator* g, int enable);
 /* Accumulated ms and launch counts per kernel group since mct_profile; *n = group count. */
 int mct_kernel_times(mct_generator* g, double* ms, int64_t* launches, int cap, int* n);
+/* How untimed batches are launched: graphs != 0 (the default) replays one captured HIP graph
+ * per batch shape, graphs == 0 launches the same kernels one by one. Results are identical.
+ * No counterpart in the reference (its batches are stream launches, terrain.cu:1292-1336). */
+int mct_set_launch_mode(mct_generator* g, int graphs);
 const char* mct_kernel_name(int i);
 
 /* Biome-only queries (no counterpart in the reference API): for block i at xyz[3i..3i+2] under

@@ -1,0 +1,766 @@
+// gfx950 kernels of the TERRAIN step. Each kernel is a thin loop around a per-item body of
+// pipeline.h (shared with the host emulation).
+//
+// Every value that changes from one batch to the next -- the chunk count, the position-key
+// codecs, the output pointers, the seed-table jobs, the counts the dedupes produce -- is read
+// from device memory (the batch descriptor and the count array), never from a kernel argument.
+// A batch is therefore one fixed sequence of launches with fixed arguments, which the engine
+// captures once as a graph and replays for every batch of the same shape.
+//
+// Work sized by device-side counts (distinct biome chunks, corner columns, surface-grid cells)
+// in light kernels runs in persistent loops: a grid sized to the GPU (workgroups per CU at the
+// kernel's occupancy, times the CU count) strides over the work. The heavy kernels (fill,
+// surface walk, ore veins, column records, aquifer cells, corners, group searches, find-top-
+// surface) run one workgroup per unit with the grid sized for the batch's chunk-count bucket
+// (part of the graph's shape) and the units past the batch's own count exiting at once: their
+// loop-free bodies keep the register use of a single unit (a grid-stride loop around them let
+// the compiler keep loop-invariant per-thread values live, 57 -> 98 VGPRs in the corner pass).
+#pragma once
+
+#include "pipeline.h"
+
+namespace mct {
+
+__constant__ ModelDev c_model;
+
+// Seed-table jobs of a batch: build the tables of `seed` into resident slot `slot`.
+struct SeedJob {
+    int64_t seed;
+    int32_t slot;
+    int32_t pad;
+};
+
+// The per-batch values every kernel reads, staged from pinned host memory at the start of the
+// batch together with the chunk list and the seed jobs.
+struct BatchDesc {
+    int32_t n;                        // chunks
+    int32_t njobs;                    // seed tables built at the start of the batch
+    int32_t col_keys, grid_keys, bio_keys;   // dedupe inputs: n x 25, n x 517, n x 9
+    int32_t pad;
+    KeyCodec kc;                      // corner columns (quart units)
+    KeyCodec kg;                      // surface-grid cells (quart units, semantics flag)
+    KeyCodec kb;                      // biome chunks (chunk units)
+    uint8_t* terrain;                 // outputs (the engine's buffers or the caller's device memory)
+    uint8_t* biomes;
+    uint8_t* aquifer;
+    float* density;                   // core configuration
+};
+
+// Slots of the device count array of a batch.
+enum BatchCount { kCntColumns = 0, kCntGrid = 1, kCntRedo = 2, kCntBiomeChunks = 3, kCntSlots = 4 };
+
+__device__ MCT_FI int global_id() { return static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x); }
+
+__device__ MCT_FI int global_threads() { return static_cast<int>(gridDim.x * blockDim.x); }
+
+// The unit of this workgroup in a one-shot launch over `units` units (the grid is sized for
+// the batch's bucket): -1 past the batch. Consecutive workgroups take consecutive units, so the
+// workgroups of one chunk run together. (Taking the units XCD by XCD instead -- workgroup b on
+// XCD b mod 8 taking that XCD's eighth of the units -- measured slower for the fill, 5.02 against
+// 4.27 ms, and the surface walk, 2.71 against 2.43 ms, on the plain three-region workload.)
+__device__ MCT_FI int chunk_unit(int units) { return static_cast<int>(blockIdx.x) < units ? static_cast<int>(blockIdx.x) : -1; }
+
+// ---- per-seed setup (the first launches of a batch) -----------------------------------------
+constexpr int kSetupThreads = 64;
+
+// One item per (job, NormalNoise layer); the permutation is shuffled in the thread's LDS row.
+__global__ __launch_bounds__(kSetupThreads) void k_seed_layers(SeedDev* seeds, const BatchDesc* B, const SeedJob* jobs, int nnormal) {
+    __shared__ uint8_t scratch[kSetupThreads][256];
+    const int items = B->njobs * nnormal;
+    for (int idx = global_id(); idx < items; idx += global_threads()) {
+        const SeedJob j = jobs[idx / nnormal];
+        seed_layer(seeds[j.slot], j.seed, idx % nnormal, scratch[threadIdx.x]);
+    }
+}
+
+// One item per (job, BlendedNoise node): 40 octaves in sequence; one per job for the
+// positional factories, clay bands and zoom seed.
+__global__ __launch_bounds__(kSetupThreads) void k_seed_serial(SeedDev* seeds, const BatchDesc* B, const SeedJob* jobs, const int* blended_first) {
+    __shared__ uint8_t scratch[kSetupThreads][256];
+    constexpr int per = mctgen::kNumBlended + 1;
+    const int items = B->njobs * per;
+    for (int idx = global_id(); idx < items; idx += global_threads()) {
+        const SeedJob j = jobs[idx / per];
+        const int k = idx % per;
+        if (k < mctgen::kNumBlended) seed_blended(seeds[j.slot], j.seed, k, blended_first[k], scratch[threadIdx.x]);
+        else seed_misc(seeds[j.slot], j.seed, c_model);
+    }
+}
+
+// ---- biome chunks: the union of the chunks' 3 x 3 neighbourhoods, on the device --------------
+// Key of neighbour k = (dz + 1) * 3 + dx + 1 of chunk c; after the dedupe, map[c * 9 + k] is
+// the neighbour's biome-chunk index.
+__global__ __launch_bounds__(256) void k_bio_keys(const BatchDesc* B, const ChunkRef* ch, uint64_t* keys) {
+    const KeyCodec K = B->kb;
+    for (int i = global_id(); i < B->bio_keys; i += global_threads()) {
+        const int c = i / 9, k = i - c * 9;
+        keys[i] = make_key(K, ch[c].seed, 0, ch[c].x + k % 3 - 1, ch[c].z + k / 3 - 1);
+    }
+}
+
+__global__ __launch_bounds__(256) void k_bio_chunks(const BatchDesc* B, const uint64_t* ukeys, const int* count, ChunkRef* bch, unsigned long long* stats) {
+    const KeyCodec K = B->kb;
+    const int nbk = *count;
+    if (global_id() == 0) atomicAdd(stats, static_cast<unsigned long long>(nbk) * kQPer);
+    for (int u = global_id(); u < nbk; u += global_threads()) {
+        int32_t s, x, z;
+        int f;
+        split_key(K, ukeys[u], s, f, x, z);
+        bch[u] = ChunkRef{x, z, s, 0};
+    }
+}
+
+// ---- biomes ---------------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_biome_columns(const SeedDev* seeds, const ChunkRef* bch, const int* nbk, float* bcol) {
+    const int items = *nbk * 16;
+    for (int idx = global_id(); idx < items; idx += global_threads()) biome_column(seeds, bch, bcol, idx);
+}
+
+__global__ __launch_bounds__(256) void k_biome_targets(const SeedDev* seeds, const ChunkRef* bch, const int* nbk, const float* bcol, int64_t* targets) {
+    const int items = *nbk * kQPer;
+    for (int idx = global_id(); idx < items; idx += global_threads()) biome_target(seeds, bch, bcol, targets, idx);
+}
+
+// The upper levels of the float R-tree (the first kTreeTop nodes of the breadth-first order,
+// 24 KB), staged in LDS once per workgroup for the searches that follow.
+constexpr int kTreeTop = 384;
+constexpr int kSearchDepth = 12;
+__device__ MCT_FI int stage_tree_top(TreeNodeF* top) {
+    const ModelDev& M = c_model;
+    const int ntop = mctgen::kTreeNodes < kTreeTop ? mctgen::kTreeNodes : kTreeTop;
+    for (int i = threadIdx.x; i < ntop * 16; i += blockDim.x) reinterpret_cast<uint32_t*>(top)[i] = reinterpret_cast<const uint32_t*>(M.treef)[i];
+    __syncthreads();
+    return ntop;
+}
+
+// L* for whole quart columns (one thread per column): a restricted search per group of the
+// column's varying coordinate instead of one search per quart.
+// (One-shot, the grid sized for the bucket's most biome chunks; workgroups past the count exit
+// before staging the tree.)
+__global__ __launch_bounds__(256) void k_biome_search_cols(const int* nbk, const int64_t* targets, int32_t* lstar, int64_t* mind) {
+    __shared__ TreeNodeF top[kTreeTop];
+    const int items = *nbk * 16;
+    if (static_cast<int>(blockIdx.x * blockDim.x) >= items) return;
+    const int ntop = stage_tree_top(top);
+    const int idx = global_id();
+    if (idx < items) biome_search_column(c_model, top, ntop, targets, lstar, mind, idx);
+}
+
+// The column search split in two: group searches (one item per column and group), then each
+// quart's L* (one per quart).
+__global__ __launch_bounds__(256) void k_biome_groups(const int* nbk, int G, const int64_t* targets, int32_t* lstar, int64_t* mind, GroupRes* res) {
+    __shared__ TreeNodeF top[kTreeTop];
+    const int items = *nbk * 16 * G;
+    if (static_cast<int>(blockIdx.x * blockDim.x) >= items) return;
+    const int ntop = stage_tree_top(top);
+    const int idx = global_id();
+    if (idx < items) biome_group_search(c_model, top, ntop, targets, lstar, mind, res, G, idx);
+}
+
+__global__ __launch_bounds__(256) void k_biome_quart_final(const int* nbk, int G, const int64_t* targets, const GroupRes* res, int32_t* lstar, int64_t* mind) {
+    const int items = *nbk * kQPer;
+    for (int idx = global_id(); idx < items; idx += global_threads()) biome_quart_final(c_model, targets, res, lstar, mind, G, idx);
+}
+
+// Quarts whose targets are beyond the float tree's exact range: the int64 search.
+__global__ __launch_bounds__(256) void k_biome_search_wide(const int* nbk, const int64_t* targets, int32_t* lstar, int64_t* mind) {
+    const int items = *nbk * kQPer;
+    for (int idx = global_id(); idx < items; idx += global_threads()) biome_search_wide(c_model, targets, lstar, mind, idx);
+}
+
+// The search without the float tree (a model whose bounds it cannot hold exactly).
+__global__ __launch_bounds__(256) void k_biome_search_scalar(const int* nbk, const int64_t* targets, int32_t* lstar, int64_t* mind, int run_len) {
+    const int items = *nbk * 16 * biome_runs(run_len);
+    for (int idx = global_id(); idx < items; idx += global_threads()) biome_search(c_model, nullptr, 0, targets, lstar, mind, idx, true, run_len);
+}
+
+__global__ __launch_bounds__(256) void k_biome_ties(const int* nbk, const int64_t* targets, const int32_t* lstar, const int64_t* mind, uint8_t* tie) {
+    const int items = *nbk * kQPer;
+    for (int idx = global_id(); idx < items; idx += global_threads()) biome_tie(c_model, targets, lstar, mind, tie, idx);
+}
+
+// The last-result rule, one wave64 per biome chunk (waves stride over the chunks). While the
+// chain is in sync (the previous result is the previous position's L*) a position's result is
+// its L* unless its tie flag is set, where the previous L* wins; from there the chain is out of
+// sync with that leaf X, which keeps winning while it is at minimal distance (all lanes test
+// it at once) until it loses or a position's L* is X itself. Each switch costs one ballot.
+__global__ __launch_bounds__(256) void k_biome_chain(const int* nbk_ptr, const int64_t* targets, const int32_t* lstar, const int64_t* mind, const uint8_t* tie,
+                                                     uint8_t* biomes, unsigned long long* stats) {
+    const int nbk = *nbk_ptr;
+    const int lane = threadIdx.x & 63;
+    const ModelDev& M = c_model;
+    unsigned long long changed = 0, changed_biome = 0;
+    for (int wave = global_id() >> 6; wave < nbk; wave += global_threads() >> 6) {
+        const size_t base = static_cast<size_t>(wave) * kQPer;
+        int prev = -1;       // result at the previous chain position
+        bool sync = true;    // prev == L* of the previous position
+        for (int p0 = 0; p0 < kQPer; p0 += 64) {
+            const int p = p0 + lane;
+            const size_t i = base + chain_quart(p);
+            const int ls = lstar[i];
+            const bool t = p > 0 && tie[base + p];
+            int res = ls;
+            int k = 0;
+            while (k < 64) {
+                if (sync) {
+                    const unsigned long long m = __ballot(t && lane >= k);
+                    if (m == 0) break;
+                    const int j = __builtin_ctzll(m);
+                    const int before = __shfl(ls, j > 0 ? j - 1 : 0);
+                    const int X = j > 0 ? before : prev;
+                    if (lane == j) res = X;
+                    prev = X;
+                    sync = false;
+                    k = j + 1;
+                } else {
+                    const bool wins = lane >= k && box_distance(M.tree[M.leaf_node[prev]], targets + i * 7) == mind[i];
+                    const unsigned long long m = __ballot(lane >= k && (!wins || ls == prev));
+                    if (m == 0) {
+                        if (lane >= k) res = prev;
+                        k = 64;
+                        break;
+                    }
+                    const int j = __builtin_ctzll(m);
+                    if (lane >= k && lane < j) res = prev;
+                    prev = __shfl(ls, j);
+                    sync = true;
+                    k = j + 1;
+                }
+            }
+            if (sync) prev = __shfl(ls, 63);
+            biomes[i] = M.leaf_biome[res];
+            changed += static_cast<unsigned long long>(__popcll(__ballot(res != ls)));
+            changed_biome += static_cast<unsigned long long>(__popcll(__ballot(res != ls && M.leaf_biome[res] != M.leaf_biome[ls])));
+        }
+    }
+    if (lane == 0 && changed) {
+        atomicAdd(stats + 1, changed);
+        atomicAdd(stats + 2, changed_biome);
+    }
+}
+
+// The biomes of each quart column of a biome chunk (a superset of any block's zoom candidates
+// taken from it), so the surface rules can skip the zoom where no candidate can matter.
+__global__ __launch_bounds__(256) void k_biome_masks(const int* nbk, const uint8_t* bb, uint64_t* mask) {
+    const int items = *nbk * 16;
+    for (int idx = global_id(); idx < items; idx += global_threads()) {
+        const uint8_t* q = bb + static_cast<size_t>(idx >> 4) * kQPer + (idx & 15);
+        uint64_t m0 = 0, m1 = 0;
+        for (int qy = 0; qy < kQH; ++qy) {
+            const int b = q[qy * 16];
+            if (b < 64) m0 |= 1ull << b;
+            else m1 |= 1ull << (b - 64);
+        }
+        mask[idx * 2] = m0;
+        mask[idx * 2 + 1] = m1;
+    }
+}
+
+// ---- density ----------------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_col_keys(const BatchDesc* B, const ChunkRef* ch, uint64_t* keys) {
+    const KeyCodec K = B->kc;
+    for (int idx = global_id(); idx < B->col_keys; idx += global_threads()) column_key(K, ch, keys, idx);
+}
+
+__global__ __launch_bounds__(256) void k_columns(const BatchDesc* B, const SeedDev* seeds, const uint64_t* ucols, const int* count, float* colv) {
+    const KeyCodec K = B->kc;
+    const int items = *count;
+    for (int u = global_id(); u < items; u += global_threads()) column_values(K, seeds, ucols, colv, u);
+}
+
+// Corner pass: a workgroup takes kCornerCols distinct columns x kRows rows (245 of 256
+// threads); the grid is sized for the bucket's most columns and the workgroups past the
+// batch's distinct count exit at once. The x / z records of every
+// octave sample site of its columns are computed once into LDS (the FP64 coordinate setup of
+// two of the three axes, shared by the 49 rows), then each thread evaluates its corner with
+// only the y axis computed per sample. Threads take the corners row by row (thread = row *
+// kCornerCols + column), so a wave holds about 13 adjacent rows: the terrain's y-dependent
+// branches (cave noises below the surface, entrances above, the ore channels' y range) then
+// mostly agree across a wave.
+constexpr int kCornerCols = 256 / kRows;
+constexpr int kCornerTab = mctgen::kCornerSites > 0 ? mctgen::kCornerSites : 1;
+__global__ __launch_bounds__(256) void k_corners(const BatchDesc* B, const SeedDev* seeds, const uint64_t* ucols, const int* count, const float* colv,
+                                                 float* corners) {
+    __shared__ mct::AxisXZ tab[kCornerCols * kCornerTab];
+    const int c0 = static_cast<int>(blockIdx.x) * kCornerCols;
+    const int ncol = min(kCornerCols, *count - c0);   // the same in every thread
+    if (ncol <= 0) return;
+    const KeyCodec K = B->kc;
+    for (int k = threadIdx.x; k < ncol * mctgen::kCornerSites; k += blockDim.x) {
+        const int c = k / mctgen::kCornerSites;
+        tab[k] = corner_site_axes(K, seeds, ucols, c0 + c, k - c * mctgen::kCornerSites);
+    }
+    __syncthreads();
+    const int row = static_cast<int>(threadIdx.x) / kCornerCols, cl = static_cast<int>(threadIdx.x) - row * kCornerCols;
+    if (row < kRows && cl < ncol)
+        corner_values(K, seeds, ucols, colv, corners, c0 + cl, row, (const MCT_LDS mct::AxisXZ*)(tab + cl * kCornerTab));
+}
+
+// ---- surface-level grid / aquifer ---------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_grid_keys(const BatchDesc* B, const ChunkRef* ch, uint64_t* keys) {
+    const KeyCodec K = B->kg;
+    for (int idx = global_id(); idx < B->grid_keys; idx += global_threads()) grid_key(K, ch, keys, idx);
+}
+
+// (One-shot, the grid sized for the bucket's most distinct cells.)
+__global__ __launch_bounds__(256) void k_fts(const BatchDesc* B, const SeedDev* seeds, const uint64_t* ukeys, const int* count, float* vals) {
+    const int u = global_id();
+    if (u < *count) fts_value(B->kg, seeds, ukeys, vals, u);
+}
+
+__global__ __launch_bounds__(256) void k_grid_scatter(const BatchDesc* B, const int* map, const float* vals, int32_t* grid, float* prelim) {
+    for (int idx = global_id(); idx < B->grid_keys; idx += global_threads()) grid_scatter(map, vals, grid, prelim, idx);
+}
+
+__global__ __launch_bounds__(256) void k_aq_skip(const BatchDesc* B, const int32_t* grid, int* skip) {
+    for (int c = global_id(); c < B->n; c += global_threads()) aquifer_skip(grid, skip, c);
+}
+
+__global__ __launch_bounds__(256) void k_aq_cells(const SeedDev* seeds, const BatchDesc* B, const ChunkRef* ch, const int32_t* grid, int32_t* loc, int32_t* fl) {
+    const int idx = global_id();
+    if (idx < B->n * kAqCells) aquifer_cell(seeds, ch, grid, loc, fl, idx);
+}
+
+// ---- fill ------------------------------------------------------------------------------------------
+// A unit is one cell row of one chunk for W threads (W / 16 z rows of 16 x, one block column
+// of the row's 8 blocks each); a chunk's units run from the top cell row down, so the columns'
+// heights are usually final before the lower rows would raise them (their atomics are then
+// skipped). The chunk's aquifer cells the row can read are staged in LDS. The blocks whose
+// aquifer barrier / pressure part decides (a few percent of the blocks, but spread over most
+// waves that reach the aquifer) are queued in LDS and resolved after the row, densely, by all
+// threads of the workgroup. Each column's 8 block classes (2 bits each) go to the class words
+// the surface walk reads (SurfArgs::clsw), half a word per cell row.
+constexpr int kFillStage = 36;   // 3 x 3 aquifer cells by at most 4 grid rows (8 block rows +-1)
+constexpr int kFillHead = kFillStage * 6 + 4;   // ints before the per-thread arrays
+static_assert(kFillHead * 4 % 16 == 0, "per-thread LDS arrays stay 16-byte aligned");
+constexpr size_t fill_lds_bytes(int w) { return static_cast<size_t>(kFillHead) * 4 + 56 * static_cast<size_t>(w); }
+
+// A staged aquifer cell in chunk-local coordinates (l = location - 16 * chunk in x and z):
+// s = |l|^2 and the factors -2 l of the cross terms, so that |l - b|^2 - |b|^2 of a block b of
+// the chunk is s + ax * bx + ay * by + az * bz.
+struct alignas(16) AqCellTerm {
+    int s, ax, ay, az;
+};
+
+struct AqCellsLds {
+    const MCT_LDS AqCellTerm* loc;   // one per staged cell
+    const MCT_LDS int* fl;           // 2 ints per staged cell
+    int base;                        // chunk cell index of staged cell 0
+    __device__ MCT_FI Fluid fluid(int i) const { return Fluid{fl[(i - base) * 2], fl[(i - base) * 2 + 1]}; }
+};
+
+// a * b + c for a, b within +-2^23, as one full-rate v_mad_i32_i24 (the compiler falls back to
+// the quarter-rate 32-bit multiply where one factor is wave-uniform).
+__device__ MCT_FI int mad_i24(int a, int b, int c) {
+    int r;
+    asm("v_mad_i32_i24 %0, %1, %2, %3" : "=v"(r) : "v"(a), "v"(b), "v"(c));
+    return r;
+}
+
+// aquifer_nearest over staged cells. Each distance is |l - b|^2 - |b|^2: the block's own |b|^2
+// is common to all 12 cells, so the order, the ties and every difference of two distances are
+// those of the plain scan, and only differences are read (d2 - d1, d3 - d1, d3 - d2). Every
+// factor is within +-2^10 (cells at chunk-local x, z in -16..25 and y in -84..332), so the
+// products are exact 24-bit multiply-adds.
+__device__ MCT_FI AqNearest aquifer_nearest(const AqCellsLds& C, int cx, int cz, int x, int y, int z) {
+    const int xa = (x - 5) >> 4, ya = grid_yf(y + 1), za = (z - 5) >> 4;
+    const int i0 = ((ya - 1 - kAqMinGY) * 3 + (za - (cz - 1))) * 3 + (xa - (cx - 1));
+    const int bx = x - cx * 16, bz = z - cz * 16;
+    const MCT_LDS AqCellTerm* p = C.loc + (i0 - C.base);
+    int d1 = INT32_MAX, d2 = INT32_MAX, d3 = INT32_MAX, d4 = INT32_MAX;
+    int o1 = 0, o2 = 0, o3 = 0;
+#pragma unroll
+    for (int ox = 0; ox <= 1; ++ox)
+#pragma unroll
+        for (int oy = 0; oy <= 2; ++oy)
+#pragma unroll
+            for (int oz = 0; oz <= 1; ++oz) {
+                const int o = oy * 9 + oz * 3 + ox;
+                const AqCellTerm t = p[o];
+                const int nd = mad_i24(t.az, bz, mad_i24(t.ay, y, mad_i24(t.ax, bx, t.s)));
+                if (d1 >= nd) { o3 = o2; o2 = o1; o1 = o; d4 = d3; d3 = d2; d2 = d1; d1 = nd; }
+                else if (d2 >= nd) { o3 = o2; o2 = o; d4 = d3; d3 = d2; d2 = nd; }
+                else if (d3 >= nd) { o3 = o; d4 = d3; d3 = nd; }
+                else if (d4 >= nd) { d4 = nd; }
+            }
+    return AqNearest{i0 + o1, i0 + o2, i0 + o3, d1, d2, d3};
+}
+
+// The 12 cells of the nearest-3 scan for the blocks of one column that share grid row ya
+// (y + 1 in [12 ya, 12 ya + 11]), as terms of the scan key (below): per cell 32 times the
+// column part s + ax * bx + az * bz of the expanded distance, plus 31 - k, and 32 ay, so that
+// a block's key is one multiply-add, ay * y + p (|32 ay| < 2^15, |p| < 2^23). ya is the same
+// for all threads of a workgroup (one cell row of one chunk), so reloads are uniform.
+struct AqColumnRow {
+    AqCellsLds C;
+    int i0 = 0, ya = INT32_MIN;
+    int p[12], ay[12];
+    __device__ MCT_FI void load(int cx, int cz, int x, int z, int y) {
+        ya = grid_yf(y + 1);
+        const int xa = (x - 5) >> 4, za = (z - 5) >> 4;
+        i0 = ((ya - 1 - kAqMinGY) * 3 + (za - (cz - 1))) * 3 + (xa - (cx - 1));
+        const int bx = x - cx * 16, bz = z - cz * 16;
+        const MCT_LDS AqCellTerm* q = C.loc + (i0 - C.base);
+#pragma unroll
+        for (int k = 0; k < 12; ++k) {
+            const AqCellTerm t = q[(k % 6 >> 1) * 9 + (k & 1) * 3 + k / 6];
+            p[k] = mad_i24(t.az, bz, mad_i24(t.ax, bx, t.s)) * 32 + (31 - k);
+            ay[k] = t.ay * 32;
+        }
+    }
+    __device__ MCT_FI Fluid fluid(int i) const { return C.fluid(i); }
+};
+
+// aquifer_nearest for a block of the loaded column row: the scan of the staged-cell form in
+// the same order k = ox * 6 + oy * 2 + oz, each cell ranked by its key 32 * distance + 31 - k.
+// The keys are distinct and ordered by distance and, among equal distances, later cells first:
+// the order the scan's insertion (d >= nd puts the new cell first) builds. The three smallest
+// keys are kept with one min and two med3; distances and cell offsets are decoded at the end.
+constexpr uint64_t kAqScanOffsets = [] {
+    uint64_t t = 0;
+    for (int k = 0; k < 12; ++k) t |= static_cast<uint64_t>((k % 6 >> 1) * 9 + (k & 1) * 3 + k / 6) << (5 * k);
+    return t;
+}();
+__device__ MCT_FI int aq_key_offset(int key) { return static_cast<int>((kAqScanOffsets >> (5 * (31 - (key & 31)))) & 31u); }
+__device__ MCT_FI int med3_i32(int a, int b, int c) {
+    int r;
+    asm("v_med3_i32 %0, %1, %2, %3" : "=v"(r) : "v"(a), "v"(b), "v"(c));
+    return r;
+}
+__device__ MCT_FI AqNearest aquifer_nearest(const AqColumnRow& R, int, int, int, int y, int) {
+    int k1 = INT32_MAX, k2 = INT32_MAX, k3 = INT32_MAX;
+#pragma unroll
+    for (int k = 0; k < 12; ++k) {
+        const int key = mad_i24(R.ay[k], y, R.p[k]);
+        k3 = med3_i32(k2, key, k3);   // k1 < k2 < k3: the new third is the median of k2, key, k3
+        k2 = med3_i32(k1, key, k2);
+        k1 = min(k1, key);
+    }
+    return AqNearest{R.i0 + aq_key_offset(k1), R.i0 + aq_key_offset(k2), R.i0 + aq_key_offset(k3), k1 >> 5, k2 >> 5, k3 >> 5};
+}
+
+// One unit of the fill: W block columns of one cell row of one chunk.
+__device__ MCT_FI void fill_unit(const FillArgs& A, int unit, int* fill_lds) {
+    const int W = static_cast<int>(blockDim.x), tid = static_cast<int>(threadIdx.x);
+    const size_t idx0 = static_cast<size_t>(unit) * static_cast<size_t>(W);
+    const int c = static_cast<int>(idx0 / (256 * kCellsY));
+    const int t0 = static_cast<int>(idx0 % (256 * kCellsY));
+    const int cy = kCellsY - 1 - (t0 >> 8);
+    const int col0 = t0 & 255;
+    int* sloc = fill_lds;
+    int* sfl = fill_lds + kFillStage * 4;
+    int* qn = fill_lds + kFillStage * 6;
+    int* hm = fill_lds + kFillHead;
+    uint32_t* fr = reinterpret_cast<uint32_t*>(fill_lds + kFillHead + W);   // class bits of the column's 8 blocks
+    float* qd = reinterpret_cast<float*>(fill_lds + kFillHead + 2 * W);
+    uint16_t* qk = reinterpret_cast<uint16_t*>(fill_lds + kFillHead + 10 * W);
+    const ChunkRef cr = A.chunks[c];
+    const int ylo = kMinY + cy * 8;
+    const int skip = A.skip[c];
+    const int gy_lo = max(grid_yf(ylo + 1) - 1, kAqMinGY), gy_hi = min(grid_yf(ylo + 8) + 1, kAqMaxGY);
+    const int base = (gy_lo - kAqMinGY) * 9;
+    if (ylo <= skip) {   // some block of the row can reach the aquifer cells
+        const int32_t* loc = A.cell_loc + static_cast<size_t>(c) * kAqCells * 4;
+        const int32_t* fl = A.cell_fluid + static_cast<size_t>(c) * kAqCells * 2;
+        for (int k = tid; k < (gy_hi - gy_lo + 1) * 9; k += W) {
+            const int lx = loc[(base + k) * 4] - cr.x * 16, ly = loc[(base + k) * 4 + 1], lz = loc[(base + k) * 4 + 2] - cr.z * 16;
+            sloc[k * 4] = lx * lx + ly * ly + lz * lz;
+            sloc[k * 4 + 1] = -2 * lx;
+            sloc[k * 4 + 2] = -2 * ly;
+            sloc[k * 4 + 3] = -2 * lz;
+            sfl[k * 2] = fl[(base + k) * 2];
+            sfl[k * 2 + 1] = fl[(base + k) * 2 + 1];
+        }
+    }
+    if (tid == 0) *qn = 0;
+    __syncthreads();
+    const AqCellsLds C{(const MCT_LDS AqCellTerm*)sloc, (const MCT_LDS int*)sfl, base};
+    AqColumnRow R{C};
+    const Octave* O = A.seeds[cr.seed].oct;
+    const int x = (col0 + tid) & 15, z = (col0 + tid) >> 4;
+    constexpr int NC = mctgen::kNumChannels > 0 ? mctgen::kNumChannels : 1;
+    float val[NC], step[NC];
+    cell_channels(A, c, x, z, cy, val, step);
+    const int bx = cr.x * 16 + x, bz = cr.z * 16 + z;
+    uint8_t* out = A.blocks + static_cast<size_t>(c) * kPer + static_cast<size_t>(cy * 8 * 256 + z * 16 + x);
+    const uint32_t below = __builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u));   // lane id
+    int top = -1;
+    uint32_t frag = 0;
+    int qwave = 0;
+    const bool one_wave = W <= 64;   // then the queue needs no LDS atomic
+#pragma unroll
+    for (int yi = 0; yi < 8; ++yi) {
+        const int by = ylo + yi;
+        const float d = mctgen::eval_block(O, bx, by, bz, val);
+        if (ylo <= skip && grid_yf(by + 1) != R.ya) R.load(cr.x, cr.z, bx, bz, by);
+        AqNearest nr;
+        const int s = aquifer_front(R, cr.x, cr.z, skip, bx, by, bz, static_cast<double>(d), nr);
+        const bool pend = s == kAqPending;
+        const uint64_t m = __builtin_amdgcn_ballot_w64(pend);
+        if (m) {
+            const int lead = __builtin_ctzll(m);
+            int slot = qwave;   // one wave per workgroup: the queue length is wave-uniform
+            if (!one_wave) {
+                if (static_cast<int>(below) == lead) slot = atomicAdd(qn, __builtin_popcountll(m));
+                slot = __builtin_amdgcn_readlane(slot, lead);
+            }
+            qwave += __builtin_popcountll(m);
+            if (pend) {
+                const int k = slot + static_cast<int>(__builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(m >> 32), __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(m), 0u)));
+                qd[k] = d;
+                qk[k] = static_cast<uint16_t>((tid << 3) | yi);
+            }
+        }
+        if (!pend) {
+            const int state = s < 0 ? mctgen::kDefaultBlock : s;
+            out[yi * 256] = static_cast<uint8_t>(state);
+            if (!((mctgen::kAirMask >> state) & 1u)) top = yi;
+            frag |= static_cast<uint32_t>(block_class(state)) << (2 * yi);
+        }
+        for (int k = 0; k < mctgen::kNumChannels; ++k) val[k] += step[k];
+    }
+    hm[tid] = top;
+    fr[tid] = frag;   // queued blocks still at 0, kClsSolid
+    if (one_wave && tid == 0) *qn = qwave;
+    __syncthreads();
+    const int q = *qn;
+    for (int e = tid; e < q; e += W) {
+        const int key = qk[e], lt = key >> 3, yi = key & 7;
+        const int ex = (col0 + lt) & 15, ez = (col0 + lt) >> 4;
+        const int ebx = cr.x * 16 + ex, eby = ylo + yi, ebz = cr.z * 16 + ez;
+        const AqNearest nr = aquifer_nearest(C, cr.x, cr.z, ebx, eby, ebz);
+        const int s = aquifer_back(O, C, ebx, eby, ebz, static_cast<double>(qd[e]), nr);
+        const int state = s < 0 ? mctgen::kDefaultBlock : s;
+        A.blocks[static_cast<size_t>(c) * kPer + static_cast<size_t>((cy * 8 + yi) * 256 + ez * 16 + ex)] = static_cast<uint8_t>(state);
+        if (!((mctgen::kAirMask >> state) & 1u)) atomicMax(&hm[lt], yi);
+        const uint32_t cl = static_cast<uint32_t>(block_class(state));
+        if (cl != kClsSolid) atomicOr(&fr[lt], cl << (2 * yi));
+    }
+    __syncthreads();
+    top = hm[tid];
+    if (A.clsw)
+        reinterpret_cast<uint16_t*>(A.clsw + (static_cast<size_t>(c) * (kHeight / 16) + (cy >> 1)) * 256 + col0 + tid)[cy & 1] = static_cast<uint16_t>(fr[tid]);
+    // WORLD_SURFACE_WG after the fill: the highest non-air block + 1 of the column, the max over
+    // its cells (the heights start at minY; a plain read skips most of the atomics)
+    if (top >= 0 && A.h0) {
+        int* h = A.h0 + static_cast<size_t>(c) * 256 + x * 16 + z;
+        const int v = ylo + top + 1;
+        if (v > *reinterpret_cast<volatile int*>(h)) atomicMax(h, v);
+    }
+}
+
+__global__ __launch_bounds__(256) void k_fill(FillArgs A, const BatchDesc* B) {
+    extern __shared__ __attribute__((aligned(16))) int fill_lds[];
+    const int u = chunk_unit(B->n * (256 * kCellsY / static_cast<int>(blockDim.x)));
+    if (u < 0) return;   // whole workgroups
+    A.blocks = B->terrain;
+    fill_unit(A, u, fill_lds);
+}
+
+// Heights start at minY before the fill raises them; the exact-pass chunk list starts empty.
+__global__ __launch_bounds__(256) void k_heights_init(const BatchDesc* B, int* h0, int* redo_count) {
+    if (global_id() == 0) *redo_count = 0;
+    const int items = B->n * 256;
+    for (int idx = global_id(); idx < items; idx += global_threads()) h0[idx] = kMinY;
+}
+
+__global__ __launch_bounds__(256) void k_fill_core(FillArgs A, const BatchDesc* B) {
+    A.blocks = B->terrain;
+    A.density = B->density;
+    const int items = B->n * 256 * kCellsY;
+    for (int idx = global_id(); idx < items; idx += global_threads()) fill_cell_core(A, idx / (256 * kCellsY), idx % (256 * kCellsY));
+}
+
+// The batch's terrain as the fill left it (before the material rules): the aquifer output.
+__global__ __launch_bounds__(256) void k_aquifer_copy(const BatchDesc* B) {
+    const size_t n16 = static_cast<size_t>(B->n) * kPer / 16;
+    const uint4* src = reinterpret_cast<const uint4*>(B->terrain);
+    uint4* dst = reinterpret_cast<uint4*>(B->aquifer);
+    for (size_t i = static_cast<size_t>(global_id()); i < n16; i += static_cast<size_t>(global_threads())) dst[i] = src[i];
+}
+
+// ---- surface ----------------------------------------------------------------------------------------
+// Column records of the surface pass (2D rule noises, snow-line noise, frozen-patch flag).
+__global__ __launch_bounds__(256) void k_surface_cols(const SeedDev* seeds, const BatchDesc* B, const ChunkRef* chunks, float* rec, const int* nb,
+                                                      const uint64_t* bmask) {
+    const int idx = global_id();
+    if (idx < B->n * 256) surface_column(seeds, c_model, chunks, rec, idx, nb, bmask);
+}
+
+// Ore-vein outcomes of the blocks the ore rules can reach, ahead of the surface walk: one item
+// per block column, adjacent threads adjacent x.
+__global__ __launch_bounds__(256) void k_ore(SurfArgs A, const BatchDesc* B) {
+    const int idx = global_id();
+    if (idx >= B->n * 256) return;
+    A.M = &c_model;
+    A.blocks = B->terrain;
+    ore_precompute(A, idx);
+}
+
+// Threads of a wave take adjacent x so the column walks read adjacent bytes of a y-layer.
+__global__ __launch_bounds__(64) void k_surface1(SurfArgs A, const BatchDesc* B, const int* h0, int* h1, int* h2) {
+    const int idx = global_id();
+    if (idx >= B->n * 256) return;
+    A.M = &c_model;
+    A.blocks = B->terrain;
+    const int c = idx >> 8, t = idx & 255;
+    surface_pass1(A, h0, h1, h2, c, (t & 15) * 16 + (t >> 4));
+}
+
+// A workgroup is 64 columns of one chunk (x 0..15 by 4 z rows, lane = (z & 3) * 16 + x), so
+// everything derived from the chunk (seed tables, column records, neighbour biome chunks) is
+// wave-uniform. What the walk reads of the blocks is their class (solid / air / water / other
+// fluid): each lane copies its column's 24 class words into LDS (word w of lane l at
+// w * 64 + l, so a y step of the wave reads 64 consecutive words); rule results go straight to
+// the chunk in global memory.
+static_assert(kHeight % 16 == 0, "class words hold 16 blocks");
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void k_surface2(SurfArgs A, const BatchDesc* B, const int* h0, const int* h1, const int* h2,
+                                                                                           int* hf) {
+    __shared__ uint32_t cls[(kHeight / 16) * 64];
+    __shared__ float recs[kColF * 64];
+    const int u = chunk_unit(B->n * 4);
+    if (u < 0) return;   // whole workgroups (256 columns per chunk)
+    A.M = &c_model;
+    A.blocks = B->terrain;
+    const int lane = static_cast<int>(threadIdx.x);
+    const int c = u >> 2;
+    const int t = (u & 3) * 64 + lane;
+    const int x = t & 15, z = t >> 4;
+    const uint32_t* cw = A.clsw + static_cast<size_t>(c) * (kHeight / 16) * 256 + t;
+    for (int w = 0; w < kHeight / 16; ++w) cls[w * 64 + lane] = cw[w * 256];
+    const float* rec = A.colrec + (static_cast<size_t>(c) * 256 + static_cast<size_t>(x * 16 + z)) * kColF;
+    for (int k = 0; k < kColF; ++k) recs[k * 64 + lane] = rec[k];
+    surface_pass2<true>(A, h0, h1, h2, hf, c, x * 16 + z, (uint32_t MCT_LDS*)(cls + lane), (const float MCT_LDS*)(recs + lane));
+}
+
+// A chunk needs the exact pass if any column's final height differs from the height the
+// speculative pass assumed for it (one workgroup of 256 per chunk at a time).
+__global__ __launch_bounds__(256) void k_surface_check(const BatchDesc* B, const int* h2, const int* hf, int* redo, int* redo_count, unsigned long long* stats) {
+    const int n = B->n;
+    for (int c = blockIdx.x; c < n; c += gridDim.x) {
+        const int t = threadIdx.x;
+        const bool bad = h2[c * 256 + t] != hf[c * 256 + t];
+        const int any = __syncthreads_or(bad ? 1 : 0);
+        if (t == 0 && any) {
+            redo[atomicAdd(redo_count, 1)] = c;
+            atomicAdd(stats + 3, 1ull);
+        }
+    }
+}
+
+// Exact pass for flagged chunks: refill the chunk, recompute its heights, then process its
+// columns by anti-diagonals (x + z = k): a column reads its x-1 and z-1 neighbours after their
+// full processing and its x+1 and z+1 neighbours before theirs, as the x-major loop does.
+__global__ __launch_bounds__(256) void k_surface_exact(FillArgs F, SurfArgs A, const BatchDesc* B, const int* redo, const int* redo_count) {
+    A.M = &c_model;
+    F.blocks = A.blocks = B->terrain;
+    __shared__ int hm[256];
+    const int total = *redo_count;
+    for (int r = blockIdx.x; r < total; r += gridDim.x) {
+        const int c = redo[r];
+        __syncthreads();
+        for (int t = threadIdx.x; t < 256 * kCellsY; t += blockDim.x) fill_cell(F, c, t, nullptr);
+        __syncthreads();
+        {
+            const int t = threadIdx.x;
+            hm[t] = column_height(c_model, F.blocks + static_cast<size_t>(c) * kPer, t >> 4, t & 15);
+        }
+        __syncthreads();
+        for (int k = 0; k <= 30; ++k) {
+            const int t = threadIdx.x;
+            const int lx = t >> 4, lz = t & 15;
+            if (lx + lz == k) surface_exact_column(A, hm, c, t);
+            __syncthreads();
+        }
+    }
+}
+
+// The centre biome chunk of each chunk to the biome output.
+__global__ __launch_bounds__(256) void k_gather_biomes(const BatchDesc* B, const int* nb, const uint8_t* bb) {
+    uint8_t* out = B->biomes;
+    const size_t items = static_cast<size_t>(B->n) * kQPer;
+    for (size_t idx = static_cast<size_t>(global_id()); idx < items; idx += static_cast<size_t>(global_threads())) {
+        const size_t c = idx / kQPer, q = idx % kQPer;
+        out[idx] = bb[static_cast<size_t>(nb[c * 9 + 4]) * kQPer + q];
+    }
+}
+
+// ---- stand-alone kernels -------------------------------------------------------------------------------
+// Biome-only queries: one block per thread (seed slot, x, y, z); 255 where the float search
+// cannot serve it (counted in *unserved).
+__global__ __launch_bounds__(256) void k_biome_query(const SeedDev* seeds, const int32_t* slot, const int32_t* xyz, int n, uint8_t* out, int* unserved) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int b = biome_query(seeds[slot[i]], c_model, xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2]);
+    if (b < 0) atomicAdd(unserved, 1);
+    out[i] = static_cast<uint8_t>(b < 0 ? 255 : b);
+}
+
+// The column-cached zoom of the surface pass against zoom_pick: one pseudo-random column per
+// thread (x, z uniform in +-29,999,984), every y. out: checked, mismatches, row refills.
+__global__ __launch_bounds__(256) void k_zoom_selftest(int64_t zoom_seed, uint64_t salt, int columns, unsigned long long* out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= columns) return;
+    uint64_t h = salt + 0x9E3779B97F4A7C15ull * static_cast<uint64_t>(i + 1);
+    h = (h ^ (h >> 30)) * 0xBF58476D1CE4E5B9ull;
+    h = (h ^ (h >> 27)) * 0x94D049BB133111EBull;
+    h ^= h >> 31;
+    const int x = static_cast<int>((h & 0xFFFFFFFFull) % 59999969ull) - 29999984;
+    const int z = static_cast<int>((h >> 32) % 59999969ull) - 29999984;
+    ZoomColumn zc;
+    zc.init(zoom_seed, x, z);
+    unsigned long long bad = 0;
+    for (int y = kMinY + kHeight - 1; y >= kMinY; --y) {
+        int a, b, c, qa, qb, qc;
+        zc.pick(y, a, b, c);
+        zoom_pick(zoom_seed, x, y, z, qa, qb, qc);
+        bad += (a != qa || b != qb || c != qc) ? 1u : 0u;
+    }
+    atomicAdd(out, static_cast<unsigned long long>(kHeight));
+    if (bad) atomicAdd(out + 1, bad);
+    atomicAdd(out + 2, static_cast<unsigned long long>(zc.refills));
+}
+
+// Top-down record of each block column of n chunks (one thread per column, t = z * 16 + x):
+// the highest non-air block below the top, then under fluid the first non-fluid block.
+// 8 bytes: top state, floor state, min(255, top_y - floor_y), 0, int16 top y, int16 floor y
+// (little-endian), the record of the reference bench's MCTD view (bench/bench_main.cpp:132-166).
+__global__ __launch_bounds__(256) void k_topdown(const uint8_t* blocks, int n, uint8_t* out) {
+    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= n * 256) return;
+    const uint8_t* b = blocks + static_cast<size_t>(idx >> 8) * kPer + (idx & 255);
+    int y = kHeight - 1;
+    while (y > 0 && ((mctgen::kAirMask >> b[y * 256]) & 1u)) --y;
+    const int top = b[y * 256], top_y = y;
+    while (y > 0 && ((mctgen::kFluidMask >> b[y * 256]) & 1u)) --y;
+    const int floor_s = b[y * 256];
+    const int ty = kMinY + top_y, fy = kMinY + y;
+    const uint32_t lo = static_cast<uint32_t>(top) | (static_cast<uint32_t>(floor_s) << 8) | (static_cast<uint32_t>(min(255, top_y - y)) << 16);
+    const uint32_t hi = (static_cast<uint32_t>(ty) & 0xFFFFu) | ((static_cast<uint32_t>(fy) & 0xFFFFu) << 16);
+    reinterpret_cast<uint2*>(out)[idx] = make_uint2(lo, hi);
+}
+
+// Per-chunk 64-bit digest of output bytes in HBM, one workgroup per chunk: the sum (mod 2^64)
+// over 8-byte words w_i of mix(w_i + (i + 1) * golden), finalised with the byte count. Used to
+// compare whole runs without copying them to the host.
+__device__ inline uint64_t digest_mix(uint64_t z) {
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+__global__ __launch_bounds__(256) void k_chunk_digest(const uint64_t* data, int64_t words_per_chunk, uint64_t* out) {
+    const uint64_t* p = data + static_cast<size_t>(blockIdx.x) * static_cast<size_t>(words_per_chunk);
+    uint64_t h = 0;
+    for (int64_t i = threadIdx.x; i < words_per_chunk; i += 256) h += digest_mix(p[i] + 0x9E3779B97F4A7C15ull * static_cast<uint64_t>(i + 1));
+    __shared__ uint64_t part[256];
+    part[threadIdx.x] = h;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (static_cast<int>(threadIdx.x) < w) part[threadIdx.x] += part[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[blockIdx.x] = digest_mix(part[0] ^ static_cast<uint64_t>(words_per_chunk * 8));
+}
+
+}  // namespace mct

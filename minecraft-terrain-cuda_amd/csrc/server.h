@@ -1,11 +1,10 @@
-// Serving loop over the generator: region requests (seed, chunk list, host outputs) arrive
-// from any thread and are admitted while the device result arena (HBM) has room for their
-// outputs. A scheduler thread forms batches from the admitted requests' chunks in arrival
-// order -- continuous batching: one batch mixes requests and seeds, and a request that
-// arrives while others are on the GPU joins the next batch -- runs them on the generator's
-// two slot streams with outputs written straight into the arena, and copies each finished
-// request back to its host buffers on a third stream while later batches compute.
-// Included by mct.hip after the Generator.
+// Serving loop over the engine: region requests (seed, chunk list, host outputs) arrive from
+// any thread and are admitted while the device result arena (HBM) has room for their outputs.
+// A scheduler thread forms batches from the admitted requests' chunks in arrival order --
+// continuous batching: one batch mixes requests and seeds, and a request that arrives while
+// others are on the GPU joins the next batch -- runs them on the engine's two lanes with
+// outputs written straight into the arena (each batch one graph launch), and copies each
+// finished request back to its host buffers on a third stream while later batches compute.
 #pragma once
 
 #include <condition_variable>
@@ -14,7 +13,9 @@
 #include <mutex>
 #include <thread>
 
-namespace {
+#include "engine.h"
+
+namespace mct {
 
 struct Request {
     int64_t id = 0, seed = 0;
@@ -38,7 +39,6 @@ struct Segment {
 class Server {
 public:
     Server(int max_batch, int seed_cap, size_t budget) : gen(max_batch, 2, seed_cap) {
-        for (auto& sl : gen.slots) sl->ensure_biome_capacity(9 * max_batch);   // worst case: scattered chunks
         const size_t per = static_cast<size_t>(kPer) + kQPer;
         const size_t fixed = gen.device_bytes();
         if (budget <= fixed + per) throw Error(MCT_ECAPACITY, "HBM budget below the generator's own buffers (" + std::to_string(fixed) + " bytes)");
@@ -46,7 +46,7 @@ public:
         hip_check(hipMalloc(&d_terrain, static_cast<size_t>(arena_chunks) * kPer), "malloc arena");
         hip_check(hipMalloc(&d_biomes, static_cast<size_t>(arena_chunks) * kQPer), "malloc arena");
         hip_check(hipStreamCreateWithFlags(&copy_stream, hipStreamNonBlocking), "stream");
-        segs.resize(gen.slots.size());
+        segs.resize(gen.lanes.size());
         worker = std::thread([this] { loop(); });
     }
 
@@ -158,10 +158,9 @@ private:
         if (allocs.empty()) head = 0;
     }
 
-    // Marks a slot's finished batch; requests with every chunk done start their copy back.
+    // Marks a lane's finished batch; requests with every chunk done start their copy back.
     void retire(size_t s) {
-        Slot& S = *gen.slots[s];
-        gen.collect(S, nullptr);
+        gen.lane_finish(static_cast<int>(s));
         for (auto& g : segs[s]) {
             Request& r = *g.req;
             r.done += g.count;
@@ -241,24 +240,22 @@ private:
                     if (stop) break;
                 }
                 bool progressed = finish_copies();
-                // retire finished batches (a seed eviction inside run_batch may have waited for
-                // a slot already, leaving it collected but not yet retired)
-                for (size_t s = 0; s < gen.slots.size(); ++s) {
-                    Slot& S = *gen.slots[s];
-                    if (!segs[s].empty() && (!S.pending || hipEventQuery(S.ev[8]) == hipSuccess)) {
+                // retire finished batches (a seed-table replacement in a launch may have waited
+                // for a lane already, leaving it collected but not yet retired)
+                for (size_t s = 0; s < gen.lanes.size(); ++s) {
+                    if (!segs[s].empty() && gen.lane_ready(static_cast<int>(s))) {
                         retire(s);
                         progressed = true;
                     }
                 }
-                for (size_t s = 0; s < gen.slots.size(); ++s) {
-                    Slot& S = *gen.slots[s];
-                    if (S.pending || !segs[s].empty()) continue;
+                for (size_t s = 0; s < gen.lanes.size(); ++s) {
+                    if (gen.lane_pending(static_cast<int>(s)) || !segs[s].empty()) continue;
                     int at = -1;
                     const int n = form(seeds, xz, sg, at);
                     if (n == 0) break;
                     segs[s] = sg;
-                    gen.run_batch(S, seeds.data(), xz.data(), n, d_terrain + static_cast<size_t>(at) * kPer, d_biomes + static_cast<size_t>(at) * kQPer,
-                                  nullptr, true);
+                    gen.launch_device(static_cast<int>(s), seeds.data(), xz.data(), n, d_terrain + static_cast<size_t>(at) * kPer,
+                                      d_biomes + static_cast<size_t>(at) * kQPer);
                     std::lock_guard<std::mutex> l(mu);
                     ++batches;
                     chunks += n;
@@ -279,7 +276,7 @@ private:
         cv.notify_all();
     }
 
-    Generator gen;
+    Engine gen;
     uint8_t *d_terrain = nullptr, *d_biomes = nullptr;
     int arena_chunks = 0, head = 0, used = 0;
     hipStream_t copy_stream = nullptr;
@@ -294,4 +291,4 @@ private:
     int64_t next_id = 1, submitted = 0, completed = 0, batches = 0, chunks = 0, max_used = 0, mixed_batches = 0;
 };
 
-}  // namespace
+}  // namespace mct

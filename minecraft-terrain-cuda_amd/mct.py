@@ -94,6 +94,7 @@ def lib():
         L.mct_info.argtypes = [P]
         L.mct_profile.argtypes = [P, I]
         L.mct_kernel_times.argtypes = [P, P, P, I, ctypes.POINTER(I)]
+        L.mct_set_launch_mode.argtypes = [P, I]
         L.mct_kernel_name.argtypes = [I]
         L.mct_kernel_name.restype = ctypes.c_char_p
         L.mct_biome_query.argtypes = [P, P, P, I, P]
@@ -115,6 +116,16 @@ def _check(rc):
 
 def _ptr(a):
     return None if a is None else a.ctypes.data
+
+
+def _host_out(a, n, per_chunk, what):
+    """A caller's host output array for n chunks of per_chunk bytes: it must be a writeable,
+    C-contiguous uint8 array of at least that size (the library writes through its pointer)."""
+    if not isinstance(a, np.ndarray) or a.dtype != np.uint8 or not a.flags['C_CONTIGUOUS'] or not a.flags['WRITEABLE']:
+        raise ValueError('%s must be a writeable C-contiguous uint8 numpy array' % what)
+    if a.nbytes < n * per_chunk:
+        raise ValueError('%s holds %d bytes, %d chunks need %d' % (what, a.nbytes, n, n * per_chunk))
+    return a
 
 
 def _xz(chunks):
@@ -148,7 +159,7 @@ class FullTerrainGenerator:
     def generate(self, chunks, biomes=True, aquifer=False, timing=None, out=None):
         xz = _xz(chunks)
         n = len(xz)
-        t = out if out is not None else np.empty((n, 384, 16, 16), np.uint8)
+        t = _host_out(out, n, BLOCKS_PER_CHUNK, 'out') if out is not None else np.empty((n, 384, 16, 16), np.uint8)
         b = np.empty((n, 96, 4, 4), np.uint8) if biomes else None
         a = np.empty((n, 384, 16, 16), np.uint8) if aquifer else None
         _check(lib().mct_generate(self.h, _ptr(xz), n, _ptr(t), _ptr(b), _ptr(a), ctypes.byref(timing) if timing is not None else None))
@@ -216,6 +227,10 @@ class FullTerrainGenerator:
 
     def profile(self, enable=True):
         _check(lib().mct_profile(self.h, 1 if enable else 0))
+
+    def launch_mode(self, graphs=True):
+        """graphs=True: each batch replays a captured HIP graph; False: launch by launch."""
+        _check(lib().mct_set_launch_mode(self.h, 1 if graphs else 0))
 
     def kernel_times(self):
         ms = np.zeros(64, np.float64)
@@ -313,6 +328,9 @@ class Server:
 
     def submit(self, seed, chunks, terrain, biomes=None):
         xz = _xz(chunks)
+        _host_out(terrain, len(xz), BLOCKS_PER_CHUNK, 'terrain')
+        if biomes is not None:
+            _host_out(biomes, len(xz), QUARTS_PER_CHUNK, 'biomes')
         rid = ctypes.c_int64()
         _check(lib().mct_server_submit(self.h, int(seed), _ptr(xz), len(xz), _ptr(terrain), _ptr(biomes), ctypes.byref(rid)))
         self._keep[rid.value] = (xz, terrain, biomes)

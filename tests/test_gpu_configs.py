@@ -1,0 +1,138 @@
+"""Every chunk of the batched and served configurations against the reference cpu/ port's own
+per-chunk digests (tests/golden/make_digests.py), compared as 64-bit chunk digests: on the
+device (mct_chunk_digests) for outputs kept in HBM, with the same digest on the host
+(mct.digest_host) for served results copied to host memory. Also the full plain workload at
+the launch shapes the bench picks (64-lane workgroups everywhere)."""
+import os
+import threading
+import time
+
+import numpy as np
+import pytest
+
+import workloads
+from conftest import GOLDEN
+
+pytestmark = pytest.mark.gpu
+
+PER, QPER = 384 * 16 * 16, 96 * 4 * 4
+
+
+def log(*a):
+    print(*a, flush=True)
+
+
+@pytest.fixture(scope='module')
+def mct():
+    import mct as m
+    m.lib()
+    return m
+
+
+def load(name):
+    with np.load(os.path.join(GOLDEN, name), allow_pickle=False) as z:
+        return {k: z[k] for k in z.files}
+
+
+def device_digests(mct, seeds, xz, **gen_args):
+    n = len(xz)
+    d_t, d_b = mct.alloc_device(n * PER), mct.alloc_device(n * QPER)
+    try:
+        g = mct.FullTerrainGenerator(**gen_args)
+        g.generate_device(seeds, xz, d_t, d_b)
+        g.close()
+        return mct.chunk_digests(d_t, n, PER), mct.chunk_digests(d_b, n, QPER)
+    finally:
+        mct.free_device(d_t)
+        mct.free_device(d_b)
+
+
+def mismatches(got, want):
+    return np.flatnonzero(got != want)
+
+
+@pytest.mark.parametrize('seed_capacity', [256, 48])
+def test_batched_workload_every_chunk_matches_reference(mct, seed_capacity):
+    """The 256-seed uneven workload (33,966 chunks, half rectangles, half scattered) in one
+    mixed-seed call; with 48 resident seed tables the call also evicts and rebuilds tables."""
+    f = load('c3_digests.npz')
+    log('C3 batched, seed capacity', seed_capacity, 'chunks', len(f['xz']))
+    dt, db = device_digests(mct, f['seed'], f['xz'], max_batch_chunks=8192, pipeline_slots=2, seed_capacity=seed_capacity)
+    bad_t, bad_b = mismatches(dt, f['terrain']), mismatches(db, f['biomes'])
+    assert len(bad_t) == 0 and len(bad_b) == 0, ('terrain', len(bad_t), f['xz'][bad_t[:8]].tolist(), 'biomes', len(bad_b))
+
+
+def test_served_configuration_every_chunk_matches_reference(mct):
+    """The served configuration: Poisson arrivals at the three fixed rates, seeds from the
+    64-seed pool, W x H regions (W, H uniform 1..32) in spiral order. The server keeps 16 seed
+    tables, so requests of other seeds evict tables while batches are in flight. Every chunk of
+    the first SERVE_CHECKED requests of each stream against the reference digests."""
+    f = load('c4_digests.npz')
+    counts = f['request_chunks']
+    off = np.concatenate([[0], np.cumsum(counts)])
+    pool = workloads.seed_pool()
+    srv = mct.Server(max_batch_chunks=1024, seed_capacity=16, hbm_budget_bytes=48 << 30)
+    k = 0
+    try:
+        for rate in workloads.SERVE_RATES:
+            reqs = workloads.requests(rate, 10.0 * workloads.SERVE_CHECKED / rate, pool=pool)[:workloads.SERVE_CHECKED]
+            log('C4 served, rate', rate, 'requests', len(reqs), 'chunks', sum(len(c) for _, _, c in reqs))
+            bufs = [(np.zeros((len(c), 384, 16, 16), np.uint8), np.zeros((len(c), 96, 4, 4), np.uint8)) for _, _, c in reqs]
+            for j, (_, seed, chunks) in enumerate(reqs):
+                i = k + j
+                assert seed == f['seed'][off[i]] and len(chunks) == counts[i]
+                assert np.array_equal(np.array(chunks, np.int32), f['xz'][off[i]:off[i + 1]])
+
+            def client():
+                t0 = time.perf_counter()
+                for j, (t, seed, chunks) in enumerate(reqs):
+                    dt = t - (time.perf_counter() - t0)
+                    if dt > 0:
+                        time.sleep(dt)
+                    srv.submit(seed, chunks, bufs[j][0], bufs[j][1])
+            th = threading.Thread(target=client)
+            th.start()
+            th.join()
+            srv.drain()
+            done = srv.poll()
+            assert len(done) == len(reqs)
+            for j in range(len(reqs)):
+                i = k + j
+                want_t, want_b = f['terrain'][off[i]:off[i + 1]], f['biomes'][off[i]:off[i + 1]]
+                bad_t = mismatches(mct.digest_host(bufs[j][0]), want_t)
+                bad_b = mismatches(mct.digest_host(bufs[j][1]), want_b)
+                assert len(bad_t) == 0 and len(bad_b) == 0, ('rate', rate, 'request', j, 'seed', int(f['seed'][off[i]]), len(bad_t), len(bad_b))
+            k += len(reqs)
+        st = srv.stats()
+        log('C4 server stats', st)
+        assert st['completed'] == k
+    finally:
+        srv.close()
+
+
+def test_plain_workload_at_64_lane_shapes_matches_reference(mct):
+    """Every chunk of the plain three-region workload with every tunable kernel at 64-lane
+    workgroups (the shapes the bench's tuning picks for the fill and the one-wave paths)."""
+    f = load('plain_digests.npz')
+    seed = int(f['seed'][0])
+    xz = f['xz']
+    g = mct.FullTerrainGenerator(max_batch_chunks=8192, pipeline_slots=2)
+    t = g.tuning()
+    t['workgroup'] = {k: (64 if v else 0) for k, v in t['workgroup'].items()}
+    g.set_tuning(t)
+    n = len(xz)
+    d_t, d_b = mct.alloc_device(n * PER), mct.alloc_device(n * QPER)
+    try:
+        g.generate_device(np.full(n, seed, np.int64), xz, d_t, d_b)
+        out = np.empty((n, 384, 16, 16), np.uint8)
+        bout = np.empty((n, 96, 4, 4), np.uint8)
+        out[:] = mct.device_to_host(d_t, n * PER).reshape(out.shape)
+        bout[:] = mct.device_to_host(d_b, n * QPER).reshape(bout.shape)
+    finally:
+        g.close()
+        mct.free_device(d_t)
+        mct.free_device(d_b)
+    import hashlib
+    dig = lambda a: np.frombuffer(hashlib.blake2b(np.ascontiguousarray(a).tobytes(), digest_size=8).digest(), np.uint8)
+    bad = [i for i in range(n) if not ((dig(out[i]) == f['terrain'][i]).all() and (dig(bout[i]) == f['biomes'][i]).all())]
+    assert not bad, ('mismatching chunks', len(bad), xz[bad[:8]].tolist())
