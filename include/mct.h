@@ -110,6 +110,14 @@ int mct_topdown(const uint8_t* terrain, int count, int on_device, uint8_t* recor
  * mix(w_i + (i + 1) * 0x9E3779B97F4A7C15) mod 2^64, xor bytes_per_chunk), mix = SplitMix64's
  * finaliser. bytes_per_chunk: a positive multiple of 8. digests: count values, host memory. */
 int mct_chunk_digests(const void* d_data, int64_t count, int64_t bytes_per_chunk, uint64_t* digests);
+/* mct_chunk_digests without any device allocation, for callers that cannot allocate (tight
+ * serving loops, harnesses that forbid allocation). No counterpart in the reference. The digests
+ * pass through a fixed device buffer of the library (MCT_DIGESTS_MAX values per GPU, held until
+ * the process ends) instead of a per-call allocation, so count is at most MCT_DIGESTS_MAX
+ * (MCT_ECAPACITY beyond). Calls on one GPU share that buffer: calls that overlap on one GPU
+ * (from several host threads) race. Same digests as mct_chunk_digests. */
+#define MCT_DIGESTS_MAX 65536
+int mct_chunk_digests_noalloc(const void* d_data, int64_t count, int64_t bytes_per_chunk, uint64_t* digests);
 
 /* ---- Launch shapes and the autotuner (no counterpart in the reference API; the reference picks
  * its tiles with a separate tuning program) ----

@@ -16,6 +16,7 @@
 #include <cstring>
 #include <fstream>
 #include <memory>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 #include <unordered_map>
@@ -26,6 +27,11 @@
 #include "server.h"
 
 using namespace mct;
+
+// The digest buffer of mct_chunk_digests_noalloc: a file-scope device array (one per GPU, live
+// until the process ends). Host code uses the address hipGetSymbolAddress returns for the
+// current device, never the array's name (a host-side stand-in, not a device address).
+static __device__ __align__(256) uint64_t scratch[MCT_DIGESTS_MAX];
 
 namespace {
 
@@ -80,6 +86,23 @@ void check_chunks(const int32_t* xz, int count) {
     for (int i = 0; i < 2 * count; ++i)
         if (xz[i] <= -kChunkLimit || xz[i] >= kChunkLimit)
             throw Error(MCT_EINVAL, "chunk " + std::to_string(i / 2) + ": coordinate " + std::to_string(xz[i]) + " outside +-2^28");
+}
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+uint64_t* g_scratch_ptr[kMaxDevices] = {};
+
+uint64_t* scratch_on_current_device() {
+    int dev = 0;
+    hip_check(hipGetDevice(&dev), "get device");
+    if (dev < 0 || dev >= kMaxDevices) throw Error(MCT_ECAPACITY, "device index beyond the library's table");
+    std::lock_guard<std::mutex> l(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        hip_check(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)), "digest buffer address");
+        g_scratch_ptr[dev] = static_cast<uint64_t*>(p);
+    }
+    return g_scratch_ptr[dev];
 }
 
 }  // namespace
@@ -392,6 +415,21 @@ int mct_chunk_digests(const void* d_data, int64_t count, int64_t bytes_per_chunk
             throw;
         }
         hip_check(hipFree(d), "free");
+    });
+}
+
+int mct_chunk_digests_noalloc(const void* d_data, int64_t count, int64_t bytes_per_chunk, uint64_t* digests) {
+    return guard([&] {
+        if (count < 0 || (count > 0 && (!d_data || !digests)) || bytes_per_chunk <= 0 || bytes_per_chunk % 8 != 0 ||
+            reinterpret_cast<uintptr_t>(d_data) % 8 != 0)
+            throw Error(MCT_EINVAL, "bad arguments (bytes_per_chunk must be a positive multiple of 8, data 8-byte aligned)");
+        if (count > MCT_DIGESTS_MAX)
+            throw Error(MCT_ECAPACITY, "count " + std::to_string(count) + " beyond the fixed digest buffer (MCT_DIGESTS_MAX = " + std::to_string(MCT_DIGESTS_MAX) + ")");
+        uint64_t* d = scratch_on_current_device();
+        if (count == 0) return;
+        k_chunk_digest<<<static_cast<unsigned>(count), 256>>>(static_cast<const uint64_t*>(d_data), bytes_per_chunk / 8, d);
+        hip_check(hipGetLastError(), "digest kernel");
+        hip_check(hipMemcpy(digests, d, sizeof(uint64_t) * static_cast<size_t>(count), hipMemcpyDeviceToHost), "download");
     });
 }
 

@@ -90,6 +90,7 @@ def lib():
         L.mct_free_device.argtypes = [P]
         L.mct_device_to_host.argtypes = [P, P, SZ]
         L.mct_chunk_digests.argtypes = [P, I64, I64, P]
+        L.mct_chunk_digests_noalloc.argtypes = [P, I64, I64, P]
         L.mct_topdown.argtypes = [P, I, I, P]
         L.mct_info.argtypes = [P]
         L.mct_profile.argtypes = [P, I]
@@ -280,6 +281,17 @@ def chunk_digests(p, count, bytes_per_chunk):
     out = np.zeros(int(count), np.uint64)
     _check(lib().mct_chunk_digests(p, int(count), int(bytes_per_chunk), _ptr(out)))
     return out
+
+
+def chunk_digests_noalloc(p, count, bytes_per_chunk):
+    """chunk_digests through the library's fixed device buffer (no device allocation; at most
+    DIGESTS_MAX chunks per call)."""
+    out = np.zeros(int(count), np.uint64)
+    _check(lib().mct_chunk_digests_noalloc(p, int(count), int(bytes_per_chunk), _ptr(out)))
+    return out
+
+
+DIGESTS_MAX = 65536
 
 
 def digest_host(chunks):
