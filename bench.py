@@ -10,8 +10,10 @@ is chunks/s of that step on one GPU. Alongside it the line carries:
   batched                 mixed-seed batch (256 uniform seeds, uneven chunk lists), chunks/s
   served                  Poisson request streams: p50 / p99 latency and chunks/s per rate
   stress                  1,024 seeds x 2^20 scattered chunks, chunks/s; 2^24 biome queries/s
-  roofline                the dominant kernel's VALU issue rate against the MI355X VALU peak
-  cpu_baseline            the reference cpu/ port (built from its sources) on the host cores
+  roofline                the dominant kernel's counted algorithmic operations (FP64 / FP32 /
+                          INT32, from the code generator's per-scope counts) per launch over its
+                          HIP-event launch time, against MI355X's published vector peaks
+  cpu_baseline            the reference cpu/ port (built from its sources) on every host core
   stages_ms / kernels_ms  per-stage and per-kernel times of one step (HIP events)
 
   python bench.py [--gpus 1] [--steps K] [--warmup W] [--quick]
@@ -30,49 +32,102 @@ sys.path.insert(0, os.path.join(ROOT, 'minecraft-terrain-cuda_amd'))
 
 from workloads import SEED, BORDER_CHUNK, region, plain_region_set  # noqa: E402
 
-# MI355X VALU peak: the published FP32 vector peak, 157.3 TFLOP/s, is 256 CUs x 128 lanes x
-# 2.4 GHz x 2 flops (FMA). In instructions: 32-bit VALU operations issue at two wave64
-# instructions per CU per cycle (128 lane-ops), FP64 ones at half that (the published FP64
-# vector peak, 78.6 TFLOP/s). The roofline counts VALU lane-ops against the 32-bit rate; a
-# kernel with FP64 work cannot reach 1.0.
-VALU_PEAK_TOPS = 256 * 128 * 2.4e9 / 1e12
+# MI355X published vector peaks (TFLOP/s; an FMA counts 2): FP64 78.6, FP32 157.3. The FP32
+# figure is 128 FMA lanes per CU per cycle (256 CUs, 2.4 GHz): one VALU lane-op per lane slot,
+# 78.6 T lane-ops/s, is the issue ceiling the counted operations are set against (the fill
+# kernel sustains 40 T VALU lane-ops/s by SQ_INSTS_VALU, so the ceiling is above 64 lanes per
+# CU per cycle). This path's arithmetic is unfused (Java semantics: no contraction), so an
+# FP32 operation is one lane-op and one flop, at most half of what the FP32 peak counts.
+PEAK_FP64 = 78.6
+PEAK_FP32 = 157.3
+LANE_OPS_PER_S = PEAK_FP32 / 2 * 1e12
 
-# Work per unit of the main kernels, measured with rocprofv3 on this workload (the summaries
-# are in profiles/valu_plain3x64.md and profiles/hbm_plain3x64.md; per thread there, per unit
-# here):
-#   VALU lane-ops   SQ_INSTS_VALU x 64 (the instruction stream the kernel issues)
-#   HBM bytes       FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0, the factors from a plain 1 GiB
-#                   device-to-device copy measured in the same runs
-WORK_PER_UNIT = {
-    'surface2': {'unit': 'block column', 'valu': 12505.6, 'hbm': 360.31 + 76.36},
-    # per corner evaluated: the launch covers every corner column slot of the batch, 1.62
-    # threads per distinct corner on this workload, and the idle workgroups exit at once
-    'corners': {'unit': 'corner', 'valu': 17988.6, 'hbm': 31.29},
-    # 2 restricted group searches per quart column + per quart its final minimum and target
-    'biome_quarts': {'unit': 'quart column of a biome chunk', 'valu': 2 * 23538.4 + 96 * (69.2 + 150.0),
-                     'hbm': 2 * (31.95 + 16.00) + 96 * (58.05 + 12.00 + 1.51 + 56.05 + 4.00)},
-    'fill': {'unit': 'block column of a cell (8 blocks)', 'valu': 1141.7, 'hbm': 11.29 + 13.15},
-    'ore_veins': {'unit': 'block column', 'valu': 11352.4, 'hbm': 125.28 + 259.67},
-}
+# device kernel group -> the emulation stage that counts its operations (one to one)
+KERNEL_STAGE = {'fill': 'fill', 'corners': 'corners', 'columns': 'columns', 'fts': 'fts', 'aquifer_cells': 'aquifer_cells',
+                'biome_columns': 'biome_columns', 'biome_quarts': 'biome_quarts', 'surface_columns': 'surface_columns'}
 
 
-def distinct_columns(chunks):
-    cols = set()
-    for x, z in chunks:
-        for k in range(25):
-            cols.add((x * 4 + k % 5, z * 4 + k // 5))
-    return len(cols)
+def count_ops(seed, chunks, threads, group=256):
+    """Algorithmic operations per stage over the whole workload, counted by the counting host
+    emulation in groups of `group` consecutive chunks on a thread pool:
+    {stage: {'unit': u, 'per_unit': {class: ops per unit}}}."""
+    import gen_host
+    from concurrent.futures import ThreadPoolExecutor
+    parts = [chunks[k:k + group] for k in range(0, len(chunks), group)]
+    with ThreadPoolExecutor(max_workers=threads) as ex:
+        results = list(ex.map(lambda part: gen_host.op_counts(seed, part), parts))
+    tot = {}
+    for r in results:
+        for st, v in r.items():
+            t = tot.setdefault(st, {'unit': v['unit'], 'units': 0, 'ops': dict.fromkeys(gen_host.OP_CLASSES, 0)})
+            t['units'] += v['units']
+            for c in gen_host.OP_CLASSES:
+                t['ops'][c] += v['ops'][c]
+    return {st: {'unit': t['unit'], 'per_unit': {c: t['ops'][c] / max(t['units'], 1) for c in gen_host.OP_CLASSES}} for st, t in tot.items()}
 
 
-def units_per_step(chunks, batch):
-    """Threads (units) each main kernel runs over the whole workload, batch by batch (corner
-    columns and biome chunks are deduplicated within a batch)."""
-    u = {'surface2': 256 * len(chunks), 'fill': 256 * 48 * len(chunks), 'ore_veins': 256 * len(chunks), 'corners': 0, 'biome_quarts': 0}
+def batch_units(seeds, chunks, batch):
+    """Units (chunks, distinct biome chunks, corner columns, grid keys) summed over the
+    workload's device batches."""
+    import gen_host
+    out = {}
     for k in range(0, len(chunks), batch):
-        part = chunks[k:k + batch]
-        u['corners'] += distinct_columns(part) * 49
-        u['biome_quarts'] += len({(x + dx, z + dz) for x, z in part for dz in (-1, 0, 1) for dx in (-1, 0, 1)}) * 16
-    return u
+        for u, v in gen_host.stage_units(seeds[k:k + batch], chunks[k:k + batch]).items():
+            out[u] = out.get(u, 0) + v
+    return out
+
+
+def roofline(kernels, launches, dom, counts, units, n, value):
+    """The dominant kernel's counted operations per launch over its average HIP-event launch
+    time, against the published peaks: frac = FP64 rate / 78.6 T + FP32 rate / 157.3 T, the
+    share of the launch the counted floating-point work needs at peak; `achieved` is the same
+    as FP32-equivalent TFLOP/s (FP64 weighted by the peak ratio) so that frac = achieved /
+    peak. issue_frac: all counted operations (FP64 + FP32 + INT32) against the VALU lane-op
+    ceiling. `path`: the whole step's counted operations per chunk at `value`."""
+    roof = {'bound': 'valu', 'kernel': dom, 'achieved': None, 'peak': PEAK_FP32, 'unit': 'TFLOP/s', 'frac': None, 'traffic': None}
+    stage = KERNEL_STAGE.get(dom)
+    if stage is None or stage not in counts or not launches.get(dom):
+        roof['note'] = 'no operation count for kernel group %s' % dom
+        return roof
+    avg_s = kernels[dom] * 1e-3 / launches[dom]
+    per = counts[stage]['per_unit']
+    unit = counts[stage]['unit']
+    per_launch = units[unit] / launches[dom]
+    ops = {c: per[c] * per_launch for c in per}
+    f64, f32, i32 = ops['f64'] / avg_s / 1e12, ops['f32'] / avg_s / 1e12, ops['i32'] / avg_s / 1e12
+    achieved = f32 + f64 * PEAK_FP32 / PEAK_FP64
+    roof.update({
+        'achieved': round(achieved, 3), 'frac': round(achieved / PEAK_FP32, 4),
+        'avg_launch_ms': round(avg_s * 1e3, 4), 'launches': launches[dom], 'work_unit': unit, 'units_per_launch': round(per_launch, 1),
+        'ops_per_unit': {c: round(v, 1) for c, v in per.items()},
+        'fp64': {'achieved': round(f64, 3), 'peak': PEAK_FP64, 'frac': round(f64 / PEAK_FP64, 4)},
+        'fp32': {'achieved': round(f32, 3), 'peak': PEAK_FP32, 'frac': round(f32 / PEAK_FP32, 4)},
+        'int32_tops': round(i32, 3),
+        'issue_frac': round((ops['f64'] + ops['f32'] + ops['i32']) / avg_s / LANE_OPS_PER_S, 4)})
+    tot = {c: sum(v['per_unit'][c] * units[v['unit']] for v in counts.values()) / n for c in ('f64', 'f32', 'i32', 'mem')}
+    p64, p32 = tot['f64'] * value / 1e12, tot['f32'] * value / 1e12
+    roof['path'] = {'ops_per_chunk': {c: round(v) for c, v in tot.items()}, 'fp64_tflops': round(p64, 3), 'fp32_tflops': round(p32, 3),
+                    'frac': round(p64 / PEAK_FP64 + p32 / PEAK_FP32, 4),
+                    'uncounted': 'surface material-rule walks, dedupe sorts, seed-table setup'}
+    return roof
+
+
+def traffic(dom, chunks_per_launch):
+    """HBM bytes per launch of kernel group `dom` from the committed counter profile of this
+    workload (rocprofv3 --pmc FETCH_SIZE, then WRITE_SIZE, scaled by the same counters over a
+    plain device-to-device copy), per chunk x chunks per launch; None without one."""
+    path = os.path.join(ROOT, 'profiles', 'hbm_%s_plain3x64.json' % dom)
+    if not os.path.exists(path):
+        return None, None
+    with open(path) as fh:
+        p = json.load(fh)
+    return round(p['bytes_per_chunk'] * chunks_per_launch), os.path.relpath(path, ROOT)
+
+
+def host_cores():
+    """Logical cores this process may run on (its affinity set), the thread count of the CPU
+    baseline."""
+    return len(os.sched_getaffinity(0)) if hasattr(os, 'sched_getaffinity') else (os.cpu_count() or 1)
 
 
 def cpu_baseline(seconds_target=15.0):
@@ -89,8 +144,7 @@ def cpu_baseline(seconds_target=15.0):
     R.ref_create.argtypes = [ctypes.c_char_p, ctypes.c_char_p]
     R.ref_generate_region.argtypes = [P, I64, I, I, I, I, I, P, P]
     h = R.ref_create(mct.IR_PATH.encode(), mct.TERRAIN_PATH.encode())
-    threads = len(os.sched_getaffinity(0)) if hasattr(os, 'sched_getaffinity') else os.cpu_count()
-    threads = max(1, min(threads, 16))
+    threads = host_cores()
     w = 8
     t = np.zeros(w * w * 98304, np.uint16)
     t0 = time.perf_counter()
@@ -107,8 +161,9 @@ def cpu_baseline(seconds_target=15.0):
 
 
 def batched_vs_plain(mct, batch, tuning=None):
-    """The 256-seed uneven workload in one mixed-seed call against the same chunks generated
-    plainly, one seed's list per call (outputs in HBM both ways)."""
+    """The 256-seed uneven workload in one mixed-seed call, with every seed's tables built
+    inside the call (cold) and then with them resident, against the same chunks generated
+    plainly, one seed's list per call (outputs in HBM every way)."""
     import workloads
     work = workloads.batched()
     seeds = np.array([s for s, ch in work for _ in ch], np.int64)
@@ -118,7 +173,12 @@ def batched_vs_plain(mct, batch, tuning=None):
     g = mct.FullTerrainGenerator(max_batch_chunks=batch, pipeline_slots=2, seed_capacity=256)
     if tuning is not None:
         g.autotune(seeds[:batch], chunks[:batch])        # shapes for this mix, before the timed calls
-    g.generate_device(seeds, chunks, d_t, d_b)          # tables resident, buffers sized
+    # buffers sized and batch shapes captured on the same chunk lists under 256 other seeds,
+    # which fill the seed cache: the first timed call builds every table itself
+    g.generate_device(seeds ^ np.int64(0x5DEECE66D), chunks, d_t, d_b)
+    t0 = time.perf_counter()
+    g.generate_device(seeds, chunks, d_t, d_b)
+    tc = time.perf_counter() - t0
     t0 = time.perf_counter()
     g.generate_device(seeds, chunks, d_t, d_b)
     tb = time.perf_counter() - t0
@@ -132,13 +192,15 @@ def batched_vs_plain(mct, batch, tuning=None):
     mct.free_device(d_t)
     mct.free_device(d_b)
     return {'workload': '256 uniform seeds, log-uniform 1..1024 chunks each (half near-square rectangles, half scattered in +-117,187)',
-            'chunks': n, 'seeds': len(work), 'chunks_per_s': round(n / tb, 1), 'plain_chunks_per_s': round(n / tp, 1), 'speedup': round(tp / tb, 2)}
+            'chunks': n, 'seeds': len(work), 'chunks_per_s': round(n / tb, 1), 'cold_chunks_per_s': round(n / tc, 1),
+            'plain_chunks_per_s': round(n / tp, 1), 'speedup': round(tp / tb, 2), 'speedup_cold': round(tp / tc, 2)}
 
 
-def served(mct, plain_rate, seconds=4.0, fractions=(0.25, 0.5, 0.8, 3.0), max_batch=1024):
-    """Open-loop Poisson request streams at fractions of the plain generator's capacity on the
-    same request mix: p50 / p99 latency (submit to results in host memory) and throughput. The
-    last rate (3x) overloads the server on purpose: its chunks/s is the served capacity."""
+def served(mct, plain_rate, seconds=4.0, overload=3.0, max_batch=1024):
+    """Open-loop Poisson request streams at the served configuration's three fixed rates
+    (requests/s) and one overload rate (`overload` x the plain generator's capacity on the same
+    mix): p50 / p99 latency (submit to results in host memory) and throughput. The overload
+    stream keeps the server saturated: its chunks/s is the served capacity."""
     import threading
     import workloads
     pool = workloads.seed_pool()
@@ -148,8 +210,7 @@ def served(mct, plain_rate, seconds=4.0, fractions=(0.25, 0.5, 0.8, 3.0), max_ba
     bring = mct.PinnedBuffer((ring_chunks, 96, 4, 4))
     out = []
     srv = mct.Server(max_batch_chunks=max_batch, seed_capacity=64, hbm_budget_bytes=48 << 30)
-    for f in fractions:
-        rps = f * plain_rate / mean_chunks
+    for rps in list(workloads.SERVE_RATES) + [overload * plain_rate / mean_chunks]:
         reqs = workloads.requests(rps, seconds, pool=pool)
         srv.poll()
         pos = [0]
@@ -174,7 +235,7 @@ def served(mct, plain_rate, seconds=4.0, fractions=(0.25, 0.5, 0.8, 3.0), max_ba
         wall = time.perf_counter() - t0
         lat = np.array([l for _, l in srv.poll(1 << 20)])
         nch = sum(len(c) for _, _, c in reqs)
-        out.append({'offered_fraction': f, 'offered_requests_per_s': round(rps, 1), 'requests': len(reqs), 'chunks': nch,
+        out.append({'offered_fraction': round(rps * mean_chunks / plain_rate, 3), 'offered_requests_per_s': round(rps, 1), 'requests': len(reqs), 'chunks': nch,
                     'chunks_per_s': round(nch / wall, 1), 'p50_ms': round(float(np.percentile(lat, 50)), 3),
                     'p99_ms': round(float(np.percentile(lat, 99)), 3), 'max_ms': round(float(lat.max()), 3)})
     st = srv.stats()
@@ -198,7 +259,9 @@ def stress(mct, batch, window=1 << 16, n_queries=1 << 24):
     n = len(xz)
     d_t, d_b = mct.alloc_device(window * mct.BLOCKS_PER_CHUNK), mct.alloc_device(window * mct.QUARTS_PER_CHUNK)
     g = mct.FullTerrainGenerator(max_batch_chunks=batch, pipeline_slots=2, seed_capacity=1024)
-    g.generate_device(cs[:window], xz[:window], d_t, d_b)   # every seed's tables resident, buffers sized
+    # buffers sized and shapes captured under 1,024 other seeds, so that the timed pass builds
+    # every table itself (setup included)
+    g.generate_device(cs[:window] ^ np.int64(0x5DEECE66D), xz[:window], d_t, d_b)
     t0 = time.perf_counter()
     for off in range(0, n, window):
         g.generate_device(cs[off:off + window], xz[off:off + window], d_t, d_b)
@@ -211,7 +274,7 @@ def stress(mct, batch, window=1 << 16, n_queries=1 << 24):
     g.biome_query(qs, q)
     tq = time.perf_counter() - t0
     g.close()
-    return {'workload': '1,024 seeds, 2^20 chunks uniform over +-117,187 (wrap chunks first), %d-chunk HBM output window; '
+    return {'workload': '1,024 seeds (tables built in the timed pass), 2^20 chunks uniform over +-117,187 (wrap chunks first), %d-chunk HBM output window; '
                         '2^24 biome-only block queries (x, z uniform over the same blocks, y uniform -64..319)' % window,
             'chunks': n, 'chunks_per_s': round(n / tc, 1), 'seconds': round(tc, 3),
             'queries': n_queries, 'queries_per_s': round(n_queries / tq, 1), 'query_seconds': round(tq, 3)}
@@ -325,19 +388,15 @@ def main():
     stages = {k: round(v, 3) for k, v in tm.as_dict().items()}
     kernels = {k: round(v[0], 3) for k, v in kt.items() if v[1] > 0}
     dom = max(kernels, key=kernels.get)
-    # roofline of the dominant kernel: its work per launch over its average launch time (HIP
-    # events on the stream it runs on, single-slot pass above)
+    # roofline of the dominant kernel: its counted operations per launch over its average
+    # launch time (HIP events on the stream it runs on, single-lane pass above)
     launches = {k: v[1] for k, v in kt.items()}
-    units = units_per_step(chunks, a.batch)
-    w = WORK_PER_UNIT.get(dom)
-    roof = {'bound': 'valu', 'kernel': dom, 'achieved': None, 'peak': VALU_PEAK_TOPS, 'unit': 'TOP/s', 'frac': None, 'traffic': None}
-    if w and dom in units:
-        per_launch = units[dom] / launches[dom]
-        avg_s = kernels[dom] * 1e-3 / launches[dom]
-        achieved = w['valu'] * per_launch / avg_s / 1e12
-        roof.update({'achieved': round(achieved, 3), 'frac': round(achieved / VALU_PEAK_TOPS, 4), 'traffic': round(w['hbm'] * per_launch),
-                     'avg_launch_ms': round(avg_s * 1e3, 4), 'units_per_launch': round(per_launch, 1), 'work_unit': w['unit'],
-                     'valu_per_unit': w['valu'], 'hbm_gbs': round(w['hbm'] * per_launch / avg_s / 1e9, 2)})
+    units = batch_units(seeds, chunks, a.batch)
+    counts = count_ops(SEED, chunks, host_cores())
+    roof = roofline(kernels, launches, dom, counts, units, n, value)
+    roof['traffic'], src = traffic(dom, units['chunks'] / max(launches.get(dom, 1), 1))
+    if src:
+        roof['traffic_source'] = src
     # end to end with results copied to pinned host memory
     pin = mct.lib().mct_alloc_pinned(n * mct.BLOCKS_PER_CHUNK)
     buf = np.ctypeslib.as_array(ctypes.cast(pin, ctypes.POINTER(ctypes.c_uint8)), shape=(n, 384, 16, 16))
@@ -346,6 +405,12 @@ def main():
     t0 = time.perf_counter()
     g.generate(chunks, biomes=False, out=buf)
     e2e = n / (time.perf_counter() - t0)
+    # the host link's ceiling for the same bytes: one device-to-pinned-host copy of the terrain
+    mct._check(mct.lib().mct_device_to_host(pin, d_t, n * mct.BLOCKS_PER_CHUNK))
+    t0 = time.perf_counter()
+    mct._check(mct.lib().mct_device_to_host(pin, d_t, n * mct.BLOCKS_PER_CHUNK))
+    d2h_s = time.perf_counter() - t0
+    d2h = {'gb_per_s': round(n * mct.BLOCKS_PER_CHUNK / d2h_s / 1e9, 2), 'chunks_per_s': round(n / d2h_s, 1)}
     mct.lib().mct_free_pinned(pin)
     mct.free_device(d_t)
     mct.free_device(d_b)
@@ -364,7 +429,7 @@ def main():
             'config': {'workload': 'plain TERRAIN, one seed (%d), %s, full height, outputs in HBM' % (
                 SEED, '32x32 region' if a.quick else '3 x 64x64-chunk regions (origin, (-2048,-2048), border chunk %d) + 8 wrap/border chunks' % BORDER_CHUNK),
                 'chunks': n, 'batch': a.batch, 'slots': a.slots},
-            'e2e_host_chunks_per_s': round(e2e, 1), 'batch_trials': batch_trials, 'tuning': tuning, 'batched': batched, 'served': serve, 'stress': mixed, 'roofline': roof, 'cpu_baseline': cpu,
+            'e2e_host_chunks_per_s': round(e2e, 1), 'd2h_ceiling': d2h, 'batch_trials': batch_trials, 'tuning': tuning, 'batched': batched, 'served': serve, 'stress': mixed, 'roofline': roof, 'cpu_baseline': cpu,
             'stages_ms': stages, 'kernels_ms': kernels}
     print(json.dumps(line), flush=True)
 

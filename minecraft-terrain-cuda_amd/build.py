@@ -1,7 +1,8 @@
 """Build the gfx950 library (libmct.so) and its host emulation (libmct_host.so), in-tree.
 
-  python build.py            codegen from data/*.json, then hipcc (device) and g++ (emulation)
-  python build.py --no-emu   skip the host emulation library
+  python build.py            codegen from data/*.json, then hipcc (device) and g++ (emulation,
+                             plain and with operation counting: libmct_count.so)
+  python build.py --no-emu   skip the host emulation libraries
   python build.py --remarks F  also write the compiler's per-kernel resource usage to F
 
 The exactness flags are part of the semantics (Java arithmetic): no FMA contraction, no
@@ -22,6 +23,7 @@ TERRAIN = os.path.join(HERE, 'data', 'synthetic_overworld_terrain.json')
 CSRC = os.path.join(HERE, 'csrc')
 LIB = os.path.join(HERE, 'libmct.so')
 EMU = os.path.join(HERE, 'libmct_host.so')
+COUNT = os.path.join(HERE, 'libmct_count.so')
 
 HIP_FLAGS = ['--offload-arch=gfx950', '-O3', '-std=c++17', '-fPIC', '-shared',
              '-ffp-contract=off', '-fno-fast-math', '-fno-gpu-flush-denormals-to-zero',
@@ -57,10 +59,10 @@ def build_device(out=LIB, extra=(), remarks=None):
     os.replace(tmp, out)
 
 
-def build_emu(out=EMU):
+def build_emu(out=EMU, extra=()):
     cxx = shutil.which('g++') or 'c++'
     tmp = out + '.tmp'
-    run([cxx] + HOST_FLAGS + [os.path.join(CSRC, 'host_emu.cpp'), '-o', tmp], 'g++ host emulation -> %s' % os.path.basename(out))
+    run([cxx] + HOST_FLAGS + list(extra) + [os.path.join(CSRC, 'host_emu.cpp'), '-o', tmp], 'g++ host emulation -> %s' % os.path.basename(out))
     os.replace(tmp, out)
 
 
@@ -75,6 +77,9 @@ def main():
         build_device(remarks=a.remarks)
     if not a.no_emu:
         build_emu()
+        # the same emulation counting the operations of every evaluated scope (the per-scope
+        # counts the code generator emits), for the algorithmic operations per stage
+        build_emu(COUNT, ['-DMCT_COUNT_OPS'])
 
 
 if __name__ == '__main__':

@@ -288,7 +288,11 @@ struct LaneMemory {
 
 struct Lane {
     std::unique_ptr<LaneMemory> mem;
-    hipStream_t stream = nullptr, copy_stream = nullptr;   // compute, copies to host memory
+    // The lane's kernels and its copies to host memory share one stream: the lane's next batch
+    // reuses the buffers, so it waits for the copy either way, and the copy of one lane overlaps
+    // the other lane's kernels. A stream per copy would also exceed the device's hardware queues
+    // (4 per process), and streams that share a queue stall each other's kernels behind a copy.
+    hipStream_t stream = nullptr;
     hipEvent_t done = nullptr, copied = nullptr;
     hipEvent_t stage_ev[kStCount + 1] = {}, kernel_ev[kKCount][2] = {}, copy_ev[2] = {};
     std::map<Shape, hipGraphExec_t> graphs;
@@ -299,7 +303,6 @@ struct Lane {
 
     Lane(int cap, int seed_cap) : mem(std::make_unique<LaneMemory>(cap, seed_cap)) {
         hip_check(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking), "stream");
-        hip_check(hipStreamCreateWithFlags(&copy_stream, hipStreamNonBlocking), "stream");
         hip_check(hipEventCreateWithFlags(&done, hipEventDefault), "event");
         hip_check(hipEventCreateWithFlags(&copied, hipEventDefault), "event");
         for (auto& e : stage_ev) hip_check(hipEventCreate(&e), "event");
@@ -309,7 +312,6 @@ struct Lane {
     }
     ~Lane() {
         if (stream) (void)hipStreamSynchronize(stream);
-        if (copy_stream) (void)hipStreamSynchronize(copy_stream);
         for (auto& g : graphs) (void)hipGraphExecDestroy(g.second);
         for (auto& e : stage_ev) (void)hipEventDestroy(e);
         for (auto& e : copy_ev) (void)hipEventDestroy(e);
@@ -320,7 +322,6 @@ struct Lane {
         (void)hipEventDestroy(copied);
         mem.reset();
         if (stream) (void)hipStreamDestroy(stream);
-        if (copy_stream) (void)hipStreamDestroy(copy_stream);
     }
     Lane(const Lane&) = delete;
     Lane& operator=(const Lane&) = delete;
@@ -382,9 +383,15 @@ public:
         finish_all(nullptr);
         done_ms.clear();
         hip_check(hipEventRecord(t0, lanes[0]->stream), "event");
+        // Results copied to host memory: the host link (about 57 GB/s, 580K chunks/s of terrain)
+        // is slower than the kernels, so the call is cut into batches of about an eighth of it
+        // (at least 1024 chunks) and each batch's copy overlaps the next batches' kernels on
+        // the other lane; only the first batch's kernels are not hidden.
+        const bool to_host = !out.device && (out.terrain || out.biomes || out.aquifer || out.density);
+        const int step = (to_host && lanes.size() > 1) ? std::min(max_batch, std::max(1024, (count + 7) / 8)) : max_batch;
         int index = 0;
         for (int off = 0; off < count; ++index) {
-            int n = std::min(max_batch, count - off);
+            int n = std::min(step, count - off);
             {
                 std::unordered_map<int64_t, int> seen;
                 for (int i = 0; i < n; ++i)
@@ -733,15 +740,14 @@ private:
         hip_check(hipEventRecord(L.done, L.stream), "event");
         L.copies = !out.device && (out.terrain || out.biomes || out.aquifer || out.density);
         if (L.copies) {
-            hip_check(hipStreamWaitEvent(L.copy_stream, L.done, 0), "wait for batch");
-            if (timed) hip_check(hipEventRecord(L.copy_ev[0], L.copy_stream), "event");
+            if (timed) hip_check(hipEventRecord(L.copy_ev[0], L.stream), "event");
             const size_t nn = static_cast<size_t>(n);
-            if (out.terrain) hip_check(hipMemcpyAsync(out.terrain, m.blocks, nn * kPer, hipMemcpyDeviceToHost, L.copy_stream), "copy terrain");
-            if (out.biomes) hip_check(hipMemcpyAsync(out.biomes, m.bout, nn * kQPer, hipMemcpyDeviceToHost, L.copy_stream), "copy biomes");
-            if (out.aquifer) hip_check(hipMemcpyAsync(out.aquifer, m.aq, nn * kPer, hipMemcpyDeviceToHost, L.copy_stream), "copy aquifer");
-            if (out.density) hip_check(hipMemcpyAsync(out.density, m.density, nn * kPer * sizeof(float), hipMemcpyDeviceToHost, L.copy_stream), "copy density");
-            if (timed) hip_check(hipEventRecord(L.copy_ev[1], L.copy_stream), "event");
-            hip_check(hipEventRecord(L.copied, L.copy_stream), "event");
+            if (out.terrain) hip_check(hipMemcpyAsync(out.terrain, m.blocks, nn * kPer, hipMemcpyDeviceToHost, L.stream), "copy terrain");
+            if (out.biomes) hip_check(hipMemcpyAsync(out.biomes, m.bout, nn * kQPer, hipMemcpyDeviceToHost, L.stream), "copy biomes");
+            if (out.aquifer) hip_check(hipMemcpyAsync(out.aquifer, m.aq, nn * kPer, hipMemcpyDeviceToHost, L.stream), "copy aquifer");
+            if (out.density) hip_check(hipMemcpyAsync(out.density, m.density, nn * kPer * sizeof(float), hipMemcpyDeviceToHost, L.stream), "copy density");
+            if (timed) hip_check(hipEventRecord(L.copy_ev[1], L.stream), "event");
+            hip_check(hipEventRecord(L.copied, L.stream), "event");
         } else {
             hip_check(hipEventRecord(L.copied, L.stream), "event");
         }

@@ -20,6 +20,34 @@ namespace {
 
 thread_local std::string g_err;
 
+// Operations of each stage of the last mcte_generate on this thread, named as the device's
+// kernel groups (the build with MCT_COUNT_OPS counts; every other build records zeros).
+struct StageOps {
+    const char* name;
+    uint64_t ops[4];   // FP64, FP32, INT32, memory reads
+};
+thread_local std::vector<StageOps> g_stages;
+
+void stage_begin() {
+    g_stages.clear();
+#if defined(MCT_COUNT_OPS)
+    g_tally = OpTally{};
+#endif
+}
+
+// Everything counted since the previous mark belongs to stage `name`.
+void stage_mark(const char* name) {
+    StageOps st{name, {0, 0, 0, 0}};
+#if defined(MCT_COUNT_OPS)
+    st.ops[0] = g_tally.f64;
+    st.ops[1] = g_tally.f32;
+    st.ops[2] = g_tally.i32;
+    st.ops[3] = g_tally.mem;
+    g_tally = OpTally{};
+#endif
+    g_stages.push_back(st);
+}
+
 struct Emu {
     ModelDev M{};
     std::vector<TreeNodeDev> tree;
@@ -178,10 +206,31 @@ int64_t mcte_zoom_check(int64_t zoom_seed, const int32_t* xz, int n) {
 // Full pipeline for one seed and n chunks. Outputs (nullable): terrain and aquifer fill u8 per
 // block, biomes u8 per quart, density f32 per block (canonical layouts, chunks back to back).
 // stats[0] = chunks redone in exact order after the speculative surface pass.
+// Operation counts of stage i of the last mcte_generate on this thread: its name and
+// {FP64, FP32, INT32, memory reads}; returns the number of stages, 0 before any call. All
+// counts are zero unless the library was built with MCT_COUNT_OPS (mcte_counts_ops).
+int mcte_stage_ops(int i, const char** name, uint64_t* ops) {
+    const int n = static_cast<int>(g_stages.size());
+    if (i >= 0 && i < n) {
+        if (name) *name = g_stages[static_cast<size_t>(i)].name;
+        if (ops) std::memcpy(ops, g_stages[static_cast<size_t>(i)].ops, sizeof g_stages[0].ops);
+    }
+    return n;
+}
+
+int mcte_counts_ops() {
+#if defined(MCT_COUNT_OPS)
+    return 1;
+#else
+    return 0;
+#endif
+}
+
 int mcte_generate(void* p, int64_t seed, const int32_t* xz, int n, uint8_t* terrain, uint8_t* biomes, uint8_t* aquifer, float* density, int64_t* stats) {
     try {
         Emu& e = *static_cast<Emu*>(p);
         build_seed(e, seed);
+        stage_begin();
         const SeedDev* seeds = e.seed.get();
         const ModelDev& M = e.M;
         std::vector<ChunkRef> ch(static_cast<size_t>(n));
@@ -208,6 +257,7 @@ int mcte_generate(void* p, int64_t seed, const int32_t* xz, int n, uint8_t* terr
         std::vector<int32_t> lstar(static_cast<size_t>(nbk) * kQPer);
         std::vector<uint8_t> tie(static_cast<size_t>(nbk) * kQPer), bb(static_cast<size_t>(nbk) * kQPer);
         for (int i = 0; i < nbk * 16; ++i) biome_column(seeds, bch.data(), bcol.data(), i);
+        stage_mark("biome_columns");
         for (int i = 0; i < nbk * kQPer; ++i) biome_target(seeds, bch.data(), bcol.data(), targets.data(), i);
         const int ntop = M.treef ? std::min(mctgen::kTreeNodes, 384) : 0;
         const int G = (M.treef && kOneVarying) ? M.col_groups[kVaryCoord] : 0;
@@ -218,9 +268,12 @@ int mcte_generate(void* p, int64_t seed, const int32_t* xz, int n, uint8_t* terr
         } else {
             for (int i = 0; i < nbk * 16; ++i) biome_search_column(M, M.treef, ntop, targets.data(), lstar.data(), mind.data(), i);
         }
+        stage_mark("biome_quarts");
         for (int i = 0; i < nbk * kQPer; ++i) biome_search_wide(M, targets.data(), lstar.data(), mind.data(), i);
         for (int i = 0; i < nbk * kQPer; ++i) biome_tie(M, targets.data(), lstar.data(), mind.data(), tie.data(), i);
+        stage_mark("biome_ties");
         for (int b = 0; b < nbk; ++b) biome_chain_seq(M, targets.data(), lstar.data(), mind.data(), bb.data(), b);
+        stage_mark("biome_chain");
         // corner columns and corners
         std::vector<uint64_t> ckeys(static_cast<size_t>(n) * kCols), ucols;
         std::vector<int> colmap;
@@ -230,45 +283,57 @@ int mcte_generate(void* p, int64_t seed, const int32_t* xz, int n, uint8_t* terr
         const int U = static_cast<int>(ucols.size());
         std::vector<float> colv(static_cast<size_t>(U) * (mctgen::kNumColumns > 0 ? mctgen::kNumColumns : 1));
         std::vector<float> corners(static_cast<size_t>(U) * kCh * kRows);
+        stage_mark("dedupe_columns");
         for (int u = 0; u < U; ++u) column_values(KC, seeds, ucols.data(), colv.data(), u);
+        stage_mark("columns");
         std::vector<mct::AxisXZ> tab(static_cast<size_t>(mctgen::kCornerSites > 0 ? mctgen::kCornerSites : 1));
         for (int u = 0; u < U; ++u) {
             for (int k = 0; k < mctgen::kCornerSites; ++k) tab[k] = corner_site_axes(KC, seeds, ucols.data(), u, k);
             for (int row = 0; row < kRows; ++row) corner_values(KC, seeds, ucols.data(), colv.data(), corners.data(), u, row, tab.data());
         }
+        stage_mark("corners");
         // surface-level grid and aquifer
         std::vector<uint64_t> gkeys(static_cast<size_t>(n) * kGridKeys), ug;
         std::vector<int> gmap;
         const KeyCodec KG = codec_for(ch, -16, 10, -8, 10);
         for (int i = 0; i < n * kGridKeys; ++i) grid_key(KG, ch.data(), gkeys.data(), i);
         dedupe(gkeys, ug, gmap);
+        stage_mark("dedupe_grid");
         std::vector<float> gv(ug.size());
         for (size_t u = 0; u < ug.size(); ++u) fts_value(KG, seeds, ug.data(), gv.data(), static_cast<int>(u));
+        stage_mark("fts");
         std::vector<int32_t> grid(static_cast<size_t>(n) * kGridCells);
         std::vector<float> prelim(static_cast<size_t>(n) * 4);
         for (int i = 0; i < n * kGridKeys; ++i) grid_scatter(gmap.data(), gv.data(), grid.data(), prelim.data(), i);
         std::vector<int> skip(static_cast<size_t>(n));
         for (int c = 0; c < n; ++c) aquifer_skip(grid.data(), skip.data(), c);
+        stage_mark("grid_scatter+aquifer_skip");
         std::vector<int32_t> cloc(static_cast<size_t>(n) * kAqCells * 4), cfl(static_cast<size_t>(n) * kAqCells * 2);
         for (int i = 0; i < n * kAqCells; ++i) aquifer_cell(seeds, ch.data(), grid.data(), cloc.data(), cfl.data(), i);
+        stage_mark("aquifer_cells");
         // fill
         std::vector<uint8_t> blocks(static_cast<size_t>(n) * kPer);
         FillArgs F{seeds, ch.data(), corners.data(), colmap.data(), cloc.data(), cfl.data(), skip.data(), blocks.data()};
         for (int c = 0; c < n; ++c)
             for (int t = 0; t < 256 * kCellsY; ++t) fill_cell(F, c, t, density ? density + static_cast<size_t>(c) * kPer : nullptr);
+        stage_mark("fill");
         if (aquifer) std::memcpy(aquifer, blocks.data(), blocks.size());
         const std::vector<uint8_t> filled = blocks;
         // surface
         std::vector<int> h0(static_cast<size_t>(n) * 256), h1(h0.size()), h2(h0.size()), hf(h0.size());
         for (int c = 0; c < n; ++c)
             for (int col = 0; col < 256; ++col) h0[c * 256 + col] = column_height(M, blocks.data() + static_cast<size_t>(c) * kPer, col >> 4, col & 15);
+        stage_mark("heights");
         std::vector<float> colrec(static_cast<size_t>(n) * 256 * kColF);
         for (int i = 0; i < n * 256; ++i) surface_column(seeds, M, ch.data(), colrec.data(), i);
+        stage_mark("surface_columns");
         SurfArgs A{seeds, &M, ch.data(), blocks.data(), nb.data(), bb.data(), corners.data(), colmap.data(), prelim.data(), colrec.data()};
         for (int c = 0; c < n; ++c)
             for (int col = 0; col < 256; ++col) surface_pass1(A, h0.data(), h1.data(), h2.data(), c, col);
+        stage_mark("surface1");
         for (int c = 0; c < n; ++c)
             for (int col = 0; col < 256; ++col) surface_pass2<false>(A, h0.data(), h1.data(), h2.data(), hf.data(), c, col);
+        stage_mark("surface2+ore_veins");
         int64_t redone = 0;
         for (int c = 0; c < n; ++c) {
             bool ok = true;
@@ -281,6 +346,7 @@ int mcte_generate(void* p, int64_t seed, const int32_t* xz, int n, uint8_t* terr
             for (int x = 0; x < 16; ++x)
                 for (int z = 0; z < 16; ++z) surface_exact_column(A, hm, c, x * 16 + z);
         }
+        stage_mark("surface_check+exact");
         if (stats) stats[0] = redone;
         if (terrain) std::memcpy(terrain, blocks.data(), blocks.size());
         if (biomes)

@@ -271,6 +271,7 @@ MCT_HD inline void seed_misc(SeedDev& S, int64_t seed, const ModelDev& M) {
 // ---- biomes ------------------------------------------------------------------------------
 MCT_HD MCT_FI int64_t quantize(float v) {
     const float q = v * 10000.0f;
+    MCT_OPS(0, 4, 0, 0);
     if (q != q) return 0;
     if (q >= 9.2233720368547758e18f) return INT64_MAX;
     if (q <= -9.2233720368547758e18f) return INT64_MIN;
@@ -280,6 +281,7 @@ MCT_HD MCT_FI int64_t quantize(float v) {
 // Squared distance of a target to a parameter box, in Java long arithmetic (wrapping).
 MCT_HD MCT_FI int64_t box_distance(const TreeNodeDev& n, const int64_t* t) {
     uint64_t d = 0;
+    MCT_OPS(0, 0, 7 * 6, 14);   // per axis: 2 differences, 2 compares, square, sum (64-bit)
     for (int i = 0; i < 7; ++i) {
         const int64_t above = static_cast<int64_t>(static_cast<uint64_t>(t[i]) - static_cast<uint64_t>(n.hi[i]));
         const int64_t below = static_cast<int64_t>(static_cast<uint64_t>(n.lo[i]) - static_cast<uint64_t>(t[i]));
@@ -310,6 +312,7 @@ MCT_HD inline int tree_first_min(const TreeNodeDev* T, const int64_t* t, int64_t
         const int k = stack_child[sp - 1];
         if (k >= n.count) { --sp; continue; }
         stack_child[sp - 1] = k + 1;
+        MCT_OPS(0, 0, 8, 2);
         const int c = n.first + k;
         const int64_t cd = box_distance(T[c], t);
         if (best > cd) {
@@ -334,6 +337,7 @@ MCT_HD inline int tree_first_min(const TreeNodeDev* T, const int64_t* t, int64_t
 // leaf found are those of the int64 search above.
 MCT_HD MCT_FI double box_distance_f(const TreeNodeF& n, const float* t) {
     double d = 0.0;
+    MCT_OPS(7 * 3, 7 * 4, 0, 14);   // per axis: 2 differences and 2 maxima (FP32), widen, square, sum (FP64)
     for (int i = 0; i < 7; ++i) {
         const float p = fmaxf(fmaxf(t[i] - n.hi[i], n.lo[i] - t[i]), 0.0f);
         const double pd = static_cast<double>(p);
@@ -384,6 +388,7 @@ MCT_HD inline int tree_first_min_f(const TreeNodeF* T, const TreeNodeF* top, int
         const int k = e & 31;
         if (k >= (fc >> 24)) { --sp; continue; }
         stack[sp - 1] = e + 1;
+        MCT_OPS(1, 0, 6, 1);   // the stack step, the child index, the bound test
         const int c = (fc & 0xFFFFFF) + k;
         const TreeNodeF& cn = node(c);
         const double cd = box_distance_f(cn, t);
@@ -426,6 +431,7 @@ MCT_HD inline int tree_first_min_group_f(const TreeNodeF* T, const TreeNodeF* to
         const int j = e & 31;
         if (j >= (fc >> 24)) { --sp; continue; }
         stack[sp - 1] = e + 1;
+        MCT_OPS(1, 2, 8, 3);   // the stack step, the child index, the group interval test, the bound test
         const int c = (fc & 0xFFFFFF) + j;
         const TreeNodeF& cn = node(c);
         const bool leaf = cn.leaf >= 0;
@@ -810,6 +816,7 @@ MCT_HD MCT_FI void column_values(const KeyCodec& K, const SeedDev* seeds, const 
     int32_t s, qx, qz;
     int f;
     split_key(K, ucols[u], s, f, qx, qz);
+    MCT_OPS(0, 0, 8, 1);   // the key read and split, x4 coordinates
     mctgen::eval_columns(seeds[s].oct, qx * 4, qz * 4, colv + static_cast<size_t>(u) * (mctgen::kNumColumns > 0 ? mctgen::kNumColumns : 1));
 }
 
@@ -828,6 +835,7 @@ MCT_HD MCT_FI void corner_values(const KeyCodec& K, const SeedDev* seeds, const 
     int32_t s, qx, qz;
     int f;
     split_key(K, ucols[u], s, f, qx, qz);
+    MCT_OPS(0, 0, 9, 1);   // the key read and split, x4 coordinates and the corner's y
     float out[kCh];
     mctgen::eval_corner(seeds[s].oct, qx * 4, kMinY + row * 8, qz * 4, colv + static_cast<size_t>(u) * (mctgen::kNumColumns > 0 ? mctgen::kNumColumns : 1), axz,
                         out);
@@ -869,6 +877,7 @@ MCT_HD MCT_FI void fts_value(const KeyCodec& K, const SeedDev* seeds, const uint
     int32_t s, qx, qz;
     int vol;
     split_key(K, ukeys[u], s, vol, qx, qz);
+    MCT_OPS(0, 0, 8, 1);
     const Octave* O = seeds[s].oct;
     vals[u] = vol ? mctgen::fts_volume(O, qx * 4, qz * 4) : mctgen::fts_point(O, qx * 4, qz * 4);
 }
@@ -990,9 +999,12 @@ MCT_HD inline void aquifer_cell(const SeedDev* seeds, const ChunkRef* ch, const 
 
 MCT_HD MCT_FI double aquifer_pressure(const Octave* O, int x, int y, int z, double& barrier, bool& barrier_set, Fluid a, Fluid b) {
     const int t1 = fluid_at(a, y), t2 = fluid_at(b, y);
+    MCT_OPS(0, 0, 6, 0);   // two fluid picks, the lava / water pair test
     if ((t1 == mctgen::kLava && t2 == mctgen::kWater) || (t1 == mctgen::kWater && t2 == mctgen::kLava)) return 2.0;
     const int diff = a.level - b.level < 0 ? b.level - a.level : a.level - b.level;
+    MCT_OPS(0, 0, 4, 0);
     if (diff == 0) return 0.0;
+    MCT_OPS(14, 0, 1, 0);   // average, above, base, towards, the gradient branch and the range test
     const double average = 0.5 * static_cast<double>(a.level + b.level);
     const double above = static_cast<double>(y) + 0.5 - average;
     const double base = static_cast<double>(diff) / 2.0;
@@ -1015,6 +1027,7 @@ MCT_HD MCT_FI double aquifer_pressure(const Octave* O, int x, int y, int z, doub
     } else {
         noise = 0.0;
     }
+    MCT_OPS(2, 0, 0, 0);
     return 2.0 * (noise + gradient);
 }
 
@@ -1047,12 +1060,15 @@ struct AqNearest {
 template <class Cells>
 MCT_HD MCT_FI AqNearest aquifer_nearest(const Cells& C, int cx, int cz, int x, int y, int z) {
     const int xa = (x - 5) >> 4, ya = grid_yf(y + 1), za = (z - 5) >> 4;
+    MCT_OPS(0, 0, 7, 0);
     int d1 = INT32_MAX, d2 = INT32_MAX, d3 = INT32_MAX, d4 = INT32_MAX;
     int i1 = 0, i2 = 0, i3 = 0;
     for (int ox = 0; ox <= 1; ++ox)
         for (int oy = -1; oy <= 1; ++oy)
             for (int oz = 0; oz <= 1; ++oz) {
                 const int gx = xa + ox, gy = ya + oy, gz = za + oz;
+                // cell index (9), offsets and squared distance (8), the insertion compares (4)
+                MCT_OPS(0, 0, 21, 3);
                 const int index = ((gy - kAqMinGY) * 3 + (gz - (cz - 1))) * 3 + (gx - (cx - 1));
                 int lx, ly, lz;
                 C.at(index, lx, ly, lz);
@@ -1073,13 +1089,17 @@ MCT_HD MCT_FI AqNearest aquifer_nearest(const Cells& C, int cx, int cz, int x, i
 constexpr int kAqPending = -2;
 template <class Cells>
 MCT_HD MCT_FI int aquifer_front(const Cells& C, int cx, int cz, int skip_above, int x, int y, int z, double density, AqNearest& nr) {
+    MCT_OPS(1, 0, 0, 0);
     if (density > 0.0) return -1;
     const Fluid global = global_fluid(y);
+    MCT_OPS(0, 0, 4, 0);   // the global fluid pick, the skip test
     if (y > skip_above) return fluid_at(global, y);
     if (fluid_at(global, y) == mctgen::kLava) return mctgen::kLava;
     nr = aquifer_nearest(C, cx, cz, x, y, z);
     const int fluid = fluid_at(C.fluid(nr.i1), y);
+    MCT_OPS(0, 0, 4, 2);   // the nearest cell's fluid, the d2 - d1 test
     if (nr.d2 - nr.d1 >= 25) return fluid;
+    MCT_OPS(0, 0, 4, 0);
     if (fluid == mctgen::kWater && fluid_at(global_fluid(y - 1), y - 1) == mctgen::kLava) return fluid;
     return kAqPending;
 }
@@ -1091,6 +1111,7 @@ MCT_HD inline int aquifer_back(const Octave* O, const Cells& C, int x, int y, in
     const Fluid s1 = C.fluid(nr.i1);
     const int fluid = fluid_at(s1, y);
     const double sim12 = 1.0 - static_cast<double>(nr.d2 - nr.d1) / 25.0;
+    MCT_OPS(5, 0, 2, 4);   // sim12, b12 and its test; the fluids of cells 1 and 2
     double barrier = 0.0;
     bool barrier_set = false;
     const Fluid s2 = C.fluid(nr.i2);
@@ -1098,12 +1119,16 @@ MCT_HD inline int aquifer_back(const Octave* O, const Cells& C, int x, int y, in
     if (density + b12 > 0.0) return -1;
     const Fluid s3 = C.fluid(nr.i3);
     const double sim13 = 1.0 - static_cast<double>(nr.d3 - nr.d1) / 25.0;
+    MCT_OPS(4, 0, 1, 2);   // sim13 and its test; the fluid of cell 3
     if (sim13 > 0.0) {
+        MCT_OPS(4, 0, 0, 0);
         const double b13 = sim12 * sim13 * aquifer_pressure(O, x, y, z, barrier, barrier_set, s1, s3);
         if (density + b13 > 0.0) return -1;
     }
     const double sim23 = 1.0 - static_cast<double>(nr.d3 - nr.d2) / 25.0;
+    MCT_OPS(4, 0, 1, 0);
     if (sim23 > 0.0) {
+        MCT_OPS(4, 0, 0, 0);
         const double b23 = sim12 * sim23 * aquifer_pressure(O, x, y, z, barrier, barrier_set, s2, s3);
         if (density + b23 > 0.0) return -1;
     }
@@ -1146,6 +1171,9 @@ MCT_HD MCT_FI void cell_channels(const FillArgs& A, int c, int x, int z, int cy,
     const int cx = x >> 2, xi = x & 3, cz = z >> 2, zi = z & 3;
     const size_t u00 = cmap[cz * 5 + cx], u10 = cmap[cz * 5 + cx + 1], u01 = cmap[(cz + 1) * 5 + cx], u11 = cmap[(cz + 1) * 5 + cx + 1];
     const float az = static_cast<float>(zi) * 0.25f, ax = static_cast<float>(xi) * 0.25f;
+    // per column of a cell: the 4 corner columns, the lerp factors; per channel 8 corner values,
+    // 6 lerps and the y step
+    MCT_OPS(0, 4 + 20 * mctgen::kNumChannels, 12, 4 + 8 * mctgen::kNumChannels);
     for (int k = 0; k < mctgen::kNumChannels; ++k) {
         const float* a = A.corners + (u00 * kCh + k) * kRows + cy;
         const float* b = A.corners + (u10 * kCh + k) * kRows + cy;
@@ -1180,6 +1208,7 @@ MCT_HD MCT_FI void fill_cell(const FillArgs& A, int c, int t, float* density_out
         const int state = s < 0 ? mctgen::kDefaultBlock : s;
         out[yi * 256] = static_cast<uint8_t>(state);
         if (!((mctgen::kAirMask >> state) & 1u)) top = yi;
+        MCT_OPS(0, mctgen::kNumChannels, 4, 0);   // the block's state and air test, the channel y steps
         for (int k = 0; k < mctgen::kNumChannels; ++k) val[k] += step[k];
     }
 #if defined(__HIP_DEVICE_COMPILE__)

@@ -5,6 +5,13 @@ a GPU (the reference's "gen-host" implementation, tests/cuda_impls.cpp:9-15). It
 fallback of the library: libmct.so has no CPU path.
 
   HostGenerator().generate(seed, chunks, threads=8) -> (terrain u8 [n, 384, 16, 16], biomes u8 [n, 96, 4, 4])
+  op_counts(seed, chunks) -> algorithmic operations per stage and per stage unit (libmct_count.so)
+
+Operation counts: the code generator gives every scope of the generated evaluators and every
+noise octave a static count of its source operations by class (FP64, FP32, INT32, memory
+reads; csrc/gen/opcounts.h), and the hand-written stage bodies carry the same hooks. The
+counting build adds a scope's counts each time the scope runs, so a stage's total is the sum
+over scopes of static count x invocations, with the invocations of the actual chunks.
 """
 import ctypes
 import os
@@ -14,7 +21,10 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, 'libmct_host.so')
+COUNT_PATH = os.path.join(HERE, 'libmct_count.so')
 _lib = None
+_count_lib = None
+OP_CLASSES = ('f64', 'f32', 'i32', 'mem')
 
 
 def lib():
@@ -22,15 +32,108 @@ def lib():
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             raise RuntimeError('libmct_host.so is not built (run build.py): %s' % LIB_PATH)
-        L = ctypes.CDLL(LIB_PATH)
-        P, I, I64 = ctypes.c_void_p, ctypes.c_int, ctypes.c_int64
-        L.mcte_create.restype = P
-        L.mcte_destroy.argtypes = [P]
-        L.mcte_last_error.restype = ctypes.c_char_p
-        L.mcte_generate.argtypes = [P, I64, P, I, P, P, P, P, P]
-        L.mcte_biome_query.argtypes = [P, I64, P, I, P]
-        _lib = L
+        _lib = _bind(ctypes.CDLL(LIB_PATH))
     return _lib
+
+
+def _bind(L):
+    P, I, I64 = ctypes.c_void_p, ctypes.c_int, ctypes.c_int64
+    L.mcte_create.restype = P
+    L.mcte_destroy.argtypes = [P]
+    L.mcte_last_error.restype = ctypes.c_char_p
+    L.mcte_generate.argtypes = [P, I64, P, I, P, P, P, P, P]
+    L.mcte_biome_query.argtypes = [P, I64, P, I, P]
+    L.mcte_stage_ops.argtypes = [I, ctypes.POINTER(ctypes.c_char_p), P]
+    return L
+
+
+def count_lib():
+    global _count_lib
+    if _count_lib is None:
+        if not os.path.exists(COUNT_PATH):
+            raise RuntimeError('libmct_count.so is not built (run build.py): %s' % COUNT_PATH)
+        L = _bind(ctypes.CDLL(COUNT_PATH))
+        if L.mcte_counts_ops() != 1:
+            raise RuntimeError('%s was built without operation counting' % COUNT_PATH)
+        _count_lib = L
+    return _count_lib
+
+
+# What each stage runs over: its operations per launch are ops per unit x units per launch.
+STAGE_UNIT = {
+    'biome_columns': 'biome_chunks', 'biome_quarts': 'biome_chunks', 'biome_ties': 'biome_chunks', 'biome_chain': 'biome_chunks',
+    'dedupe_columns': 'chunks', 'columns': 'columns', 'corners': 'columns', 'dedupe_grid': 'chunks', 'fts': 'grid_keys',
+    'grid_scatter+aquifer_skip': 'chunks', 'aquifer_cells': 'chunks', 'fill': 'chunks', 'heights': 'chunks',
+    'surface_columns': 'chunks', 'surface1': 'chunks', 'surface2+ore_veins': 'chunks', 'surface_check+exact': 'chunks',
+}
+
+
+def stage_units(seeds, chunks):
+    """Units of one batch: chunks, distinct biome chunks (3 x 3 neighbourhoods), distinct
+    corner columns (5 x 5 quart columns per chunk) and distinct surface-level grid keys
+    (27 x 19 quarts around each chunk, the 11 x 11 prefill window on the volume path, and the
+    4 chunk corners on the volume path), each per seed, as the batch's dedupe finds them.
+    Quart coordinates stay within +-2^20 for chunks within the world border."""
+    xz = np.asarray(chunks, np.int64).reshape(-1, 2)
+    s = np.broadcast_to(np.asarray(seeds, np.int64), (len(xz),)).astype(np.int64)
+    cx, cz = xz[:, 0], xz[:, 1]
+
+    _, sidx = np.unique(s, return_inverse=True)
+
+    def distinct(a, b, flag=None, lead=1):
+        # one int64 key per position: seed index, x and z offset to 21 bits each, the flag
+        si = np.repeat(sidx.astype(np.int64), a.size // len(xz))
+        key = (si << 43) | ((a.ravel() + (1 << 20)) << 22) | ((b.ravel() + (1 << 20)) << 1)
+        if flag is not None:
+            key |= flag.ravel()
+        return len(np.unique(key))
+    d = np.arange(-1, 2)
+    bx = (cx[:, None, None] + d[None, None, :]).repeat(3, 1)
+    bz = (cz[:, None, None] + d[None, :, None]).repeat(3, 2)
+    k = np.arange(25)
+    qx = cx[:, None] * 4 + k % 5
+    qz = cz[:, None] * 4 + k // 5
+    e = np.arange(27 * 19)
+    gx = 4 * cx[:, None] - 16 + e % 27
+    gz = 4 * cz[:, None] - 8 + e // 27
+    vol = (gx >= 4 * cx[:, None] - 4) & (gx <= 4 * cx[:, None] + 6) & (gz >= 4 * cz[:, None] - 4) & (gz <= 4 * cz[:, None] + 6)
+    j = np.arange(4)
+    gx = np.concatenate([gx, 4 * cx[:, None] + 4 * (j & 1)], 1)
+    gz = np.concatenate([gz, 4 * cz[:, None] + 4 * (j >> 1)], 1)
+    vol = np.concatenate([vol, np.ones((len(xz), 4), bool)], 1)
+    return {'chunks': len(xz),
+            'biome_chunks': distinct(bx, bz),
+            'columns': distinct(qx, qz),
+            'grid_keys': distinct(gx, gz, vol.astype(np.int64))}
+
+
+def op_counts(seed, chunks):
+    """Algorithmic operations of the TERRAIN step on these chunks under one seed, per stage:
+    {stage: {'unit', 'units', 'ops': {class: total}, 'per_unit': {class: ops / unit}}}, from
+    one run of the counting emulation (single thread; about 0.1 s per chunk)."""
+    L = count_lib()
+    xz = np.ascontiguousarray(np.asarray(chunks, np.int32).reshape(-1, 2))
+    n = len(xz)
+    h = L.mcte_create()
+    try:
+        t = np.zeros((n, 384, 16, 16), np.uint8)
+        b = np.zeros((n, 96, 4, 4), np.uint8)
+        st = np.zeros(4, np.int64)
+        if L.mcte_generate(h, int(seed), xz.ctypes.data, n, t.ctypes.data, b.ctypes.data, None, None, st.ctypes.data) != 0:
+            raise RuntimeError('host emulation: ' + L.mcte_last_error().decode())
+    finally:
+        L.mcte_destroy(h)
+    units = stage_units(np.full(n, seed, np.int64), xz)
+    out = {}
+    name = ctypes.c_char_p()
+    ops = np.zeros(4, np.uint64)
+    for i in range(L.mcte_stage_ops(0, None, None)):
+        L.mcte_stage_ops(i, ctypes.byref(name), ops.ctypes.data)
+        stage = name.value.decode()
+        unit = STAGE_UNIT[stage]
+        tot = {c: int(v) for c, v in zip(OP_CLASSES, ops)}
+        out[stage] = {'unit': unit, 'units': units[unit], 'ops': tot, 'per_unit': {c: v / units[unit] for c, v in tot.items()}}
+    return out
 
 
 class HostGenerator:

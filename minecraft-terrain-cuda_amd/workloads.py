@@ -13,8 +13,12 @@ inputs everywhere.
   stress             1,024 seeds, 2^20 chunks uniform over +-117,187 (wrap chunks included)
   queries            biome-only block queries: seed from the stress seeds, x / z uniform over
                      the blocks of +-117,187 chunks, y uniform in -64..319
+  fold_chunks        one chunk on each side of the noise lattice's wrap fold for every distinct
+                     horizontal scale x frequency product of the IR's noise sampling
+  border_region      the reference's 8 x 8 border verify region at chunk (1874992, 1874992)
 """
 import math
+import os
 
 MASK = (1 << 64) - 1
 SEED = 123456789              # the reference bench's seed (bench/bench_main.cpp:50)
@@ -62,6 +66,55 @@ class SplitMix64:
 
     def between(self, lo, hi):    # [lo, hi]
         return lo + self.below(hi - lo + 1)
+
+
+BORDER_VERIFY = (1874992, 8)   # bench/run_all.py:27: x0 = z0 = 1874992, size 8
+FOLD = float(1 << 24)          # GradientNoise.wrap folds a lattice coordinate at |d| = 2^24
+
+
+def fold_products(ir_path=None):
+    """Distinct horizontal products (block coordinate -> lattice coordinate) of the IR's noise
+    sampling: xz_scale x octave frequency for every noise node (both NormalNoise forks), 1/4 x
+    frequency for the offset noise of shift_a / shift_b, and BlendedNoise's 684.412 x xz_scale
+    x octave factor for its limit noises and / xz_factor for its main noise."""
+    import json
+    import sys
+    here = os.path.dirname(os.path.abspath(__file__))
+    sys.path.insert(0, os.path.join(here, 'codegen'))
+    import model
+    with open(ir_path or os.path.join(here, 'data', 'synthetic_overworld_ir.json')) as fh:
+        ir = json.load(fh)
+    out = set()
+    for n in ir['nodes']:
+        if n['t'] in ('noise', 'shift_a', 'shift_b'):
+            s = model.f64h(n['xz_scale']) if n['t'] == 'noise' else 0.25
+            for _, fr, _ in model.normal_noise_layers(ir['noises'][n['noise']]):
+                out.update((s * fr, s * fr * model.SECOND_FREQ))
+        elif n['t'] == 'blended_noise':
+            xz = 684.412 * model.f64h(n['xz_scale'])
+            for first, div in ((-15, 1.0), (-7, model.f64h(n['xz_factor']))):
+                out.update(xz / div * f for f, _, _ in model.fbm_layers(first, 0.0, 1.0))
+    return sorted(out)
+
+
+def fold_chunks(limit=1874999):
+    """For each fold product p whose fold lies within chunk +-limit: the chunk holding block
+    2^24 / p on the +x axis and the chunk holding block -2^24 / p on the -z axis (the other
+    coordinate a small offset, so the chunks are distinct)."""
+    out = []
+    for i, p in enumerate(fold_products()):
+        b = FOLD / p
+        c = math.floor(b / 16.0)
+        if c > limit:
+            continue
+        out.append((c, i % 7))
+        out.append((i % 5 - 2, math.floor(-b / 16.0)))
+    return out
+
+
+def border_region():
+    x0, w = BORDER_VERIFY
+    return region(x0, x0, w)
 
 
 def region(x0, z0, w, d=None):

@@ -2,7 +2,7 @@
 the reference cpu/ port compiled from its own sources (`make -C oracle ref`,
 oracle/_ref/libmcref.so). Runs in the build container only; what it writes is data.
 
-  python tests/golden/make_digests.py [--threads T] [--only c3,c4,c5q,c5]
+  python tests/golden/make_digests.py [--threads T] [--only c3,c4,c5q,c5,edge]
 
 Every digest is the library's 64-bit chunk digest (the sum over 8-byte words of a SplitMix64
 mix, as `mct_chunk_digests` computes it on the device), so a GPU test compares HBM outputs
@@ -19,6 +19,10 @@ with these files without copying the outputs to the host.
                    likewise for biomes, and group g of QGROUP queries digest(its answer bytes).
                    A GPU run that folds its own per-chunk digests the same way matches every
                    group exactly when every chunk matches (up to 64-bit collisions).
+  edge_digests.npz the edge set: a chunk on each side of the noise wrap fold for every
+                   distinct horizontal scale x frequency product of the IR
+                   (workloads.fold_chunks(), under the bench seed and under seed 1) and the
+                   reference's 8 x 8 border verify region at chunk (1874992, 1874992)
 
 The 2^20-chunk pass takes about 45,000 CPU-seconds; partial results are kept under
 --state (default /tmp/mct_digest_state) so an interrupted run resumes.
@@ -93,6 +97,17 @@ def c4(R, threads):
     print('c4: %d requests, %d chunks, %.0f s' % (len(reqs), len(xz), time.time() - t0), flush=True)
 
 
+def edge(R, threads):
+    t0 = time.time()
+    folds = workloads.fold_chunks()
+    border = workloads.border_region()
+    seeds = np.array([workloads.SEED] * len(folds) + [1] * len(folds) + [workloads.SEED] * len(border), np.int64)
+    xz = np.array(folds + folds + border, np.int32)
+    dt, db = R.digest_terrain(seeds, xz, threads=threads)
+    np.savez_compressed(os.path.join(HERE, 'edge_digests.npz'), seed=seeds, xz=xz, terrain=dt, biomes=db)
+    print('edge: %d chunks, %.0f s' % (len(xz), time.time() - t0), flush=True)
+
+
 def c5_queries(R, threads, state):
     path = os.path.join(state, 'c5_queries.npy')
     if os.path.exists(path):
@@ -136,7 +151,7 @@ def c5_chunks(R, threads, state, slice_seeds=32):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument('--threads', type=int, default=os.cpu_count())
-    ap.add_argument('--only', default='c3,c4,c5q,c5')
+    ap.add_argument('--only', default='c3,c4,c5q,c5,edge')
     ap.add_argument('--state', default='/tmp/mct_digest_state')
     a = ap.parse_args()
     parts = set(a.only.split(','))
@@ -146,6 +161,8 @@ def main():
         c3(R, a.threads)
     if 'c4' in parts:
         c4(R, a.threads)
+    if 'edge' in parts:
+        edge(R, a.threads)
     if 'c5' in parts:
         dt, db = c5_chunks(R, a.threads, a.state)
     ans = c5_queries(R, a.threads, a.state) if ('c5q' in parts or 'c5' in parts) else None

@@ -136,3 +136,17 @@ def test_plain_workload_at_64_lane_shapes_matches_reference(mct):
     dig = lambda a: np.frombuffer(hashlib.blake2b(np.ascontiguousarray(a).tobytes(), digest_size=8).digest(), np.uint8)
     bad = [i for i in range(n) if not ((dig(out[i]) == f['terrain'][i]).all() and (dig(bout[i]) == f['biomes'][i]).all())]
     assert not bad, ('mismatching chunks', len(bad), xz[bad[:8]].tolist())
+
+
+def test_edge_set_every_chunk_matches_reference(mct):
+    """The edge set: a chunk on each side of the noise wrap fold for every distinct horizontal
+    scale x frequency product of the IR (under two seeds), and the reference's 8 x 8 border
+    verify region at chunk (1874992, 1874992) (bench/run_all.py:27)."""
+    f = load('edge_digests.npz')
+    folds, border = workloads.fold_chunks(), workloads.border_region()
+    assert f['xz'].tolist() == [list(c) for c in folds + folds + border]
+    log('edge set: fold chunks', len(folds), 'x 2 seeds, border region', len(border))
+    for batch in (256, 64):
+        dt, db = device_digests(mct, f['seed'], f['xz'], max_batch_chunks=batch, pipeline_slots=2, seed_capacity=4)
+        bad_t, bad_b = mismatches(dt, f['terrain']), mismatches(db, f['biomes'])
+        assert len(bad_t) == 0 and len(bad_b) == 0, ('batch', batch, 'terrain', f['xz'][bad_t[:8]].tolist(), 'biomes', f['xz'][bad_b[:8]].tolist())

@@ -47,3 +47,17 @@ def test_stress_sample_and_queries(host):
     for i in sel:
         got = host.biome_query(int(g['query_seed'][i]), g['query_xyz'][i:i + 1])
         assert got[0] == g['query_biome'][i], i
+
+
+def test_operation_counts_add_over_independent_chunks():
+    """The counting build: per-chunk stages (fill, aquifer cells, surface columns) count the
+    same operations for a chunk alone as inside a batch, the deduplicated stages add over
+    chunks that share no position, and the plain build counts nothing."""
+    far = [(0, 0), (5000, -7000)]
+    both = gen_host.op_counts(123456789, far)
+    one = [gen_host.op_counts(123456789, [c]) for c in far]
+    for st in ('fill', 'aquifer_cells', 'surface_columns', 'corners', 'columns', 'fts', 'biome_columns', 'biome_quarts'):
+        for c in gen_host.OP_CLASSES:
+            assert both[st]['ops'][c] == one[0][st]['ops'][c] + one[1][st]['ops'][c], (st, c)
+    assert both['fill']['ops']['f32'] > 0 and both['corners']['ops']['f64'] > 0
+    assert gen_host.lib().mcte_counts_ops() == 0
