@@ -160,7 +160,9 @@ int mct_biome_query(mct_generator* g, const int64_t* seeds, const int32_t* xyz, 
 
 /* zoom_selftest (terrain_gpu.h:57-59): the column-cached zoom that the surface pass uses against
  * the plain FP64 BiomeManager zoom, on about `count` blocks (whole columns at pseudo-random
- * x, z in +-29,999,984, every y). out: blocks checked, mismatches, quart-row refills. */
+ * x, z in +-29,999,984, every y). out: blocks checked, mismatches, near ties (the reference's
+ * "fallbacks": picks with a second candidate within 8e-6 of the minimal fiddled distance, which
+ * the reference's FP32 filter hands to FP64; here every distance is FP64). */
 int mct_zoom_selftest(int64_t world_seed, int64_t count, int64_t out[3]);
 
 /* ---- Serving (no counterpart in the reference, which serves one synchronous call at a time) ----

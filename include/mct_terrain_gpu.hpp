@@ -119,7 +119,7 @@ private:
 };
 
 struct ZoomSelfTest {     // terrain_gpu.h:56-58 (field names kept)
-    int64_t checked, mismatches, fallbacks;   // fallbacks: here the column cache's quart-row refills (no FP32 filter)
+    int64_t checked, mismatches, fallbacks;   // fallbacks: near ties (a second candidate within 8e-6), resolved in FP64
 };
 
 // The surface pass's column-cached zoom against the plain FP64 zoom on about `count` blocks.

@@ -11,11 +11,11 @@
 // written, the core or full configuration and the launch-shape tuning. Timed batches (kernel
 // or stage times asked for) run the same program launch by launch with events around groups.
 //
-// Lanes: each lane is a compute stream with its own device buffers, staging block and graphs;
-// batches go to the lanes in turn, so one lane's batch computes while the other's results are
-// copied. Copies to host memory run on the lane's copy stream after the batch's completion
-// event (the lanes' copies can run at once, on separate copy engines); a lane's next batch
-// waits for its copy before the buffers are reused.
+// Lanes: each lane is a stream with its own device buffers, staging block and graphs; batches
+// go to the lanes in turn, so one lane's batch computes while the other's results are copied.
+// Copies to host memory follow the batch's program on the lane's stream, so a lane's next
+// batch starts once its buffers are free. A call whose results go to host memory is cut
+// into smaller batches, so that the copies overlap the kernels.
 //
 // Seed tables: seed_capacity resident tables in HBM, least recently used replaced. A batch
 // lists the tables it needs built as jobs, and its program builds them first. A table is only
@@ -158,8 +158,8 @@ struct Shape {
     }
 };
 
-// Where a batch's results go: device pointers the kernels write, or host pointers the copy
-// stream fills from the lane's buffers.
+// Where a batch's results go: device pointers the kernels write, or host pointers the lane's
+// copies fill from its buffers.
 struct BatchOut {
     uint8_t *terrain = nullptr, *biomes = nullptr, *aquifer = nullptr;
     float* density = nullptr;

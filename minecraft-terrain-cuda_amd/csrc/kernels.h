@@ -696,8 +696,11 @@ __global__ __launch_bounds__(256) void k_biome_query(const SeedDev* seeds, const
     out[i] = static_cast<uint8_t>(b < 0 ? 255 : b);
 }
 
-// The column-cached zoom of the surface pass against zoom_pick: one pseudo-random column per
-// thread (x, z uniform in +-29,999,984), every y. out: checked, mismatches, row refills.
+// The column-cached zoom of the surface pass (ZoomColumn) against the plain FP64 zoom
+// (zoom_pick): one pseudo-random column per thread (x, z uniform in +-29,999,984), every y.
+// out: blocks checked, mismatches, near ties (picks with another candidate within 8e-6 of the
+// minimal distance, which an FP32 distance filter cannot decide; this path computes every
+// distance in FP64, so a near tie needs no fallback here).
 __global__ __launch_bounds__(256) void k_zoom_selftest(int64_t zoom_seed, uint64_t salt, int columns, unsigned long long* out) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= columns) return;
@@ -709,16 +712,17 @@ __global__ __launch_bounds__(256) void k_zoom_selftest(int64_t zoom_seed, uint64
     const int z = static_cast<int>((h >> 32) % 59999969ull) - 29999984;
     ZoomColumn zc;
     zc.init(zoom_seed, x, z);
-    unsigned long long bad = 0;
+    unsigned long long bad = 0, ties = 0;
     for (int y = kMinY + kHeight - 1; y >= kMinY; --y) {
         int a, b, c, qa, qb, qc;
         zc.pick(y, a, b, c);
         zoom_pick(zoom_seed, x, y, z, qa, qb, qc);
         bad += (a != qa || b != qb || c != qc) ? 1u : 0u;
+        ties += zc.candidates(y, 8.0e-6) > 1 ? 1u : 0u;
     }
     atomicAdd(out, static_cast<unsigned long long>(kHeight));
     if (bad) atomicAdd(out + 1, bad);
-    atomicAdd(out + 2, static_cast<unsigned long long>(zc.refills));
+    if (ties) atomicAdd(out + 2, ties);
 }
 
 // Top-down record of each block column of n chunks (one thread per column, t = z * 16 + x):

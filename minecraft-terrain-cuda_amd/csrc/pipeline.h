@@ -1374,7 +1374,6 @@ struct ZoomColumn {
     int64_t seed;
     int px, pz, fx, fz, py = INT32_MIN;
     uint32_t zc[8];
-    int refills = 0;
 
     MCT_HD MCT_FI void init(int64_t zoom_seed, int x, int z) {
         seed = zoom_seed;
@@ -1403,7 +1402,6 @@ struct ZoomColumn {
                 layer(ry + 1, zc + 4);
             }
             py = ry;
-            ++refills;
         }
         int best = 0;
         double best_d = u2d(0x7FF0000000000000ull);
@@ -1420,6 +1418,21 @@ struct ZoomColumn {
         qx = (best & 4) ? px + 1 : px;
         qy = (best & 2) ? ry + 1 : ry;
         qz = (best & 1) ? pz + 1 : pz;
+    }
+    // Candidates of the last pick within `margin` of its distance (1 unless a near tie): the
+    // picks an FP32 distance filter with that margin could not decide.
+    MCT_HD MCT_FI int candidates(int y, double margin) const {
+        const int fy = (y - 2) & 3;
+        double d[8], dmin = u2d(0x7FF0000000000000ull);
+        for (int i = 0; i < 8; ++i) {
+            const bool xf = (i & 4) != 0, yf = (i & 2) != 0, zf = (i & 1) != 0;
+            const uint32_t pk = zc[(yf ? 4 : 0) + (xf ? 2 : 0) + (zf ? 1 : 0)];
+            d[i] = (zoom_sq(fz, zf, static_cast<int>(pk >> 20)) + zoom_sq(fy, yf, static_cast<int>((pk >> 10) & 1023u))) + zoom_sq(fx, xf, static_cast<int>(pk & 1023u));
+            dmin = d[i] < dmin ? d[i] : dmin;
+        }
+        int n = 0;
+        for (int i = 0; i < 8; ++i) n += d[i] <= dmin + margin ? 1 : 0;
+        return n;
     }
 };
 

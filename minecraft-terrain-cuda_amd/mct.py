@@ -253,7 +253,9 @@ class FullTerrainGenerator:
 
 
 def zoom_selftest(world_seed, count):
-    """(checked, mismatches, refills) of the column-cached zoom against the plain FP64 zoom."""
+    """(checked, mismatches, near_ties) of the column-cached zoom against the plain FP64 zoom;
+    near ties are picks with a second candidate within 8e-6 of the minimal distance (the
+    reference's FP64 fallbacks, terrain_gpu.h:56-59)."""
     out = np.zeros(3, np.int64)
     _check(lib().mct_zoom_selftest(int(world_seed), int(count), _ptr(out)))
     return tuple(int(v) for v in out)

@@ -144,6 +144,6 @@ def test_biome_query_rejects_coordinates_beyond_range(mct):
 
 def test_cached_zoom_matches_plain_zoom(mct):
     for seed in (123456789, -1, 0):
-        checked, bad, refills = mct.zoom_selftest(seed, 1 << 22)
-        log('zoom selftest', seed, checked, bad, refills)
-        assert checked >= 1 << 22 and bad == 0
+        checked, bad, ties = mct.zoom_selftest(seed, 1 << 22)
+        log('zoom selftest', seed, checked, bad, 'near ties', ties)
+        assert checked >= 1 << 22 and bad == 0 and 0 <= ties < checked // 100
