@@ -167,6 +167,8 @@ struct BatchOut {
 };
 
 // Device memory of one lane, sized for `cap` chunks (biome chunks: at most 9 per chunk).
+constexpr size_t kDescBytes = 1024;   // the batch descriptor's slot at the start of the staging block
+
 struct LaneMemory {
     int cap;
     size_t bytes = 0;
@@ -200,10 +202,10 @@ struct LaneMemory {
     int *dva = nullptr, *dvb = nullptr, *dflag = nullptr, *dhist = nullptr, *dhsum = nullptr, *dtsum = nullptr, *dtotal = nullptr;
 
     LaneMemory(int c, int seed_cap) : cap(c) {
-        jobs_off = 256;
+        jobs_off = kDescBytes;
         chunks_off = jobs_off + ((sizeof(SeedJob) * static_cast<size_t>(seed_cap) + 255) & ~size_t{255});
         block_bytes = chunks_off + sizeof(ChunkRef) * static_cast<size_t>(cap);
-        static_assert(sizeof(BatchDesc) <= 256, "the batch descriptor fits its 256-byte slot");
+        static_assert(sizeof(BatchDesc) <= kDescBytes, "the batch descriptor fits its slot of the staging block");
         hip_check(hipHostMalloc(reinterpret_cast<void**>(&h_block), block_bytes, hipHostMallocDefault), "pinned staging");
         std::memset(h_block, 0, block_bytes);
         M(&d_block, block_bytes);
@@ -598,27 +600,13 @@ private:
     static int passes(int bits) { return (bits + 7) / 8; }
 
     // Position-key codec of a batch's chunks: x / z in units of `unit` per chunk extended by
-    // [lo, hi] units, seed slots. 64-bit intermediate arithmetic (chunk coordinates are int32).
+    // [lo, hi] units, seed slots (make_codec).
     KeyCodec codec(const ChunkRef* ch, int n, int unit, int xlo, int xhi, int zlo, int zhi) const {
-        int64_t mnx = INT64_MAX, mxx = INT64_MIN, mnz = INT64_MAX, mxz = INT64_MIN, mns = INT64_MAX, mxs = INT64_MIN;
-        for (int i = 0; i < n; ++i) {
-            mnx = std::min<int64_t>(mnx, ch[i].x); mxx = std::max<int64_t>(mxx, ch[i].x);
-            mnz = std::min<int64_t>(mnz, ch[i].z); mxz = std::max<int64_t>(mxz, ch[i].z);
-            mns = std::min<int64_t>(mns, ch[i].seed); mxs = std::max<int64_t>(mxs, ch[i].seed);
+        try {
+            return make_codec(ch, n, unit, xlo, xhi, zlo, zhi);
+        } catch (const std::range_error& e) {
+            throw Error(MCT_EINVAL, e.what());
         }
-        auto bits = [](int64_t range) { int b = 0; while ((int64_t{1} << b) < range) ++b; return b; };
-        const int64_t x0 = unit * mnx + xlo, z0 = unit * mnz + zlo;
-        if (x0 < INT32_MIN || z0 < INT32_MIN || unit * mxx + xhi > INT32_MAX || unit * mxz + zhi > INT32_MAX)
-            throw Error(MCT_EINVAL, "chunk coordinates outside the range the pipeline's position keys hold");
-        KeyCodec k{};
-        k.x0 = static_cast<int32_t>(x0);
-        k.z0 = static_cast<int32_t>(z0);
-        k.s0 = static_cast<int32_t>(mns);
-        k.bx = bits(unit * mxx + xhi - x0 + 1);
-        k.bz = bits(unit * mxz + zhi - z0 + 1);
-        k.bits = bits(mxs - mns + 1) + 1 + k.bx + k.bz;
-        if (k.bits > 64) throw Error(MCT_ECAPACITY, "chunk range too wide for 64-bit position keys");
-        return k;
     }
 
     // Tables of the batch's seeds: resident ones are reused, missing ones get a table (least

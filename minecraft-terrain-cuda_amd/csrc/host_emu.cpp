@@ -112,23 +112,7 @@ void build_seed(Emu& e, int64_t seed) {
 }
 
 KeyCodec codec_for(const std::vector<ChunkRef>& ch, int xlo, int xhi, int zlo, int zhi) {
-    int64_t mnx = INT64_MAX, mxx = INT64_MIN, mnz = INT64_MAX, mxz = INT64_MIN, mns = INT64_MAX, mxs = INT64_MIN;
-    for (const ChunkRef& c : ch) {
-        mnx = std::min<int64_t>(mnx, c.x); mxx = std::max<int64_t>(mxx, c.x); mnz = std::min<int64_t>(mnz, c.z); mxz = std::max<int64_t>(mxz, c.z);
-        mns = std::min<int64_t>(mns, c.seed); mxs = std::max<int64_t>(mxs, c.seed);
-    }
-    auto bits = [](int64_t range) { int b = 0; while ((int64_t{1} << b) < range) ++b; return b; };
-    const int64_t x0 = 4 * mnx + xlo, z0 = 4 * mnz + zlo;
-    if (x0 < INT32_MIN || z0 < INT32_MIN || 4 * mxx + xhi > INT32_MAX || 4 * mxz + zhi > INT32_MAX)
-        throw std::runtime_error("chunk coordinates outside the range the position keys hold");
-    KeyCodec k{};
-    k.x0 = static_cast<int32_t>(x0);
-    k.z0 = static_cast<int32_t>(z0);
-    k.s0 = static_cast<int32_t>(mns);
-    k.bx = bits(4 * mxx + xhi - x0 + 1);
-    k.bz = bits(4 * mxz + zhi - z0 + 1);
-    k.bits = bits(mxs - mns + 1) + 1 + k.bx + k.bz;
-    return k;
+    return make_codec(ch.data(), static_cast<int>(ch.size()), 4, xlo, xhi, zlo, zhi);
 }
 
 // unique sorted keys and the map key -> unique index

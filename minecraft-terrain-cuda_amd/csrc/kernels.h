@@ -91,7 +91,7 @@ __global__ __launch_bounds__(kSetupThreads) void k_seed_serial(SeedDev* seeds, c
 // Key of neighbour k = (dz + 1) * 3 + dx + 1 of chunk c; after the dedupe, map[c * 9 + k] is
 // the neighbour's biome-chunk index.
 __global__ __launch_bounds__(256) void k_bio_keys(const BatchDesc* B, const ChunkRef* ch, uint64_t* keys) {
-    const KeyCodec K = B->kb;
+    const KeyCodec& K = B->kb;
     for (int i = global_id(); i < B->bio_keys; i += global_threads()) {
         const int c = i / 9, k = i - c * 9;
         keys[i] = make_key(K, ch[c].seed, 0, ch[c].x + k % 3 - 1, ch[c].z + k / 3 - 1);
@@ -99,7 +99,7 @@ __global__ __launch_bounds__(256) void k_bio_keys(const BatchDesc* B, const Chun
 }
 
 __global__ __launch_bounds__(256) void k_bio_chunks(const BatchDesc* B, const uint64_t* ukeys, const int* count, ChunkRef* bch, unsigned long long* stats) {
-    const KeyCodec K = B->kb;
+    const KeyCodec& K = B->kb;
     const int nbk = *count;
     if (global_id() == 0) atomicAdd(stats, static_cast<unsigned long long>(nbk) * kQPer);
     for (int u = global_id(); u < nbk; u += global_threads()) {
@@ -258,12 +258,12 @@ __global__ __launch_bounds__(256) void k_biome_masks(const int* nbk, const uint8
 
 // ---- density ----------------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_col_keys(const BatchDesc* B, const ChunkRef* ch, uint64_t* keys) {
-    const KeyCodec K = B->kc;
+    const KeyCodec& K = B->kc;
     for (int idx = global_id(); idx < B->col_keys; idx += global_threads()) column_key(K, ch, keys, idx);
 }
 
 __global__ __launch_bounds__(256) void k_columns(const BatchDesc* B, const SeedDev* seeds, const uint64_t* ucols, const int* count, float* colv) {
-    const KeyCodec K = B->kc;
+    const KeyCodec& K = B->kc;
     const int items = *count;
     for (int u = global_id(); u < items; u += global_threads()) column_values(K, seeds, ucols, colv, u);
 }
@@ -285,7 +285,7 @@ __global__ __launch_bounds__(256) void k_corners(const BatchDesc* B, const SeedD
     const int c0 = static_cast<int>(blockIdx.x) * kCornerCols;
     const int ncol = min(kCornerCols, *count - c0);   // the same in every thread
     if (ncol <= 0) return;
-    const KeyCodec K = B->kc;
+    const KeyCodec& K = B->kc;
     for (int k = threadIdx.x; k < ncol * mctgen::kCornerSites; k += blockDim.x) {
         const int c = k / mctgen::kCornerSites;
         tab[k] = corner_site_axes(K, seeds, ucols, c0 + c, k - c * mctgen::kCornerSites);
@@ -298,7 +298,7 @@ __global__ __launch_bounds__(256) void k_corners(const BatchDesc* B, const SeedD
 
 // ---- surface-level grid / aquifer ---------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_grid_keys(const BatchDesc* B, const ChunkRef* ch, uint64_t* keys) {
-    const KeyCodec K = B->kg;
+    const KeyCodec& K = B->kg;
     for (int idx = global_id(); idx < B->grid_keys; idx += global_threads()) grid_key(K, ch, keys, idx);
 }
 

@@ -16,6 +16,8 @@
 #pragma once
 
 #include <algorithm>
+#include <functional>
+#include <stdexcept>
 #include <type_traits>
 #include <vector>
 
@@ -162,24 +164,120 @@ inline void column_groups(const std::vector<TreeNodeF>& treef, ModelDev& M) {
 }
 
 // ---- keys for the position dedupe -------------------------------------------------------
-// A key packs (seed slot, flag, quart x, quart z) relative to the batch's minima, with each
-// field as wide as the batch's range needs, so the radix sort runs as few passes as possible:
-// key = (((slot - s0) * 2 + flag) << (bx + bz)) | ((qx - x0) << bz) | (qz - z0).
+// A key packs (seed slot, flag, quart x, quart z), each field as narrow as the batch needs, so
+// the radix sort runs as few passes as possible:
+//   key = (((slot - s0) * 2 + flag) << (bx + bz)) | (ex << bz) | ez.
+// ex / ez are the coordinates compressed per axis: the positions a batch keys lie in at most
+// kCodecSegs segments along each axis (the union of the chunks' key ranges, the smallest gaps
+// merged), and ex is the offset of x in the concatenation of the segments. Batches of a few
+// far-apart regions (the border region beside the origin, say) then need the bits of their
+// regions' total extent, not of the distance between them.
+constexpr int kCodecSegs = 16;
+struct AxisCodec {
+    int32_t n;                  // segments (1..kCodecSegs)
+    int32_t a[kCodecSegs];      // first coordinate of each segment, increasing
+    uint32_t o[kCodecSegs];     // compressed offset of each segment, increasing, o[0] = 0
+};
 struct KeyCodec {
-    int32_t x0, z0, s0;
+    AxisCodec x, z;
+    int32_t s0;
     int32_t bx, bz;
     int32_t bits;     // total key bits
 };
+MCT_HD MCT_FI uint32_t axis_encode(const AxisCodec& c, int32_t q) {
+    int j = 0;
+    for (int i = 1; i < c.n; ++i) j = q >= c.a[i] ? i : j;
+    return c.o[j] + static_cast<uint32_t>(q - c.a[j]);
+}
+MCT_HD MCT_FI int32_t axis_decode(const AxisCodec& c, uint32_t v) {
+    int j = 0;
+    for (int i = 1; i < c.n; ++i) j = v >= c.o[i] ? i : j;
+    return static_cast<int32_t>(static_cast<int64_t>(c.a[j]) + static_cast<int64_t>(v - c.o[j]));
+}
 MCT_HD MCT_FI uint64_t make_key(const KeyCodec& k, int32_t slot, int flag, int32_t qx, int32_t qz) {
     const uint64_t hi = static_cast<uint64_t>(slot - k.s0) * 2u + static_cast<uint64_t>(flag);
-    return (hi << (k.bx + k.bz)) | (static_cast<uint64_t>(static_cast<uint32_t>(qx - k.x0)) << k.bz) | static_cast<uint64_t>(static_cast<uint32_t>(qz - k.z0));
+    return (hi << (k.bx + k.bz)) | (static_cast<uint64_t>(axis_encode(k.x, qx)) << k.bz) | static_cast<uint64_t>(axis_encode(k.z, qz));
 }
 MCT_HD MCT_FI void split_key(const KeyCodec& k, uint64_t key, int32_t& slot, int& flag, int32_t& qx, int32_t& qz) {
-    qz = static_cast<int32_t>(key & ((1ull << k.bz) - 1)) + k.z0;
-    qx = static_cast<int32_t>((key >> k.bz) & ((1ull << k.bx) - 1)) + k.x0;
+    qz = axis_decode(k.z, static_cast<uint32_t>(key & ((1ull << k.bz) - 1)));
+    qx = axis_decode(k.x, static_cast<uint32_t>((key >> k.bz) & ((1ull << k.bx) - 1)));
     const uint64_t hi = key >> (k.bx + k.bz);
     flag = static_cast<int>(hi & 1u);
     slot = static_cast<int32_t>(hi >> 1) + k.s0;
+}
+
+// Codec of a batch (host): per chunk the key range [unit * c + lo, unit * c + hi] along each axis.
+// Throws std::range_error when a coordinate leaves int32 or the key 64 bits.
+inline int codec_bits(uint64_t range) {
+    int b = 0;
+    while (b < 64 && (uint64_t{1} << b) < range) ++b;
+    return b;
+}
+inline AxisCodec axis_codec(std::vector<int64_t> c, int unit, int lo, int hi, uint64_t& extent) {
+    std::sort(c.begin(), c.end());
+    c.erase(std::unique(c.begin(), c.end()), c.end());
+    std::vector<std::pair<int64_t, int64_t>> iv;   // disjoint, increasing
+    for (int64_t v : c) {
+        const int64_t a = unit * v + lo, b = unit * v + hi;
+        if (a < INT32_MIN || b > INT32_MAX) throw std::range_error("chunk coordinates outside the range the pipeline's position keys hold");
+        if (!iv.empty() && a <= iv.back().second + 1) iv.back().second = std::max(iv.back().second, b);
+        else iv.emplace_back(a, b);
+    }
+    if (static_cast<int>(iv.size()) > kCodecSegs) {   // keep the kCodecSegs - 1 widest gaps
+        std::vector<int64_t> gaps;
+        for (size_t i = 1; i < iv.size(); ++i) gaps.push_back(iv[i].first - iv[i - 1].second);
+        std::vector<int64_t> g = gaps;
+        std::nth_element(g.begin(), g.begin() + (kCodecSegs - 2), g.end(), std::greater<int64_t>());
+        const int64_t cut = g[kCodecSegs - 2];
+        int above = 0;
+        for (int64_t x : gaps) above += x > cut ? 1 : 0;
+        int ties = kCodecSegs - 1 - above;   // gaps equal to the cut that still split
+        std::vector<std::pair<int64_t, int64_t>> m{iv[0]};
+        for (size_t i = 1; i < iv.size(); ++i) {
+            const int64_t gp = gaps[i - 1];
+            const bool split = gp > cut || (gp == cut && ties > 0);
+            if (gp == cut && split) --ties;
+            if (split) m.push_back(iv[i]);
+            else m.back().second = iv[i].second;
+        }
+        iv.swap(m);
+    }
+    AxisCodec ac{};
+    ac.n = static_cast<int32_t>(iv.size());
+    uint64_t off = 0;
+    for (int i = 0; i < ac.n; ++i) {
+        ac.a[i] = static_cast<int32_t>(iv[static_cast<size_t>(i)].first);
+        ac.o[i] = static_cast<uint32_t>(off);
+        off += static_cast<uint64_t>(iv[static_cast<size_t>(i)].second - iv[static_cast<size_t>(i)].first + 1);
+        if (off > (uint64_t{1} << 32)) throw std::range_error("chunk range too wide for the position keys");
+    }
+    extent = off;
+    return ac;
+}
+inline KeyCodec make_codec(const ChunkRef* ch, int n, int unit, int xlo, int xhi, int zlo, int zhi) {
+    std::vector<int64_t> xs(static_cast<size_t>(n)), zs(static_cast<size_t>(n));
+    int64_t mns = INT64_MAX, mxs = INT64_MIN;
+    for (int i = 0; i < n; ++i) {
+        xs[static_cast<size_t>(i)] = ch[i].x;
+        zs[static_cast<size_t>(i)] = ch[i].z;
+        mns = std::min<int64_t>(mns, ch[i].seed);
+        mxs = std::max<int64_t>(mxs, ch[i].seed);
+    }
+    if (n == 0) mns = mxs = 0;
+    KeyCodec k{};
+    uint64_t ex = 1, ez = 1;
+    if (n > 0) {
+        k.x = axis_codec(xs, unit, xlo, xhi, ex);
+        k.z = axis_codec(zs, unit, zlo, zhi, ez);
+    } else {
+        k.x.n = k.z.n = 1;
+    }
+    k.s0 = static_cast<int32_t>(mns);
+    k.bx = codec_bits(ex);
+    k.bz = codec_bits(ez);
+    k.bits = codec_bits(static_cast<uint64_t>(mxs - mns + 1)) + 1 + k.bx + k.bz;
+    if (k.bits > 64) throw std::range_error("chunk range too wide for 64-bit position keys");
+    return k;
 }
 
 // ---- per-seed setup ----------------------------------------------------------------------
