@@ -193,6 +193,7 @@ struct LaneMemory {
     int32_t *grid = nullptr, *cloc = nullptr, *cfl = nullptr;
     // outputs and surface state
     uint8_t *blocks = nullptr, *bout = nullptr, *aq = nullptr, *ore = nullptr;
+    uint8_t *blocks2 = nullptr, *bout2 = nullptr;   // second result buffers (host outputs)
     float* density = nullptr;
     int *h0 = nullptr, *h1 = nullptr, *h2 = nullptr, *hf = nullptr;
     uint32_t* clsw = nullptr;
@@ -290,12 +291,13 @@ struct LaneMemory {
 
 struct Lane {
     std::unique_ptr<LaneMemory> mem;
-    // The lane's kernels and its copies to host memory share one stream: the lane's next batch
-    // reuses the buffers, so it waits for the copy either way, and the copy of one lane overlaps
-    // the other lane's kernels. A stream per copy would also exceed the device's hardware queues
-    // (4 per process), and streams that share a queue stall each other's kernels behind a copy.
     hipStream_t stream = nullptr;
-    hipEvent_t done = nullptr, copied = nullptr;
+    hipEvent_t done = nullptr, copied = nullptr;   // the latest batch's program / copies done
+    // Host outputs alternate between two result buffers per lane, so the lane's next batch
+    // computes while the previous one's results are still being copied; buf_free[k]: the copy
+    // that last read buffer k is done.
+    int obuf = 0;
+    hipEvent_t buf_free[2] = {};
     hipEvent_t stage_ev[kStCount + 1] = {}, kernel_ev[kKCount][2] = {}, copy_ev[2] = {};
     std::map<Shape, hipGraphExec_t> graphs;
     int64_t seq = -1;             // the lane's latest batch
@@ -307,10 +309,12 @@ struct Lane {
         hip_check(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking), "stream");
         hip_check(hipEventCreateWithFlags(&done, hipEventDefault), "event");
         hip_check(hipEventCreateWithFlags(&copied, hipEventDefault), "event");
+        for (auto& e : buf_free) hip_check(hipEventCreateWithFlags(&e, hipEventDisableTiming), "event");
         for (auto& e : stage_ev) hip_check(hipEventCreate(&e), "event");
         for (auto& e : copy_ev) hip_check(hipEventCreate(&e), "event");
         hip_check(hipEventRecord(done, stream), "event");
         hip_check(hipEventRecord(copied, stream), "event");
+        for (auto& e : buf_free) hip_check(hipEventRecord(e, stream), "event");
     }
     ~Lane() {
         if (stream) (void)hipStreamSynchronize(stream);
@@ -322,6 +326,7 @@ struct Lane {
                 if (e) (void)hipEventDestroy(e);
         (void)hipEventDestroy(done);
         (void)hipEventDestroy(copied);
+        for (auto& e : buf_free) (void)hipEventDestroy(e);
         mem.reset();
         if (stream) (void)hipStreamDestroy(stream);
     }
@@ -348,6 +353,12 @@ public:
     double kernel_ms[kKCount] = {};
     int64_t kernel_launches[kKCount] = {};
     std::vector<double> done_ms;
+    std::vector<hipEvent_t> batch_ev;   // completion of batch i of the current call (results in place)
+
+    // Copies of host outputs, for every lane (created with the first one): one stream beside
+    // the lanes' keeps the process within the device's 4 hardware queues per process (streams
+    // beyond that share a queue, and a stream's wait for a copy then stalls another's kernels).
+    hipStream_t copy_stream = nullptr;
 
     Engine(int mb, int nlanes, int sc) : max_batch(mb), seed_cap(sc), model(), seeds(sc) {
         hip_check(hipGetDevice(&device), "get device");
@@ -361,6 +372,8 @@ public:
     ~Engine() {
         (void)hipDeviceSynchronize();
         lanes.clear();
+        if (copy_stream) (void)hipStreamDestroy(copy_stream);
+        for (hipEvent_t e : batch_ev) (void)hipEventDestroy(e);
         void* q[] = {d_qxyz, d_qslot, d_qbad, d_qout};
         for (void* x : q) (void)hipFree(x);
         (void)hipFree(d_stats);
@@ -387,13 +400,13 @@ public:
         hip_check(hipEventRecord(t0, lanes[0]->stream), "event");
         // Results copied to host memory: the host link (about 57 GB/s, 580K chunks/s of terrain)
         // is slower than the kernels, so the call is cut into batches of about an eighth of it
-        // (at least 1024 chunks) and each batch's copy overlaps the next batches' kernels on
-        // the other lane; only the first batch's kernels are not hidden.
+        // (at least 1024 chunks), the first one half that, so that the copies start early and
+        // then run back to back while the lanes compute the next batches.
         const bool to_host = !out.device && (out.terrain || out.biomes || out.aquifer || out.density);
         const int step = (to_host && lanes.size() > 1) ? std::min(max_batch, std::max(1024, (count + 7) / 8)) : max_batch;
         int index = 0;
         for (int off = 0; off < count; ++index) {
-            int n = std::min(step, count - off);
+            int n = std::min(index == 0 && step < max_batch ? step / 2 : step, count - off);
             {
                 std::unordered_map<int64_t, int> seen;
                 for (int i = 0; i < n; ++i)
@@ -431,12 +444,18 @@ public:
     bool lane_pending(int lane) const { return lanes[static_cast<size_t>(lane)]->pending; }
     bool lane_ready(int lane) const {
         const Lane& L = *lanes[static_cast<size_t>(lane)];
-        return !L.pending || hipEventQuery(L.copies ? L.copied : L.done) == hipSuccess;
+        return !L.pending || hipEventQuery(L.done) == hipSuccess;
     }
     void lane_finish(int lane) { finish(*lanes[static_cast<size_t>(lane)], nullptr); }
 
+    // Every lane's batch and every copy done; the call's per-batch completion times.
     void finish_all(mct_timing* tm) {
         for (auto& l : lanes) finish(*l, tm);
+        for (auto& l : lanes) hip_check(hipEventSynchronize(l->copied), "copies");
+        for (size_t i = 0; i < done_ms.size() && i < batch_ev.size(); ++i) {
+            float ms = 0;
+            if (hipEventElapsedTime(&ms, t0, batch_ev[i]) == hipSuccess) done_ms[i] = ms;
+        }
     }
 
     // ---- launch shapes -----------------------------------------------------------------------
@@ -557,17 +576,13 @@ private:
         return g;
     }
 
-    // Collects lane L's batch: waits for its completion, accumulates its times.
+    // Collects lane L's batch: waits for its program (the lane's staging block and scratch are
+    // then free; copies of host outputs may still run, finish_all waits for those) and
+    // accumulates its times.
     void finish(Lane& L, mct_timing* tm) {
         if (!L.pending) return;
-        hipEvent_t last = L.copies ? L.copied : L.done;
-        hip_check(hipEventSynchronize(last), "batch");
+        hip_check(hipEventSynchronize(L.done), "batch");
         L.pending = false;
-        if (L.batch_index >= 0 && L.batch_index < static_cast<int>(done_ms.size())) {
-            float ms = 0;
-            hip_check(hipEventElapsedTime(&ms, t0, last), "event time");
-            done_ms[static_cast<size_t>(L.batch_index)] = ms;
-        }
         if (!L.timed) return;
         L.timed = false;
         for (int k = 0; k < kKCount; ++k) {
@@ -592,6 +607,7 @@ private:
         tm->surface_ms += t[kStSurface];
         if (L.copies) {
             float c = 0;
+            hip_check(hipEventSynchronize(L.copy_ev[1]), "copy");
             hip_check(hipEventElapsedTime(&c, L.copy_ev[0], L.copy_ev[1]), "event time");
             tm->d2h_ms += c;
         }
@@ -690,14 +706,30 @@ private:
         sh.pass_col = passes(D.kc.bits);
         sh.pass_grid = core ? 0 : passes(D.kg.bits);
         sh.pass_bio = core ? 0 : passes(D.kb.bits);
+        // Host outputs: the batch writes result buffer k of the lane (alternating, so the copy
+        // of the lane's previous batch can still read the other one) unless aquifer or density
+        // outputs are asked for, which have one buffer each (then buffer 0 and both waits).
+        const bool to_host = !out.device && (out.terrain || out.biomes || out.aquifer || out.density);
+        const bool single = out.aquifer || out.density;
+        int k = 0;
+        if (to_host && !single) {
+            k = L.obuf;
+            L.obuf ^= 1;
+            if (k == 1) {
+                m.ensure(&m.blocks2, static_cast<size_t>(m.cap) * kPer);
+                m.ensure(&m.bout2, static_cast<size_t>(m.cap) * kQPer);
+            }
+        }
+        uint8_t* tbuf = k ? m.blocks2 : m.blocks;
+        uint8_t* bbuf = k ? m.bout2 : m.bout;
         if (out.device) {
             D.terrain = out.terrain;
             D.biomes = out.biomes;
             D.aquifer = out.aquifer;
             D.density = out.density;
         } else {
-            D.terrain = out.terrain ? m.blocks : nullptr;
-            if (out.biomes) D.biomes = m.bout;
+            D.terrain = out.terrain ? tbuf : nullptr;
+            if (out.biomes) D.biomes = bbuf;
             if (out.aquifer) {
                 m.ensure(&m.aq, static_cast<size_t>(m.cap) * kPer);
                 D.aquifer = m.aq;
@@ -707,13 +739,17 @@ private:
                 D.density = m.density;
             }
         }
-        if (!core && !D.terrain) D.terrain = m.blocks;   // the full step always fills
+        if (!core && !D.terrain) D.terrain = tbuf;   // the full step always fills
         sh.biomes = D.biomes != nullptr;
         sh.aquifer = D.aquifer != nullptr;
         sh.density = D.density != nullptr;
         sh.fill = D.terrain != nullptr;
-        // the lane's buffers are free once its previous copy back is done
-        hip_check(hipStreamWaitEvent(L.stream, L.copied, 0), "wait for copy back");
+        // the buffers this batch writes are free once the copies that read them are done
+        if (to_host && !single) {
+            hip_check(hipStreamWaitEvent(L.stream, L.buf_free[k], 0), "wait for copy back");
+        } else {
+            for (hipEvent_t e : L.buf_free) hip_check(hipStreamWaitEvent(L.stream, e, 0), "wait for copy back");
+        }
         L.seq = seq;
         L.timed = timed;
         if (timed) {
@@ -726,20 +762,42 @@ private:
             run_program(L, sh, false);
         }
         hip_check(hipEventRecord(L.done, L.stream), "event");
-        L.copies = !out.device && (out.terrain || out.biomes || out.aquifer || out.density);
+        L.copies = to_host;
         if (L.copies) {
-            if (timed) hip_check(hipEventRecord(L.copy_ev[0], L.stream), "event");
+            if (!copy_stream) hip_check(hipStreamCreateWithFlags(&copy_stream, hipStreamNonBlocking), "stream");
+            hipStream_t cs = copy_stream;
+            hip_check(hipStreamWaitEvent(cs, L.done, 0), "wait for batch");
+            if (timed) hip_check(hipEventRecord(L.copy_ev[0], cs), "event");
             const size_t nn = static_cast<size_t>(n);
-            if (out.terrain) hip_check(hipMemcpyAsync(out.terrain, m.blocks, nn * kPer, hipMemcpyDeviceToHost, L.stream), "copy terrain");
-            if (out.biomes) hip_check(hipMemcpyAsync(out.biomes, m.bout, nn * kQPer, hipMemcpyDeviceToHost, L.stream), "copy biomes");
-            if (out.aquifer) hip_check(hipMemcpyAsync(out.aquifer, m.aq, nn * kPer, hipMemcpyDeviceToHost, L.stream), "copy aquifer");
-            if (out.density) hip_check(hipMemcpyAsync(out.density, m.density, nn * kPer * sizeof(float), hipMemcpyDeviceToHost, L.stream), "copy density");
-            if (timed) hip_check(hipEventRecord(L.copy_ev[1], L.stream), "event");
-            hip_check(hipEventRecord(L.copied, L.stream), "event");
+            if (out.terrain) hip_check(hipMemcpyAsync(out.terrain, tbuf, nn * kPer, hipMemcpyDeviceToHost, cs), "copy terrain");
+            if (out.biomes) hip_check(hipMemcpyAsync(out.biomes, bbuf, nn * kQPer, hipMemcpyDeviceToHost, cs), "copy biomes");
+            if (out.aquifer) hip_check(hipMemcpyAsync(out.aquifer, m.aq, nn * kPer, hipMemcpyDeviceToHost, cs), "copy aquifer");
+            if (out.density) hip_check(hipMemcpyAsync(out.density, m.density, nn * kPer * sizeof(float), hipMemcpyDeviceToHost, cs), "copy density");
+            if (timed) hip_check(hipEventRecord(L.copy_ev[1], cs), "event");
+            record_batch(L, cs);
+            hip_check(hipEventRecord(L.copied, cs), "event");
+            if (single) {
+                for (hipEvent_t e : L.buf_free) hip_check(hipEventRecord(e, cs), "event");
+            } else {
+                hip_check(hipEventRecord(L.buf_free[k], cs), "event");
+            }
         } else {
+            record_batch(L, L.stream);
             hip_check(hipEventRecord(L.copied, L.stream), "event");
         }
         L.pending = true;
+    }
+
+    // The completion event of the lane's batch in the current call (its results in place),
+    // recorded on `s` before the lane's `copied` event, which finish_all waits for.
+    void record_batch(const Lane& L, hipStream_t s) {
+        if (L.batch_index < 0) return;
+        while (batch_ev.size() <= static_cast<size_t>(L.batch_index)) {
+            hipEvent_t e = nullptr;
+            hip_check(hipEventCreate(&e), "event");
+            batch_ev.push_back(e);
+        }
+        hip_check(hipEventRecord(batch_ev[static_cast<size_t>(L.batch_index)], s), "event");
     }
 
     hipGraphExec_t capture(Lane& L, const Shape& sh) {
