@@ -12,10 +12,10 @@
 // or stage times asked for) run the same program launch by launch with events around groups.
 //
 // Lanes: each lane is a stream with its own device buffers, staging block and graphs; batches
-// go to the lanes in turn, so one lane's batch computes while the other's results are copied.
-// Copies to host memory follow the batch's program on the lane's stream, so a lane's next
-// batch starts once its buffers are free. A call whose results go to host memory is cut
-// into smaller batches, so that the copies overlap the kernels.
+// go to the lanes in turn. Host outputs alternate between two result buffers per lane and are
+// copied on one copy stream shared by the lanes, so a lane's next batch computes while its
+// previous results are copied; a call whose results go to host memory is cut into smaller
+// batches, so that the copies start early and run back to back.
 //
 // Seed tables: seed_capacity resident tables in HBM, least recently used replaced. A batch
 // lists the tables it needs built as jobs, and its program builds them first. A table is only
