@@ -150,3 +150,45 @@ def test_edge_set_every_chunk_matches_reference(mct):
         dt, db = device_digests(mct, f['seed'], f['xz'], max_batch_chunks=batch, pipeline_slots=2, seed_capacity=4)
         bad_t, bad_b = mismatches(dt, f['terrain']), mismatches(db, f['biomes'])
         assert len(bad_t) == 0 and len(bad_b) == 0, ('batch', batch, 'terrain', f['xz'][bad_t[:8]].tolist(), 'biomes', f['xz'][bad_b[:8]].tolist())
+
+
+def test_stress_every_chunk_and_query_matches_reference(mct):
+    """The stress configuration at full size: all 2^20 chunks of the 1,024-seed list (every
+    seed's tables resident, in 65,536-chunk HBM windows) and the first 2^20 of its 2^24
+    biome-only queries (the reference answers a query with a whole chunk's biomes, so all 2^24
+    would take it hours), against the reference's group digests: each group of 1,024 chunk
+    digests (terrain and biomes) and of 4,096 query answers folded with the chunk digest, so a
+    group matches when every member matches (up to 64-bit collisions). The first group is also
+    compared chunk by chunk, to name the chunks of a mismatch."""
+    f = load('c5_digests.npz')
+    G, QG = int(f['group']), int(f['qgroup'])
+    seeds, which, xz = workloads.stress()
+    n = len(xz)
+    assert n == len(f['terrain_groups']) * G
+    window = 1 << 16
+    dt, db = np.empty(n, np.uint64), np.empty(n, np.uint64)
+    d_t, d_b = mct.alloc_device(window * PER), mct.alloc_device(window * QPER)
+    g = mct.FullTerrainGenerator(max_batch_chunks=16384, pipeline_slots=2, seed_capacity=1024)
+    try:
+        for off in range(0, n, window):
+            k = min(window, n - off)
+            if off % (1 << 18) == 0:
+                log('C5 chunks from', off)
+            g.generate_device(seeds[which[off:off + k]], xz[off:off + k], d_t, d_b)
+            dt[off:off + k] = mct.chunk_digests(d_t, k, PER)
+            db[off:off + k] = mct.chunk_digests(d_b, k, QPER)
+        head_t, head_b = mismatches(dt[:G], f['head_terrain']), mismatches(db[:G], f['head_biomes'])
+        assert len(head_t) == 0 and len(head_b) == 0, ('first group', xz[head_t[:8]].tolist(), xz[head_b[:8]].tolist())
+        bad_t = mismatches(mct.digest_host(dt.reshape(-1, G)), f['terrain_groups'])
+        bad_b = mismatches(mct.digest_host(db.reshape(-1, G)), f['biome_groups'])
+        assert len(bad_t) == 0 and len(bad_b) == 0, ('terrain groups', bad_t[:8].tolist(), 'biome groups', bad_b[:8].tolist())
+        qs, q = workloads.queries(int(f['queries']), seeds)
+        log('C5 queries', len(q))
+        ans = g.biome_query(qs, q)
+        assert np.array_equal(ans[:QG], f['head_answers']), np.flatnonzero(ans[:QG] != f['head_answers'])[:8].tolist()
+        bad_q = mismatches(mct.digest_host(ans.reshape(-1, QG)), f['query_groups'])
+        assert len(bad_q) == 0, ('query groups', len(bad_q), bad_q[:8].tolist())
+    finally:
+        g.close()
+        mct.free_device(d_t)
+        mct.free_device(d_b)
