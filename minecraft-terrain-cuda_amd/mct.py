@@ -297,8 +297,9 @@ DIGESTS_MAX = 65536
 
 
 def digest_host(chunks):
-    """The same digest computed on the host (numpy), for arrays of whole chunks (n, ...)."""
-    a = np.ascontiguousarray(chunks).reshape(len(chunks), -1)
+    """The same digest computed on the host (numpy), for arrays of whole chunks (n, ...) of any
+    element type (the digest is of each chunk's bytes; a multiple of 8 per chunk)."""
+    a = np.ascontiguousarray(chunks).reshape(len(chunks), -1).view(np.uint8)
     w = a.view('<u8')
     with np.errstate(over='ignore'):
         def mix(z):

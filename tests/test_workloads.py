@@ -40,3 +40,19 @@ def test_stage_units_count_distinct_positions():
         gk.update((s, 4 * x + 4 * (j & 1), 4 * z + 4 * (j >> 1), True) for j in range(4))
     u = gen_host.stage_units(sd, ch)
     assert u == {'chunks': len(ch), 'biome_chunks': len(bc), 'columns': len(cols), 'grid_keys': len(gk)}
+
+
+def test_host_digest_folds_groups_like_the_fixture_writer():
+    """mct.digest_host over uint64 group arrays (the stress test's folds of per-chunk digests)
+    and over uint8 answers equals the fold tests/golden/make_digests.py wrote the C5 groups with."""
+    import importlib.util
+    import os
+    import mct
+    spec = importlib.util.spec_from_file_location('make_digests', os.path.join(os.path.dirname(__file__), 'golden', 'make_digests.py'))
+    md = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(md)
+    rng = np.random.default_rng(7)
+    words = rng.integers(0, 2 ** 63, (6, 1024), dtype=np.uint64)
+    answers = rng.integers(0, 256, (4, 4096), dtype=np.uint8)
+    assert np.array_equal(mct.digest_host(words), md.fold(words))
+    assert np.array_equal(mct.digest_host(answers), md.fold(answers))
