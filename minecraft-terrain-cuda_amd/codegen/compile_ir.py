@@ -96,7 +96,7 @@ class Compiler:
         self.function('eval_climate_columns', 'void eval_climate_columns(%s, int32_t x, int32_t z, float* out)' % O, ccols,
                       self.outputs(ccols, plan.climate_columns))
         corner = Context('Corner', columns=plan.corner_columns, pass_interpolated=True, tabulated=True)
-        self.function('eval_corner', 'void eval_corner(%s, int32_t x, int32_t y, int32_t z, const float* col, const MCT_LDS mct::AxisXZ* axz, float* out)' % O,
+        self.function('eval_corner', 'void eval_corner(%s, int32_t x, int32_t y, int32_t z, const float* col, const MCT_LDS mct::AxisXZ* axz, const mct::Axis* yt, float* out)' % O,
                       corner, self.outputs(corner, [g.nodes[c]['in'] for c in plan.all_channels]))
         block = Context('Block', channels=plan.channels)
         self.function('eval_block', 'float eval_block(%s, int32_t x, int32_t y, int32_t z, const float* ch)' % O, block,
@@ -161,6 +161,18 @@ class Compiler:
         o.append('    const int layer = kCornerSiteLayer[site];')
         o.append('    return mct::site_record(O[layer], x, z, kCornerSiteScale[site] * kLayerFreq[layer]);')
         o.append('}')
+        Y = self.sites.ysites
+        o.append('// the corner stage\'s y sites: per site the layer, the y scale and whether the octave is smeared;')
+        o.append('// a seed tabulates each site\'s y axis record at every corner row (kCornerYRows rows, 8 blocks apart)')
+        o.append('constexpr int kCornerYSites = %d;' % len(Y))
+        o.append('MCT_TABLE int32_t kCornerYLayer[%d] = {%s};' % (max(1, len(Y)), ', '.join(str(l) for l, _, _ in Y) or '0'))
+        o.append('MCT_TABLE double kCornerYScale[%d] = {%s};' % (max(1, len(Y)), ', '.join(dlit(v) for _, v, _ in Y) or '0.0'))
+        o.append('MCT_TABLE int32_t kCornerYSmeared[%d] = {%s};' % (max(1, len(Y)), ', '.join('1' if sm else '0' for _, _, sm in Y) or '0'))
+        o.append('MCT_HD MCT_FI mct::Axis corner_y_axis(const mct::Octave* O, int32_t y, int site) {')
+        o.append('    const int layer = kCornerYLayer[site];')
+        o.append('    const double s = kCornerYScale[site] * kLayerFreq[layer];')
+        o.append('    return kCornerYSmeared[site] ? mct::axis_smeared(y, s, O[layer].oy, kLayerSmear[layer]) : mct::axis_plain(y, s, O[layer].oy);')
+        o.append('}')
         for k in ('surface', 'surface_secondary', 'clay_bands_offset', 'badlands_pillar', 'badlands_pillar_roof', 'badlands_surface',
                   'iceberg_pillar', 'iceberg_pillar_roof', 'iceberg_surface'):
             first, count = self.P.noise_range[self.P.key_index['minecraft:' + k]]
@@ -173,7 +185,8 @@ class Compiler:
         o = [HEADER, '#pragma once', 'namespace mctgen {',
              '// operations by class (FP64, FP32, INT32, memory reads), counted at the level of source operations',
              'struct OpCount { int f64, f32, i32, mem; };']
-        for name, v in (('kOctaveVolume', OCTAVE_OPS['volume']), ('kOctaveVolumeTab', OCTAVE_OPS['volume_tab']), ('kOctavePoint', OCTAVE_OPS['point']),
+        for name, v in (('kOctaveVolume', OCTAVE_OPS['volume']), ('kOctaveVolumeTab', OCTAVE_OPS['volume_tab']),
+                        ('kOctaveVolumeTabY', OCTAVE_OPS['volume_tab_y']), ('kOctavePoint', OCTAVE_OPS['point']),
                         ('kSmearExtra', SMEAR_EXTRA), ('kSiteRecord', TABLE_RECORD_OPS)):
             o.append('constexpr OpCount %s{%d, %d, %d, %d};' % ((name,) + tuple(v)))
         o.append('// static counts per function, without the noise stacks (their octaves count themselves):')

@@ -36,6 +36,9 @@ OCTAVE_OPS = {
     'volume': ops(f64=3 * 10 + 2, f32=3 * 7 + 3 + 8 * 5 + 21 + 3, i32=16 + 24, mem=14 + 5),
     # volume path, x / z from the column table (two records read), y computed
     'volume_tab': ops(f64=10 + 1, f32=7 + 3 + 8 * 5 + 21 + 3, i32=12 + 24, mem=12 + 2 + 4),
+    # volume path, x / z from the column table and y from the seed's corner-row table (three
+    # records read, no axis computed)
+    'volume_tab_y': ops(f32=3 + 8 * 5 + 21 + 3, i32=12 + 24, mem=12 + 2 + 4 + 3),
     # point path: per axis frequency multiply, wrap compares, offset add, floor, two
     # conversions, subtract, to float (9 FP64); fractions minus one, smoothsteps
     'point': ops(f64=3 * 9, f32=3 + 3 * 7 + 8 * 5 + 21 + 2, i32=16 + 24, mem=14 + 5),
@@ -89,11 +92,15 @@ class Context:
 
 class NoiseSites:
     """The corner stage's octave sample sites: per (stack, xz scale) one site per layer,
-    whose x / z axis records the stage tabulates per block column."""
+    whose x / z axis records the stage tabulates per block column; and per (stack, y scale,
+    smeared) one y site per layer, whose y axis records at the corner rows (the same for every
+    column of a seed) the seed setup tabulates."""
 
     def __init__(self):
         self.sites = []          # (layer, xz scale)
         self.first = {}
+        self.ysites = []         # (layer, y scale, smeared)
+        self.yfirst = {}
 
     def site(self, first, count, scale):
         k = (first, count, struct.pack('<d', scale))
@@ -101,6 +108,13 @@ class NoiseSites:
             self.first[k] = len(self.sites)
             self.sites += [(first + j, scale) for j in range(count)]
         return self.first[k]
+
+    def ysite(self, first, count, scale, smeared):
+        k = (first, count, struct.pack('<d', scale), bool(smeared))
+        if k not in self.yfirst:
+            self.yfirst[k] = len(self.ysites)
+            self.ysites += [(first + j, scale, bool(smeared)) for j in range(count)]
+        return self.yfirst[k]
 
 
 class Lowering:
@@ -171,6 +185,9 @@ class Lowering:
             amp = amplitude or '1.0f'
             if tab:
                 site = self.sites.site(first, count, xz)
+                if by == self.ctx.base[1]:   # at the corner's own y: y records from the seed's table
+                    ysite = self.sites.ysite(first, count, ys, smeared)
+                    return 'mct::stack_volume_tab_y<%d, %d, kCornerYRows>(O, axz + %d, yt + %d * kCornerYRows, %s)' % (first, count, site, ysite, amp)
                 fn = 'mct::stack_volume_tab%s<%d, %d>' % ('_smeared' if smeared else '', first, count)
                 return '%s(O, axz + %d, %s, %s, %s)' % (fn, site, by, dlit(ys), amp)
             fn = 'mct::stack_volume%s<%d, %d>' % ('_smeared' if smeared else '', first, count)

@@ -46,6 +46,7 @@ class ModelHeader:
         D = self.decl
         D('constexpr uint64_t kIrHash = %s, kTerrainHash = %s;' % (hex64(self.ir_hash), hex64(self.terrain_hash)))
         D('constexpr int kMinY = %d, kHeight = %d, kSeaLevel = %d;' % (g.min_y, g.height, g.sea_level))
+        D('constexpr int kCornerYRows = %d;   // corner rows of the cell grid, 8 blocks apart' % (g.height // 8 + 1))
         # noise layers
         D('constexpr int kNumLayers = %d, kNumNoises = %d, kNumBlended = %d;' % (P.count, len(P.keys), len(P.blended_ids)))
         D('constexpr int kNumChannels = %d, kNumOreChannels = %d, kNumColumns = %d, kNumClimateColumns = %d;' % (

@@ -878,6 +878,7 @@ private:
         group(L, timed, kKSeedSetup, [&] {
             k_seed_layers<<<grid((const void*)k_seed_layers, kSetupThreads), kSetupThreads, 0, s>>>(seeds.d, B, m.d_jobs(), model.nnormal);
             k_seed_serial<<<grid((const void*)k_seed_serial, kSetupThreads), kSetupThreads, 0, s>>>(seeds.d, B, m.d_jobs(), model.blended_first);
+            k_seed_yaxes<<<grid((const void*)k_seed_yaxes, 256), 256, 0, s>>>(seeds.d, B, m.d_jobs());
         });
         hip_check(hipGetLastError(), "seed setup kernels");
         const int* cnt = m.counts;
@@ -1081,6 +1082,7 @@ private:
         hip_check(hipMemcpyAsync(m.d_block, m.h_block, m.block_bytes, hipMemcpyHostToDevice, L.stream), "upload jobs");
         k_seed_layers<<<grid((const void*)k_seed_layers, kSetupThreads), kSetupThreads, 0, L.stream>>>(seeds.d, m.d_desc(), m.d_jobs(), model.nnormal);
         k_seed_serial<<<grid((const void*)k_seed_serial, kSetupThreads), kSetupThreads, 0, L.stream>>>(seeds.d, m.d_desc(), m.d_jobs(), model.blended_first);
+        k_seed_yaxes<<<grid((const void*)k_seed_yaxes, 256), 256, 0, L.stream>>>(seeds.d, m.d_desc(), m.d_jobs());
         hip_check(hipGetLastError(), "seed setup kernels");
         hip_check(hipEventRecord(L.done, L.stream), "event");
         hip_check(hipEventRecord(L.copied, L.stream), "event");

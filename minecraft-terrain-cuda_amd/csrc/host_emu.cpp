@@ -107,6 +107,7 @@ void build_seed(Emu& e, int64_t seed) {
     }
     for (int b = 0; b < mctgen::kNumBlended; ++b) seed_blended(S, seed, b, blended_first[b], scratch);
     seed_misc(S, seed, e.M);
+    for (int i = 0; i < kCornerYTab; ++i) seed_yaxis(S, i);
     e.seed_value = seed;
     e.has_seed = true;
 }

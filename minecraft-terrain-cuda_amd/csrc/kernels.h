@@ -87,6 +87,12 @@ __global__ __launch_bounds__(kSetupThreads) void k_seed_serial(SeedDev* seeds, c
     }
 }
 
+// The corner-row y tables of the batch's new seeds (after their octaves).
+__global__ __launch_bounds__(256) void k_seed_yaxes(SeedDev* seeds, const BatchDesc* B, const SeedJob* jobs) {
+    const int items = B->njobs * kCornerYTab;
+    for (int idx = global_id(); idx < items; idx += global_threads()) seed_yaxis(seeds[jobs[idx / kCornerYTab].slot], idx % kCornerYTab);
+}
+
 // ---- biome chunks: the union of the chunks' 3 x 3 neighbourhoods, on the device --------------
 // Key of neighbour k = (dz + 1) * 3 + dx + 1 of chunk c; after the dedupe, map[c * 9 + k] is
 // the neighbour's biome-chunk index.

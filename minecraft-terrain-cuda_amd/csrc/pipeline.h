@@ -34,6 +34,7 @@ constexpr int kQH = kHeight / 4;                   // quart rows
 constexpr int kQPer = 16 * kQH;                    // quarts per chunk
 constexpr int kPer = 256 * kHeight;                // blocks per chunk
 constexpr int kRows = kHeight / 8 + 1;             // corner rows (4x8 cells)
+static_assert(kRows == mctgen::kCornerYRows, "the corner-row y tables have one record per corner row");
 constexpr int kCh = mctgen::kNumChannels + mctgen::kNumOreChannels;
 constexpr int kCols = 25;                          // corner columns per chunk (5 x 5)
 constexpr int kGridX = 27, kGridZ = 19, kGridCells = kGridX * kGridZ;   // surface-level quarts per chunk
@@ -58,12 +59,16 @@ struct ChunkRef {
 };
 
 // Per-seed state, built on the device from the seed (seed_layer / seed_blended / seed_misc).
+// y axis records of the corner stage's y sites at every corner row: [site][row]
+constexpr int kCornerYTab = (mctgen::kCornerYSites > 0 ? mctgen::kCornerYSites : 1) * mctgen::kCornerYRows;
+
 struct SeedDev {
     Octave oct[mctgen::kNumLayers];
     Positional random, aquifer, ore, grad[kMaxRandomNames];
     int64_t zoom_seed;
     int64_t seed;
     uint8_t clay[192];
+    Axis ytab[kCornerYTab];
 };
 
 struct TreeNodeDev {
@@ -364,6 +369,12 @@ MCT_HD inline void seed_misc(SeedDev& S, int64_t seed, const ModelDev& M) {
         if (start + 1 < 192 && next_bool(r)) c[start + 1] = static_cast<uint8_t>(LGRAY);
         ++i;
     }
+}
+
+// Record idx = site * kCornerYRows + row of the seed's corner-row y table (after its octaves).
+MCT_HD MCT_FI void seed_yaxis(SeedDev& S, int idx) {
+    const int site = idx / mctgen::kCornerYRows, row = idx % mctgen::kCornerYRows;
+    S.ytab[idx] = mctgen::corner_y_axis(S.oct, kMinY + row * 8, site);
 }
 
 // ---- biomes ------------------------------------------------------------------------------
@@ -936,7 +947,7 @@ MCT_HD MCT_FI void corner_values(const KeyCodec& K, const SeedDev* seeds, const 
     MCT_OPS(0, 0, 9, 1);   // the key read and split, x4 coordinates and the corner's y
     float out[kCh];
     mctgen::eval_corner(seeds[s].oct, qx * 4, kMinY + row * 8, qz * 4, colv + static_cast<size_t>(u) * (mctgen::kNumColumns > 0 ? mctgen::kNumColumns : 1), axz,
-                        out);
+                        seeds[s].ytab + row, out);
     for (int ch = 0; ch < kCh; ++ch) corners[(static_cast<size_t>(u) * kCh + ch) * kRows + row] = out[ch];
 }
 

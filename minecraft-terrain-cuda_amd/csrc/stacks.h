@@ -7,6 +7,8 @@
 //                               acc = acc + (amplitude * amp) * sample(block * (scale * freq))
 //   stack_volume_tab(_smeared)  the same with the x / z axis records of each layer read from
 //                               the corner stage's per-column table
+//   stack_volume_tab_y          the same with the y records read from the seed's corner-row
+//                               table too
 //   stack_point(_smeared)       point path at (x, y, z) already scaled: value += amp * sample
 //
 // (reference: cpu/include/mccuda/noise_eval.h:267-304, cpu/src/seed_tables.cpp:106-184).
@@ -81,6 +83,20 @@ MCT_HD MCT_NI float stack_volume_tab_smeared(const Octave* O, const MCT_LDS Axis
         MCT_OPS_OF(mctgen::kSmearExtra);
         const AxisXZ a = A[i - First];
         const Axis Y = axis_smeared(by, ys * mctgen::kLayerFreq[i], o.oy, mctgen::kLayerSmear[i]);
+        return acc + (amplitude * mctgen::kLayerAmp[i]) * volume_sample_xz(P, a, Y);
+    });
+}
+
+// The same with the y axis records read too, from the seed's corner-row table: Yt[(i - First)
+// * Rows] is layer i's record at the corner's row (the records stack_volume_tab(_smeared)
+// computes, tabulated per seed by corner_y_axis).
+template <int First, int Count, int Rows>
+MCT_HD MCT_NI float stack_volume_tab_y(const Octave* O, const MCT_LDS AxisXZ* A, const Axis* Yt, float amplitude) {
+    return octave_sum<First, Count>(O, [&](float acc, const Octave& o, const auto& P, int i) MCT_CLOSURE {
+        MCT_OPS_OF(mctgen::kOctaveVolumeTabY);
+        (void)o;
+        const AxisXZ a = A[i - First];
+        const Axis Y = Yt[(i - First) * Rows];
         return acc + (amplitude * mctgen::kLayerAmp[i]) * volume_sample_xz(P, a, Y);
     });
 }
