@@ -23,12 +23,24 @@ struct Octave {
     uint8_t p[512];
 };
 
-// GRADIENT[h & 15] components packed as 2-bit fields (value + 1), entry h at bits 2h..2h+1.
+// GRADIENT[h & 15] components as signed 2-bit fields (two's complement: -1 = 0b11), entry h
+// at bits 2h..2h+1, so one sign-extending bit-field extract gives the component:
 //   gx: 1,-1,1,-1,1,-1,1,-1,0,0,0,0,1,0,-1,0
 //   gy: 1,1,-1,-1,0,0,0,0,1,-1,1,-1,1,-1,1,-1
 //   gz: 0,0,0,0,1,1,-1,-1,1,1,-1,-1,0,1,0,-1
-constexpr uint32_t kGX = 0x46552222u, kGY = 0x2222550Au, kGZ = 0x190A0A55u;
-MCT_HD MCT_FI float gcomp(uint32_t packed, int h) { return static_cast<float>(static_cast<int>((packed >> ((h & 15) << 1)) & 3u) - 1); }
+constexpr uint32_t kGX = 0x3100DDDDu, kGY = 0xDDDD00F5u, kGZ = 0xC4F5F500u;
+MCT_HD MCT_FI float gcomp(uint32_t packed, int h) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    // v_bfe_i32 written out: the compiler's own sign-extending extract, followed by the int to
+    // float conversion, was compiled to an unsigned conversion (-1 became 4294967295.0). The
+    // instruction reads the offset's low 5 bits only, so 2h needs no mask: they are 2 (h & 15).
+    int g;
+    asm("v_bfe_i32 %0, %1, %2, 2" : "=v"(g) : "s"(packed), "v"(h << 1));
+    return static_cast<float>(g);
+#else
+    return static_cast<float>(static_cast<int32_t>(packed << (30 - ((h & 15) << 1))) >> 30);
+#endif
+}
 
 // Math.nextDown(2^24): GradientNoise.wrap leaves |x| below it untouched.
 constexpr double kWrapLimit = 0x1.fffffffffffffp+23;   // bits 0x416FFFFFFFFFFFFF
