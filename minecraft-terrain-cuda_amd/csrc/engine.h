@@ -181,7 +181,7 @@ struct LaneMemory {
     int* nb = nullptr;
     ChunkRef* bch = nullptr;
     float* bcol = nullptr;
-    int64_t *targets = nullptr, *mind = nullptr;
+    int64_t *targets = nullptr, *mind = nullptr, *tvary = nullptr;
     int32_t* lstar = nullptr;
     uint8_t *tie = nullptr, *bb = nullptr;
     uint64_t* bmask = nullptr;
@@ -217,6 +217,7 @@ struct LaneMemory {
         M(&bch, nbk);
         M(&bcol, nbk * 16 * ncc);
         M(&targets, nbk * kQPer * 7);
+        M(&tvary, nbk * kQPer);
         M(&mind, nbk * kQPer);
         M(&lstar, nbk * kQPer);
         M(&tie, nbk * kQPer);
@@ -894,13 +895,13 @@ private:
                 k_biome_columns<<<grid((const void*)k_biome_columns, W(kKBiomeColumns)), W(kKBiomeColumns), 0, s>>>(sd, m.bch, nbk, m.bcol);
             });
             group(L, timed, kKBiomeQuarts, [&] {
-                k_biome_targets<<<grid((const void*)k_biome_targets, 256), 256, 0, s>>>(sd, m.bch, nbk, m.bcol, m.targets);
+                k_biome_targets<<<grid((const void*)k_biome_targets, 256), 256, 0, s>>>(sd, m.bch, nbk, m.bcol, m.targets, m.tvary);
                 const int w = W(kKBiomeQuarts);
                 const size_t bk_max = 9 * static_cast<size_t>(sh.grid_n);
                 if (model.treef) {
                     if (model.vary_groups > 0) {
                         k_biome_groups<<<blocks_for(bk_max * 16 * model.vary_groups, w), w, 0, s>>>(nbk, model.vary_groups, m.targets, m.lstar, m.mind, m.gres);
-                        k_biome_quart_final<<<grid((const void*)k_biome_quart_final, 256), 256, 0, s>>>(nbk, model.vary_groups, m.targets, m.gres, m.lstar, m.mind);
+                        k_biome_quart_final<<<grid((const void*)k_biome_quart_final, 256), 256, 0, s>>>(nbk, model.vary_groups, m.tvary, m.gres, m.lstar, m.mind);
                     } else {
                         k_biome_search_cols<<<blocks_for(bk_max * 16, w), w, 0, s>>>(nbk, m.targets, m.lstar, m.mind);
                     }

@@ -243,13 +243,14 @@ int mcte_generate(void* p, int64_t seed, const int32_t* xz, int n, uint8_t* terr
         std::vector<uint8_t> tie(static_cast<size_t>(nbk) * kQPer), bb(static_cast<size_t>(nbk) * kQPer);
         for (int i = 0; i < nbk * 16; ++i) biome_column(seeds, bch.data(), bcol.data(), i);
         stage_mark("biome_columns");
-        for (int i = 0; i < nbk * kQPer; ++i) biome_target(seeds, bch.data(), bcol.data(), targets.data(), i);
+        std::vector<int64_t> tvary(static_cast<size_t>(nbk) * kQPer);
+        for (int i = 0; i < nbk * kQPer; ++i) biome_target(seeds, bch.data(), bcol.data(), targets.data(), i, tvary.data());
         const int ntop = M.treef ? std::min(mctgen::kTreeNodes, 384) : 0;
         const int G = (M.treef && kOneVarying) ? M.col_groups[kVaryCoord] : 0;
         if (G > 0) {
             std::vector<GroupRes> gres(static_cast<size_t>(nbk) * 16 * G);
             for (int i = 0; i < nbk * 16 * G; ++i) biome_group_search(M, M.treef, ntop, targets.data(), lstar.data(), mind.data(), gres.data(), G, i);
-            for (int i = 0; i < nbk * kQPer; ++i) biome_quart_final(M, targets.data(), gres.data(), lstar.data(), mind.data(), G, i);
+            for (int i = 0; i < nbk * kQPer; ++i) biome_quart_final(M, tvary.data(), gres.data(), lstar.data(), mind.data(), G, i);
         } else {
             for (int i = 0; i < nbk * 16; ++i) biome_search_column(M, M.treef, ntop, targets.data(), lstar.data(), mind.data(), i);
         }

@@ -122,9 +122,10 @@ __global__ __launch_bounds__(256) void k_biome_columns(const SeedDev* seeds, con
     for (int idx = global_id(); idx < items; idx += global_threads()) biome_column(seeds, bch, bcol, idx);
 }
 
-__global__ __launch_bounds__(256) void k_biome_targets(const SeedDev* seeds, const ChunkRef* bch, const int* nbk, const float* bcol, int64_t* targets) {
+__global__ __launch_bounds__(256) void k_biome_targets(const SeedDev* seeds, const ChunkRef* bch, const int* nbk, const float* bcol, int64_t* targets,
+                                                       int64_t* tvary) {
     const int items = *nbk * kQPer;
-    for (int idx = global_id(); idx < items; idx += global_threads()) biome_target(seeds, bch, bcol, targets, idx);
+    for (int idx = global_id(); idx < items; idx += global_threads()) biome_target(seeds, bch, bcol, targets, idx, tvary);
 }
 
 // The upper levels of the float R-tree (the first kTreeTop nodes of the breadth-first order,
@@ -163,9 +164,9 @@ __global__ __launch_bounds__(256) void k_biome_groups(const int* nbk, int G, con
     if (idx < items) biome_group_search(c_model, top, ntop, targets, lstar, mind, res, G, idx);
 }
 
-__global__ __launch_bounds__(256) void k_biome_quart_final(const int* nbk, int G, const int64_t* targets, const GroupRes* res, int32_t* lstar, int64_t* mind) {
+__global__ __launch_bounds__(256) void k_biome_quart_final(const int* nbk, int G, const int64_t* tvary, const GroupRes* res, int32_t* lstar, int64_t* mind) {
     const int items = *nbk * kQPer;
-    for (int idx = global_id(); idx < items; idx += global_threads()) biome_quart_final(c_model, targets, res, lstar, mind, G, idx);
+    for (int idx = global_id(); idx < items; idx += global_threads()) biome_quart_final(c_model, tvary, res, lstar, mind, G, idx);
 }
 
 // Quarts whose targets are beyond the float tree's exact range: the int64 search.

@@ -572,7 +572,16 @@ constexpr int kBiomeRunDefault = 8;
 MCT_HD MCT_FI int biome_runs(int run_len) { return (kQH + run_len - 1) / run_len; }
 
 // Climate target of one quart (idx = biome chunk * kQPer + (qy * 4 + qz) * 4 + qx).
-MCT_HD MCT_FI void biome_target(const SeedDev* seeds, const ChunkRef* bch, const float* bcol, int64_t* targets, int idx) {
+// Climate target coordinates that may vary along a quart column: all but the column-record reads
+// the code generator marks and the offset coordinate (always 0).
+constexpr int kMayVary = 0x3F & ~mctgen::kClimateColumnConst;
+constexpr bool kOneVarying = (kMayVary & (kMayVary - 1)) == 0;
+constexpr int kVaryCoord = kMayVary == 0 ? 6 : (kMayVary & 1 ? 0 : kMayVary & 2 ? 1 : kMayVary & 4 ? 2 : kMayVary & 8 ? 3 : kMayVary & 16 ? 4 : 5);
+
+// tvary (nullable) gets the quart's varying coordinate (kVaryCoord) again, contiguous per quart:
+// the per-quart final minimum reads that one value, 8 bytes where the strided target costs a
+// memory sector.
+MCT_HD MCT_FI void biome_target(const SeedDev* seeds, const ChunkRef* bch, const float* bcol, int64_t* targets, int idx, int64_t* tvary = nullptr) {
     const int b = idx / kQPer, q = idx % kQPer;
     const int qx = q & 3, qz = (q >> 2) & 3, qy = q >> 4;
     const ChunkRef c = bch[b];
@@ -580,8 +589,13 @@ MCT_HD MCT_FI void biome_target(const SeedDev* seeds, const ChunkRef* bch, const
     float clim[6];
     mctgen::eval_climate(seeds[c.seed].oct, c.x * 16 + qx * 4, kMinY + qy * 4, c.z * 16 + qz * 4, cv, clim);
     int64_t* t = targets + static_cast<size_t>(idx) * 7;
-    for (int k = 0; k < 6; ++k) t[k] = quantize(clim[k]);
+    int64_t tv = 0;
+    for (int k = 0; k < 6; ++k) {
+        t[k] = quantize(clim[k]);
+        if (k == kVaryCoord) tv = t[k];
+    }
     t[6] = 0;
+    if (tvary) tvary[idx] = tv;
 }
 
 // L* and its distance for a run of run_len quarts down one quart column; each search is
@@ -635,12 +649,6 @@ MCT_HD inline void biome_search_wide(const ModelDev& M, const int64_t* targets, 
     lstar[idx] = tree_first_min(M.tree, targets + static_cast<size_t>(idx) * 7, INT64_MAX, d);
     mind[idx] = d;
 }
-
-// Climate target coordinates that may vary along a quart column: all but the column-record reads
-// the code generator marks and the offset coordinate (always 0).
-constexpr int kMayVary = 0x3F & ~mctgen::kClimateColumnConst;
-constexpr bool kOneVarying = (kMayVary & (kMayVary - 1)) == 0;
-constexpr int kVaryCoord = kMayVary == 0 ? 6 : (kMayVary & 1 ? 0 : kMayVary & 2 ? 1 : kMayVary & 4 ? 2 : kMayVary & 8 ? 3 : kMayVary & 16 ? 4 : 5);
 
 // L* and its distance for all quarts of one quart column (idx = biome chunk * 16 + qz * 4 + qx).
 // Along a column usually one climate coordinate varies (the depth) and the leaves take a few
@@ -787,12 +795,13 @@ MCT_HD inline void biome_group_search(const ModelDev& M, const TreeNodeF* top, i
 }
 
 // L* and its distance of quart idx (biome chunk * kQPer + quart) from its column's G groups.
-MCT_HD inline void biome_quart_final(const ModelDev& M, const int64_t* targets, const GroupRes* res, int32_t* lstar, int64_t* mind, int G, int idx) {
+// tvary: each quart's varying coordinate (biome_target's tvary).
+MCT_HD inline void biome_quart_final(const ModelDev& M, const int64_t* tvary, const GroupRes* res, int32_t* lstar, int64_t* mind, int G, int idx) {
     const int b = idx / kQPer, q = idx - b * kQPer;
     const GroupRes* r = res + static_cast<size_t>(b * 16 + (q & 15)) * G;
     if (r[0].leaf == -3) return;
     const int k = kVaryCoord;
-    const int64_t tv = targets[static_cast<size_t>(idx) * 7 + k];
+    const int64_t tv = tvary[idx];
     if (tv < -static_cast<int64_t>(kTreeExact) || tv > static_cast<int64_t>(kTreeExact)) {
         lstar[idx] = -2;
         mind[idx] = -1;
