@@ -15,7 +15,7 @@ with these files without copying the outputs to the host.
                    spiral order): terrain and biome digest of every chunk, in request order
   c5_digests.npz   the stress configuration: all 2^20 chunks of the 1,024-seed list and all
                    2^24 biome-only block queries (the reference answers a query by generating
-                   its chunk's biomes, about 100 s per 2^20 queries on 8 cores). Stored as
+                   its chunk's biomes, about 1,450 s for the 2^24 on 8 cores). Stored as
                    digests of digests: group g of
                    GROUP chunks holds digest(terrain digests of its chunks as 8-byte words),
                    likewise for biomes, and group g of QGROUP queries digest(its answer bytes).
