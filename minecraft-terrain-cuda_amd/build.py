@@ -30,7 +30,7 @@ HIP_FLAGS = ['--offload-arch=gfx950', '-O3', '-std=c++17', '-fPIC', '-shared',
              '-fhip-fp32-correctly-rounded-divide-sqrt',
              '-Xclang', '-target-feature', '-Xclang', '-packed-fp32-ops',
              '-Wno-unused-command-line-argument']
-HOST_FLAGS = ['-std=c++17', '-O2', '-fPIC', '-shared', '-ffp-contract=off', '-fno-fast-math']
+HOST_FLAGS = ['-std=c++17', '-O2', '-fPIC', '-shared', '-pthread', '-ffp-contract=off', '-fno-fast-math']
 
 
 def run(cmd, log=None):

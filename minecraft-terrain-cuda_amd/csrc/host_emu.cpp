@@ -9,6 +9,7 @@
 #include <memory>
 #include <stdexcept>
 #include <string>
+#include <thread>
 #include <unordered_map>
 #include <vector>
 
@@ -125,6 +126,21 @@ void dedupe(const std::vector<uint64_t>& keys, std::vector<uint64_t>& uniq, std:
     for (size_t i = 0; i < keys.size(); ++i) map[i] = static_cast<int>(std::lower_bound(uniq.begin(), uniq.end(), keys[i]) - uniq.begin());
 }
 
+// Mismatches (bit patterns) of the device's squeeze division against t / 24.0f over the
+// floats of magnitude bits [lo, hi), both signs. Compiled twice: plain, and for CPUs with a
+// fused multiply-add instruction, where __builtin_fmaf is that instruction instead of a call.
+MCT_FI int64_t squeeze_div_range(uint32_t lo, uint32_t hi) {
+    int64_t bad = 0;
+    for (uint32_t u = lo; u < hi; ++u)
+        for (uint32_t s = 0; s < 2; ++s) {
+            const float t = u2f(u | (s << 31));
+            bad += f2u(squeeze_div24(t)) != f2u(t / 24.0f) ? 1 : 0;
+        }
+    return bad;
+}
+int64_t squeeze_div_range_plain(uint32_t lo, uint32_t hi) { return squeeze_div_range(lo, hi); }
+__attribute__((target("fma"))) int64_t squeeze_div_range_fma(uint32_t lo, uint32_t hi) { return squeeze_div_range(lo, hi); }
+
 }  // namespace
 
 extern "C" {
@@ -201,6 +217,75 @@ int mcte_stage_ops(int i, const char** name, uint64_t* ops) {
         if (ops) std::memcpy(ops, g_stages[static_cast<size_t>(i)].ops, sizeof g_stages[0].ops);
     }
     return n;
+}
+
+// The device's divisions by constants against the IEEE division, bit for bit:
+//   out[0]      squeeze_div24 over every float in [-1, 1] (threads host threads);
+//   out[1 + i]  div_known(x, d_i, RN(1 / d_i)) over every half-integer |x| <= 2^19, for the
+//               aquifer divisors d = 25, 1.5, 2.5, 3, 10;
+//   out[6 + i]  the uncorrected product x * RN(1 / d_i) over the same x, which the check must
+//               see differ (its own sensitivity);
+//   out[11]     the device's fused gradient terms gmad(g, v, c) = fma(g, v, c) against the
+//               unfused g * v + c, for g in {-1, 0, 1}, over 2^24 pseudo-random (v, c) with
+//               |v| < 2 and |c| < 4 (the dots' fractions and partial sums) and every pair of
+//               +-0, +-2^-149, +-2^-126, +-1.
+// Mismatch counts; all but out[6..10] are zero when the device's rewrites are exact.
+int mcte_check_divisions(int threads, int64_t* out) {
+    const uint32_t end = 0x3F800001u;   // magnitudes 0 .. 1.0f
+    threads = std::max(1, std::min(threads, 64));
+    const bool fma = __builtin_cpu_supports("fma");
+    std::vector<int64_t> part(static_cast<size_t>(threads), 0);
+    std::vector<std::thread> pool;
+    for (int t = 0; t < threads; ++t)
+        pool.emplace_back([&, t] {
+            const uint32_t lo = static_cast<uint32_t>(static_cast<uint64_t>(end) * t / threads);
+            const uint32_t hi = static_cast<uint32_t>(static_cast<uint64_t>(end) * (t + 1) / threads);
+            part[static_cast<size_t>(t)] = fma ? squeeze_div_range_fma(lo, hi) : squeeze_div_range_plain(lo, hi);
+        });
+    for (auto& th : pool) th.join();
+    out[0] = 0;
+    for (int64_t v : part) out[0] += v;
+    const double ds[5] = {25.0, 1.5, 2.5, 3.0, 10.0};
+    for (int i = 0; i < 5; ++i) {
+        const double d = ds[i], y = 1.0 / d;
+        int64_t bad = 0, raw = 0;
+        for (int64_t k = -(int64_t(1) << 20); k <= (int64_t(1) << 20); ++k) {
+            const double x = static_cast<double>(k) * 0.5;
+            uint64_t a, b, c;
+            const double q = x / d, f = div_known(x, d, y), p = x * y;
+            std::memcpy(&a, &q, 8);
+            std::memcpy(&b, &f, 8);
+            std::memcpy(&c, &p, 8);
+            bad += a != b ? 1 : 0;
+            raw += a != c ? 1 : 0;
+        }
+        out[1 + i] = bad;
+        out[6 + i] = raw;
+    }
+    int64_t gbad = 0;
+    const auto gcheck = [&gbad](float v, float c) {
+        for (int g = -1; g <= 1; ++g) {
+            const float gf = static_cast<float>(g);
+            const float a = __builtin_fmaf(gf, v, c), b = gf * v + c;
+            gbad += f2u(a) != f2u(b) ? 1 : 0;
+        }
+    };
+    uint64_t st = 0;
+    for (int i = 0; i < (1 << 24); ++i) {
+        uint64_t r = (st += 0x9E3779B97F4A7C15ull);   // SplitMix64
+        r = (r ^ (r >> 30)) * 0xBF58476D1CE4E5B9ull;
+        r = (r ^ (r >> 27)) * 0x94D049BB133111EBull;
+        r ^= r >> 31;
+        // sign, an exponent below 128 (magnitude < 2) or 129 (< 4), any mantissa
+        const uint32_t v = (static_cast<uint32_t>(r) & 0x807FFFFFu) | ((static_cast<uint32_t>(r >> 23) % 128u) << 23);
+        const uint32_t c = (static_cast<uint32_t>(r >> 32) & 0x807FFFFFu) | ((static_cast<uint32_t>(r >> 55) % 129u) << 23);
+        gcheck(u2f(v), u2f(c));
+    }
+    const float sp[8] = {0.0f, -0.0f, 0x1p-149f, -0x1p-149f, 0x1p-126f, -0x1p-126f, 1.0f, -1.0f};
+    for (float v : sp)
+        for (float c : sp) gcheck(v, c);
+    out[11] = gbad;
+    return 0;
 }
 
 int mcte_counts_ops() {

@@ -134,10 +134,56 @@ MCT_HD MCT_FI float smooth(float t) {
 
 MCT_HD MCT_FI float cubef(float x) { return (x * x) * x; }
 
+// ---- division by a constant ------------------------------------------------------------
+// x / d from y = RN(1/d): the quotient q = RN(x * y), its remainder r = q * d - x (exact
+// under one fused multiply-add) and one fused correction q - r * y. This is the one-FMA
+// method of Brisebarre, Muller and Raina, "Accelerating correctly rounded floating-point
+// division when the divisor is known in advance" (IEEE Trans. Computers 53(8), 2004). It is
+// not correctly rounded for every x and d, so each caller names the input domain on which it
+// equals the IEEE division bit for bit, checked exhaustively on the CPU
+// (mcte_check_divisions). The remainder is taken as q * d - x, not x - q * d, so that a
+// zero quotient keeps the sign of x. The device's IEEE division is about ten instructions,
+// one of them at quarter rate; this is three.
+MCT_HD MCT_FI float div_known(float x, float d, float y) {
+    const float q = x * y;
+    const float r = __builtin_fmaf(q, d, -x);
+    return __builtin_fmaf(-r, y, q);
+}
+MCT_HD MCT_FI double div_known(double x, double d, double y) {
+    const double q = x * y;
+    const double r = __builtin_fma(q, d, -x);
+    return __builtin_fma(-r, y, q);
+}
+
+// t / 24 as the squeeze takes it on the device: div_known where |t| >= 2^-120 and the IEEE
+// division below, where the quotient is subnormal and div_known can be one unit off. Equal to
+// t / 24.0f for every float t in [-1, 1], the squeeze's range. The division runs only in
+// waves that hold such a t.
+MCT_HD MCT_FI float squeeze_div24(float t) {
+    const bool tiny = fabsf(t) < 0x1p-120f;
+    float q = div_known(t, 24.0f, 1.0f / 24.0f);
+#if defined(__HIP_DEVICE_COMPILE__)
+    if (__builtin_amdgcn_ballot_w64(tiny)) {
+        // an opaque copy, so that the division stays behind the branch instead of being
+        // computed for every wave and selected
+        float tt = t;
+        asm volatile("" : "+v"(tt));
+        if (tiny) q = tt / 24.0f;
+    }
+#else
+    if (tiny) q = t / 24.0f;
+#endif
+    return q;
+}
+
 // ---- density-function unary ops (UnaryFunction) ----------------------------------
 MCT_HD MCT_FI float op_squeeze(float x) {
     const float c = clampf(x, -1.0f, 1.0f);
+#if defined(__HIP_DEVICE_COMPILE__)
+    return c / 2.0f - squeeze_div24(cubef(c));
+#else
     return c / 2.0f - cubef(c) / 24.0f;
+#endif
 }
 MCT_HD MCT_FI float op_leaky(float x, float k) { return x > 0.0f ? x : x * k; }
 // Mth.sqrt(float) = (float) Math.sqrt(x): the double sqrt of a float input, rounded to float.

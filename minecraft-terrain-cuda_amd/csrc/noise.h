@@ -140,12 +140,21 @@ MCT_HD MCT_FI Hash8 hash8_ab(const Perm& P, int a, int b, int32_t fy, int32_t fz
     return h;
 }
 
+// g * v + c for a gradient component g in {-1, 0, 1}. The product is exact (v, -v or a zero
+// with the product's sign), so RN(g * v + c) under one fused multiply-add equals the unfused
+// RN(RN(g * v) + c), the sign of a zero sum included: the device fuses, the host does not,
+// and both give the reference's sums bit for bit.
+MCT_HD MCT_FI float gmad(uint32_t packed, int h, float v, float c) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __builtin_fmaf(gcomp(packed, h), v, c);
+#else
+    return gcomp(packed, h) * v + c;
+#endif
+}
+
 // Gradient dot for the point path: (gx*x + gy*y) + gz*z.
 MCT_HD MCT_FI float dot3(int h, float x, float y, float z) {
-    const float a = gcomp(kGX, h) * x;
-    const float b = gcomp(kGY, h) * y;
-    const float c = gcomp(kGZ, h) * z;
-    return (a + b) + c;
+    return gmad(kGZ, h, z, gmad(kGY, h, y, gcomp(kGX, h) * x));
 }
 
 // PerlinNoise.sampleAndLerp (point path); ty is the y fraction the smoothstep uses.
@@ -240,16 +249,16 @@ MCT_HD MCT_FI Axis axis_smeared(int32_t b, double s, double offset, double smear
 MCT_HD MCT_FI float volume_corners(const Hash8& h, const Axis& X, const Axis& Y, const Axis& Z) {
     const float rx = X.r, rz = Z.r, rx1 = rx - 1.0f, rz1 = rz - 1.0f;
     const float ry = Y.r, ry1 = ry - 1.0f;
-#define MCT_XZ(hh, xx, zz) ((gcomp(kGX, hh) * (xx)) + (gcomp(kGZ, hh) * (zz)))
-    const float c000 = MCT_XZ(h.h000, rx, rz) + gcomp(kGY, h.h000) * ry;
-    const float c100 = MCT_XZ(h.h100, rx1, rz) + gcomp(kGY, h.h100) * ry;
-    const float c010 = MCT_XZ(h.h010, rx, rz) + gcomp(kGY, h.h010) * ry1;
-    const float c110 = MCT_XZ(h.h110, rx1, rz) + gcomp(kGY, h.h110) * ry1;
-    const float c001 = MCT_XZ(h.h001, rx, rz1) + gcomp(kGY, h.h001) * ry;
-    const float c101 = MCT_XZ(h.h101, rx1, rz1) + gcomp(kGY, h.h101) * ry;
-    const float c011 = MCT_XZ(h.h011, rx, rz1) + gcomp(kGY, h.h011) * ry1;
-    const float c111 = MCT_XZ(h.h111, rx1, rz1) + gcomp(kGY, h.h111) * ry1;
-#undef MCT_XZ
+#define MCT_XZY(hh, xx, yy, zz) gmad(kGY, hh, yy, gmad(kGZ, hh, zz, gcomp(kGX, hh) * (xx)))
+    const float c000 = MCT_XZY(h.h000, rx, ry, rz);
+    const float c100 = MCT_XZY(h.h100, rx1, ry, rz);
+    const float c010 = MCT_XZY(h.h010, rx, ry1, rz);
+    const float c110 = MCT_XZY(h.h110, rx1, ry1, rz);
+    const float c001 = MCT_XZY(h.h001, rx, ry, rz1);
+    const float c101 = MCT_XZY(h.h101, rx1, ry, rz1);
+    const float c011 = MCT_XZY(h.h011, rx, ry1, rz1);
+    const float c111 = MCT_XZY(h.h111, rx1, ry1, rz1);
+#undef MCT_XZY
     return lerp3f(X.a, Y.a, Z.a, c000, c100, c010, c110, c001, c101, c011, c111);
 }
 

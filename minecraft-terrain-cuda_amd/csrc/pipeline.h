@@ -1115,6 +1115,18 @@ MCT_HD inline void aquifer_cell(const SeedDev* seeds, const ChunkRef* ch, const 
     cell_fluid[static_cast<size_t>(idx) * 2 + 1] = res.type;
 }
 
+// x / d for the pressures' divisions, whose x are half-integers of magnitude below 2^18
+// (fluid levels, y and 0.5 offsets; levels lie in [kWayBelow, kHeight]) and whose d are 25,
+// 1.5, 2.5, 3 and 10. On the device div_known, equal to the IEEE division on every
+// half-integer of magnitude up to 2^19 for these d (checked exhaustively on the CPU).
+MCT_HD MCT_FI double aq_div(double x, double d) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return div_known(x, d, 1.0 / d);
+#else
+    return x / d;
+#endif
+}
+
 MCT_HD MCT_FI double aquifer_pressure(const Octave* O, int x, int y, int z, double& barrier, bool& barrier_set, Fluid a, Fluid b) {
     const int t1 = fluid_at(a, y), t2 = fluid_at(b, y);
     MCT_OPS(0, 0, 6, 0);   // two fluid picks, the lava / water pair test
@@ -1130,10 +1142,10 @@ MCT_HD MCT_FI double aquifer_pressure(const Octave* O, int x, int y, int z, doub
     double gradient;
     if (above > 0.0) {
         const double center = 0.0 + towards;
-        gradient = center > 0.0 ? center / 1.5 : center / 2.5;
+        gradient = center > 0.0 ? aq_div(center, 1.5) : aq_div(center, 2.5);
     } else {
         const double center = 3.0 + towards;
-        gradient = center > 0.0 ? center / 3.0 : center / 10.0;
+        gradient = center > 0.0 ? aq_div(center, 3.0) : aq_div(center, 10.0);
     }
     double noise;
     if (!(gradient < -2.0) && !(gradient > 2.0)) {
@@ -1228,7 +1240,7 @@ template <class Cells>
 MCT_HD inline int aquifer_back(const Octave* O, const Cells& C, int x, int y, int z, double density, const AqNearest& nr) {
     const Fluid s1 = C.fluid(nr.i1);
     const int fluid = fluid_at(s1, y);
-    const double sim12 = 1.0 - static_cast<double>(nr.d2 - nr.d1) / 25.0;
+    const double sim12 = 1.0 - aq_div(static_cast<double>(nr.d2 - nr.d1), 25.0);
     MCT_OPS(5, 0, 2, 4);   // sim12, b12 and its test; the fluids of cells 1 and 2
     double barrier = 0.0;
     bool barrier_set = false;
@@ -1236,14 +1248,14 @@ MCT_HD inline int aquifer_back(const Octave* O, const Cells& C, int x, int y, in
     const double b12 = sim12 * aquifer_pressure(O, x, y, z, barrier, barrier_set, s1, s2);
     if (density + b12 > 0.0) return -1;
     const Fluid s3 = C.fluid(nr.i3);
-    const double sim13 = 1.0 - static_cast<double>(nr.d3 - nr.d1) / 25.0;
+    const double sim13 = 1.0 - aq_div(static_cast<double>(nr.d3 - nr.d1), 25.0);
     MCT_OPS(4, 0, 1, 2);   // sim13 and its test; the fluid of cell 3
     if (sim13 > 0.0) {
         MCT_OPS(4, 0, 0, 0);
         const double b13 = sim12 * sim13 * aquifer_pressure(O, x, y, z, barrier, barrier_set, s1, s3);
         if (density + b13 > 0.0) return -1;
     }
-    const double sim23 = 1.0 - static_cast<double>(nr.d3 - nr.d2) / 25.0;
+    const double sim23 = 1.0 - aq_div(static_cast<double>(nr.d3 - nr.d2), 25.0);
     MCT_OPS(4, 0, 1, 0);
     if (sim23 > 0.0) {
         MCT_OPS(4, 0, 0, 0);

@@ -44,6 +44,7 @@ def _bind(L):
     L.mcte_generate.argtypes = [P, I64, P, I, P, P, P, P, P]
     L.mcte_biome_query.argtypes = [P, I64, P, I, P]
     L.mcte_stage_ops.argtypes = [I, ctypes.POINTER(ctypes.c_char_p), P]
+    L.mcte_check_divisions.argtypes = [I, P]
     return L
 
 
@@ -134,6 +135,23 @@ def op_counts(seed, chunks):
         tot = {c: int(v) for c, v in zip(OP_CLASSES, ops)}
         out[stage] = {'unit': unit, 'units': units[unit], 'ops': tot, 'per_unit': {c: v / units[unit] for c, v in tot.items()}}
     return out
+
+
+DIVISORS = (25.0, 1.5, 2.5, 3.0, 10.0)
+
+
+def check_divisions(threads=8):
+    """The device's rewrites of divisions and products, checked bit for bit on the CPU with the
+    same source (libmct_host.so, mcte_check_divisions): mismatch counts of
+      'squeeze'   t / 24 (squeeze_div24) over every float t in [-1, 1];
+      'aquifer'   x / d (div_known) for d in DIVISORS over every half-integer |x| <= 2^19;
+      'raw'       the uncorrected x * RN(1/d) over the same x (must mismatch: sensitivity);
+      'gradient'  fma(g, v, c) against g * v + c for g in {-1, 0, 1} (noise dots)."""
+    out = np.zeros(12, np.int64)
+    if lib().mcte_check_divisions(int(threads), out.ctypes.data) != 0:
+        raise RuntimeError('mcte_check_divisions failed')
+    return {'squeeze': int(out[0]), 'aquifer': dict(zip(DIVISORS, out[1:6].tolist())),
+            'raw': dict(zip(DIVISORS, out[6:11].tolist())), 'gradient': int(out[11])}
 
 
 class HostGenerator:

@@ -61,3 +61,18 @@ def test_operation_counts_add_over_independent_chunks():
             assert both[st]['ops'][c] == one[0][st]['ops'][c] + one[1][st]['ops'][c], (st, c)
     assert both['fill']['ops']['f32'] > 0 and both['corners']['ops']['f64'] > 0
     assert gen_host.lib().mcte_counts_ops() == 0
+
+
+def test_device_division_rewrites_are_exact():
+    """The device computes the squeeze's t / 24 and the aquifer pressures' divisions by 25, 1.5,
+    2.5, 3 and 10 as a product corrected once by a fused multiply-add, and fuses the noise
+    gradient terms g * v + c. Each rewrite equals the IEEE operation bit for bit on its whole
+    input domain (every float of the squeeze's range, every half-integer the pressures can
+    take); the uncorrected product does not, which shows the check can fail."""
+    if not os.path.exists(gen_host.LIB_PATH):
+        pytest.skip('libmct_host.so not built')
+    r = gen_host.check_divisions(threads=8)
+    assert r['squeeze'] == 0
+    assert all(v == 0 for v in r['aquifer'].values()), r['aquifer']
+    assert all(v > 0 for v in r['raw'].values()), r['raw']
+    assert r['gradient'] == 0
