@@ -162,7 +162,8 @@ int mcte_seed_tables(void* p, int64_t seed, double* offsets, uint8_t* perms, int
     const SeedDev& S = *e.seed;
     for (int l = 0; l < mctgen::kNumLayers; ++l) {
         if (offsets) { offsets[3 * l] = S.oct[l].ox; offsets[3 * l + 1] = S.oct[l].oy; offsets[3 * l + 2] = S.oct[l].oz; }
-        if (perms) std::memcpy(perms + 256 * l, S.oct[l].p, 256);
+        if (perms)
+            for (int i = 0; i < 256; ++i) perms[256 * l + i] = static_cast<uint8_t>(S.oct[l].q[i] & 0xFF);
     }
     if (zoom_seed) *zoom_seed = S.zoom_seed;
     if (positional6) {

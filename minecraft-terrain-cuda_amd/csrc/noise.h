@@ -15,12 +15,16 @@
 
 namespace mct {
 
-// One gradient-noise octave of one seed: random origin and permutation, the permutation stored
-// twice (p[256 + i] = p[i]) so that the lattice hashes index it without masking: with the
-// cell coordinates reduced mod 256 once, every lookup index is at most 511.
+// One gradient-noise octave of one seed: random origin and permutation p. The permutation is
+// stored as entry pairs q[i] = p[i & 255] | p[(i + 1) & 255] << 8 for i in [0, 511]: the
+// lattice hashes index it without masking (with the cell coordinates reduced mod 256 once,
+// every lookup index is at most 511, as in a doubled table), and each of their lookups goes
+// in adjacent pairs (p[k], p[k + 1]), one 16-bit load each. A hash sample then takes 6 or 7
+// loads instead of 12 or 14; the loads are what bounds the noise kernels (their vector
+// memory data path, not their arithmetic).
 struct Octave {
     double ox, oy, oz;
-    uint8_t p[512];
+    uint16_t q[512];
 };
 
 // GRADIENT[h & 15] components as signed 2-bit fields (two's complement: -1 = 0b11), entry h
@@ -64,11 +68,12 @@ MCT_HD MCT_FI double wrap(double x) {
 // [0, 256), never NaN): Mth.floor without the saturation and NaN cases, which cannot occur.
 MCT_HD MCT_FI int32_t lattice_floor(double v) { return static_cast<int32_t>(floor(v)); }
 
-// Permutation lookups of one octave, index in [0, 511] (the doubled table). PermRef reads
-// through the octave pointer.
+// Permutation lookups of one octave, index in [0, 511]: P(i) = p[i & 255], P.pair(i) =
+// P(i) | P(i + 1) << 8. PermRef reads through the octave pointer.
 struct PermRef {
-    const uint8_t* p;
-    MCT_HD MCT_FI int operator()(int32_t i) const { return p[i]; }
+    const uint16_t* q;
+    MCT_HD MCT_FI int operator()(int32_t i) const { return q[i] & 0xFF; }
+    MCT_HD MCT_FI uint32_t pair(int32_t i) const { return q[i]; }
 };
 
 #if defined(__HIP_DEVICE_COMPILE__)
@@ -86,58 +91,56 @@ __device__ MCT_FI __amdgpu_buffer_rsrc_t octave_rsrc(const Octave* table, int n)
     return __builtin_amdgcn_make_buffer_rsrc(const_cast<Octave*>(table), 0, n * static_cast<int>(sizeof(Octave)), 0x00020000);
 }
 
-// PermBuf: lookups of octave k of a wave-uniform table as raw buffer byte loads, the
-// octave's byte offset as scalar offset and the index as 32-bit vector offset (no per-lane
-// 64-bit address arithmetic).
+// PermBuf: lookups of octave k of a wave-uniform table as raw buffer 16-bit loads, the
+// octave's byte offset as scalar offset and the pair index as 32-bit vector offset (no
+// per-lane 64-bit address arithmetic).
 struct PermBuf {
     __amdgpu_buffer_rsrc_t r;
     int off;
-    __device__ MCT_FI PermBuf(__amdgpu_buffer_rsrc_t rsrc, int k) : r(rsrc), off(k * static_cast<int>(sizeof(Octave)) + static_cast<int>(offsetof(Octave, p))) {}
-    __device__ MCT_FI int operator()(int32_t i) const { return static_cast<int>(__builtin_amdgcn_raw_buffer_load_b8(r, i, off, 0)); }
+    __device__ MCT_FI PermBuf(__amdgpu_buffer_rsrc_t rsrc, int k) : r(rsrc), off(k * static_cast<int>(sizeof(Octave)) + static_cast<int>(offsetof(Octave, q))) {}
+    __device__ MCT_FI uint32_t pair(int32_t i) const { return __builtin_amdgcn_raw_buffer_load_b16(r, i * 2, off, 0); }
+    __device__ MCT_FI int operator()(int32_t i) const { return static_cast<int>(pair(i) & 0xFF); }
 };
 #endif
 
-// Lattice hashes of the 8 cell corners, h[xyz] with bit 4 = x, 2 = y, 1 = z.
+// Lattice hashes of the 8 cell corners, h[xyz] with bit 4 = x, 2 = y, 1 = z. Only bits 0-3
+// of a hash are meaningful (the gradient index h & 15); higher bits may be set.
 struct Hash8 {
     int h000, h100, h010, h110, h001, h101, h011, h111;
 };
 
-// p[(a + f) & 255] for the Java int sums a + f of the reference equals p2[a + (f & 255)] with
-// the doubled table (a <= 255), so each coordinate is reduced once and no lookup is masked.
+// p[(a + f) & 255] for the Java int sums a + f of the reference equals P(a + (f & 255)) (the
+// pair table covers indexes up to 511, a <= 255), so each coordinate is reduced once and no
+// lookup is masked. The lookups go in adjacent pairs (p[k], p[k + 1]), one load each.
+template <class Perm>
+MCT_HD MCT_FI Hash8 hash8_pairs(const Perm& P, int a, int b, int Y, int Z) {
+    const uint32_t A = P.pair(a + Y), B = P.pair(b + Y);   // a0 | a1 << 8, b0 | b1 << 8
+    const uint32_t H0 = P.pair(static_cast<int>(A & 0xFF) + Z), H1 = P.pair(static_cast<int>(B & 0xFF) + Z);
+    const uint32_t H2 = P.pair(static_cast<int>(A >> 8) + Z), H3 = P.pair(static_cast<int>(B >> 8) + Z);
+    // the gradient picks read bits 0-3 of a hash only, so the z = 0 corners keep the pair's
+    // upper byte instead of a mask
+    Hash8 h;
+    h.h000 = static_cast<int>(H0);
+    h.h001 = static_cast<int>(H0 >> 8);
+    h.h100 = static_cast<int>(H1);
+    h.h101 = static_cast<int>(H1 >> 8);
+    h.h010 = static_cast<int>(H2);
+    h.h011 = static_cast<int>(H2 >> 8);
+    h.h110 = static_cast<int>(H3);
+    h.h111 = static_cast<int>(H3 >> 8);
+    return h;
+}
+
 template <class Perm>
 MCT_HD MCT_FI Hash8 hash8(const Perm& P, int32_t fx, int32_t fy, int32_t fz) {
-    const int X = fx & 0xFF, Y = fy & 0xFF, Z = fz & 0xFF;
-    const int a = P(X), b = P(X + 1);
-    const int a0 = P(a + Y), a1 = P(a + Y + 1);
-    const int b0 = P(b + Y), b1 = P(b + Y + 1);
-    Hash8 h;
-    h.h000 = P(a0 + Z);
-    h.h100 = P(b0 + Z);
-    h.h010 = P(a1 + Z);
-    h.h110 = P(b1 + Z);
-    h.h001 = P(a0 + Z + 1);
-    h.h101 = P(b0 + Z + 1);
-    h.h011 = P(a1 + Z + 1);
-    h.h111 = P(b1 + Z + 1);
-    return h;
+    const uint32_t ab = P.pair(fx & 0xFF);
+    return hash8_pairs(P, static_cast<int>(ab & 0xFF), static_cast<int>(ab >> 8), fy & 0xFF, fz & 0xFF);
 }
 
 // hash8 from the first-level entries a = p[x & 255], b = p[(x & 255) + 1].
 template <class Perm>
 MCT_HD MCT_FI Hash8 hash8_ab(const Perm& P, int a, int b, int32_t fy, int32_t fz) {
-    const int Y = fy & 0xFF, Z = fz & 0xFF;
-    const int a0 = P(a + Y), a1 = P(a + Y + 1);
-    const int b0 = P(b + Y), b1 = P(b + Y + 1);
-    Hash8 h;
-    h.h000 = P(a0 + Z);
-    h.h100 = P(b0 + Z);
-    h.h010 = P(a1 + Z);
-    h.h110 = P(b1 + Z);
-    h.h001 = P(a0 + Z + 1);
-    h.h101 = P(b0 + Z + 1);
-    h.h011 = P(a1 + Z + 1);
-    h.h111 = P(b1 + Z + 1);
-    return h;
+    return hash8_pairs(P, a, b, fy & 0xFF, fz & 0xFF);
 }
 
 // g * v + c for a gradient component g in {-1, 0, 1}. The product is exact (v, -v or a zero
@@ -184,7 +187,7 @@ MCT_HD MCT_FI float perlin_point(const Octave& o, const Perm& P, double px, doub
     return sample_lerp(P, fx, fy, fz, rx, ry, rz, ry);
 }
 
-MCT_HD MCT_FI float perlin_point(const Octave& o, double px, double py, double pz) { return perlin_point(o, PermRef{o.p}, px, py, pz); }
+MCT_HD MCT_FI float perlin_point(const Octave& o, double px, double py, double pz) { return perlin_point(o, PermRef{o.q}, px, py, pz); }
 
 // SmearedPerlinNoise.computeFudgeY
 MCT_HD MCT_FI double fudge_y(double original_y, double rel_y, double smear) {
@@ -207,7 +210,7 @@ MCT_HD MCT_FI float smeared_point(const Octave& o, const Perm& P, double smear, 
 }
 
 MCT_HD MCT_FI float smeared_point(const Octave& o, double smear, double px, double py, double pz) {
-    return smeared_point(o, PermRef{o.p}, smear, px, py, pz);
+    return smeared_point(o, PermRef{o.q}, smear, px, py, pz);
 }
 
 // Coordinate setup of one axis of the volume path, the only FP64 work of a sample:
@@ -283,7 +286,7 @@ MCT_HD MCT_FI float perlin_volume(const Octave& o, const Perm& P, int32_t bx, in
 }
 
 MCT_HD MCT_FI float perlin_volume(const Octave& o, int32_t bx, int32_t by, int32_t bz, double xs, double ys) {
-    return perlin_volume(o, PermRef{o.p}, bx, by, bz, xs, ys);
+    return perlin_volume(o, PermRef{o.q}, bx, by, bz, xs, ys);
 }
 
 // SmearedPerlinNoise.addToVolume, one sample.
@@ -296,7 +299,7 @@ MCT_HD MCT_FI float smeared_volume(const Octave& o, const Perm& P, double smear,
 }
 
 MCT_HD MCT_FI float smeared_volume(const Octave& o, double smear, int32_t bx, int32_t by, int32_t bz, double xs, double ys) {
-    return smeared_volume(o, PermRef{o.p}, smear, bx, by, bz, xs, ys);
+    return smeared_volume(o, PermRef{o.q}, smear, bx, by, bz, xs, ys);
 }
 
 }  // namespace mct

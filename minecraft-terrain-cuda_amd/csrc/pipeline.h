@@ -298,7 +298,7 @@ MCT_HD MCT_FI void make_octave(Xoro& r, uint8_t* p, Octave& out) {
         p[i] = p[o + i];
         p[o + i] = t;
     }
-    for (int i = 0; i < 256; ++i) out.p[i] = out.p[256 + i] = p[i];
+    for (int i = 0; i < 512; ++i) out.q[i] = static_cast<uint16_t>(p[i & 255] | p[(i + 1) & 255] << 8);
 }
 
 MCT_HD MCT_FI Positional world_random(int64_t seed) {

@@ -42,7 +42,7 @@ MCT_HD MCT_FI float octave_sum(const Octave* O, Sample sample) {
         }
     }
 #else
-    for (int i = First; i < First + Count; ++i) acc = sample(acc, O[i], PermRef{O[i].p}, i);
+    for (int i = First; i < First + Count; ++i) acc = sample(acc, O[i], PermRef{O[i].q}, i);
 #endif
     return acc;
 }
@@ -127,7 +127,7 @@ MCT_HD MCT_FI AxisXZ site_record(const Octave& o, int32_t x, int32_t z, double x
     MCT_OPS_OF(mctgen::kSiteRecord);
     Axis X = axis_plain(x, xs, o.ox);
     const int xi = X.f & 0xFF;
-    X.f = static_cast<int32_t>(o.p[xi]) | (static_cast<int32_t>(o.p[xi + 1]) << 16);
+    X.f = static_cast<int32_t>(o.q[xi] & 0xFF) | (static_cast<int32_t>(o.q[xi] >> 8) << 16);
     return AxisXZ{X, axis_plain(z, xs, o.oz)};
 }
 
