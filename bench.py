@@ -124,6 +124,11 @@ def traffic(dom, chunks_per_launch):
     return round(p['bytes_per_chunk'] * chunks_per_launch), os.path.relpath(path, ROOT)
 
 
+def note(msg):
+    """One progress line on stderr as each leg ends (the JSON line stays alone on stdout)."""
+    print('bench: %s' % msg, file=sys.stderr, flush=True)
+
+
 def host_cores():
     """Logical cores this process may run on (its affinity set), the thread count of the CPU
     baseline."""
@@ -359,6 +364,7 @@ def main():
             batch_trials[b] = round(n / (time.perf_counter() - t0), 1)
             gt.close()
         a.batch = max(batch_trials, key=batch_trials.get)
+        note('batch trials %s' % batch_trials)
     g = mct.FullTerrainGenerator(max_batch_chunks=a.batch, pipeline_slots=a.slots, seed_capacity=256)
     tuning = None
     if a.tuning_json:
@@ -373,6 +379,7 @@ def main():
         g.generate_device(seeds, chunks, d_t, d_b)
     dt = (time.perf_counter() - t0) / a.steps
     value = n / dt
+    note('plain: %.1f chunks/s' % value)
     # one profiled step on a single-slot generator (no overlap between batches, so each
     # kernel's HIP-event time is its own): stage and kernel times
     gp = mct.FullTerrainGenerator(max_batch_chunks=a.batch, pipeline_slots=1, seed_capacity=256)
@@ -395,6 +402,7 @@ def main():
     counts = count_ops(SEED, chunks, host_cores())
     roof = roofline(kernels, launches, dom, counts, units, n, value)
     roof['traffic'], src = traffic(dom, units['chunks'] / max(launches.get(dom, 1), 1))
+    note('profiled pass and operation counts done')
     if src:
         roof['traffic_source'] = src
     # end to end with results copied to pinned host memory
@@ -412,14 +420,20 @@ def main():
     d2h_s = time.perf_counter() - t0
     d2h = {'gb_per_s': round(n * mct.BLOCKS_PER_CHUNK / d2h_s / 1e9, 2), 'chunks_per_s': round(n / d2h_s, 1)}
     mct.lib().mct_free_pinned(pin)
+    note('host output: %.1f chunks/s' % e2e)
     mct.free_device(d_t)
     mct.free_device(d_b)
     g.close()
     batched = None if (a.quick or a.plain_only) else batched_vs_plain(mct, a.batch, None if a.no_tune else tuning)
+    if batched:
+        note('batched: %s' % batched['chunks_per_s'])
     serve = None
     if not a.quick and not a.no_serve and not a.plain_only:
         serve = served(mct, plain_request_capacity(mct))
+        note('served done')
     mixed = None if (a.quick or a.plain_only or a.no_stress) else stress(mct, a.batch)
+    if mixed:
+        note('stress done')
     if a.copy_calibration:
         copy_calibration(a.copy_calibration)
     cpu = None if a.no_cpu else cpu_baseline()
