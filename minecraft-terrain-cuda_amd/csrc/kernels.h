@@ -556,7 +556,7 @@ __device__ MCT_FI void fill_unit(const FillArgs& A, int unit, int* fill_lds) {
     }
 }
 
-__global__ __launch_bounds__(256) void k_fill(FillArgs A, const BatchDesc* B) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k_fill(FillArgs A, const BatchDesc* B) {
     extern __shared__ __attribute__((aligned(16))) int fill_lds[];
     const int u = chunk_unit(B->n * (256 * kCellsY / static_cast<int>(blockDim.x)));
     if (u < 0) return;   // whole workgroups

@@ -1304,6 +1304,9 @@ MCT_HD MCT_FI void cell_channels(const FillArgs& A, int c, int x, int z, int cy,
     // per column of a cell: the 4 corner columns, the lerp factors; per channel 8 corner values,
     // 6 lerps and the y step
     MCT_OPS(0, 4 + 20 * mctgen::kNumChannels, 12, 4 + 8 * mctgen::kNumChannels);
+    // unrolled, so that all channels' corner loads are in flight together (a rolled loop
+    // waits for each channel's loads before it issues the next channel's)
+#pragma unroll
     for (int k = 0; k < mctgen::kNumChannels; ++k) {
         const float* a = A.corners + (u00 * kCh + k) * kRows + cy;
         const float* b = A.corners + (u10 * kCh + k) * kRows + cy;
