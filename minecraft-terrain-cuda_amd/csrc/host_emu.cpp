@@ -141,6 +141,21 @@ MCT_FI int64_t squeeze_div_range(uint32_t lo, uint32_t hi) {
 int64_t squeeze_div_range_plain(uint32_t lo, uint32_t hi) { return squeeze_div_range(lo, hi); }
 __attribute__((target("fma"))) int64_t squeeze_div_range_fma(uint32_t lo, uint32_t hi) { return squeeze_div_range(lo, hi); }
 
+// java.lang.Math.min / max (float) and Mth.clamp(float) as the JDK writes them: a NaN first
+// operand is returned, then negative zero is below positive zero, then the compare. The
+// references for the select forms in jmath.h.
+float java_minf(float a, float b) {
+    if (a != a) return a;
+    if (a == 0.0f && b == 0.0f && (f2u(b) >> 31)) return b;
+    return a <= b ? a : b;
+}
+float java_maxf(float a, float b) {
+    if (a != a) return a;
+    if (a == 0.0f && b == 0.0f && (f2u(a) >> 31)) return b;
+    return a >= b ? a : b;
+}
+float java_clampf(float v, float lo, float hi) { return v < lo ? lo : java_minf(v, hi); }
+
 }  // namespace
 
 extern "C" {
@@ -286,6 +301,55 @@ int mcte_check_divisions(int threads, int64_t* out) {
     for (float v : sp)
         for (float c : sp) gcheck(v, c);
     out[11] = gbad;
+    return 0;
+}
+
+// jminf / jmaxf / clampf (jmath.h, select forms) against the JDK's branching definitions,
+// bit for bit: out[0] every pair of 20 special floats (zeros, NaNs of both signs, infinities,
+// subnormals, ones) for min and max, and every triple for clamp; out[1] 2^26 pseudo-random
+// bit-pattern pairs for min and max; out[2] clamp of every float v for the bounds (-1, 1),
+// (0, 4), (-0.0, -0.0), (0.0, -0.0) (threads host threads). Mismatch counts.
+int mcte_check_minmax(int threads, int64_t* out) {
+    const float sp[20] = {0.0f, -0.0f, 1.0f, -1.0f, u2f(0x7FC00000u), u2f(0xFFC00000u), u2f(0x7F800001u), u2f(0x7F800000u), u2f(0xFF800000u),
+                          u2f(1u), u2f(0x80000001u), 0x1p-126f, -0x1p-126f, 4.0f, -64.0f, 320.0f, 0.5f, 3.5f, -3.5f, 0x1p127f};
+    int64_t bad = 0;
+    for (float a : sp)
+        for (float b : sp) {
+            bad += f2u(jminf(a, b)) != f2u(java_minf(a, b)) ? 1 : 0;
+            bad += f2u(jmaxf(a, b)) != f2u(java_maxf(a, b)) ? 1 : 0;
+            for (float c : sp) bad += f2u(clampf(a, b, c)) != f2u(java_clampf(a, b, c)) ? 1 : 0;
+        }
+    out[0] = bad;
+    bad = 0;
+    uint64_t st = 7;
+    for (int i = 0; i < (1 << 26); ++i) {
+        uint64_t r = (st += 0x9E3779B97F4A7C15ull);   // SplitMix64
+        r = (r ^ (r >> 30)) * 0xBF58476D1CE4E5B9ull;
+        r = (r ^ (r >> 27)) * 0x94D049BB133111EBull;
+        r ^= r >> 31;
+        const float a = u2f(static_cast<uint32_t>(r)), b = u2f(static_cast<uint32_t>(r >> 32));
+        bad += f2u(jminf(a, b)) != f2u(java_minf(a, b)) ? 1 : 0;
+        bad += f2u(jmaxf(a, b)) != f2u(java_maxf(a, b)) ? 1 : 0;
+    }
+    out[1] = bad;
+    const float bounds[4][2] = {{-1.0f, 1.0f}, {0.0f, 4.0f}, {-0.0f, -0.0f}, {0.0f, -0.0f}};
+    threads = std::max(1, std::min(threads, 64));
+    std::vector<int64_t> part(static_cast<size_t>(threads), 0);
+    std::vector<std::thread> pool;
+    for (int t = 0; t < threads; ++t)
+        pool.emplace_back([&, t] {
+            const uint64_t lo = (uint64_t(1) << 32) * static_cast<uint64_t>(t) / static_cast<uint64_t>(threads);
+            const uint64_t hi = (uint64_t(1) << 32) * static_cast<uint64_t>(t + 1) / static_cast<uint64_t>(threads);
+            int64_t b = 0;
+            for (uint64_t u = lo; u < hi; ++u) {
+                const float v = u2f(static_cast<uint32_t>(u));
+                for (const auto& bd : bounds) b += f2u(clampf(v, bd[0], bd[1])) != f2u(java_clampf(v, bd[0], bd[1])) ? 1 : 0;
+            }
+            part[static_cast<size_t>(t)] = b;
+        });
+    for (auto& th : pool) th.join();
+    out[2] = 0;
+    for (int64_t v : part) out[2] += v;
     return 0;
 }
 

@@ -72,16 +72,18 @@ MCT_HD MCT_FI int32_t fdiv(int32_t a, int32_t b) {
 }
 MCT_HD MCT_FI int32_t fmod_(int32_t a, int32_t b) { return a - fdiv(a, b) * b; }
 
-// Math.min / Math.max (float): NaN wins, -0.0 < +0.0.
+// Math.min / Math.max (float): NaN wins, -0.0 < +0.0. Written as selects, so that the device
+// code has no branch: a NaN a is kept by the last select, a NaN b falls through the compare
+// (a <= NaN is false) and is picked, and of two zeros the one Java returns is picked.
 MCT_HD MCT_FI float jminf(float a, float b) {
-    if (a != a) return a;
-    if (a == 0.0f && b == 0.0f && (f2u(b) >> 31)) return b;
-    return a <= b ? a : b;
+    float r = a <= b ? a : b;
+    r = (a == 0.0f && b == 0.0f && (f2u(b) >> 31)) ? b : r;
+    return a != a ? a : r;
 }
 MCT_HD MCT_FI float jmaxf(float a, float b) {
-    if (a != a) return a;
-    if (a == 0.0f && b == 0.0f && (f2u(a) >> 31)) return b;
-    return a >= b ? a : b;
+    float r = a >= b ? a : b;
+    r = (a == 0.0f && b == 0.0f && (f2u(a) >> 31)) ? b : r;
+    return a != a ? a : r;
 }
 MCT_HD MCT_FI double jmind(double a, double b) {
     if (a != a) return a;
@@ -100,7 +102,12 @@ MCT_HD MCT_FI int32_t jmaxi(int32_t a, int32_t b) { return a >= b ? a : b; }
 MCT_HD MCT_FI float jabsf(float a) { return u2f(f2u(a) & 0x7FFFFFFFu); }
 
 // Mth.clamp(float|double|int): v < lo ? lo : Math.min(v, hi)
-MCT_HD MCT_FI float clampf(float v, float lo, float hi) { return v < lo ? lo : jminf(v, hi); }
+// For v not below lo, Math.min(v, hi) is hi exactly when v > hi, when hi is NaN and v is not,
+// or when both are zeros and hi is -0.0 (a NaN v is kept); as selects.
+MCT_HD MCT_FI float clampf(float v, float lo, float hi) {
+    const bool to_hi = v > hi || (hi != hi && v == v) || (v == 0.0f && hi == 0.0f && (f2u(hi) >> 31));
+    return v < lo ? lo : (to_hi ? hi : v);
+}
 MCT_HD MCT_FI double clampd(double v, double lo, double hi) { return v < lo ? lo : jmind(v, hi); }
 MCT_HD MCT_FI int32_t clampi(int32_t v, int32_t lo, int32_t hi) { return jmini(jmaxi(v, lo), hi); }
 

@@ -521,8 +521,9 @@ __device__ MCT_FI void fill_unit(const FillArgs& A, int unit, int* fill_lds) {
         if (!pend) {
             const int state = s < 0 ? mctgen::kDefaultBlock : s;
             out[yi * 256] = static_cast<uint8_t>(state);
-            if (!((mctgen::kAirMask >> state) & 1u)) top = yi;
-            frag |= static_cast<uint32_t>(block_class(state)) << (2 * yi);
+            const int cl = block_class_small(state);
+            if (cl != kClsAir) top = yi;
+            frag |= static_cast<uint32_t>(cl) << (2 * yi);
         }
         for (int k = 0; k < mctgen::kNumChannels; ++k) val[k] += step[k];
     }
@@ -539,8 +540,8 @@ __device__ MCT_FI void fill_unit(const FillArgs& A, int unit, int* fill_lds) {
         const int s = aquifer_back(O, C, ebx, eby, ebz, static_cast<double>(qd[e]), nr);
         const int state = s < 0 ? mctgen::kDefaultBlock : s;
         A.blocks[static_cast<size_t>(c) * kPer + static_cast<size_t>((cy * 8 + yi) * 256 + ez * 16 + ex)] = static_cast<uint8_t>(state);
-        if (!((mctgen::kAirMask >> state) & 1u)) atomicMax(&hm[lt], yi);
-        const uint32_t cl = static_cast<uint32_t>(block_class(state));
+        const uint32_t cl = static_cast<uint32_t>(block_class_small(state));
+        if (cl != kClsAir) atomicMax(&hm[lt], yi);
         if (cl != kClsSolid) atomicOr(&fr[lt], cl << (2 * yi));
     }
     __syncthreads();

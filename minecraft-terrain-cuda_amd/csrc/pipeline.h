@@ -1613,11 +1613,21 @@ constexpr int kOreRowsPad = (kOreRows + 15) & ~15;
 // Block classes the surface walk distinguishes: solid, air, water (the water state itself) and
 // other fluids.
 constexpr int kClsSolid = 0, kClsAir = 1, kClsWater = 2, kClsFluid = 3;
-MCT_HD MCT_FI int block_class(int s) {
+MCT_HD constexpr int block_class(int s) {
     if ((mctgen::kAirMask >> s) & 1u) return kClsAir;
     if (s == mctgen::kWater) return kClsWater;
     return ((mctgen::kFluidMask >> s) & 1u) ? kClsFluid : kClsSolid;
 }
+// block_class of palette indices 0..31, 2 bits each, so that a state known to be below 32 (the
+// fill's outcomes: the default block, air and the fluids) takes one shift and one mask.
+constexpr uint64_t kClassTable = [] {
+    uint64_t t = 0;
+    for (int s = 0; s < 32; ++s) t |= static_cast<uint64_t>(block_class(s)) << (2 * s);
+    return t;
+}();
+MCT_HD MCT_FI int block_class_small(int s) { return static_cast<int>((kClassTable >> (2 * s)) & 3u); }
+static_assert(mctgen::kDefaultBlock < 32 && mctgen::kAir < 32 && mctgen::kWater < 32 && mctgen::kLava < 32 && mctgen::kDefaultFluid < 32,
+              "the fill's outcomes index the 32-entry class table");
 
 
 struct Surf {

@@ -45,6 +45,7 @@ def _bind(L):
     L.mcte_biome_query.argtypes = [P, I64, P, I, P]
     L.mcte_stage_ops.argtypes = [I, ctypes.POINTER(ctypes.c_char_p), P]
     L.mcte_check_divisions.argtypes = [I, P]
+    L.mcte_check_minmax.argtypes = [I, P]
     return L
 
 
@@ -152,6 +153,17 @@ def check_divisions(threads=8):
         raise RuntimeError('mcte_check_divisions failed')
     return {'squeeze': int(out[0]), 'aquifer': dict(zip(DIVISORS, out[1:6].tolist())),
             'raw': dict(zip(DIVISORS, out[6:11].tolist())), 'gradient': int(out[11])}
+
+
+def check_minmax(threads=8):
+    """Java Math.min / max (float) and Mth.clamp as the device computes them (select forms in
+    jmath.h), against the JDK's own branching definitions on the CPU with the same source
+    (libmct_host.so, mcte_check_minmax): mismatch counts over 'special' value pairs / triples,
+    'random' bit-pattern pairs and 'clamp' of every float for four bound pairs."""
+    out = np.zeros(3, np.int64)
+    if lib().mcte_check_minmax(int(threads), out.ctypes.data) != 0:
+        raise RuntimeError('mcte_check_minmax failed')
+    return {'special': int(out[0]), 'random': int(out[1]), 'clamp': int(out[2])}
 
 
 class HostGenerator:

@@ -76,3 +76,14 @@ def test_device_division_rewrites_are_exact():
     assert all(v == 0 for v in r['aquifer'].values()), r['aquifer']
     assert all(v > 0 for v in r['raw'].values()), r['raw']
     assert r['gradient'] == 0
+
+
+def test_java_min_max_clamp_select_forms():
+    """Math.min / Math.max (float) and Mth.clamp are computed with selects instead of the JDK's
+    early returns; they return the JDK's bits for NaNs of either sign, zeros of either sign,
+    infinities and subnormals, on random bit patterns, and for every float under the clamp
+    bounds the density functions use."""
+    if not os.path.exists(gen_host.LIB_PATH):
+        pytest.skip('libmct_host.so not built')
+    r = gen_host.check_minmax(threads=8)
+    assert r == {'special': 0, 'random': 0, 'clamp': 0}, r
