@@ -293,6 +293,7 @@ __global__ __launch_bounds__(256) void k_corners(const BatchDesc* B, const SeedD
     const int ncol = min(kCornerCols, *count - c0);   // the same in every thread
     if (ncol <= 0) return;
     const KeyCodec& K = B->kc;
+    grad_table_fill(static_cast<int>(threadIdx.x));
     for (int k = threadIdx.x; k < ncol * mctgen::kCornerSites; k += blockDim.x) {
         const int c = k / mctgen::kCornerSites;
         tab[k] = corner_site_axes(K, seeds, ucols, c0 + c, k - c * mctgen::kCornerSites);

@@ -265,6 +265,38 @@ MCT_HD MCT_FI float volume_corners(const Hash8& h, const Axis& X, const Axis& Y,
     return lerp3f(X.a, Y.a, Z.a, c000, c100, c010, c110, c001, c101, c011, c111);
 }
 
+#if defined(__HIPCC__)
+// GRADIENT[h & 15] as the float bits of (gx, gy, gz) and a zero word in LDS, for the kernels
+// that fill it first (grad_table_fill): one 16-byte LDS read per corner instead of three
+// bit-field extracts and three conversions on the VALU. The 16 entries are few enough that the
+// lanes of a wave mostly share them (broadcast reads). The zero word is OR-ed into gy's bits,
+// which keeps the read at 16 bytes (a 12-byte LDS read is several times slower).
+__shared__ int4 g_grad[16];
+__device__ MCT_FI void grad_table_fill(int t) {
+    if (t < 16) g_grad[t] = make_int4(static_cast<int>(f2u(gcomp(kGX, t))), static_cast<int>(f2u(gcomp(kGY, t))), static_cast<int>(f2u(gcomp(kGZ, t))), 0);
+}
+
+// volume_corners with the gradients read from g_grad: the same products and fused sums.
+__device__ MCT_FI float volume_corners_tab(const Hash8& h, const Axis& X, const Axis& Y, const Axis& Z) {
+    const float rx = X.r, rz = Z.r, rx1 = rx - 1.0f, rz1 = rz - 1.0f;
+    const float ry = Y.r, ry1 = ry - 1.0f;
+    const auto dot = [](int hh, float xx, float yy, float zz) {
+        const int4 g = g_grad[hh & 15];
+        const float gx = u2f(static_cast<uint32_t>(g.x)), gy = u2f(static_cast<uint32_t>(g.y | g.w)), gz = u2f(static_cast<uint32_t>(g.z));
+        return __builtin_fmaf(gy, yy, __builtin_fmaf(gz, zz, gx * xx));
+    };
+    const float c000 = dot(h.h000, rx, ry, rz);
+    const float c100 = dot(h.h100, rx1, ry, rz);
+    const float c010 = dot(h.h010, rx, ry1, rz);
+    const float c110 = dot(h.h110, rx1, ry1, rz);
+    const float c001 = dot(h.h001, rx, ry, rz1);
+    const float c101 = dot(h.h101, rx1, ry, rz1);
+    const float c011 = dot(h.h011, rx, ry1, rz1);
+    const float c111 = dot(h.h111, rx1, ry1, rz1);
+    return lerp3f(X.a, Y.a, Z.a, c000, c100, c010, c110, c001, c101, c011, c111);
+}
+#endif
+
 template <class Perm>
 MCT_HD MCT_FI float volume_sample(const Perm& P, const Axis& X, const Axis& Y, const Axis& Z) {
     return volume_corners(hash8(P, X.f, Y.f, Z.f), X, Y, Z);
@@ -274,6 +306,17 @@ MCT_HD MCT_FI float volume_sample(const Perm& P, const Axis& X, const Axis& Y, c
 template <class Perm>
 MCT_HD MCT_FI float volume_sample_xz(const Perm& P, const AxisXZ& A, const Axis& Y) {
     return volume_corners(hash8_ab(P, A.X.f & 0xFFFF, A.X.f >> 16, Y.f, A.Z.f), A.X, Y, A.Z);
+}
+
+// volume_sample_xz for the kernels that hold the gradient table in LDS (on the host, the same
+// function as volume_sample_xz).
+template <class Perm>
+MCT_HD MCT_FI float volume_sample_xz_tab(const Perm& P, const AxisXZ& A, const Axis& Y) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return volume_corners_tab(hash8_ab(P, A.X.f & 0xFFFF, A.X.f >> 16, Y.f, A.Z.f), A.X, Y, A.Z);
+#else
+    return volume_sample_xz(P, A, Y);
+#endif
 }
 
 // PerlinNoise.addToVolume, one sample (before the amplitude multiply).

@@ -89,7 +89,8 @@ MCT_HD MCT_NI float stack_volume_tab_smeared(const Octave* O, const MCT_LDS Axis
 
 // The same with the y axis records read too, from the seed's corner-row table: Yt[(i - First)
 // * Rows] is layer i's record at the corner's row (the records stack_volume_tab(_smeared)
-// computes, tabulated per seed by corner_y_axis).
+// computes, tabulated per seed by corner_y_axis). On the device the gradients come from the
+// LDS table (the corner kernel fills it).
 template <int First, int Count, int Rows>
 MCT_HD MCT_NI float stack_volume_tab_y(const Octave* O, const MCT_LDS AxisXZ* A, const Axis* Yt, float amplitude) {
     return octave_sum<First, Count>(O, [&](float acc, const Octave& o, const auto& P, int i) MCT_CLOSURE {
@@ -97,7 +98,7 @@ MCT_HD MCT_NI float stack_volume_tab_y(const Octave* O, const MCT_LDS AxisXZ* A,
         (void)o;
         const AxisXZ a = A[i - First];
         const Axis Y = Yt[(i - First) * Rows];
-        return acc + (amplitude * mctgen::kLayerAmp[i]) * volume_sample_xz(P, a, Y);
+        return acc + (amplitude * mctgen::kLayerAmp[i]) * volume_sample_xz_tab(P, a, Y);
     });
 }
 
