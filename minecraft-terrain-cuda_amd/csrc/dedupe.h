@@ -243,4 +243,113 @@ __global__ __launch_bounds__(256) void k_unique_scatter(const uint64_t* sorted, 
     if (blockIdx.x == 0 && threadIdx.x == 0) *count = *total;
 }
 
+// ---- dense form -----------------------------------------------------------------------------
+// For a batch whose key space [0, dense_size(K)) is small (contiguous regions: the codec
+// compresses each axis to the batch's ranges), the same outputs without a sort: a flag per
+// possible key, the exclusive scan of the flags is each present key's rank (the keys' order is
+// the dense order), unique[rank] = the key, map[i] = the rank of key i. Six launches whatever the
+// key width; the per-key work is two scattered int accesses into an array of dense_size ints.
+// f holds at least dense_size(K) ints; tsum a tile sum per kDenseTile flags, plus one.
+constexpr int kDenseTile = 2048;   // flags per scan tile: 256 threads x 8
+
+__device__ MCT_FI int dense_n(const KeyCodec* K) { return static_cast<int>(dense_size(*K)); }
+
+__global__ __launch_bounds__(256) void k_dense_clear(const KeyCodec* K, int* f) {
+    const int n = dense_n(K);
+    for (int i = global_id(); i < n; i += global_threads()) f[i] = 0;
+}
+
+__global__ __launch_bounds__(256) void k_dense_mark(const uint64_t* keys, const int* np, const KeyCodec* K, int* f) {
+    const int n = *np;
+    for (int i = global_id(); i < n; i += global_threads()) f[dense_index(*K, keys[i])] = 1;
+}
+
+// Flags set per tile of kDenseTile (workgroups stride over the tiles: any grid).
+__global__ __launch_bounds__(256) void k_dense_tiles(const KeyCodec* K, const int* f, int* tsum) {
+    const int n = dense_n(K), tiles = (n + kDenseTile - 1) / kDenseTile;
+    __shared__ int part[256];
+    for (int tile = static_cast<int>(blockIdx.x); tile < tiles; tile += static_cast<int>(gridDim.x)) {
+        const int base = tile * kDenseTile;
+        int a = 0;
+        for (int j = 0; j < kDenseTile / 256; ++j) {
+            const int i = base + j * 256 + static_cast<int>(threadIdx.x);
+            if (i < n) a += f[i];
+        }
+        part[threadIdx.x] = a;
+        __syncthreads();
+        for (int w = 128; w > 0; w >>= 1) {
+            if (static_cast<int>(threadIdx.x) < w) part[threadIdx.x] += part[threadIdx.x + w];
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) tsum[tile] = part[0];
+        __syncthreads();
+    }
+}
+
+// Exclusive scan of the tile sums in one workgroup; *count = the distinct keys.
+__global__ __launch_bounds__(1024) void k_dense_offsets(const KeyCodec* K, int* tsum, int* count) {
+    const int tiles = (dense_n(K) + kDenseTile - 1) / kDenseTile;
+    __shared__ int part[1024];
+    const int t = static_cast<int>(threadIdx.x);
+    const int per = (tiles + 1023) / 1024;
+    const int b = t * per, e = b + per < tiles ? b + per : tiles;
+    int sum = 0;
+    for (int i = b; i < e; ++i) sum += tsum[i];
+    part[t] = sum;
+    __syncthreads();
+    for (int off = 1; off < 1024; off <<= 1) {
+        const int a = t >= off ? part[t - off] : 0;
+        __syncthreads();
+        part[t] += a;
+        __syncthreads();
+    }
+    int run = part[t] - sum;
+    for (int i = b; i < e; ++i) {
+        const int x = tsum[i];
+        tsum[i] = run;
+        run += x;
+    }
+    if (t == 1023) *count = part[1023];
+}
+
+// Per tile: each set flag becomes its key's rank (the tile's offset plus the set flags before
+// it in the tile) and unique[rank] the key (workgroups stride over the tiles: any grid).
+__global__ __launch_bounds__(256) void k_dense_ranks(const KeyCodec* K, int* f, const int* tsum, uint64_t* unique) {
+    const int n = dense_n(K), tiles = (n + kDenseTile - 1) / kDenseTile;
+    constexpr int kPer = kDenseTile / 256;
+    __shared__ int ts[256];
+    const int t = static_cast<int>(threadIdx.x);
+    for (int tile = static_cast<int>(blockIdx.x); tile < tiles; tile += static_cast<int>(gridDim.x)) {
+        const int base = tile * kDenseTile + t * kPer;   // a thread's kPer consecutive flags
+        int v[kPer];
+        int a = 0;
+        for (int k = 0; k < kPer; ++k) {
+            v[k] = base + k < n ? f[base + k] : 0;
+            a += v[k];
+        }
+        ts[t] = a;
+        __syncthreads();
+        for (int off = 1; off < 256; off <<= 1) {
+            const int x = t >= off ? ts[t - off] : 0;
+            __syncthreads();
+            ts[t] += x;
+            __syncthreads();
+        }
+        int run = tsum[tile] + ts[t] - a;
+        for (int k = 0; k < kPer; ++k) {
+            if (v[k]) {
+                f[base + k] = run;
+                unique[run] = dense_key(*K, static_cast<uint32_t>(base + k));
+                ++run;
+            }
+        }
+        __syncthreads();
+    }
+}
+
+__global__ __launch_bounds__(256) void k_dense_map(const uint64_t* keys, const int* np, const KeyCodec* K, const int* f, int* map) {
+    const int n = *np;
+    for (int i = global_id(); i < n; i += global_threads()) map[i] = f[dense_index(*K, keys[i])];
+}
+
 }  // namespace mct

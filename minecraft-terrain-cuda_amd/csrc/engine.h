@@ -146,6 +146,10 @@ struct SeedCache {
 
 // What one batch program looks like (the graph cache key): the chunk-count bucket the
 // one-shot kernels' grids are sized for, the radix passes of the three dedupes, the outputs.
+// A dedupe whose pass count is kDensePass takes the dense form (its key space fits the lane's
+// dense array) instead of the radix sort.
+constexpr int kDensePass = -1;
+
 struct Shape {
     bool core = false;
     int grid_n = 0;
@@ -201,6 +205,9 @@ struct LaneMemory {
     int dcap = 0;
     uint64_t *dka = nullptr, *dkb = nullptr;
     int *dva = nullptr, *dvb = nullptr, *dflag = nullptr, *dhist = nullptr, *dhsum = nullptr, *dtsum = nullptr, *dtotal = nullptr;
+    // the dense form's flag / rank array (dense_cap ints) and its tile sums
+    size_t dense_cap = 0;
+    int *dense_f = nullptr, *dense_t = nullptr;
 
     LaneMemory(int c, int seed_cap) : cap(c) {
         jobs_off = kDescBytes;
@@ -261,6 +268,11 @@ struct LaneMemory {
         M(&dhsum, (256 * tiles + kScanTile - 1) / kScanTile + 1);
         M(&dtsum, tiles + 1);
         M(&dtotal, 1);
+        // four ints per grid key of capacity (at most 2^26): the spaces of contiguous batches
+        // (a 64 x 64 region's grid keys span about 150K) with room to spare
+        dense_cap = std::min<size_t>(4 * gk, size_t{1} << 26);
+        M(&dense_f, dense_cap);
+        M(&dense_t, dense_cap / kDenseTile + 1);
     }
     ~LaneMemory() {
         for (void* p : owned) (void)hipFree(p);
@@ -288,6 +300,7 @@ struct LaneMemory {
     const SeedJob* d_jobs() const { return reinterpret_cast<const SeedJob*>(d_block + jobs_off); }
     const ChunkRef* d_chunks() const { return reinterpret_cast<const ChunkRef*>(d_block + chunks_off); }
     const int* d_count_of(size_t field) const { return reinterpret_cast<const int*>(d_block + field); }
+    const KeyCodec* d_codec(size_t field) const { return reinterpret_cast<const KeyCodec*>(d_block + field); }
 };
 
 struct Lane {
@@ -704,9 +717,10 @@ private:
             const int step = std::max(64, (m.cap + 7) / 8);
             sh.grid_n = std::min(m.cap, (n + step - 1) / step * step);
         }
-        sh.pass_col = passes(D.kc.bits);
-        sh.pass_grid = core ? 0 : passes(D.kg.bits);
-        sh.pass_bio = core ? 0 : passes(D.kb.bits);
+        const auto pick = [&](const KeyCodec& k) { return dense_size(k) <= m.dense_cap ? kDensePass : passes(k.bits); };
+        sh.pass_col = pick(D.kc);
+        sh.pass_grid = core ? 0 : pick(D.kg);
+        sh.pass_bio = core ? 0 : pick(D.kb);
         // Host outputs: the batch writes result buffer k of the lane (alternating, so the copy
         // of the lane's previous batch can still read the other one) unless aquifer or density
         // outputs are asked for, which have one buffer each (then buffer 0 and both waits).
@@ -838,10 +852,21 @@ private:
         if (timed) hip_check(hipEventRecord(L.stage_ev[s], L.stream), "event");
     }
 
-    // Dedupe of the keys at `keys` (count at device `np`): unique keys ascending, map, count.
-    void dedupe(Lane& L, const uint64_t* keys, const int* np, int npass, int cap, uint64_t* unique, int* map, int* count) {
+    // Dedupe of the keys at `keys` (count at device `np`, codec at device `K`): unique keys
+    // ascending, map, count. npass: the radix passes, or kDensePass for the dense form.
+    void dedupe(Lane& L, const uint64_t* keys, const int* np, const KeyCodec* K, int npass, int cap, uint64_t* unique, int* map, int* count) {
         LaneMemory& m = *L.mem;
         hipStream_t s = L.stream;
+        if (npass == kDensePass) {
+            k_dense_clear<<<grid((const void*)k_dense_clear, 256), 256, 0, s>>>(K, m.dense_f);
+            k_dense_mark<<<grid((const void*)k_dense_mark, 256), 256, 0, s>>>(keys, np, K, m.dense_f);
+            k_dense_tiles<<<grid((const void*)k_dense_tiles, 256), 256, 0, s>>>(K, m.dense_f, m.dense_t);
+            k_dense_offsets<<<1, 1024, 0, s>>>(K, m.dense_t, count);
+            k_dense_ranks<<<grid((const void*)k_dense_ranks, 256), 256, 0, s>>>(K, m.dense_f, m.dense_t, unique);
+            k_dense_map<<<grid((const void*)k_dense_map, 256), 256, 0, s>>>(keys, np, K, m.dense_f, map);
+            hip_check(hipGetLastError(), "dense dedupe kernels");
+            return;
+        }
         const unsigned tiles = blocks_for(static_cast<size_t>(cap), kRsTile);
         const unsigned stiles = blocks_for(256 * static_cast<size_t>(tiles), kScanTile);
         const uint64_t* kin = keys;
@@ -888,7 +913,7 @@ private:
             const int* nbk = cnt + kCntBiomeChunks;
             group(L, timed, kKBiomeUnion, [&] {
                 k_bio_keys<<<grid((const void*)k_bio_keys, 256), 256, 0, s>>>(B, ch, m.bkeys);
-                dedupe(L, m.bkeys, m.d_count_of(offsetof(BatchDesc, bio_keys)), sh.pass_bio, m.cap * 9, m.ubk, m.nb, m.counts + kCntBiomeChunks);
+                dedupe(L, m.bkeys, m.d_count_of(offsetof(BatchDesc, bio_keys)), m.d_codec(offsetof(BatchDesc, kb)), sh.pass_bio, m.cap * 9, m.ubk, m.nb, m.counts + kCntBiomeChunks);
                 k_bio_chunks<<<grid((const void*)k_bio_chunks, 256), 256, 0, s>>>(B, m.ubk, nbk, m.bch, d_stats);
             });
             group(L, timed, kKBiomeColumns, [&] {
@@ -923,7 +948,7 @@ private:
         stage(L, timed, kStNoise);
         k_col_keys<<<grid((const void*)k_col_keys, 256), 256, 0, s>>>(B, ch, m.ckeys);
         group(L, timed, kKDedupeCols, [&] {
-            dedupe(L, m.ckeys, m.d_count_of(offsetof(BatchDesc, col_keys)), sh.pass_col, m.cap * kCols, m.ucols, m.colmap, m.counts + kCntColumns);
+            dedupe(L, m.ckeys, m.d_count_of(offsetof(BatchDesc, col_keys)), m.d_codec(offsetof(BatchDesc, kc)), sh.pass_col, m.cap * kCols, m.ucols, m.colmap, m.counts + kCntColumns);
         });
         group(L, timed, kKColumns, [&] {
             k_columns<<<grid((const void*)k_columns, W(kKColumns)), W(kKColumns), 0, s>>>(B, sd, m.ucols, cnt + kCntColumns, m.colv);
@@ -945,7 +970,7 @@ private:
         // surface-level grid and aquifer cells
         k_grid_keys<<<grid((const void*)k_grid_keys, 256), 256, 0, s>>>(B, ch, m.gkeys);
         group(L, timed, kKDedupeGrid, [&] {
-            dedupe(L, m.gkeys, m.d_count_of(offsetof(BatchDesc, grid_keys)), sh.pass_grid, m.cap * kGridKeys, m.ug, m.gmap, m.counts + kCntGrid);
+            dedupe(L, m.gkeys, m.d_count_of(offsetof(BatchDesc, grid_keys)), m.d_codec(offsetof(BatchDesc, kg)), sh.pass_grid, m.cap * kGridKeys, m.ug, m.gmap, m.counts + kCntGrid);
         });
         const size_t gn = static_cast<size_t>(sh.grid_n);
         group(L, timed, kKFts, [&] { k_fts<<<blocks_for(gn * kGridKeys, W(kKFts)), W(kKFts), 0, s>>>(B, sd, m.ug, cnt + kCntGrid, m.gv); });

@@ -353,6 +353,58 @@ int mcte_check_minmax(int threads, int64_t* out) {
     return 0;
 }
 
+// The dense dedupe's key index (dense_index / dense_key, pipeline.h) over every key of random
+// batches: n chunks (seeds from `nseeds` slots) around a few centres within +-spread chunks, for
+// the column, grid and biome codecs: out[0] keys whose index is outside [0, dense_size), out[1]
+// keys that dense_key does not give back, out[2] adjacent distinct sorted keys whose indexes are
+// not increasing, out[3] batches checked.
+int mcte_check_dense(int batches, int n, int nseeds, int spread, int64_t* out) {
+    std::memset(out, 0, 4 * sizeof(int64_t));
+    uint64_t st = 12345;
+    const auto rnd = [&st](uint64_t m) {
+        uint64_t r = (st += 0x9E3779B97F4A7C15ull);
+        r = (r ^ (r >> 30)) * 0xBF58476D1CE4E5B9ull;
+        r = (r ^ (r >> 27)) * 0x94D049BB133111EBull;
+        return (r ^ (r >> 31)) % m;
+    };
+    for (int b = 0; b < batches; ++b) {
+        std::vector<ChunkRef> ch(static_cast<size_t>(n));
+        const int centres = 1 + static_cast<int>(rnd(4));
+        int cx[4], cz[4];
+        for (int k = 0; k < centres; ++k) {
+            cx[k] = static_cast<int>(rnd(2 * 117187 + 1)) - 117187;
+            cz[k] = static_cast<int>(rnd(2 * 117187 + 1)) - 117187;
+        }
+        for (auto& c : ch) {
+            const int k = static_cast<int>(rnd(static_cast<uint64_t>(centres)));
+            c.x = cx[k] + static_cast<int>(rnd(2 * static_cast<uint64_t>(spread) + 1)) - spread;
+            c.z = cz[k] + static_cast<int>(rnd(2 * static_cast<uint64_t>(spread) + 1)) - spread;
+            c.seed = static_cast<int>(rnd(static_cast<uint64_t>(nseeds)));
+            c.pad = 0;
+        }
+        const struct { int unit, xlo, xhi, zlo, zhi; } cs[3] = {{4, 0, 4, 0, 4}, {4, -16, 10, -8, 10}, {1, -1, 1, -1, 1}};
+        for (const auto& c : cs) {
+            const KeyCodec K = make_codec(ch.data(), n, c.unit, c.xlo, c.xhi, c.zlo, c.zhi);
+            std::vector<uint64_t> keys;
+            for (const auto& r : ch)
+                for (int dx = c.xlo; dx <= c.xhi; ++dx)
+                    for (int dz = c.zlo; dz <= c.zhi; ++dz)
+                        for (int f = 0; f < 2; ++f) keys.push_back(make_key(K, r.seed, f, r.x * c.unit + dx, r.z * c.unit + dz));
+            std::sort(keys.begin(), keys.end());
+            keys.erase(std::unique(keys.begin(), keys.end()), keys.end());
+            const uint64_t D = dense_size(K);
+            for (size_t i = 0; i < keys.size(); ++i) {
+                const uint32_t d = dense_index(K, keys[i]);
+                out[0] += d >= D ? 1 : 0;
+                out[1] += dense_key(K, d) != keys[i] ? 1 : 0;
+                if (i > 0) out[2] += d <= dense_index(K, keys[i - 1]) ? 1 : 0;
+            }
+        }
+        ++out[3];
+    }
+    return 0;
+}
+
 int mcte_counts_ops() {
 #if defined(MCT_COUNT_OPS)
     return 1;

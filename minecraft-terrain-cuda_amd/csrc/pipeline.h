@@ -188,6 +188,8 @@ struct KeyCodec {
     int32_t s0;
     int32_t bx, bz;
     int32_t bits;     // total key bits
+    uint32_t ex, ez;  // exact compressed extents of the two axes
+    uint32_t nhi;     // values of the high part, (seed slot - s0) * 2 + flag: 2 x the slot range
 };
 MCT_HD MCT_FI uint32_t axis_encode(const AxisCodec& c, int32_t q) {
     int j = 0;
@@ -203,6 +205,20 @@ MCT_HD MCT_FI uint64_t make_key(const KeyCodec& k, int32_t slot, int flag, int32
     const uint64_t hi = static_cast<uint64_t>(slot - k.s0) * 2u + static_cast<uint64_t>(flag);
     return (hi << (k.bx + k.bz)) | (static_cast<uint64_t>(axis_encode(k.x, qx)) << k.bz) | static_cast<uint64_t>(axis_encode(k.z, qz));
 }
+// A key's index in the dense space [0, nhi * ex * ez) of the codec, the keys' own order
+// (high part, then x, then z); for batches whose space is small enough to be marked densely.
+MCT_HD MCT_FI uint32_t dense_index(const KeyCodec& k, uint64_t key) {
+    const uint32_t ze = static_cast<uint32_t>(key & ((1ull << k.bz) - 1));
+    const uint32_t xe = static_cast<uint32_t>((key >> k.bz) & ((1ull << k.bx) - 1));
+    const uint32_t hi = static_cast<uint32_t>(key >> (k.bx + k.bz));
+    return (hi * k.ex + xe) * k.ez + ze;
+}
+MCT_HD MCT_FI uint64_t dense_key(const KeyCodec& k, uint32_t d) {
+    const uint32_t ze = d % k.ez, r = d / k.ez, xe = r % k.ex, hi = r / k.ex;
+    return (static_cast<uint64_t>(hi) << (k.bx + k.bz)) | (static_cast<uint64_t>(xe) << k.bz) | ze;
+}
+MCT_HD MCT_FI uint64_t dense_size(const KeyCodec& k) { return static_cast<uint64_t>(k.nhi) * k.ex * k.ez; }
+
 MCT_HD MCT_FI void split_key(const KeyCodec& k, uint64_t key, int32_t& slot, int& flag, int32_t& qx, int32_t& qz) {
     qz = axis_decode(k.z, static_cast<uint32_t>(key & ((1ull << k.bz) - 1)));
     qx = axis_decode(k.x, static_cast<uint32_t>((key >> k.bz) & ((1ull << k.bx) - 1)));
@@ -280,6 +296,9 @@ inline KeyCodec make_codec(const ChunkRef* ch, int n, int unit, int xlo, int xhi
     k.s0 = static_cast<int32_t>(mns);
     k.bx = codec_bits(ex);
     k.bz = codec_bits(ez);
+    k.ex = static_cast<uint32_t>(std::min<uint64_t>(ex, 0xFFFFFFFFu));   // (a space this wide is never dense)
+    k.ez = static_cast<uint32_t>(std::min<uint64_t>(ez, 0xFFFFFFFFu));
+    k.nhi = static_cast<uint32_t>(std::min<int64_t>(2 * (mxs - mns + 1), int64_t{1} << 31));
     k.bits = codec_bits(static_cast<uint64_t>(mxs - mns + 1)) + 1 + k.bx + k.bz;
     if (k.bits > 64) throw std::range_error("chunk range too wide for 64-bit position keys");
     return k;

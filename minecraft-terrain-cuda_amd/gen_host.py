@@ -46,6 +46,7 @@ def _bind(L):
     L.mcte_stage_ops.argtypes = [I, ctypes.POINTER(ctypes.c_char_p), P]
     L.mcte_check_divisions.argtypes = [I, P]
     L.mcte_check_minmax.argtypes = [I, P]
+    L.mcte_check_dense.argtypes = [I, I, I, I, P]
     return L
 
 
@@ -164,6 +165,15 @@ def check_minmax(threads=8):
     if lib().mcte_check_minmax(int(threads), out.ctypes.data) != 0:
         raise RuntimeError('mcte_check_minmax failed')
     return {'special': int(out[0]), 'random': int(out[1]), 'clamp': int(out[2])}
+
+
+def check_dense(batches=40, n=64, nseeds=3, spread=40):
+    """The dense dedupe's key index on random batches (mcte_check_dense): counts of keys indexed
+    outside the dense space, not given back by dense_key, and out of order; batches checked."""
+    out = np.zeros(4, np.int64)
+    if lib().mcte_check_dense(int(batches), int(n), int(nseeds), int(spread), out.ctypes.data) != 0:
+        raise RuntimeError('mcte_check_dense failed')
+    return {'outside': int(out[0]), 'roundtrip': int(out[1]), 'order': int(out[2]), 'batches': int(out[3])}
 
 
 class HostGenerator:

@@ -87,3 +87,14 @@ def test_java_min_max_clamp_select_forms():
         pytest.skip('libmct_host.so not built')
     r = gen_host.check_minmax(threads=8)
     assert r == {'special': 0, 'random': 0, 'clamp': 0}, r
+
+
+def test_dense_dedupe_index_is_an_order_preserving_bijection():
+    """The dense dedupe ranks keys by their index in the codec's dense space: every key of a
+    batch maps inside the space, dense_key inverts the map, and distinct keys keep their order
+    (so the dense form returns the radix sort's unique keys and map). Random batches of 1 to 4
+    clusters and 3 seed slots, for the column, grid and biome codecs."""
+    if not os.path.exists(gen_host.LIB_PATH):
+        pytest.skip('libmct_host.so not built')
+    r = gen_host.check_dense()
+    assert r == {'outside': 0, 'roundtrip': 0, 'order': 0, 'batches': 40}, r
