@@ -185,7 +185,8 @@ struct LaneMemory {
     int* nb = nullptr;
     ChunkRef* bch = nullptr;
     float* bcol = nullptr;
-    int64_t *targets = nullptr, *mind = nullptr, *tvary = nullptr;
+    int64_t *targets = nullptr, *mind = nullptr, *tvary = nullptr, *tcol = nullptr;
+    Targets tgt() const { return Targets{targets, tcol, tvary}; }
     int32_t* lstar = nullptr;
     uint8_t *tie = nullptr, *bb = nullptr;
     uint64_t* bmask = nullptr;
@@ -223,7 +224,8 @@ struct LaneMemory {
         M(&nb, nbk);
         M(&bch, nbk);
         M(&bcol, nbk * 16 * ncc);
-        M(&targets, nbk * kQPer * 7);
+        M(&targets, kOneVarying ? 0 : nbk * kQPer * 7);
+        M(&tcol, kOneVarying ? nbk * 16 * 7 : 0);
         M(&tvary, nbk * kQPer);
         M(&mind, nbk * kQPer);
         M(&lstar, nbk * kQPer);
@@ -920,26 +922,26 @@ private:
                 k_biome_columns<<<grid((const void*)k_biome_columns, W(kKBiomeColumns)), W(kKBiomeColumns), 0, s>>>(sd, m.bch, nbk, m.bcol);
             });
             group(L, timed, kKBiomeQuarts, [&] {
-                k_biome_targets<<<grid((const void*)k_biome_targets, 256), 256, 0, s>>>(sd, m.bch, nbk, m.bcol, m.targets, m.tvary);
+                k_biome_targets<<<grid((const void*)k_biome_targets, 256), 256, 0, s>>>(sd, m.bch, nbk, m.bcol, m.tgt());
                 const int w = W(kKBiomeQuarts);
                 const size_t bk_max = 9 * static_cast<size_t>(sh.grid_n);
                 if (model.treef) {
                     if (model.vary_groups > 0) {
-                        k_biome_groups<<<blocks_for(bk_max * 16 * model.vary_groups, w), w, 0, s>>>(nbk, model.vary_groups, m.targets, m.lstar, m.mind, m.gres);
+                        k_biome_groups<<<blocks_for(bk_max * 16 * model.vary_groups, w), w, 0, s>>>(nbk, model.vary_groups, m.tgt(), m.lstar, m.mind, m.gres);
                         k_biome_quart_final<<<grid((const void*)k_biome_quart_final, 256), 256, 0, s>>>(nbk, model.vary_groups, m.tvary, m.gres, m.lstar, m.mind);
                     } else {
-                        k_biome_search_cols<<<blocks_for(bk_max * 16, w), w, 0, s>>>(nbk, m.targets, m.lstar, m.mind);
+                        k_biome_search_cols<<<blocks_for(bk_max * 16, w), w, 0, s>>>(nbk, m.tgt(), m.lstar, m.mind);
                     }
-                    k_biome_search_wide<<<grid((const void*)k_biome_search_wide, 256), 256, 0, s>>>(nbk, m.targets, m.lstar, m.mind);
+                    k_biome_search_wide<<<grid((const void*)k_biome_search_wide, 256), 256, 0, s>>>(nbk, m.tgt(), m.lstar, m.mind);
                 } else {
-                    k_biome_search_scalar<<<grid((const void*)k_biome_search_scalar, w), w, 0, s>>>(nbk, m.targets, m.lstar, m.mind, run_len);
+                    k_biome_search_scalar<<<grid((const void*)k_biome_search_scalar, w), w, 0, s>>>(nbk, m.tgt(), m.lstar, m.mind, run_len);
                 }
             });
             group(L, timed, kKBiomeTies, [&] {
-                k_biome_ties<<<grid((const void*)k_biome_ties, W(kKBiomeTies)), W(kKBiomeTies), 0, s>>>(nbk, m.targets, m.lstar, m.mind, m.tie);
+                k_biome_ties<<<grid((const void*)k_biome_ties, W(kKBiomeTies)), W(kKBiomeTies), 0, s>>>(nbk, m.tgt(), m.lstar, m.mind, m.tie);
             });
             group(L, timed, kKBiomeChain, [&] {
-                k_biome_chain<<<grid((const void*)k_biome_chain, W(kKBiomeChain)), W(kKBiomeChain), 0, s>>>(nbk, m.targets, m.lstar, m.mind, m.tie, m.bb, d_stats);
+                k_biome_chain<<<grid((const void*)k_biome_chain, W(kKBiomeChain)), W(kKBiomeChain), 0, s>>>(nbk, m.tgt(), m.lstar, m.mind, m.tie, m.bb, d_stats);
             });
             k_biome_masks<<<grid((const void*)k_biome_masks, 256), 256, 0, s>>>(nbk, m.bb, m.bmask);
             hip_check(hipGetLastError(), "biome kernels");

@@ -440,27 +440,29 @@ int mcte_generate(void* p, int64_t seed, const int32_t* xz, int n, uint8_t* terr
                 }
         const int nbk = static_cast<int>(bch.size());
         std::vector<float> bcol(static_cast<size_t>(nbk) * 16 * (mctgen::kNumClimateColumns > 0 ? mctgen::kNumClimateColumns : 1));
-        std::vector<int64_t> targets(static_cast<size_t>(nbk) * kQPer * 7), mind(static_cast<size_t>(nbk) * kQPer);
+        std::vector<int64_t> targets(kOneVarying ? 1 : static_cast<size_t>(nbk) * kQPer * 7), mind(static_cast<size_t>(nbk) * kQPer);
+        std::vector<int64_t> tcol(kOneVarying ? static_cast<size_t>(nbk) * 16 * 7 : 1);
         std::vector<int32_t> lstar(static_cast<size_t>(nbk) * kQPer);
         std::vector<uint8_t> tie(static_cast<size_t>(nbk) * kQPer), bb(static_cast<size_t>(nbk) * kQPer);
         for (int i = 0; i < nbk * 16; ++i) biome_column(seeds, bch.data(), bcol.data(), i);
         stage_mark("biome_columns");
         std::vector<int64_t> tvary(static_cast<size_t>(nbk) * kQPer);
-        for (int i = 0; i < nbk * kQPer; ++i) biome_target(seeds, bch.data(), bcol.data(), targets.data(), i, tvary.data());
+        const Targets T{targets.data(), tcol.data(), tvary.data()};
+        for (int i = 0; i < nbk * kQPer; ++i) biome_target(seeds, bch.data(), bcol.data(), T, i);
         const int ntop = M.treef ? std::min(mctgen::kTreeNodes, 384) : 0;
         const int G = (M.treef && kOneVarying) ? M.col_groups[kVaryCoord] : 0;
         if (G > 0) {
             std::vector<GroupRes> gres(static_cast<size_t>(nbk) * 16 * G);
-            for (int i = 0; i < nbk * 16 * G; ++i) biome_group_search(M, M.treef, ntop, targets.data(), lstar.data(), mind.data(), gres.data(), G, i);
+            for (int i = 0; i < nbk * 16 * G; ++i) biome_group_search(M, M.treef, ntop, T, lstar.data(), mind.data(), gres.data(), G, i);
             for (int i = 0; i < nbk * kQPer; ++i) biome_quart_final(M, tvary.data(), gres.data(), lstar.data(), mind.data(), G, i);
         } else {
-            for (int i = 0; i < nbk * 16; ++i) biome_search_column(M, M.treef, ntop, targets.data(), lstar.data(), mind.data(), i);
+            for (int i = 0; i < nbk * 16; ++i) biome_search_column(M, M.treef, ntop, T, lstar.data(), mind.data(), i);
         }
         stage_mark("biome_quarts");
-        for (int i = 0; i < nbk * kQPer; ++i) biome_search_wide(M, targets.data(), lstar.data(), mind.data(), i);
-        for (int i = 0; i < nbk * kQPer; ++i) biome_tie(M, targets.data(), lstar.data(), mind.data(), tie.data(), i);
+        for (int i = 0; i < nbk * kQPer; ++i) biome_search_wide(M, T, lstar.data(), mind.data(), i);
+        for (int i = 0; i < nbk * kQPer; ++i) biome_tie(M, T, lstar.data(), mind.data(), tie.data(), i);
         stage_mark("biome_ties");
-        for (int b = 0; b < nbk; ++b) biome_chain_seq(M, targets.data(), lstar.data(), mind.data(), bb.data(), b);
+        for (int b = 0; b < nbk; ++b) biome_chain_seq(M, T, lstar.data(), mind.data(), bb.data(), b);
         stage_mark("biome_chain");
         // corner columns and corners
         std::vector<uint64_t> ckeys(static_cast<size_t>(n) * kCols), ucols;

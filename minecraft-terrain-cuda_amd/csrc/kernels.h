@@ -122,10 +122,9 @@ __global__ __launch_bounds__(256) void k_biome_columns(const SeedDev* seeds, con
     for (int idx = global_id(); idx < items; idx += global_threads()) biome_column(seeds, bch, bcol, idx);
 }
 
-__global__ __launch_bounds__(256) void k_biome_targets(const SeedDev* seeds, const ChunkRef* bch, const int* nbk, const float* bcol, int64_t* targets,
-                                                       int64_t* tvary) {
+__global__ __launch_bounds__(256) void k_biome_targets(const SeedDev* seeds, const ChunkRef* bch, const int* nbk, const float* bcol, Targets T) {
     const int items = *nbk * kQPer;
-    for (int idx = global_id(); idx < items; idx += global_threads()) biome_target(seeds, bch, bcol, targets, idx, tvary);
+    for (int idx = global_id(); idx < items; idx += global_threads()) biome_target(seeds, bch, bcol, T, idx);
 }
 
 // The upper levels of the float R-tree (the first kTreeTop nodes of the breadth-first order,
@@ -144,24 +143,24 @@ __device__ MCT_FI int stage_tree_top(TreeNodeF* top) {
 // column's varying coordinate instead of one search per quart.
 // (One-shot, the grid sized for the bucket's most biome chunks; workgroups past the count exit
 // before staging the tree.)
-__global__ __launch_bounds__(256) void k_biome_search_cols(const int* nbk, const int64_t* targets, int32_t* lstar, int64_t* mind) {
+__global__ __launch_bounds__(256) void k_biome_search_cols(const int* nbk, Targets T, int32_t* lstar, int64_t* mind) {
     __shared__ TreeNodeF top[kTreeTop];
     const int items = *nbk * 16;
     if (static_cast<int>(blockIdx.x * blockDim.x) >= items) return;
     const int ntop = stage_tree_top(top);
     const int idx = global_id();
-    if (idx < items) biome_search_column(c_model, top, ntop, targets, lstar, mind, idx);
+    if (idx < items) biome_search_column(c_model, top, ntop, T, lstar, mind, idx);
 }
 
 // The column search split in two: group searches (one item per column and group), then each
 // quart's L* (one per quart).
-__global__ __launch_bounds__(256) void k_biome_groups(const int* nbk, int G, const int64_t* targets, int32_t* lstar, int64_t* mind, GroupRes* res) {
+__global__ __launch_bounds__(256) void k_biome_groups(const int* nbk, int G, Targets T, int32_t* lstar, int64_t* mind, GroupRes* res) {
     __shared__ TreeNodeF top[kTreeTop];
     const int items = *nbk * 16 * G;
     if (static_cast<int>(blockIdx.x * blockDim.x) >= items) return;
     const int ntop = stage_tree_top(top);
     const int idx = global_id();
-    if (idx < items) biome_group_search(c_model, top, ntop, targets, lstar, mind, res, G, idx);
+    if (idx < items) biome_group_search(c_model, top, ntop, T, lstar, mind, res, G, idx);
 }
 
 __global__ __launch_bounds__(256) void k_biome_quart_final(const int* nbk, int G, const int64_t* tvary, const GroupRes* res, int32_t* lstar, int64_t* mind) {
@@ -170,20 +169,20 @@ __global__ __launch_bounds__(256) void k_biome_quart_final(const int* nbk, int G
 }
 
 // Quarts whose targets are beyond the float tree's exact range: the int64 search.
-__global__ __launch_bounds__(256) void k_biome_search_wide(const int* nbk, const int64_t* targets, int32_t* lstar, int64_t* mind) {
+__global__ __launch_bounds__(256) void k_biome_search_wide(const int* nbk, Targets T, int32_t* lstar, int64_t* mind) {
     const int items = *nbk * kQPer;
-    for (int idx = global_id(); idx < items; idx += global_threads()) biome_search_wide(c_model, targets, lstar, mind, idx);
+    for (int idx = global_id(); idx < items; idx += global_threads()) biome_search_wide(c_model, T, lstar, mind, idx);
 }
 
 // The search without the float tree (a model whose bounds it cannot hold exactly).
-__global__ __launch_bounds__(256) void k_biome_search_scalar(const int* nbk, const int64_t* targets, int32_t* lstar, int64_t* mind, int run_len) {
+__global__ __launch_bounds__(256) void k_biome_search_scalar(const int* nbk, Targets T, int32_t* lstar, int64_t* mind, int run_len) {
     const int items = *nbk * 16 * biome_runs(run_len);
-    for (int idx = global_id(); idx < items; idx += global_threads()) biome_search(c_model, nullptr, 0, targets, lstar, mind, idx, true, run_len);
+    for (int idx = global_id(); idx < items; idx += global_threads()) biome_search(c_model, nullptr, 0, T, lstar, mind, idx, true, run_len);
 }
 
-__global__ __launch_bounds__(256) void k_biome_ties(const int* nbk, const int64_t* targets, const int32_t* lstar, const int64_t* mind, uint8_t* tie) {
+__global__ __launch_bounds__(256) void k_biome_ties(const int* nbk, Targets T, const int32_t* lstar, const int64_t* mind, uint8_t* tie) {
     const int items = *nbk * kQPer;
-    for (int idx = global_id(); idx < items; idx += global_threads()) biome_tie(c_model, targets, lstar, mind, tie, idx);
+    for (int idx = global_id(); idx < items; idx += global_threads()) biome_tie(c_model, T, lstar, mind, tie, idx);
 }
 
 // The last-result rule, one wave64 per biome chunk (waves stride over the chunks). While the
@@ -191,7 +190,7 @@ __global__ __launch_bounds__(256) void k_biome_ties(const int* nbk, const int64_
 // its L* unless its tie flag is set, where the previous L* wins; from there the chain is out of
 // sync with that leaf X, which keeps winning while it is at minimal distance (all lanes test
 // it at once) until it loses or a position's L* is X itself. Each switch costs one ballot.
-__global__ __launch_bounds__(256) void k_biome_chain(const int* nbk_ptr, const int64_t* targets, const int32_t* lstar, const int64_t* mind, const uint8_t* tie,
+__global__ __launch_bounds__(256) void k_biome_chain(const int* nbk_ptr, Targets T, const int32_t* lstar, const int64_t* mind, const uint8_t* tie,
                                                      uint8_t* biomes, unsigned long long* stats) {
     const int nbk = *nbk_ptr;
     const int lane = threadIdx.x & 63;
@@ -220,7 +219,12 @@ __global__ __launch_bounds__(256) void k_biome_chain(const int* nbk_ptr, const i
                     sync = false;
                     k = j + 1;
                 } else {
-                    const bool wins = lane >= k && box_distance(M.tree[M.leaf_node[prev]], targets + i * 7) == mind[i];
+                    bool wins = false;
+                    if (lane >= k) {
+                        int64_t tg[7];
+                        T.get(i, tg);
+                        wins = box_distance(M.tree[M.leaf_node[prev]], tg) == mind[i];
+                    }
                     const unsigned long long m = __ballot(lane >= k && (!wins || ls == prev));
                     if (m == 0) {
                         if (lane >= k) res = prev;

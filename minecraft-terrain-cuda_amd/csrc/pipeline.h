@@ -597,31 +597,58 @@ constexpr int kMayVary = 0x3F & ~mctgen::kClimateColumnConst;
 constexpr bool kOneVarying = (kMayVary & (kMayVary - 1)) == 0;
 constexpr int kVaryCoord = kMayVary == 0 ? 6 : (kMayVary & 1 ? 0 : kMayVary & 2 ? 1 : kMayVary & 4 ? 2 : kMayVary & 8 ? 3 : kMayVary & 16 ? 4 : 5);
 
-// tvary (nullable) gets the quart's varying coordinate (kVaryCoord) again, contiguous per quart:
-// the per-quart final minimum reads that one value, 8 bytes where the strided target costs a
-// memory sector.
-MCT_HD MCT_FI void biome_target(const SeedDev* seeds, const ChunkRef* bch, const float* bcol, int64_t* targets, int idx, int64_t* tvary = nullptr) {
+// The climate targets (7 int64 coordinates) of the quarts of a batch's biome chunks, quart
+// i = biome chunk * kQPer + (qy * 4 + qz) * 4 + qx. When at most one coordinate can vary along
+// a quart column (kOneVarying: the code generator marks the others as column-record reads, the
+// same for every quart of the column), a quart column's coordinates are held once (col) and
+// each quart's varying one on its own (vary): 8 bytes written per quart instead of 56.
+// Otherwise every quart's record is held in full. vary (coordinate kVaryCoord of every quart,
+// contiguous) is kept either way: the per-quart final minimum reads that one value.
+struct Targets {
+    int64_t* full;   // biome chunks x kQPer x 7 (when not kOneVarying)
+    int64_t* col;    // biome chunks x 16 x 7: each quart column's coordinates (when kOneVarying)
+    int64_t* vary;   // biome chunks x kQPer
+    MCT_HD MCT_FI void get(size_t i, int64_t* t) const {
+        if constexpr (kOneVarying) {
+            const int64_t* c = col + ((i / kQPer) * 16 + (i % kQPer & 15)) * 7;
+            for (int k = 0; k < 7; ++k) t[k] = c[k];
+            t[kVaryCoord] = vary[i];
+        } else {
+            for (int k = 0; k < 7; ++k) t[k] = full[i * 7 + k];
+        }
+    }
+    MCT_HD MCT_FI int64_t at(size_t i, int k) const {
+        if constexpr (kOneVarying) return k == kVaryCoord ? vary[i] : col[((i / kQPer) * 16 + (i % kQPer & 15)) * 7 + k];
+        else return full[i * 7 + k];
+    }
+};
+
+MCT_HD MCT_FI void biome_target(const SeedDev* seeds, const ChunkRef* bch, const float* bcol, const Targets& T, int idx) {
     const int b = idx / kQPer, q = idx % kQPer;
     const int qx = q & 3, qz = (q >> 2) & 3, qy = q >> 4;
     const ChunkRef c = bch[b];
     const float* cv = bcol + (static_cast<size_t>(b) * 16 + qz * 4 + qx) * mctgen::kNumClimateColumns;
     float clim[6];
     mctgen::eval_climate(seeds[c.seed].oct, c.x * 16 + qx * 4, kMinY + qy * 4, c.z * 16 + qz * 4, cv, clim);
-    int64_t* t = targets + static_cast<size_t>(idx) * 7;
-    int64_t tv = 0;
-    for (int k = 0; k < 6; ++k) {
-        t[k] = quantize(clim[k]);
-        if (k == kVaryCoord) tv = t[k];
-    }
+    int64_t t[7];
+    for (int k = 0; k < 6; ++k) t[k] = quantize(clim[k]);
     t[6] = 0;
-    if (tvary) tvary[idx] = tv;
+    T.vary[idx] = t[kVaryCoord];
+    if constexpr (kOneVarying) {
+        if (qy == 0) {
+            int64_t* o = T.col + (static_cast<size_t>(b) * 16 + (q & 15)) * 7;
+            for (int k = 0; k < 7; ++k) o[k] = t[k];
+        }
+    } else {
+        for (int k = 0; k < 7; ++k) T.full[static_cast<size_t>(idx) * 7 + k] = t[k];
+    }
 }
 
 // L* and its distance for a run of run_len quarts down one quart column; each search is
 // bounded by the previous quart's leaf distance + 1. Targets beyond the float tree's exact
 // range take the int64 search here when `wide` is set, else they are marked (L* = -2) for a
 // separate pass (biome_search_wide).
-MCT_HD inline void biome_search(const ModelDev& M, const TreeNodeF* top, int ntop, const int64_t* targets, int32_t* lstar, int64_t* mind, int idx,
+MCT_HD inline void biome_search(const ModelDev& M, const TreeNodeF* top, int ntop, const Targets& T, int32_t* lstar, int64_t* mind, int idx,
                                 bool wide, int run_len) {
     const int runs = biome_runs(run_len);
     const int b = idx / (16 * runs), rem = idx % (16 * runs);
@@ -632,7 +659,8 @@ MCT_HD inline void biome_search(const ModelDev& M, const TreeNodeF* top, int nto
     const int q1 = (run + 1) * run_len < kQH ? (run + 1) * run_len : kQH;
     for (int qy = run * run_len; qy < q1; ++qy) {
         const size_t i = static_cast<size_t>(b) * kQPer + (qy * 4 + qz) * 4 + qx;
-        const int64_t* t = targets + i * 7;
+        int64_t t[7];
+        T.get(i, t);
         bool small = M.treef != nullptr;
         float td[7];
         for (int k = 0; k < 7; ++k) {
@@ -662,10 +690,11 @@ MCT_HD inline void biome_search(const ModelDev& M, const TreeNodeF* top, int nto
 }
 
 // The quarts biome_search marked (targets beyond 2^24): int64 search, unbounded.
-MCT_HD inline void biome_search_wide(const ModelDev& M, const int64_t* targets, int32_t* lstar, int64_t* mind, int idx) {
+MCT_HD inline void biome_search_wide(const ModelDev& M, const Targets& T, int32_t* lstar, int64_t* mind, int idx) {
     if (lstar[idx] != -2) return;
-    int64_t d = 0;
-    lstar[idx] = tree_first_min(M.tree, targets + static_cast<size_t>(idx) * 7, INT64_MAX, d);
+    int64_t d = 0, t[7];
+    T.get(static_cast<size_t>(idx), t);
+    lstar[idx] = tree_first_min(M.tree, t, INT64_MAX, d);
     mind[idx] = d;
 }
 
@@ -680,10 +709,11 @@ MCT_HD inline void biome_search_wide(const ModelDev& M, const int64_t* targets, 
 // the unrestricted search gives. Columns that vary in more than one coordinate, or in one with
 // too many groups, or whose targets leave the float tree's exact range, take the per-quart
 // search.
-MCT_HD inline void biome_search_column(const ModelDev& M, const TreeNodeF* top, int ntop, const int64_t* targets, int32_t* lstar, int64_t* mind, int idx) {
+MCT_HD inline void biome_search_column(const ModelDev& M, const TreeNodeF* top, int ntop, const Targets& T, int32_t* lstar, int64_t* mind, int idx) {
     const int b = idx >> 4, qx = idx & 3, qz = (idx >> 2) & 3;
     const size_t col = static_cast<size_t>(b) * kQPer + qz * 4 + qx;   // + qy * 16: the column's quarts
-    const int64_t* t0 = targets + col * 7;
+    int64_t t0[7];
+    T.get(col, t0);
     auto in_range = [](int64_t v) { return v >= -static_cast<int64_t>(kTreeExact) && v <= static_cast<int64_t>(kTreeExact); };
     // The code generator marks the climate values that are column-record reads (the same for
     // every quart of the column); the target's offset coordinate is always 0. When at most one
@@ -699,7 +729,8 @@ MCT_HD inline void biome_search_column(const ModelDev& M, const TreeNodeF* top, 
     } else {
         scanned = true;
         for (int qy = 0; qy < kQH; ++qy) {
-            const int64_t* t = targets + (col + static_cast<size_t>(qy) * 16) * 7;
+            int64_t t[7];
+            T.get(col + static_cast<size_t>(qy) * 16, t);
             for (int j = 0; j < 7; ++j) {
                 if (t[j] != t0[j]) vary |= 1 << j;
                 small = small && in_range(t[j]);
@@ -716,7 +747,7 @@ MCT_HD inline void biome_search_column(const ModelDev& M, const TreeNodeF* top, 
         }
     }
     if (k < 0) {
-        biome_search(M, top, ntop, targets, lstar, mind, idx, false, kQH);
+        biome_search(M, top, ntop, T, lstar, mind, idx, false, kQH);
         return;
     }
     auto node = [&](int i) -> const TreeNodeF& { return i < ntop ? top[i] : M.treef[i]; };
@@ -749,7 +780,7 @@ MCT_HD inline void biome_search_column(const ModelDev& M, const TreeNodeF* top, 
     }
     for (int qy = 0; qy < kQH; ++qy) {
         const size_t i = col + static_cast<size_t>(qy) * 16;
-        const int64_t tv = targets[i * 7 + k];
+        const int64_t tv = T.at(i, k);
         if (!scanned && !in_range(tv)) {
             lstar[i] = -2;
             mind[i] = -1;
@@ -784,17 +815,18 @@ struct GroupRes {
 };
 
 // Group g of quart column c (idx = c * G + g).
-MCT_HD inline void biome_group_search(const ModelDev& M, const TreeNodeF* top, int ntop, const int64_t* targets, int32_t* lstar, int64_t* mind,
+MCT_HD inline void biome_group_search(const ModelDev& M, const TreeNodeF* top, int ntop, const Targets& T, int32_t* lstar, int64_t* mind,
                                       GroupRes* out, int G, int idx) {
     const int colidx = idx / G, g = idx - colidx * G, k = kVaryCoord;
     const int b = colidx >> 4, qx = colidx & 3, qz = (colidx >> 2) & 3;
     const size_t col = static_cast<size_t>(b) * kQPer + qz * 4 + qx;
-    const int64_t* t0 = targets + col * 7;
+    int64_t t0[7];
+    T.get(col, t0);
     bool small = M.treef != nullptr;
     for (int j = 0; j < 7; ++j)
         if (j != k) small = small && t0[j] >= -static_cast<int64_t>(kTreeExact) && t0[j] <= static_cast<int64_t>(kTreeExact);
     if (!small) {
-        if (g == 0) biome_search(M, top, ntop, targets, lstar, mind, colidx, false, kQH);
+        if (g == 0) biome_search(M, top, ntop, T, lstar, mind, colidx, false, kQH);
         out[idx] = GroupRes{0.0, -3, 0};
         return;
     }
@@ -921,23 +953,31 @@ MCT_HD inline int biome_query(const SeedDev& S, const ModelDev& M, int32_t x, in
 // tie[p]: the previous chain position's L* differs from L* at p and is also at minimal
 // distance at p. While the chain is in sync (previous result == previous L*), the result at p
 // differs from L* exactly where tie[p] is set.
-MCT_HD MCT_FI void biome_tie(const ModelDev& M, const int64_t* targets, const int32_t* lstar, const int64_t* mind, uint8_t* tie, int idx) {
+MCT_HD MCT_FI void biome_tie(const ModelDev& M, const Targets& T, const int32_t* lstar, const int64_t* mind, uint8_t* tie, int idx) {
     const int b = idx / kQPer, p = idx % kQPer;
     if (p == 0) { tie[idx] = 0; return; }
     const size_t i = static_cast<size_t>(b) * kQPer + chain_quart(p), ip = static_cast<size_t>(b) * kQPer + chain_quart(p - 1);
     const int prev = lstar[ip];
-    tie[idx] = (prev != lstar[i] && box_distance(M.tree[M.leaf_node[prev]], targets + i * 7) == mind[i]) ? 1 : 0;
+    bool t = false;
+    if (prev != lstar[i]) {
+        int64_t tg[7];
+        T.get(i, tg);
+        t = box_distance(M.tree[M.leaf_node[prev]], tg) == mind[i];
+    }
+    tie[idx] = t ? 1 : 0;
 }
 
 // The last-result rule along the chain of one biome chunk (sequential form): the result is
 // the previous result if it reaches the minimal distance, else L*.
-MCT_HD inline void biome_chain_seq(const ModelDev& M, const int64_t* targets, const int32_t* lstar, const int64_t* mind, uint8_t* biomes, int b) {
+MCT_HD inline void biome_chain_seq(const ModelDev& M, const Targets& T, const int32_t* lstar, const int64_t* mind, uint8_t* biomes, int b) {
     const size_t base = static_cast<size_t>(b) * kQPer;
     int prev = -1;
     for (int p = 0; p < kQPer; ++p) {
         const size_t i = base + chain_quart(p);
         int cur = lstar[i];
-        if (prev >= 0 && box_distance(M.tree[M.leaf_node[prev]], targets + i * 7) == mind[i]) cur = prev;
+        int64_t tg[7];
+        T.get(i, tg);
+        if (prev >= 0 && box_distance(M.tree[M.leaf_node[prev]], tg) == mind[i]) cur = prev;
         biomes[i] = M.leaf_biome[cur];
         prev = cur;
     }
