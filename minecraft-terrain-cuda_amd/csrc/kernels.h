@@ -541,7 +541,9 @@ __device__ MCT_FI void fill_unit(const FillArgs& A, int unit, int* fill_lds) {
         const int key = qk[e], lt = key >> 3, yi = key & 7;
         const int ex = (col0 + lt) & 15, ez = (col0 + lt) >> 4;
         const int ebx = cr.x * 16 + ex, eby = ylo + yi, ebz = cr.z * 16 + ez;
-        const AqNearest nr = aquifer_nearest(C, cr.x, cr.z, ebx, eby, ebz);
+        AqColumnRow Rq{C};   // the block's 12 cells as scan keys: min / med3 instead of the insertion chain
+        Rq.load(cr.x, cr.z, ebx, ebz, eby);
+        const AqNearest nr = aquifer_nearest(Rq, cr.x, cr.z, ebx, eby, ebz);
         const int s = aquifer_back(O, C, ebx, eby, ebz, static_cast<double>(qd[e]), nr);
         const int state = s < 0 ? mctgen::kDefaultBlock : s;
         A.blocks[static_cast<size_t>(c) * kPer + static_cast<size_t>((cy * 8 + yi) * 256 + ez * 16 + ex)] = static_cast<uint8_t>(state);
