@@ -124,12 +124,15 @@ int mct_chunk_digests_noalloc(const void* d_data, int64_t count, int64_t bytes_p
  * workgroup[k]: workgroup size of kernel group k (mct_kernel_name(k)); 64, 128 or 256 where the
  * group's kernels take any size, 0 where the kernel's layout fixes it (and 0 in mct_set_tuning
  * also means the default, 256). run_len: quarts per R-tree search thread (1..96, default 8).
- * ms[k]: from mct_autotune, group k's time on the sample with the chosen shape. Results never
- * depend on the shapes. */
+ * flags: MCT_TUNE_REPLAY_ALL makes the exact surface pass walk every column of each chunk it
+ * redoes again, in place of only the columns whose steep test the final heights change (a check
+ * of the selective replay); other bits must be 0. ms[k]: from mct_autotune, group k's time on
+ * the sample with the chosen shape. Results never depend on the shapes or the flags. */
+#define MCT_TUNE_REPLAY_ALL 1
 typedef struct mct_tuning {
     int32_t workgroup[24];
     int32_t run_len;
-    int32_t reserved;
+    int32_t flags;
     double ms[24];
 } mct_tuning;
 /* Times the shapes on a sample (its first max_batch_chunks chunks; chunk i under seeds[i]; outputs

@@ -364,6 +364,7 @@ public:
     // launch shapes: workgroup size per tunable kernel group, quarts per scalar R-tree search
     int wg[kKCount] = {};
     int run_len = kBiomeRunDefault;
+    bool replay_all = false;   // exact surface pass: every column of a redone chunk
     bool use_graphs = true;
     bool profile = false;
     double kernel_ms[kKCount] = {};
@@ -493,6 +494,7 @@ public:
         *out = mct_tuning{};
         for (int k = 0; k < kKCount; ++k) out->workgroup[k] = tunable(k) ? W(k) : 0;
         out->run_len = run_len;
+        out->flags = replay_all ? MCT_TUNE_REPLAY_ALL : 0;
     }
 
     void set_tuning(const mct_tuning& t) {
@@ -502,6 +504,9 @@ public:
                 throw Error(MCT_EINVAL, std::string("workgroup size of ") + kKernelNames[k] + " must be " + (tunable(k) ? "0, 64, 128 or 256" : "0"));
         }
         if (t.run_len < 1 || t.run_len > kQH) throw Error(MCT_EINVAL, "run_len must be 1.." + std::to_string(kQH));
+        if (t.flags & ~MCT_TUNE_REPLAY_ALL) throw Error(MCT_EINVAL, "unknown tuning flags");
+        finish_all(nullptr);
+        replay_all = (t.flags & MCT_TUNE_REPLAY_ALL) != 0;
         set_shapes(t.workgroup, t.run_len);
     }
 
@@ -1008,7 +1013,7 @@ private:
         group(L, timed, kKSurface2, [&] { k_surface2<<<static_cast<unsigned>(gn * 4), 64, 0, s>>>(A, B, m.h0, m.h1, m.h2, m.hf); });
         group(L, timed, kKSurfaceExact, [&] {
             k_surface_check<<<grid((const void*)k_surface_check, 256), 256, 0, s>>>(B, m.h2, m.hf, m.flags, m.counts + kCntRedo, d_stats);
-            k_surface_exact<<<64, 256, 0, s>>>(F, A, B, m.flags, m.counts + kCntRedo);
+            k_surface_exact<<<64, 256, 0, s>>>(F, A, B, m.h0, m.h1, m.h2, m.hf, m.flags, m.counts + kCntRedo, replay_all ? 1 : 0);
         });
         if (sh.biomes) group(L, timed, kKGather, [&] { k_gather_biomes<<<grid((const void*)k_gather_biomes, 256), 256, 0, s>>>(B, m.nb, m.bb); });
         hip_check(hipGetLastError(), "surface kernels");

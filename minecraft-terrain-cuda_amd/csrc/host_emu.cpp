@@ -58,6 +58,7 @@ struct Emu {
     std::vector<int32_t> leaf_rank;
     std::unique_ptr<SeedDev> seed;
     int64_t seed_value = 0;
+    bool replay_all = false;
     bool has_seed = false;
 };
 
@@ -170,6 +171,8 @@ void* mcte_create() {
 
 void mcte_destroy(void* p) { delete static_cast<Emu*>(p); }
 
+void mcte_set_replay_all(void* p, int on) { static_cast<Emu*>(p)->replay_all = on != 0; }
+
 // Seed tables as the device builds them: per layer {ox, oy, oz} and 256 perm bytes.
 int mcte_seed_tables(void* p, int64_t seed, double* offsets, uint8_t* perms, int64_t* zoom_seed, uint64_t* positional6, int32_t* clay192) {
     Emu& e = *static_cast<Emu*>(p);
@@ -220,9 +223,6 @@ int64_t mcte_zoom_check(int64_t zoom_seed, const int32_t* xz, int n) {
     return bad;
 }
 
-// Full pipeline for one seed and n chunks. Outputs (nullable): terrain and aquifer fill u8 per
-// block, biomes u8 per quart, density f32 per block (canonical layouts, chunks back to back).
-// stats[0] = chunks redone in exact order after the speculative surface pass.
 // Operation counts of stage i of the last mcte_generate on this thread: its name and
 // {FP64, FP32, INT32, memory reads}; returns the number of stages, 0 before any call. All
 // counts are zero unless the library was built with MCT_COUNT_OPS (mcte_counts_ops).
@@ -413,6 +413,12 @@ int mcte_counts_ops() {
 #endif
 }
 
+// Full pipeline for one seed and n chunks. Outputs (nullable): terrain and aquifer fill u8 per
+// block, biomes u8 per quart, density f32 per block (canonical layouts, chunks back to back).
+// stats[0] = chunks redone in exact order after the speculative surface pass, stats[1] = their
+// columns walked again, stats[2] = those of the chunks whose blocks the replay changed. With replay_all set
+// (mcte_set_replay_all) every column of a redone chunk is refilled and walked again, in the
+// x-major order, instead of only those whose steep test the live heights change.
 int mcte_generate(void* p, int64_t seed, const int32_t* xz, int n, uint8_t* terrain, uint8_t* biomes, uint8_t* aquifer, float* density, int64_t* stats) {
     try {
         Emu& e = *static_cast<Emu*>(p);
@@ -508,7 +514,6 @@ int mcte_generate(void* p, int64_t seed, const int32_t* xz, int n, uint8_t* terr
             for (int t = 0; t < 256 * kCellsY; ++t) fill_cell(F, c, t, density ? density + static_cast<size_t>(c) * kPer : nullptr);
         stage_mark("fill");
         if (aquifer) std::memcpy(aquifer, blocks.data(), blocks.size());
-        const std::vector<uint8_t> filled = blocks;
         // surface
         std::vector<int> h0(static_cast<size_t>(n) * 256), h1(h0.size()), h2(h0.size()), hf(h0.size());
         for (int c = 0; c < n; ++c)
@@ -524,20 +529,38 @@ int mcte_generate(void* p, int64_t seed, const int32_t* xz, int n, uint8_t* terr
         for (int c = 0; c < n; ++c)
             for (int col = 0; col < 256; ++col) surface_pass2<false>(A, h0.data(), h1.data(), h2.data(), hf.data(), c, col);
         stage_mark("surface2+ore_veins");
-        int64_t redone = 0;
+        int64_t redone = 0, recols = 0, changed = 0;
         for (int c = 0; c < n; ++c) {
             bool ok = true;
             for (int col = 0; col < 256; ++col) ok &= hf[c * 256 + col] == h2[c * 256 + col];
             if (ok) continue;
             ++redone;
-            std::memcpy(blocks.data() + static_cast<size_t>(c) * kPer, filled.data() + static_cast<size_t>(c) * kPer, kPer);
+            // the device's selective replay in x-major order (a linear order of its
+            // anti-diagonals): a column whose steep test the live heights leave as the
+            // speculative pass had it keeps its result, any other is refilled and walked again
+            const int* h0c = h0.data() + static_cast<size_t>(c) * 256;
+            const int* h1c = h1.data() + static_cast<size_t>(c) * 256;
+            const int* h2c = h2.data() + static_cast<size_t>(c) * 256;
+            const std::vector<uint8_t> speculative(blocks.begin() + static_cast<ptrdiff_t>(c) * kPer, blocks.begin() + static_cast<ptrdiff_t>(c + 1) * kPer);
             int hm[256];
             for (int col = 0; col < 256; ++col) hm[col] = h0[c * 256 + col];
-            for (int x = 0; x < 16; ++x)
-                for (int z = 0; z < 16; ++z) surface_exact_column(A, hm, c, x * 16 + z);
+            for (int col = 0; col < 256; ++col) {
+                if (!e.replay_all && !surface_replay_needed(hm, h0c, h1c, h2c, col)) {
+                    hm[col] = hf[c * 256 + col];
+                    continue;
+                }
+                for (int cy = 0; cy < kCellsY; ++cy) fill_cell(F, c, (col >> 4) + (col & 15) * 16 + cy * 256, nullptr);
+                surface_exact_column(A, hm, c, col);
+                ++recols;
+            }
+            changed += std::memcmp(speculative.data(), blocks.data() + static_cast<size_t>(c) * kPer, kPer) != 0;
         }
         stage_mark("surface_check+exact");
-        if (stats) stats[0] = redone;
+        if (stats) {
+            stats[0] = redone;
+            stats[1] = recols;
+            stats[2] = changed;
+        }
         if (terrain) std::memcpy(terrain, blocks.data(), blocks.size());
         if (biomes)
             for (int c = 0; c < n; ++c) std::memcpy(biomes + static_cast<size_t>(c) * kQPer, bb.data() + static_cast<size_t>(nb[c * 9 + 4]) * kQPer, kQPer);

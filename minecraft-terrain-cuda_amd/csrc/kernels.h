@@ -663,29 +663,50 @@ __global__ __launch_bounds__(256) void k_surface_check(const BatchDesc* B, const
     }
 }
 
-// Exact pass for flagged chunks: refill the chunk, recompute its heights, then process its
-// columns by anti-diagonals (x + z = k): a column reads its x-1 and z-1 neighbours after their
-// full processing and its x+1 and z+1 neighbours before theirs, as the x-major loop does.
-__global__ __launch_bounds__(256) void k_surface_exact(FillArgs F, SurfArgs A, const BatchDesc* B, const int* redo, const int* redo_count) {
+// Exact pass for flagged chunks, one workgroup per chunk at a time. Columns go by anti-diagonals
+// (x + z = k): in the x-major loop a column sees its x-1 and z-1 neighbours after their full
+// processing and its x+1 and z+1 neighbours before theirs. A column whose steep test comes out
+// as in the speculative pass keeps its speculative blocks and height hf; any other is refilled
+// (its cells as the fill left them, all the workgroup's threads over the diagonal's redone
+// columns) and walked again with the live heights. hm holds the final height of every
+// processed column and h0 of the rest. replay_all: every column is walked again.
+__global__ __launch_bounds__(256) void k_surface_exact(FillArgs F, SurfArgs A, const BatchDesc* B, const int* h0, const int* h1, const int* h2,
+                                                       const int* hf, const int* redo, const int* redo_count, int replay_all) {
     A.M = &c_model;
     F.blocks = A.blocks = B->terrain;
     __shared__ int hm[256];
+    __shared__ int list[16];
+    __shared__ int nlist;
+    const int t = threadIdx.x;
+    const int lx = t >> 4, lz = t & 15;
     const int total = *redo_count;
     for (int r = blockIdx.x; r < total; r += gridDim.x) {
         const int c = redo[r];
+        const int* h0c = h0 + static_cast<size_t>(c) * 256;
+        const int* h1c = h1 + static_cast<size_t>(c) * 256;
+        const int* h2c = h2 + static_cast<size_t>(c) * 256;
         __syncthreads();
-        for (int t = threadIdx.x; t < 256 * kCellsY; t += blockDim.x) fill_cell(F, c, t, nullptr);
-        __syncthreads();
-        {
-            const int t = threadIdx.x;
-            hm[t] = column_height(c_model, F.blocks + static_cast<size_t>(c) * kPer, t >> 4, t & 15);
-        }
+        hm[t] = h0c[t];
+        if (t == 0) nlist = 0;
         __syncthreads();
         for (int k = 0; k <= 30; ++k) {
-            const int t = threadIdx.x;
-            const int lx = t >> 4, lz = t & 15;
-            if (lx + lz == k) surface_exact_column(A, hm, c, t);
+            if (lx + lz == k) {
+                if (replay_all || surface_replay_needed(hm, h0c, h1c, h2c, t)) list[atomicAdd(&nlist, 1)] = t;
+                else hm[t] = hf[static_cast<size_t>(c) * 256 + t];
+            }
             __syncthreads();
+            const int nr = nlist;
+            if (nr > 0) {
+                for (int i = t; i < nr * kCellsY; i += 256) {
+                    const int col = list[i % nr];
+                    fill_cell(F, c, (col >> 4) + (col & 15) * 16 + (i / nr) * 256, nullptr);
+                }
+                __syncthreads();
+                if (t < nr) surface_exact_column(A, hm, c, list[t]);
+                __syncthreads();
+                if (t == 0) nlist = 0;
+                __syncthreads();
+            }
         }
     }
 }

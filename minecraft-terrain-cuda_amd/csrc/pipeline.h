@@ -2314,6 +2314,24 @@ MCT_HD MCT_FI void surface_pass2(const SurfArgs& A, const int* h0, const int* h1
     hf[hidx] = hm;
 }
 
+// Whether column col of a chunk that failed the speculative check must be walked again in the
+// exact order. A walk reads the other columns' heights only through the slope of its steep
+// test (grad_x <= -4 || grad_z >= 4, from the x+-1 and z+-1 neighbours, clamped to the chunk);
+// the speculative pass took earlier columns (x-major) at h2, later ones at h0 and the column
+// itself at h1. hm holds the final heights of the columns already processed and h0 of the rest,
+// so the live slope takes those, and h1 for the column itself (its own badlands extension runs
+// first in either order). The same steep answer means the same walk, its blocks and height hf.
+MCT_HD MCT_FI bool surface_replay_needed(const int* hm, const int* h0c, const int* h1c, const int* h2c, int col) {
+    const int lx = col >> 4, lz = col & 15;
+    const int xp = lx < 15 ? col + 16 : col, xm = lx > 0 ? col - 16 : col;
+    const int zp = lz < 15 ? col + 1 : col, zm = lz > 0 ? col - 1 : col;
+    auto spec = [&](int j) { return j == col ? h1c[j] : (j < col ? h2c[j] : h0c[j]); };
+    auto live = [&](int j) { return j == col ? h1c[j] : hm[j]; };
+    const bool s_spec = spec(xp) - spec(xm) <= -4 || spec(zp) - spec(zm) >= 4;
+    const bool s_live = live(xp) - live(xm) <= -4 || live(zp) - live(zm) >= 4;
+    return s_spec != s_live;
+}
+
 // One column in the exact sequential order, with a live height array hm[256] of the chunk
 // (used for chunks whose speculative pass 2 saw a height change it did not assume).
 MCT_HD MCT_FI void surface_exact_column(const SurfArgs& A, int* hm, int ci, int col) {

@@ -47,6 +47,7 @@ def _bind(L):
     L.mcte_check_divisions.argtypes = [I, P]
     L.mcte_check_minmax.argtypes = [I, P]
     L.mcte_check_dense.argtypes = [I, I, I, I, P]
+    L.mcte_set_replay_all.argtypes = [P, I]
     return L
 
 
@@ -177,14 +178,22 @@ def check_dense(batches=40, n=64, nseeds=3, spread=40):
 
 
 class HostGenerator:
+    def __init__(self, replay_all=False):
+        """replay_all: the exact surface pass walks every column of a chunk it redoes again (the
+        plain sequential order) instead of only the columns whose steep test changed."""
+        self.replay_all = replay_all
+        self.exact_redone = self.exact_columns = self.exact_changed = 0
+
     def _run(self, seed, xz, t, b):
         L = lib()
         h = L.mcte_create()
         try:
+            L.mcte_set_replay_all(h, 1 if self.replay_all else 0)
             st = np.zeros(4, np.int64)
             rc = L.mcte_generate(h, int(seed), xz.ctypes.data, len(xz), t.ctypes.data, b.ctypes.data, None, None, st.ctypes.data)
             if rc != 0:
                 raise RuntimeError('host emulation: ' + L.mcte_last_error().decode())
+            return st[:3].copy()
         finally:
             L.mcte_destroy(h)
 
@@ -197,12 +206,13 @@ class HostGenerator:
         b = np.zeros((n, 96, 4, 4), np.uint8)
         parts = [p for p in np.array_split(np.arange(n), max(1, min(threads, n))) if len(p)]
         errors = []
+        redone = []
 
         def work(p):
             try:
                 lo, hi = int(p[0]), int(p[-1]) + 1
                 tt, bb = t[lo:hi], b[lo:hi]
-                self._run(seed, np.ascontiguousarray(xz[lo:hi]), tt, bb)
+                redone.append(self._run(seed, np.ascontiguousarray(xz[lo:hi]), tt, bb))
             except Exception as e:   # reported after the join
                 errors.append(e)
         ths = [threading.Thread(target=work, args=(p,)) for p in parts]
@@ -212,6 +222,8 @@ class HostGenerator:
             th.join()
         if errors:
             raise errors[0]
+        # chunks the exact surface pass redid, their columns walked again, the chunks it changed
+        self.exact_redone, self.exact_columns, self.exact_changed = (int(v) for v in np.sum(redone, axis=0)) if redone else (0, 0, 0)
         return t, b
 
     def biome_query(self, seed, xyz):

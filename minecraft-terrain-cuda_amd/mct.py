@@ -27,14 +27,20 @@ class MctError(RuntimeError):
         self.code = code
 
 
+TUNE_REPLAY_ALL = 1   # mct_tuning.flags: the exact surface pass walks every column of a redone chunk
+
+
 class Tuning(ctypes.Structure):
-    _fields_ = [('workgroup', ctypes.c_int32 * 24), ('run_len', ctypes.c_int32), ('reserved', ctypes.c_int32), ('ms', ctypes.c_double * 24)]
+    _fields_ = [('workgroup', ctypes.c_int32 * 24), ('run_len', ctypes.c_int32), ('flags', ctypes.c_int32), ('ms', ctypes.c_double * 24)]
 
     def as_dict(self):
         n = kernel_group_count()
         names = [lib().mct_kernel_name(i).decode() for i in range(n)]
-        return {'workgroup': {names[i]: int(self.workgroup[i]) for i in range(n) if self.workgroup[i]}, 'run_len': int(self.run_len),
-                'ms': {names[i]: round(float(self.ms[i]), 4) for i in range(n) if self.ms[i] > 0}}
+        d = {'workgroup': {names[i]: int(self.workgroup[i]) for i in range(n) if self.workgroup[i]}, 'run_len': int(self.run_len),
+             'ms': {names[i]: round(float(self.ms[i]), 4) for i in range(n) if self.ms[i] > 0}}
+        if self.flags & TUNE_REPLAY_ALL:
+            d['surface_replay_all'] = True
+        return d
 
     @classmethod
     def from_dict(cls, d):
@@ -44,6 +50,7 @@ class Tuning(ctypes.Structure):
         for k, v in d.get('workgroup', {}).items():
             t.workgroup[names.index(k)] = int(v)
         t.run_len = int(d.get('run_len', 8))
+        t.flags = TUNE_REPLAY_ALL if d.get('surface_replay_all') else 0
         return t
 
 

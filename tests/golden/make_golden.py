@@ -4,7 +4,8 @@ container only (the reference sources are not on the GPU machines); the fixtures
 data: inputs and the reference's outputs, on the synthetic overworld IR and terrain model.
 
   python tests/golden/make_golden.py            fixtures.npz (+ plain_digests.npz with --plain,
-                                                stress_golden.npz with --stress)
+                                                stress_golden.npz with --stress,
+                                                exact_golden.npz with --exact)
 
 fixtures.npz holds, per part of the path:
   layers_<seed>_<key>   per-seed noise layers {frequency, amplitude, ox, oy, oz, smear} (f64),
@@ -23,6 +24,12 @@ stress_golden.npz: a 1,536-chunk sample of the 1,024-seed, 2^20-chunk stress lis
 chunks + evenly spaced indices; digests of terrain and biomes, one chunk per reference call) and
 the answers of the first 4,096 biome-only block queries (the biome the reference stores for the
 block's zoomed quart, quart y clamped to the world height).
+exact_golden.npz: the chunks among the first 2^18 of the stress list whose columns' final
+heights differ from the heights the speculative surface pass assumed for them (so the exact
+surface pass replays them), with the reference's digests. The chunks are picked with the host
+twin of the library (libmct_host.so, its exact-pass count), among chunks with a frozen-ocean
+biome at one of nine sample points at y = 60 (the height-changing surface step is the frozen
+ocean's); the expected outputs are the reference's.
 """
 import hashlib
 import json
@@ -232,8 +239,63 @@ def stress():
     print('stress sample + queries: %.1fs' % (time.time() - t0), flush=True)
 
 
+def exact(n_scan=1 << 18):
+    import re
+    import threading
+    import gen_host
+    t0 = time.time()
+    seeds, which, xz = workloads.stress()
+    which, xz = which[:n_scan], xz[:n_scan]
+    model = open(os.path.join(ROOT, 'minecraft-terrain-cuda_amd', 'csrc', 'gen', 'model.h')).read()
+    frozen = [int(re.search(r'\b%s = (\d+)' % k, model).group(1)) for k in ('kBiomeFrozenOcean', 'kBiomeDeepFrozenOcean')]
+    hg = gen_host.HostGenerator()
+    cand = np.zeros(n_scan, bool)
+    for s in np.unique(which):
+        sel = np.nonzero(which == s)[0]
+        for dx in (2, 8, 13):
+            for dz in (2, 8, 13):
+                p = np.stack([xz[sel, 0] * 16 + dx, np.full(len(sel), 60), xz[sel, 1] * 16 + dz], 1).astype(np.int32)
+                cand[sel] |= np.isin(hg.biome_query(int(seeds[s]), p), frozen)
+    cand = np.nonzero(cand)[0]
+    print('frozen-ocean candidates: %d (%.1fs)' % (len(cand), time.time() - t0), flush=True)
+    L = gen_host.lib()
+    found, lock = [], threading.Lock()
+
+    def scan(part):
+        t = np.zeros((1, 384, 16, 16), np.uint8)
+        b = np.zeros((1, 96, 4, 4), np.uint8)
+        for i in part:
+            h = L.mcte_create()
+            st = np.zeros(4, np.int64)
+            c = np.ascontiguousarray(xz[i:i + 1])
+            rc = L.mcte_generate(h, int(seeds[which[i]]), c.ctypes.data, 1, t.ctypes.data, b.ctypes.data, None, None, st.ctypes.data)
+            L.mcte_destroy(h)
+            if rc != 0:
+                raise RuntimeError('host twin failed')
+            if st[0]:
+                with lock:
+                    found.append(int(i))
+    ths = [threading.Thread(target=scan, args=(p,)) for p in np.array_split(cand, 8)]
+    for th in ths:
+        th.start()
+    for th in ths:
+        th.join()
+    idx = np.array(sorted(found), np.int64)
+    print('exact-pass chunks: %d (%.1fs)' % (len(idx), time.time() - t0), flush=True)
+    R = oracle.Reference()
+    dt = np.zeros((len(idx), 8), np.uint8)
+    db = np.zeros((len(idx), 8), np.uint8)
+    for k, i in enumerate(idx):
+        t, b, _ = R.generate(int(seeds[which[i]]), [tuple(xz[i])])
+        dt[k], db[k] = digest(t[0].astype(np.uint8)), digest(b[0])
+    np.savez_compressed(os.path.join(HERE, 'exact_golden.npz'), index=idx, seed=seeds[which[idx]], xz=xz[idx], terrain=dt, biomes=db)
+    print('exact-pass chunks + digests: %.1fs' % (time.time() - t0), flush=True)
+
+
 if __name__ == '__main__':
-    if '--stress' in sys.argv and '--plain' not in sys.argv and '--all' not in sys.argv:
+    if '--exact' in sys.argv:
+        exact()
+    elif '--stress' in sys.argv and '--plain' not in sys.argv and '--all' not in sys.argv:
         stress()
     else:
         main()

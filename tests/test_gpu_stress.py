@@ -46,6 +46,27 @@ def test_stress_sample_matches_reference_digests(mct, stress_golden):
     assert not bad, (len(bad), [(int(g['seed'][k]), tuple(g['xz'][k])) for k in bad[:5]])
 
 
+def test_exact_surface_chunks_match_reference_digests(mct):
+    """Chunks whose columns end at heights the speculative surface pass did not assume go
+    through the exact pass: selective (only columns whose steep test changed are walked again)
+    and with every column refilled and walked again (MCT_TUNE_REPLAY_ALL), both equal to the
+    reference's digests."""
+    with np.load(os.path.join(GOLDEN, 'exact_golden.npz'), allow_pickle=False) as z:
+        g = {k: z[k] for k in z.files}
+    n = len(g['index'])
+    gen = mct.FullTerrainGenerator(max_batch_chunks=64, pipeline_slots=1, seed_capacity=64)
+    for rnd, tune in enumerate(({}, {'surface_replay_all': True})):
+        log('exact chunks', n, tune)
+        if tune:
+            gen.set_tuning(tune)
+            assert gen.tuning().get('surface_replay_all')
+        t, b = gen.generate_multi(g['seed'], [tuple(c) for c in g['xz']])
+        bad = [k for k in range(n) if not ((digest(t[k]) == g['terrain'][k]).all() and (digest(b[k]) == g['biomes'][k]).all())]
+        assert not bad, (tune, bad)
+        assert gen.biome_tie_stats()['exact_surface_redo'] == (rnd + 1) * n
+    gen.close()
+
+
 def test_device_digest_matches_host_digest(mct):
     gen = mct.FullTerrainGenerator(max_batch_chunks=64, pipeline_slots=1)
     chunks = [(0, 0), (-3, 7), (6128, 0)]

@@ -98,3 +98,20 @@ def test_dense_dedupe_index_is_an_order_preserving_bijection():
         pytest.skip('libmct_host.so not built')
     r = gen_host.check_dense()
     assert r == {'outside': 0, 'roundtrip': 0, 'order': 0, 'batches': 40}, r
+
+
+def test_exact_surface_chunks(host):
+    """Chunks whose columns end at heights the speculative surface pass did not assume: each
+    goes through the exact pass and matches the reference's digests, both with the selective
+    replay the device runs (only columns whose steep test the live heights change are walked
+    again) and with every column of the chunk refilled and walked again in x-major order."""
+    with np.load(os.path.join(GOLDEN, 'exact_golden.npz'), allow_pickle=False) as z:
+        g = {k: z[k] for k in z.files}
+    assert len(g['index']) >= 8
+    full = gen_host.HostGenerator(replay_all=True)
+    for k in range(len(g['index'])):
+        for h, cols in ((host, None), (full, 256)):
+            t, b = h.generate(int(g['seed'][k]), [tuple(g['xz'][k])], threads=1)
+            assert h.exact_redone == 1, k
+            assert cols is None or h.exact_columns == cols, (k, h.exact_columns)
+            assert (digest(t[0]) == g['terrain'][k]).all() and (digest(b[0]) == g['biomes'][k]).all(), (k, cols)
