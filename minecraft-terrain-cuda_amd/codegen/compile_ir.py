@@ -120,7 +120,7 @@ class Compiler:
                 const |= 1 << k
         self.const_lines = ['// bit k: climate value k is a column-record read, the same for every quart of a quart column',
                             'constexpr int kClimateColumnConst = 0x%x;' % const]
-        fts = Context('Fts', base=XZ0, sem='point')
+        fts = Context('Fts', base=XZ0, sem='point', signs={'pp': '0', 'pn': '(-1)'})
         for name, sem in (('fts_volume', 'volume'), ('fts_point', 'point')):
             self.function(name, 'float %s(%s, int32_t x, int32_t z)' % (name, O), fts, self.probe(sem))
         point = Context('Point', sem='point')
@@ -130,7 +130,11 @@ class Compiler:
 
     def probe(self, upper_sem):
         """find_top_surface at column (x, z): from the cell at or below the upper bound down to
-        the lower bound, the first probe height whose density (point semantics) is positive."""
+        the lower bound, the first probe height whose density (point semantics) is positive.
+        The probes run as two loops, the heights >= 0 (pp) and then the negative ones (pn): the
+        same sequence (the top is a multiple of the cell height, so it passes 0), with the
+        sign of the height fixed in each loop, so that noises with a zero y scale are computed
+        once per loop instead of once per probe."""
         p = self.plan
         cell, lower = p.fts_cell, p.fts_lower
 
@@ -140,10 +144,16 @@ class Compiler:
             root.add('const int32_t %s = mct::mfloor(static_cast<double>(%s / static_cast<float>(%d))) * %d;' % (top, ub, cell, cell))
             root.add('if (%s <= %d) return static_cast<float>(%d);' % (top, lower, lower))
 
-            def body(s):
-                d = b.value((p.fts_density, ('x', 'probe', 'z'), 'point'), s)
-                s.add('if (%s > 0.0f) return static_cast<float>(probe);' % d)
-            b.loop(root, 'for (int32_t probe = %s; probe >= %d; probe -= %d)' % (top, lower, cell), 'probe', body)
+            def body(var):
+                def f(s):
+                    d = b.value((p.fts_density, ('x', var, 'z'), 'point'), s)
+                    s.add('if (%s > 0.0f) return static_cast<float>(%s);' % (d, var))
+                return f
+            if lower >= 0:
+                b.loop(root, 'for (int32_t pp = %s; pp >= %d; pp -= %d)' % (top, lower, cell), 'pp', body('pp'))
+            else:
+                b.loop(root, 'for (int32_t pp = %s; pp >= 0; pp -= %d)' % (top, cell), 'pp', body('pp'))
+                b.loop(root, 'for (int32_t pn = %s < 0 ? %s : %d; pn >= %d; pn -= %d)' % (top, top, -cell, lower, cell), 'pn', body('pn'))
             root.add('return static_cast<float>(%d);' % lower)
         return fill
 

@@ -80,8 +80,11 @@ def dlit(v):
 
 
 class Context:
-    def __init__(self, name, base=('x', 'y', 'z'), sem='volume', columns=(), channels=(), pass_interpolated=False, tabulated=False):
+    def __init__(self, name, base=('x', 'y', 'z'), sem='volume', columns=(), channels=(), pass_interpolated=False, tabulated=False,
+                 signs=None):
         self.name = name
+        # loop variables whose sign is fixed in their loop: name -> an int literal of that sign
+        self.signs = signs or {}
         self.base = base
         self.sem = sem
         self.columns = {n: k for k, n in enumerate(columns)}
@@ -139,12 +142,17 @@ class Lowering:
     def alias(self, key):
         """The key whose value a pass-through node takes: cache / blend_density markers, slices
         not read from the column record, interpolation evaluated in place, and MIN / MAX nodes
-        whose operand ranges decide them at compile time."""
+        whose operand ranges decide them at compile time. A point-path noise whose y scale is
+        zero reads y only as (double) y * scale, a signed zero fixed by the sign of y: inside a
+        loop whose variable has one sign it takes the key at a literal of that sign, which is
+        loop-invariant (the same double coordinates, so the same value)."""
         node, var, sem = key
         n = self.g.nodes[node]
         t = n['t']
         if t in ('cache', 'blend_density'):
             return (n['in'], var, sem)
+        if sem == 'point' and var[1] in self.ctx.signs and self.zero_y_noise(n):
+            return (node, (var[0], self.ctx.signs[var[1]], var[2]), sem)
         if t == 'slice':
             if self.column_slot(node, var, sem) is not None:
                 return None
@@ -158,6 +166,13 @@ class Lowering:
             if op == 'pass':
                 return (a, var, sem)
         return None
+
+    @staticmethod
+    def zero_y_noise(n):
+        """A noise read whose only use of y is (double) y * 0.0 (either zero)."""
+        if n['t'] == 'shift_a':
+            return True
+        return n['t'] == 'noise' and n['shift_x_zero'] and n['shift_y_zero'] and n['shift_z_zero'] and f64_from_hex(n['y_scale']) == 0.0
 
     def column_slot(self, node, var, sem):
         ctx = self.ctx
