@@ -850,9 +850,14 @@ MCT_HD inline void biome_group_search(const ModelDev& M, const TreeNodeF* top, i
 MCT_HD inline void biome_quart_final(const ModelDev& M, const int64_t* tvary, const GroupRes* res, int32_t* lstar, int64_t* mind, int G, int idx) {
     const int b = idx / kQPer, q = idx - b * kQPer;
     const GroupRes* r = res + static_cast<size_t>(b * 16 + (q & 15)) * G;
-    if (r[0].leaf == -3) return;
-    const int k = kVaryCoord;
+    // every load of the quart at once (G <= kColGroups; tvary holds every quart's coordinate)
     const int64_t tv = tvary[idx];
+    GroupRes rg[kColGroups];
+#pragma unroll
+    for (int g = 0; g < kColGroups; ++g)
+        if (g < G) rg[g] = r[g];
+    if (rg[0].leaf == -3) return;
+    const int k = kVaryCoord;
     if (tv < -static_cast<int64_t>(kTreeExact) || tv > static_cast<int64_t>(kTreeExact)) {
         lstar[idx] = -2;
         mind[idx] = -1;
@@ -861,13 +866,15 @@ MCT_HD inline void biome_quart_final(const ModelDev& M, const int64_t* tvary, co
     const float v = static_cast<float>(tv);
     double best = u2d(0x7FF0000000000000ull);
     int bl = -1, br = INT32_MAX;
-    for (int g = 0; g < G; ++g) {
+#pragma unroll
+    for (int g = 0; g < kColGroups; ++g) {
+        if (g >= G) break;
         const float p = fmaxf(fmaxf(v - M.col_hi[k][g], M.col_lo[k][g] - v), 0.0f);
-        const double d = r[g].a + static_cast<double>(p) * static_cast<double>(p);
-        if (r[g].leaf >= 0 && (d < best || (d == best && r[g].rank < br))) {
+        const double d = rg[g].a + static_cast<double>(p) * static_cast<double>(p);
+        if (rg[g].leaf >= 0 && (d < best || (d == best && rg[g].rank < br))) {
             best = d;
-            bl = r[g].leaf;
-            br = r[g].rank;
+            bl = rg[g].leaf;
+            br = rg[g].rank;
         }
     }
     lstar[idx] = bl;
