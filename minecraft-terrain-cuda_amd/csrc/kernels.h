@@ -212,7 +212,7 @@ __global__ __launch_bounds__(256) void k_biome_chain(const int* nbk_ptr, Targets
                     const unsigned long long m = __ballot(t && lane >= k);
                     if (m == 0) break;
                     const int j = __builtin_ctzll(m);
-                    const int before = __shfl(ls, j > 0 ? j - 1 : 0);
+                    const int before = __builtin_amdgcn_readlane(ls, j > 0 ? j - 1 : 0);   // j is wave-uniform
                     const int X = j > 0 ? before : prev;
                     if (lane == j) res = X;
                     prev = X;
@@ -233,12 +233,12 @@ __global__ __launch_bounds__(256) void k_biome_chain(const int* nbk_ptr, Targets
                     }
                     const int j = __builtin_ctzll(m);
                     if (lane >= k && lane < j) res = prev;
-                    prev = __shfl(ls, j);
+                    prev = __builtin_amdgcn_readlane(ls, j);
                     sync = true;
                     k = j + 1;
                 }
             }
-            if (sync) prev = __shfl(ls, 63);
+            if (sync) prev = __builtin_amdgcn_readlane(ls, 63);
             biomes[i] = M.leaf_biome[res];
             changed += static_cast<unsigned long long>(__popcll(__ballot(res != ls)));
             changed_biome += static_cast<unsigned long long>(__popcll(__ballot(res != ls && M.leaf_biome[res] != M.leaf_biome[ls])));
